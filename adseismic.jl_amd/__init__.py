@@ -1,0 +1,18 @@
+"""MI355X-native wave-propagation hot path of ADSeismic.jl (acoustic PML and elastic
+staggered-grid time steppers, hand-written adjoints, batched checkpointed FWI gradients).
+
+All compute runs in ``lib/libadseismic_hip.so`` (HIP, gfx950) through the C-ABI declared in
+``include/adseismic.h``; this package is the host-side mirror of the reference's Julia API.
+The directory name is not a Python identifier: load it with ``__graft_entry__.load_package()``
+(registered as the module ``adseismic_amd``).
+"""
+from ._lib import ADSError, load as load_library  # noqa: F401
+from .params import (AcousticPropagatorParams, AcousticReceiver, AcousticSource,  # noqa: F401
+                     ElasticPropagatorParams, ElasticReceiver, ElasticSource, Geometry)
+from .acoustic import (AcousticPlan, AcousticPropagator, AcousticPropagatorSolver,  # noqa: F401
+                       SimulatedObservation_, acoustic_one_step, acoustic_one_step_customop,
+                       acoustic_traces, check_cfl, compute_PML_Params_, getid2)
+from .utils import (Gauss, Ricker, compute_lame_parameters, compute_properties,  # noqa: F401
+                    ricker_freq)
+
+SimulatedObservation = SimulatedObservation_

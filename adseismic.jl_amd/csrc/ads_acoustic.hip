@@ -1,0 +1,953 @@
+// ads_acoustic.hip -- acoustic PML time stepper for gfx950: forward, hand-written adjoint,
+// checkpointed multi-shot batches, full-history drop-in.
+//
+// Scheme (src/Core.jl:528-549 / AcousticOneStepCpu.h:1-48), interior cell (i, j):
+//   u^n = [(2 - s t dt^2) w + C lap(w) + ex (phi_{i+1} - phi_{i-1}) + ey (psi_{j+1} - psi_{j-1})
+//          - (1 - (s+t) dt/2) wold] / (1 + (s+t) dt/2),
+//   lap(w) = cx (w_{i+1} + w_{i-1} - 2w) + cy (w_{j+1} + w_{j-1} - 2w),  cx = dt^2/hx^2
+//   phi^n = (1 - dt s) phi + C (t - s) fx (v_{i+1} - v_{i-1}),  fx = dt/(2hx)
+//   psi^n = (1 - dt t) psi + C (s - t) fy (v_{j+1} - v_{j-1})
+// with w = u^{n-1}, wold = u^{n-2}, v = u^n before source injection (PropagatorKernel 0) or
+// v = w (PropagatorKernel 1/2).  s = sigma_x(i), t = tau_y(j) (separable, Core.jl:738-761).
+// phi/psi are non-zero only where s or t is (the PML strips); interior cells skip them.
+//
+// Adjoint (exact transpose of the discrete loop; P^n = dL/du^n / d, d = 1 + (s+t) dt/2):
+//   dL/du^n = R^n + (2 - s t dt^2) P^{n+1} + lap(C P^{n+1}) - (1 - (s+t) dt/2) P^{n+2}
+//             + fx [Cp_{i-1} F_{i-1} - Cp_{i+1} F_{i+1}] + fy [Cq_{j-1} G_{j-1} - Cq_{j+1} G_{j+1}]
+//   F^n = (1 - dt s) F^{n+1} + ex (P^{n+1}_{i-1} - P^{n+1}_{i+1})    (dL/dphi^n), G likewise
+//   dL/dC += P^n lap(u^{n-1}) + F^n (t - s) fx (v_{i+1} - v_{i-1}) + G^n (s - t) fy (...)
+// where Cp = C (t - s), Cq = C (s - t) and F in the dL/du^n line is F^{n+1} (kernel 1) or F^n
+// (kernel 0).  Receiver residuals R^n are added after the stencil pass by a per-cell gather.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "ads_common.h"
+
+namespace ads {
+
+template <typename T> struct AcCoef {
+    T cx, cy, ex, ey, fx, fy, dt, hdt, dt2;
+};
+
+template <typename T> AcCoef<T> make_coef(double dt, double hx, double hy)
+{
+    AcCoef<T> k;
+    k.cx = (T)(dt * dt / (hx * hx));
+    k.cy = (T)(dt * dt / (hy * hy));
+    k.ex = (T)(dt * dt / (2.0 * hx));
+    k.ey = (T)(dt * dt / (2.0 * hy));
+    k.fx = (T)(dt / (2.0 * hx));
+    k.fy = (T)(dt / (2.0 * hy));
+    k.dt = (T)dt;
+    k.hdt = (T)(dt / 2.0);
+    k.dt2 = (T)(dt * dt);
+    return k;
+}
+
+// Field addressing: level pointer + shot * stride.
+template <typename T> struct Lv {
+    T *p;
+    int64_t stride;
+    __device__ __host__ T *at(int s) const { return p + (int64_t)s * stride; }
+};
+
+constexpr int BX = 64;  // lanes along j (depth, contiguous)
+constexpr int BY = 4;   // rows along i
+
+// ---------------------------------------------------------------------------------------------
+// forward step: u^n (pre-source) and, for kernel 1, phi^n/psi^n.  One thread per cell.
+// ---------------------------------------------------------------------------------------------
+template <typename T, int V>
+__global__ __launch_bounds__(BX *BY) void ac_fwd_step(
+    Lv<T> uo, Lv<const T> w_, Lv<const T> wold_, Lv<const T> phi_, Lv<const T> psi_, Lv<T> phio,
+    Lv<T> psio, const T *__restrict__ C, const T *__restrict__ sx, const T *__restrict__ ty,
+    const uint8_t *__restrict__ nearx, const uint8_t *__restrict__ neary, int NX, int NY,
+    AcCoef<T> k)
+{
+    const int j = blockIdx.x * BX + threadIdx.x;
+    const int i = blockIdx.y * BY + threadIdx.y;
+    const int s = blockIdx.z;
+    const int S = NY + 2;
+    if (j >= S || i >= NX + 2) return;
+    const int64_t ij = (int64_t)i * S + j;
+    T *__restrict__ u = uo.at(s);
+    if (i == 0 || i == NX + 1 || j == 0 || j == NY + 1) {
+        u[ij] = T(0);
+        return;
+    }
+    const T *__restrict__ w = w_.at(s);
+    const T wc = w[ij], wn = w[ij - S], ws = w[ij + S], ww = w[ij - 1], we = w[ij + 1];
+    const T Cij = C[ij];
+    const T lap = k.cx * (ws + wn - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
+    const T wo = wold_.at(s)[ij];
+    if (!(nearx[i] | neary[j])) {
+        u[ij] = T(2) * wc - wo + Cij * lap;
+        return;
+    }
+    const T sg = sx[i], tu = ty[j];
+    const T *__restrict__ phi = phi_.at(s);
+    const T *__restrict__ psi = psi_.at(s);
+    T num = (T(2) - sg * tu * k.dt2) * wc + Cij * lap + k.ex * (phi[ij + S] - phi[ij - S]) +
+            k.ey * (psi[ij + 1] - psi[ij - 1]) - (T(1) - (sg + tu) * k.hdt) * wo;
+    u[ij] = num / (T(1) + (sg + tu) * k.hdt);
+    if (V == 1 && (sg != T(0) || tu != T(0))) {
+        phio.at(s)[ij] = (T(1) - k.dt * sg) * phi[ij] + Cij * (tu - sg) * k.fx * (ws - wn);
+        psio.at(s)[ij] = (T(1) - k.dt * tu) * psi[ij] + Cij * (sg - tu) * k.fy * (we - ww);
+    }
+}
+
+// kernel 0: phi^n/psi^n from the new u (before sources), PML cells only.
+template <typename T>
+__global__ __launch_bounds__(BX *BY) void ac_fwd_phipsi(
+    Lv<const T> u_, Lv<const T> phi_, Lv<const T> psi_, Lv<T> phio, Lv<T> psio,
+    const T *__restrict__ C, const T *__restrict__ sx, const T *__restrict__ ty,
+    const uint8_t *__restrict__ pmlx, const uint8_t *__restrict__ pmly, int NX, int NY,
+    AcCoef<T> k)
+{
+    const int j = blockIdx.x * BX + threadIdx.x;
+    const int i = blockIdx.y * BY + threadIdx.y;
+    const int s = blockIdx.z;
+    const int S = NY + 2;
+    if (j < 1 || j > NY || i < 1 || i > NX) return;
+    if (!(pmlx[i] | pmly[j])) return;
+    const int64_t ij = (int64_t)i * S + j;
+    const T *__restrict__ u = u_.at(s);
+    const T sg = sx[i], tu = ty[j], Cij = C[ij];
+    phio.at(s)[ij] = (T(1) - k.dt * sg) * phi_.at(s)[ij] + Cij * (tu - sg) * k.fx * (u[ij + S] - u[ij - S]);
+    psio.at(s)[ij] = (T(1) - k.dt * tu) * psi_.at(s)[ij] + Cij * (sg - tu) * k.fy * (u[ij + 1] - u[ij - 1]);
+}
+
+// Per-shot source injection at level n (u[cell] += dt^2 * srcv[n-1, q] in source order,
+// Core.jl:593-601), then receiver sampling trace[n, r] = u[rcv_r] (Core.jl:726-730).
+// One workgroup per shot; shots are in plan order.
+template <typename T>
+__global__ __launch_bounds__(256) void ac_src_rcv(
+    Lv<T> u_, int n, const int64_t *__restrict__ us_ptr, const int64_t *__restrict__ us_cell,
+    const int64_t *__restrict__ us_eptr, const int64_t *__restrict__ us_ent,
+    const T *__restrict__ srcv, const int64_t *__restrict__ srcv_off,
+    const int64_t *__restrict__ nsrc, T dt2, const int64_t *__restrict__ rcv_ptr,
+    const int64_t *__restrict__ rcv_cell, T *__restrict__ traces,
+    const int64_t *__restrict__ trace_off, int record)
+{
+    const int s = blockIdx.x;
+    T *__restrict__ u = u_.at(s);
+    const int64_t ns = nsrc[s];
+    const T *__restrict__ row = srcv + srcv_off[s] + (int64_t)(n - 1) * ns;
+    for (int64_t c = us_ptr[s] + threadIdx.x; c < us_ptr[s + 1]; c += blockDim.x) {
+        const int64_t cell = us_cell[c];
+        T acc = u[cell];
+        for (int64_t e = us_eptr[c]; e < us_eptr[c + 1]; e++) acc += row[us_ent[e]] * dt2;
+        u[cell] = acc;
+    }
+    if (!record) return;
+    __syncthreads();
+    const int64_t r0 = rcv_ptr[s], nr = rcv_ptr[s + 1] - r0;
+    T *__restrict__ tr = traces + trace_off[s] + (int64_t)n * nr;
+    for (int64_t r = threadIdx.x; r < nr; r += blockDim.x) tr[r] = u[rcv_cell[r0 + r]];
+}
+
+// ---------------------------------------------------------------------------------------------
+// adjoint
+// ---------------------------------------------------------------------------------------------
+// kernel 0 only: F^n, G^n on PML cells and their dL/dC term with v = u^n before sources.
+template <typename T>
+__global__ __launch_bounds__(BX *BY) void ac_adj_phipsi(
+    Lv<const T> P1_, Lv<const T> F1_, Lv<const T> G1_, Lv<T> F0_, Lv<T> G0_, Lv<const T> un_,
+    Lv<T> gC_, int n, const T *__restrict__ C, const T *__restrict__ sx, const T *__restrict__ ty,
+    const uint8_t *__restrict__ pmlx, const uint8_t *__restrict__ pmly,
+    const int64_t *__restrict__ us_ptr, const int64_t *__restrict__ us_cell,
+    const int64_t *__restrict__ us_eptr, const int64_t *__restrict__ us_ent,
+    const T *__restrict__ srcv, const int64_t *__restrict__ srcv_off,
+    const int64_t *__restrict__ nsrc, int NX, int NY, AcCoef<T> k)
+{
+    const int j = blockIdx.x * BX + threadIdx.x;
+    const int i = blockIdx.y * BY + threadIdx.y;
+    const int s = blockIdx.z;
+    const int S = NY + 2;
+    if (j < 1 || j > NY || i < 1 || i > NX) return;
+    if (!(pmlx[i] | pmly[j])) return;
+    const int64_t ij = (int64_t)i * S + j;
+    const T *__restrict__ P1 = P1_.at(s);
+    const T sg = sx[i], tu = ty[j];
+    const T F0 = (T(1) - k.dt * sg) * F1_.at(s)[ij] + k.ex * (P1[ij - S] - P1[ij + S]);
+    const T G0 = (T(1) - k.dt * tu) * G1_.at(s)[ij] + k.ey * (P1[ij - 1] - P1[ij + 1]);
+    F0_.at(s)[ij] = F0;
+    G0_.at(s)[ij] = G0;
+    const T *__restrict__ u = un_.at(s);
+    T vn = u[ij - S], vs = u[ij + S], vw = u[ij - 1], ve = u[ij + 1];
+    // remove this level's sources from the neighbours (v is u^n before injection)
+    const int64_t ns = nsrc[s];
+    const T *__restrict__ row = srcv + srcv_off[s] + (int64_t)(n - 1) * ns;
+    for (int64_t c = us_ptr[s]; c < us_ptr[s + 1]; c++) {
+        const int64_t cell = us_cell[c];
+        if (cell != ij - S && cell != ij + S && cell != ij - 1 && cell != ij + 1) continue;
+        T add = T(0);
+        for (int64_t e = us_eptr[c]; e < us_eptr[c + 1]; e++) add += row[us_ent[e]] * k.dt2;
+        if (cell == ij - S) vn -= add;
+        if (cell == ij + S) vs -= add;
+        if (cell == ij - 1) vw -= add;
+        if (cell == ij + 1) ve -= add;
+    }
+    const T Cij = C[ij];
+    (void)Cij;
+    gC_.at(s)[ij] += F0 * (tu - sg) * k.fx * (vs - vn) + G0 * (sg - tu) * k.fy * (ve - vw);
+}
+
+template <typename T, int V>
+__global__ __launch_bounds__(BX *BY) void ac_adj_step(
+    Lv<T> P_, Lv<const T> P1_, Lv<const T> Fn_, Lv<const T> Gn_, Lv<T> F0_, Lv<T> G0_,
+    Lv<const T> um1_, Lv<const T> dense_, Lv<T> gC_, const T *__restrict__ C,
+    const T *__restrict__ sx, const T *__restrict__ ty, const uint8_t *__restrict__ nearx,
+    const uint8_t *__restrict__ neary, int NX, int NY, AcCoef<T> k)
+{
+    const int j = blockIdx.x * BX + threadIdx.x;
+    const int i = blockIdx.y * BY + threadIdx.y;
+    const int s = blockIdx.z;
+    const int S = NY + 2;
+    if (j < 1 || j > NY || i < 1 || i > NX) return;
+    const int64_t ij = (int64_t)i * S + j;
+    const T *__restrict__ P1 = P1_.at(s);
+    T *__restrict__ P = P_.at(s);
+    const T p1c = P1[ij], p1n = P1[ij - S], p1s = P1[ij + S], p1w = P1[ij - 1], p1e = P1[ij + 1];
+    const T Cc = C[ij], Cn = C[ij - S], Cs = C[ij + S], Cw = C[ij - 1], Ce = C[ij + 1];
+    const T qc = Cc * p1c;
+    const T lapq = k.cx * (Cs * p1s + Cn * p1n - T(2) * qc) + k.cy * (Ce * p1e + Cw * p1w - T(2) * qc);
+    const T p2 = P[ij];
+    const bool near = nearx[i] | neary[j];
+    T sg = T(0), tu = T(0);
+    T U;
+    if (!near) {
+        U = T(2) * p1c + lapq - p2;
+    } else {
+        sg = sx[i];
+        tu = ty[j];
+        const T *__restrict__ Fn = Fn_.at(s);
+        const T *__restrict__ Gn = Gn_.at(s);
+        U = (T(2) - sg * tu * k.dt2) * p1c + lapq - (T(1) - (sg + tu) * k.hdt) * p2 +
+            k.fx * (Cn * (tu - sx[i - 1]) * Fn[ij - S] - Cs * (tu - sx[i + 1]) * Fn[ij + S]) +
+            k.fy * (Cw * (sg - ty[j - 1]) * Gn[ij - 1] - Ce * (sg - ty[j + 1]) * Gn[ij + 1]);
+    }
+    if (dense_.p) U += dense_.at(s)[ij];
+    const T Pn = near ? U / (T(1) + (sg + tu) * k.hdt) : U;
+    P[ij] = Pn;
+    const T *__restrict__ um = um1_.at(s);
+    const T uc = um[ij], un = um[ij - S], us = um[ij + S], uw = um[ij - 1], ue = um[ij + 1];
+    T g = Pn * (k.cx * (us + un - T(2) * uc) + k.cy * (ue + uw - T(2) * uc));
+    if (V == 1 && near && (sg != T(0) || tu != T(0))) {
+        const T F0 = (T(1) - k.dt * sg) * Fn_.at(s)[ij] + k.ex * (p1n - p1s);
+        const T G0 = (T(1) - k.dt * tu) * Gn_.at(s)[ij] + k.ey * (p1w - p1e);
+        F0_.at(s)[ij] = F0;
+        G0_.at(s)[ij] = G0;
+        g += F0 * (tu - sg) * k.fx * (us - un) + G0 * (sg - tu) * k.fy * (ue - uw);
+    }
+    gC_.at(s)[ij] += g;
+}
+
+// Receiver residual injection at level n after the stencil pass: for each distinct interior
+// receiver cell, R = sum of its receivers' residuals; P += R/d; dL/dC += (R/d) lap(u^{n-1}).
+template <typename T>
+__global__ __launch_bounds__(256) void ac_adj_inject(
+    Lv<T> P_, Lv<const T> um1_, Lv<T> gC_, int n, const int64_t *__restrict__ ur_ptr,
+    const int64_t *__restrict__ ur_cell, const int64_t *__restrict__ ur_eptr,
+    const int64_t *__restrict__ ur_ent, const int64_t *__restrict__ rcv_ptr,
+    const T *__restrict__ resid, const int64_t *__restrict__ trace_off, const T *__restrict__ sx,
+    const T *__restrict__ ty, int NX, int NY, AcCoef<T> k)
+{
+    const int s = blockIdx.y;
+    const int64_t c = ur_ptr[s] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ur_ptr[s + 1]) return;
+    const int S = NY + 2;
+    const int64_t cell = ur_cell[c];
+    const int i = (int)(cell / S), j = (int)(cell % S);
+    if (i < 1 || i > NX || j < 1 || j > NY) return;  // ring cells carry no gradient
+    const int64_t nr = rcv_ptr[s + 1] - rcv_ptr[s];
+    const T *__restrict__ row = resid + trace_off[s] + (int64_t)n * nr;
+    T R = T(0);
+    for (int64_t e = ur_eptr[c]; e < ur_eptr[c + 1]; e++) R += row[ur_ent[e]];
+    const T add = R / (T(1) + (sx[i] + ty[j]) * k.hdt);
+    P_.at(s)[cell] += add;
+    const T *__restrict__ um = um1_.at(s);
+    const T uc = um[cell];
+    gC_.at(s)[cell] += add * (k.cx * (um[cell + S] + um[cell - S] - T(2) * uc) +
+                              k.cy * (um[cell + 1] + um[cell - 1] - T(2) * uc));
+}
+
+// misc elementwise kernels
+template <typename T>
+__global__ void k_square(const T *__restrict__ m, T *__restrict__ C, int64_t n, int sq)
+{
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) C[q] = sq ? m[q] * m[q] : m[q];
+}
+
+template <typename T>
+__global__ void k_reduce_grad(const T *__restrict__ gC, int64_t stride, int nshots,
+                              const T *__restrict__ m, T *__restrict__ g, int64_t n, int sq)
+{
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    T acc = T(0);
+    for (int s = 0; s < nshots; s++) acc += gC[(int64_t)s * stride + q];
+    g[q] = sq ? T(2) * m[q] * acc : acc;
+}
+
+// resid = 2 (traces - observed); loss += sum (traces - observed)^2 (double accumulation).
+template <typename T>
+__global__ __launch_bounds__(256) void k_misfit(const T *__restrict__ tr, const T *__restrict__ obs,
+                                                T *__restrict__ resid, int64_t n,
+                                                double *__restrict__ loss)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        T d = tr[q] - obs[q];
+        resid[q] = T(2) * d;
+        acc += (double)d * (double)d;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------------------------
+struct AcPlanBase {
+    virtual ~AcPlanBase() = default;
+    ads_acoustic_params p{};
+    int dtype = ADS_F32;
+    int variant = 0;  // 0 or 1
+    int64_t NX = 0, NY = 0, S = 0, Npad = 0, NMAX = 0, nshots = 0;
+    std::vector<int64_t> order;        // plan position -> caller shot
+    std::vector<int64_t> nstep_sorted; // per plan position
+    std::vector<int64_t> srcv_off_caller, trace_off_caller;  // per caller shot
+    int64_t srcv_elems = 0, trace_elems = 0;
+    std::vector<int> active;  // active[n] = #shots with nstep >= n
+    std::vector<double> sx, ty;
+    std::vector<uint8_t> pmlx, pmly, nearx, neary;
+    CellLists usrc, urcv;
+    // device tables
+    DevBuf d_sx, d_ty, d_pmlx, d_pmly, d_nearx, d_neary;
+    DevBuf d_us_ptr, d_us_cell, d_us_eptr, d_us_ent, d_nsrc, d_srcv_off;
+    DevBuf d_ur_ptr, d_ur_cell, d_ur_eptr, d_ur_ent, d_rcv_ptr, d_rcv_cell, d_trace_off;
+    int64_t max_ucells_rcv = 0;
+};
+
+static int validate_params(const ads_acoustic_params *p)
+{
+    if (!p) ADS_FAIL(ADS_ERR_ARG, "null params");
+    if (p->NX < 1 || p->NY < 1 || p->NSTEP < 1)
+        ADS_FAIL(ADS_ERR_SHAPE, "NX, NY, NSTEP must be >= 1 (got %lld, %lld, %lld)",
+                 (long long)p->NX, (long long)p->NY, (long long)p->NSTEP);
+    if ((p->NX + 2) * (p->NY + 2) >= (int64_t)1 << 31)
+        ADS_FAIL(ADS_ERR_SHAPE, "grid too large for 32-bit cell indexing within a shot");
+    if (!(p->DELTAT > 0) || !(p->DELTAX > 0) || !(p->DELTAY > 0))
+        ADS_FAIL(ADS_ERR_ARG, "DELTAT, DELTAX, DELTAY must be > 0");
+    if (p->PropagatorKernel < 0 || p->PropagatorKernel > 2)
+        ADS_FAIL(ADS_ERR_ARG, "PropagatorKernel must be 0, 1 or 2");
+    return ADS_OK;
+}
+
+template <typename T> struct AcPlan : AcPlanBase {
+    AcCoef<T> k;
+
+    // ---- launch helpers ----
+    dim3 grid_cells(int act) const { return dim3(cdiv(S, BX), cdiv(NX + 2, BY), act); }
+
+    int fwd_step(int n, int act, Lv<T> uo, Lv<const T> w, Lv<const T> wo, Lv<const T> phi,
+                 Lv<const T> psi, Lv<T> phio, Lv<T> psio, const T *C, const T *srcv, T *traces,
+                 int record, hipStream_t st)
+    {
+        if (act <= 0) return ADS_OK;
+        dim3 b(BX, BY);
+        if (variant == 1)
+            ac_fwd_step<T, 1><<<grid_cells(act), b, 0, st>>>(uo, w, wo, phi, psi, phio, psio, C,
+                                                             d_sx.as<T>(), d_ty.as<T>(),
+                                                             d_nearx.as<uint8_t>(),
+                                                             d_neary.as<uint8_t>(), (int)NX, (int)NY, k);
+        else {
+            ac_fwd_step<T, 0><<<grid_cells(act), b, 0, st>>>(uo, w, wo, phi, psi, phio, psio, C,
+                                                             d_sx.as<T>(), d_ty.as<T>(),
+                                                             d_nearx.as<uint8_t>(),
+                                                             d_neary.as<uint8_t>(), (int)NX, (int)NY, k);
+            ac_fwd_phipsi<T><<<grid_cells(act), b, 0, st>>>(
+                Lv<const T>{uo.p, uo.stride}, phi, psi, phio, psio, C, d_sx.as<T>(), d_ty.as<T>(),
+                d_pmlx.as<uint8_t>(), d_pmly.as<uint8_t>(), (int)NX, (int)NY, k);
+        }
+        ac_src_rcv<T><<<act, 256, 0, st>>>(uo, n, d_us_ptr.as<int64_t>(), d_us_cell.as<int64_t>(),
+                                           d_us_eptr.as<int64_t>(), d_us_ent.as<int64_t>(), srcv,
+                                           d_srcv_off.as<int64_t>(), d_nsrc.as<int64_t>(), k.dt2,
+                                           d_rcv_ptr.as<int64_t>(), d_rcv_cell.as<int64_t>(),
+                                           traces, d_trace_off.as<int64_t>(), record);
+        ADS_LAUNCH_CHECK();
+        return ADS_OK;
+    }
+
+    int adj_level(int n, int act, Lv<T> P, Lv<const T> P1, Lv<T> F0, Lv<T> G0, Lv<const T> F1,
+                  Lv<const T> G1, Lv<const T> um1, Lv<const T> un, Lv<const T> dense, Lv<T> gC,
+                  const T *C, const T *srcv, const T *resid, hipStream_t st)
+    {
+        if (act <= 0) return ADS_OK;
+        dim3 b(BX, BY);
+        if (variant == 1) {
+            ac_adj_step<T, 1><<<grid_cells(act), b, 0, st>>>(
+                P, P1, F1, G1, F0, G0, um1, dense, gC, C, d_sx.as<T>(), d_ty.as<T>(),
+                d_nearx.as<uint8_t>(), d_neary.as<uint8_t>(), (int)NX, (int)NY, k);
+        } else {
+            ac_adj_phipsi<T><<<grid_cells(act), b, 0, st>>>(
+                P1, F1, G1, F0, G0, un, gC, n, C, d_sx.as<T>(), d_ty.as<T>(), d_pmlx.as<uint8_t>(),
+                d_pmly.as<uint8_t>(), d_us_ptr.as<int64_t>(), d_us_cell.as<int64_t>(),
+                d_us_eptr.as<int64_t>(), d_us_ent.as<int64_t>(), srcv, d_srcv_off.as<int64_t>(),
+                d_nsrc.as<int64_t>(), (int)NX, (int)NY, k);
+            ac_adj_step<T, 0><<<grid_cells(act), b, 0, st>>>(
+                P, P1, Lv<const T>{F0.p, F0.stride}, Lv<const T>{G0.p, G0.stride}, F0, G0, um1,
+                dense, gC, C, d_sx.as<T>(), d_ty.as<T>(), d_nearx.as<uint8_t>(),
+                d_neary.as<uint8_t>(), (int)NX, (int)NY, k);
+        }
+        if (resid && max_ucells_rcv > 0) {
+            dim3 g(cdiv(max_ucells_rcv, 256), act);
+            ac_adj_inject<T><<<g, 256, 0, st>>>(P, um1, gC, n, d_ur_ptr.as<int64_t>(),
+                                                d_ur_cell.as<int64_t>(), d_ur_eptr.as<int64_t>(),
+                                                d_ur_ent.as<int64_t>(), d_rcv_ptr.as<int64_t>(),
+                                                resid, d_trace_off.as<int64_t>(), d_sx.as<T>(),
+                                                d_ty.as<T>(), (int)NX, (int)NY, k);
+        }
+        ADS_LAUNCH_CHECK();
+        return ADS_OK;
+    }
+
+    // ---- workspace layout for checkpoint interval K ----
+    struct WS {
+        int64_t K, NS, M;  // interval, slots, segments
+        T *seg, *phi[2], *psi[2], *P[2], *F[2], *G[2], *gC, *ckpt, *C, *resid;
+        double *loss;
+    };
+    int64_t seg_count(int64_t K) const { return NMAX >= 2 ? (NMAX - 1 + K - 1) / K : 0; }
+    void layout(WS &w, Carve &cv, int64_t K) const
+    {
+        if (K < 1) K = 1;
+        if (K > std::max<int64_t>(NMAX - 1, 1)) K = std::max<int64_t>(NMAX - 1, 1);
+        w.K = K;
+        w.M = seg_count(K);
+        w.NS = std::min<int64_t>(K + 2, NMAX + 1);
+        const size_t F = (size_t)nshots * Npad * sizeof(T);
+        w.seg = (T *)cv.take(F * w.NS);
+        for (int q = 0; q < 2; q++) {
+            w.phi[q] = (T *)cv.take(F);
+            w.psi[q] = (T *)cv.take(F);
+            w.P[q] = (T *)cv.take(F);
+            w.F[q] = (T *)cv.take(F);
+            w.G[q] = (T *)cv.take(F);
+        }
+        w.gC = (T *)cv.take(F);
+        w.ckpt = (T *)cv.take(F * 4 * (size_t)std::max<int64_t>(w.M - 1, 0));
+        w.C = (T *)cv.take((size_t)Npad * sizeof(T));
+        w.resid = (T *)cv.take((size_t)std::max<int64_t>(trace_elems, 1) * sizeof(T));
+        w.loss = (double *)cv.take(64);
+    }
+    size_t ws_size(int64_t K) const
+    {
+        WS w;
+        Carve cv(nullptr, 0);
+        layout(w, cv, K);
+        return cv.off + 256;
+    }
+
+    Lv<T> slot(const WS &w, int64_t n) const
+    {
+        return Lv<T>{w.seg + (size_t)(n % w.NS) * nshots * Npad, Npad};
+    }
+    static Lv<const T> cst(Lv<T> a) { return Lv<const T>{a.p, a.stride}; }
+
+    int prepare_C(const WS &w, const void *model, hipStream_t st)
+    {
+        k_square<T><<<cdiv(Npad, 256), 256, 0, st>>>((const T *)model, w.C, Npad, p.square_velocity);
+        ADS_LAUNCH_CHECK();
+        return ADS_OK;
+    }
+
+    // forward over levels lo+1..hi from the state held in slots/phi (record traces if asked)
+    int run_forward(const WS &w, int64_t lo, int64_t hi, const T *srcv, T *traces, int record,
+                    hipStream_t st)
+    {
+        for (int64_t n = lo + 1; n <= hi; n++) {
+            int act = active[n];
+            Lv<T> pi{w.phi[(n - 1) & 1], Npad}, si{w.psi[(n - 1) & 1], Npad};
+            Lv<T> po{w.phi[n & 1], Npad}, so{w.psi[n & 1], Npad};
+            ADS_TRY(fwd_step((int)n, act, slot(w, n), cst(slot(w, n - 1)), cst(slot(w, n - 2)),
+                             cst(pi), cst(si), po, so, w.C, srcv, traces, record, st));
+        }
+        return ADS_OK;
+    }
+
+    int save_ckpt(const WS &w, int64_t m, int64_t a, hipStream_t st)
+    {
+        // state at level a: u^a, u^{a-1}, phi^a, psi^a (active shots are a prefix)
+        int act = active[std::min<int64_t>(a + 1, NMAX)];
+        size_t bytes = (size_t)act * Npad * sizeof(T);
+        T *dst = w.ckpt + (size_t)(m - 1) * 4 * nshots * Npad;
+        size_t F = (size_t)nshots * Npad;
+        ADS_HIP(hipMemcpyAsync(dst, slot(w, a).p, bytes, hipMemcpyDeviceToDevice, st));
+        ADS_HIP(hipMemcpyAsync(dst + F, slot(w, a - 1).p, bytes, hipMemcpyDeviceToDevice, st));
+        ADS_HIP(hipMemcpyAsync(dst + 2 * F, w.phi[a & 1], bytes, hipMemcpyDeviceToDevice, st));
+        ADS_HIP(hipMemcpyAsync(dst + 3 * F, w.psi[a & 1], bytes, hipMemcpyDeviceToDevice, st));
+        return ADS_OK;
+    }
+
+    int restore_ckpt(const WS &w, int64_t m, int64_t a, hipStream_t st)
+    {
+        int act = active[std::min<int64_t>(a + 1, NMAX)];
+        size_t bytes = (size_t)act * Npad * sizeof(T);
+        size_t F = (size_t)nshots * Npad;
+        if (m == 0) {  // level 1: everything zero
+            ADS_HIP(hipMemsetAsync(slot(w, 0).p, 0, bytes, st));
+            ADS_HIP(hipMemsetAsync(slot(w, 1).p, 0, bytes, st));
+            ADS_HIP(hipMemsetAsync(w.phi[1], 0, bytes, st));
+            ADS_HIP(hipMemsetAsync(w.psi[1], 0, bytes, st));
+            return ADS_OK;
+        }
+        const T *src = w.ckpt + (size_t)(m - 1) * 4 * nshots * Npad;
+        ADS_HIP(hipMemcpyAsync(slot(w, a).p, src, bytes, hipMemcpyDeviceToDevice, st));
+        ADS_HIP(hipMemcpyAsync(slot(w, a - 1).p, src + F, bytes, hipMemcpyDeviceToDevice, st));
+        ADS_HIP(hipMemcpyAsync(w.phi[a & 1], src + 2 * F, bytes, hipMemcpyDeviceToDevice, st));
+        ADS_HIP(hipMemcpyAsync(w.psi[a & 1], src + 3 * F, bytes, hipMemcpyDeviceToDevice, st));
+        return ADS_OK;
+    }
+
+    int forward(const void *model, const void *srcv, void *traces, void *ws, size_t wsb,
+                int64_t K, hipStream_t st)
+    {
+        WS w;
+        Carve cv(ws, wsb);
+        layout(w, cv, K);
+        if (!ws || !cv.ok())
+            ADS_FAIL(ADS_ERR_WORKSPACE, "workspace too small: need %zu bytes, got %zu", cv.off, wsb);
+        const size_t F = (size_t)nshots * Npad * sizeof(T);
+        ADS_TRY(prepare_C(w, model, st));
+        ADS_HIP(hipMemsetAsync(slot(w, 0).p, 0, F, st));
+        ADS_HIP(hipMemsetAsync(slot(w, 1).p, 0, F, st));
+        for (int q = 0; q < 2; q++) {
+            ADS_HIP(hipMemsetAsync(w.phi[q], 0, F, st));
+            ADS_HIP(hipMemsetAsync(w.psi[q], 0, F, st));
+        }
+        if (traces && trace_elems) ADS_HIP(hipMemsetAsync(traces, 0, trace_elems * sizeof(T), st));
+        for (int64_t m = 0; m < w.M; m++) {
+            int64_t a = 1 + m * w.K, b = std::min<int64_t>(a + w.K, NMAX);
+            if (m > 0) ADS_TRY(save_ckpt(w, m, a, st));
+            ADS_TRY(run_forward(w, a, b, (const T *)srcv, (T *)traces, traces != nullptr, st));
+        }
+        return ADS_OK;
+    }
+
+    int backward(const void *model, const void *srcv, const T *resid, void *grad, void *ws,
+                 size_t wsb, int64_t K, hipStream_t st)
+    {
+        WS w;
+        Carve cv(ws, wsb);
+        layout(w, cv, K);
+        if (!ws || !cv.ok())
+            ADS_FAIL(ADS_ERR_WORKSPACE, "workspace too small: need %zu bytes, got %zu", cv.off, wsb);
+        const size_t F = (size_t)nshots * Npad * sizeof(T);
+        for (int q = 0; q < 2; q++) {
+            ADS_HIP(hipMemsetAsync(w.P[q], 0, F, st));
+            ADS_HIP(hipMemsetAsync(w.F[q], 0, F, st));
+            ADS_HIP(hipMemsetAsync(w.G[q], 0, F, st));
+        }
+        ADS_HIP(hipMemsetAsync(w.gC, 0, F, st));
+        const T *sv = (const T *)srcv;
+        for (int64_t m = w.M - 1; m >= 0; m--) {
+            int64_t a = 1 + m * w.K, b = std::min<int64_t>(a + w.K, NMAX);
+            if (m < w.M - 1) {
+                ADS_TRY(restore_ckpt(w, m, a, st));
+                ADS_TRY(run_forward(w, a, b, sv, nullptr, 0, st));
+            }
+            for (int64_t n = b; n > a; n--) {
+                int act = active[n];
+                Lv<T> P{w.P[n & 1], Npad};
+                Lv<T> P1{w.P[(n + 1) & 1], Npad};
+                Lv<T> F0{w.F[n & 1], Npad}, G0{w.G[n & 1], Npad};
+                Lv<T> F1{w.F[(n + 1) & 1], Npad}, G1{w.G[(n + 1) & 1], Npad};
+                ADS_TRY(adj_level((int)n, act, P, cst(P1), F0, G0, cst(F1), cst(G1),
+                                  cst(slot(w, n - 1)), cst(slot(w, n)), Lv<const T>{nullptr, 0},
+                                  Lv<T>{w.gC, Npad}, w.C, sv, resid, st));
+            }
+        }
+        k_reduce_grad<T><<<cdiv(Npad, 256), 256, 0, st>>>(w.gC, Npad, (int)nshots,
+                                                          (const T *)model, (T *)grad, Npad,
+                                                          p.square_velocity);
+        ADS_LAUNCH_CHECK();
+        return ADS_OK;
+    }
+
+    int misfit(const WS &w, const T *traces, const T *obs, hipStream_t st)
+    {
+        ADS_HIP(hipMemsetAsync(w.loss, 0, sizeof(double), st));
+        unsigned nb = std::min<unsigned>(cdiv(trace_elems, 256), 2048u);
+        if (nb == 0) nb = 1;
+        k_misfit<T><<<nb, 256, 0, st>>>(traces, obs, w.resid, trace_elems, w.loss);
+        ADS_LAUNCH_CHECK();
+        return ADS_OK;
+    }
+
+    // ---- full-history drop-in (every shot has NSTEP levels) ----
+    int forward_history(const void *model, const void *srcv, void *u_hist, void *phi_hist,
+                        void *psi_hist, void *scratch, hipStream_t st)
+    {
+        const int64_t L = NMAX + 1;
+        const size_t H = (size_t)nshots * L * Npad * sizeof(T);
+        T *C = (T *)scratch;
+        T *pp[2] = {(T *)scratch + Npad, (T *)scratch + Npad + (size_t)nshots * Npad};
+        T *qq[2] = {(T *)scratch + Npad + 2 * (size_t)nshots * Npad,
+                    (T *)scratch + Npad + 3 * (size_t)nshots * Npad};
+        k_square<T><<<cdiv(Npad, 256), 256, 0, st>>>((const T *)model, C, Npad, p.square_velocity);
+        ADS_HIP(hipMemsetAsync(u_hist, 0, H, st));
+        if (phi_hist) ADS_HIP(hipMemsetAsync(phi_hist, 0, H, st));
+        if (psi_hist) ADS_HIP(hipMemsetAsync(psi_hist, 0, H, st));
+        if (!phi_hist || !psi_hist)
+            for (int q = 0; q < 2; q++) {
+                ADS_HIP(hipMemsetAsync(pp[q], 0, (size_t)nshots * Npad * sizeof(T), st));
+                ADS_HIP(hipMemsetAsync(qq[q], 0, (size_t)nshots * Npad * sizeof(T), st));
+            }
+        const int64_t hs = L * Npad;
+        auto lvl = [&](void *base, int64_t n) { return Lv<T>{(T *)base + n * Npad, hs}; };
+        for (int64_t n = 2; n <= NMAX; n++) {
+            Lv<T> pi = phi_hist ? lvl(phi_hist, n - 1) : Lv<T>{pp[(n - 1) & 1], Npad};
+            Lv<T> po = phi_hist ? lvl(phi_hist, n) : Lv<T>{pp[n & 1], Npad};
+            Lv<T> si = psi_hist ? lvl(psi_hist, n - 1) : Lv<T>{qq[(n - 1) & 1], Npad};
+            Lv<T> so = psi_hist ? lvl(psi_hist, n) : Lv<T>{qq[n & 1], Npad};
+            ADS_TRY(fwd_step((int)n, (int)nshots, lvl(u_hist, n), cst(lvl(u_hist, n - 1)),
+                             cst(lvl(u_hist, n - 2)), cst(pi), cst(si), po, so, C,
+                             (const T *)srcv, nullptr, 0, st));
+        }
+        return ADS_OK;
+    }
+
+    int backward_history(const void *model, const void *srcv, const void *u_hist,
+                         const void *g_hist, void *grad, void *scratch, hipStream_t st)
+    {
+        const int64_t L = NMAX + 1;
+        const size_t F = (size_t)nshots * Npad;
+        T *C = (T *)scratch;
+        T *b = (T *)scratch + Npad;
+        T *P[2] = {b, b + F}, *Fa[2] = {b + 2 * F, b + 3 * F}, *Ga[2] = {b + 4 * F, b + 5 * F};
+        T *gC = b + 6 * F;
+        k_square<T><<<cdiv(Npad, 256), 256, 0, st>>>((const T *)model, C, Npad, p.square_velocity);
+        ADS_HIP(hipMemsetAsync(b, 0, 7 * F * sizeof(T), st));
+        const int64_t hs = L * Npad;
+        auto lvl = [&](const void *base, int64_t n) {
+            return Lv<const T>{(const T *)base + n * Npad, hs};
+        };
+        for (int64_t n = NMAX; n >= 2; n--) {
+            ADS_TRY(adj_level((int)n, (int)nshots, Lv<T>{P[n & 1], Npad},
+                              Lv<const T>{P[(n + 1) & 1], Npad}, Lv<T>{Fa[n & 1], Npad},
+                              Lv<T>{Ga[n & 1], Npad}, Lv<const T>{Fa[(n + 1) & 1], Npad},
+                              Lv<const T>{Ga[(n + 1) & 1], Npad}, lvl(u_hist, n - 1),
+                              lvl(u_hist, n), lvl(g_hist, n), Lv<T>{gC, Npad}, C,
+                              (const T *)srcv, nullptr, st));
+        }
+        k_reduce_grad<T><<<cdiv(Npad, 256), 256, 0, st>>>(gC, Npad, (int)nshots, (const T *)model,
+                                                          (T *)grad, Npad, p.square_velocity);
+        ADS_LAUNCH_CHECK();
+        return ADS_OK;
+    }
+    size_t history_scratch() const { return ((size_t)Npad + 7 * (size_t)nshots * Npad) * sizeof(T); }
+};
+
+template <typename T>
+static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, AcPlan<T> *P)
+{
+    ADS_TRY(validate_params(p));
+    if (!g || g->nshots < 1 || !g->src_ptr || !g->rcv_ptr)
+        ADS_FAIL(ADS_ERR_ARG, "geometry: need nshots >= 1, src_ptr and rcv_ptr");
+    P->p = *p;
+    P->variant = p->PropagatorKernel == 0 ? 0 : 1;
+    P->NX = p->NX;
+    P->NY = p->NY;
+    P->S = p->NY + 2;
+    P->Npad = (p->NX + 2) * (p->NY + 2);
+    P->nshots = g->nshots;
+    P->k = make_coef<T>(p->DELTAT, p->DELTAX, p->DELTAY);
+    const int64_t ns = g->nshots;
+    if (g->src_ptr[0] != 0 || g->rcv_ptr[0] != 0) ADS_FAIL(ADS_ERR_ARG, "src_ptr/rcv_ptr must start at 0");
+    std::vector<int64_t> nstep(ns);
+    for (int64_t s = 0; s < ns; s++) {
+        nstep[s] = g->nstep ? g->nstep[s] : p->NSTEP;
+        if (nstep[s] < 1 || nstep[s] > p->NSTEP)
+            ADS_FAIL(ADS_ERR_SHAPE, "shot %lld: nstep %lld outside [1, NSTEP=%lld]", (long long)s,
+                     (long long)nstep[s], (long long)p->NSTEP);
+        if (g->src_ptr[s + 1] < g->src_ptr[s] || g->rcv_ptr[s + 1] < g->rcv_ptr[s])
+            ADS_FAIL(ADS_ERR_ARG, "src_ptr/rcv_ptr must be non-decreasing");
+    }
+    // plan order: longest record first, so the shots active at level n are a prefix
+    P->order.resize(ns);
+    for (int64_t s = 0; s < ns; s++) P->order[s] = s;
+    std::stable_sort(P->order.begin(), P->order.end(),
+                     [&](int64_t a, int64_t b) { return nstep[a] > nstep[b]; });
+    P->NMAX = nstep[P->order[0]];
+    P->nstep_sorted.resize(ns);
+    for (int64_t q = 0; q < ns; q++) P->nstep_sorted[q] = nstep[P->order[q]];
+    P->active.assign(P->NMAX + 2, 0);
+    for (int64_t n = 0; n <= P->NMAX + 1; n++) {
+        int c = 0;
+        for (int64_t q = 0; q < ns; q++) c += P->nstep_sorted[q] >= n;
+        P->active[n] = c;
+    }
+    // caller-order offsets of srcv and traces
+    P->srcv_off_caller.resize(ns);
+    P->trace_off_caller.resize(ns);
+    int64_t so = 0, to = 0;
+    for (int64_t s = 0; s < ns; s++) {
+        P->srcv_off_caller[s] = so;
+        P->trace_off_caller[s] = to;
+        so += nstep[s] * (g->src_ptr[s + 1] - g->src_ptr[s]);
+        to += (nstep[s] + 1) * (g->rcv_ptr[s + 1] - g->rcv_ptr[s]);
+    }
+    P->srcv_elems = so;
+    P->trace_elems = to;
+    // flat indices, validated, in plan order
+    auto flat = [&](const int64_t *I, const int64_t *J, int64_t e, const char *what,
+                    int64_t *out) -> int {
+        int64_t i = I[e], j = J[e];
+        if (i < 1 || i > p->NX + 2 || j < 1 || j > p->NY + 2)
+            ADS_FAIL(ADS_ERR_INDEX, "%s %lld at (%lld, %lld) is outside the padded grid [1, %lld] x [1, %lld]",
+                     what, (long long)e, (long long)i, (long long)j, (long long)(p->NX + 2),
+                     (long long)(p->NY + 2));
+        *out = (i - 1) * (p->NY + 2) + (j - 1);
+        return ADS_OK;
+    };
+    std::vector<int64_t> sp(ns + 1, 0), rp(ns + 1, 0), sflat, rflat, nsrc(ns), soff(ns), toff(ns);
+    for (int64_t q = 0; q < ns; q++) {
+        int64_t s = P->order[q];
+        if (g->src_ptr[s + 1] > g->src_ptr[s] && (!g->srci || !g->srcj))
+            ADS_FAIL(ADS_ERR_ARG, "srci/srcj are null");
+        if (g->rcv_ptr[s + 1] > g->rcv_ptr[s] && (!g->rcvi || !g->rcvj))
+            ADS_FAIL(ADS_ERR_ARG, "rcvi/rcvj are null");
+        for (int64_t e = g->src_ptr[s]; e < g->src_ptr[s + 1]; e++) {
+            int64_t f;
+            ADS_TRY(flat(g->srci, g->srcj, e, "source", &f));
+            sflat.push_back(f);
+        }
+        for (int64_t e = g->rcv_ptr[s]; e < g->rcv_ptr[s + 1]; e++) {
+            int64_t f;
+            ADS_TRY(flat(g->rcvi, g->rcvj, e, "receiver", &f));
+            rflat.push_back(f);
+        }
+        sp[q + 1] = (int64_t)sflat.size();
+        rp[q + 1] = (int64_t)rflat.size();
+        nsrc[q] = g->src_ptr[s + 1] - g->src_ptr[s];
+        soff[q] = P->srcv_off_caller[s];
+        toff[q] = P->trace_off_caller[s];
+    }
+    P->usrc = build_cell_lists(ns, sp.data(), sflat);
+    P->urcv = build_cell_lists(ns, rp.data(), rflat);
+    P->max_ucells_rcv = 0;
+    for (int64_t q = 0; q < ns; q++)
+        P->max_ucells_rcv = std::max(P->max_ucells_rcv, P->urcv.shot_ptr[q + 1] - P->urcv.shot_ptr[q]);
+    // PML profiles and flags
+    P->sx.resize(p->NX + 2);
+    P->ty.resize(p->NY + 2);
+    ADS_TRY(ads_acoustic_pml_profiles(p, P->sx.data(), P->ty.data()));
+    std::vector<T> sxT(P->sx.begin(), P->sx.end()), tyT(P->ty.begin(), P->ty.end());
+    auto flags = [](const std::vector<T> &v, std::vector<uint8_t> &pm, std::vector<uint8_t> &nr) {
+        size_t L = v.size();
+        pm.assign(L, 0);
+        nr.assign(L, 0);
+        for (size_t q = 0; q < L; q++) pm[q] = v[q] != T(0);
+        for (size_t q = 0; q < L; q++)
+            nr[q] = pm[q] | (q > 0 ? pm[q - 1] : 0) | (q + 1 < L ? pm[q + 1] : 0);
+    };
+    flags(sxT, P->pmlx, P->nearx);
+    flags(tyT, P->pmly, P->neary);
+    ADS_TRY(P->d_sx.upload(sxT));
+    ADS_TRY(P->d_ty.upload(tyT));
+    ADS_TRY(P->d_pmlx.upload(P->pmlx));
+    ADS_TRY(P->d_pmly.upload(P->pmly));
+    ADS_TRY(P->d_nearx.upload(P->nearx));
+    ADS_TRY(P->d_neary.upload(P->neary));
+    ADS_TRY(P->d_us_ptr.upload(P->usrc.shot_ptr));
+    ADS_TRY(P->d_us_cell.upload(P->usrc.cell));
+    ADS_TRY(P->d_us_eptr.upload(P->usrc.ent_ptr));
+    ADS_TRY(P->d_us_ent.upload(P->usrc.ent));
+    ADS_TRY(P->d_nsrc.upload(nsrc));
+    ADS_TRY(P->d_srcv_off.upload(soff));
+    ADS_TRY(P->d_ur_ptr.upload(P->urcv.shot_ptr));
+    ADS_TRY(P->d_ur_cell.upload(P->urcv.cell));
+    ADS_TRY(P->d_ur_eptr.upload(P->urcv.ent_ptr));
+    ADS_TRY(P->d_ur_ent.upload(P->urcv.ent));
+    ADS_TRY(P->d_rcv_ptr.upload(rp));
+    ADS_TRY(P->d_rcv_cell.upload(rflat));
+    ADS_TRY(P->d_trace_off.upload(toff));
+    return ADS_OK;
+}
+
+}  // namespace ads
+
+using namespace ads;
+
+struct ads_acoustic_plan {
+    std::unique_ptr<AcPlanBase> impl;
+};
+
+template <typename F> static int dispatch(ads_acoustic_plan *pl, F &&f)
+{
+    if (!pl || !pl->impl) ADS_FAIL(ADS_ERR_ARG, "null plan");
+    if (pl->impl->dtype == ADS_F64) return f(*static_cast<AcPlan<double> *>(pl->impl.get()));
+    return f(*static_cast<AcPlan<float> *>(pl->impl.get()));
+}
+
+extern "C" {
+
+int ads_acoustic_plan_create(const ads_acoustic_params *p, const ads_shot_geometry *g, int dtype,
+                             ads_acoustic_plan **out)
+{
+    clear_error();
+    if (!out) ADS_FAIL(ADS_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (dtype != ADS_F32 && dtype != ADS_F64) ADS_FAIL(ADS_ERR_ARG, "dtype must be ADS_F32 or ADS_F64");
+    auto pl = std::make_unique<ads_acoustic_plan>();
+    int rc;
+    if (dtype == ADS_F64) {
+        auto P = std::make_unique<AcPlan<double>>();
+        P->dtype = dtype;
+        rc = build_plan<double>(p, g, P.get());
+        pl->impl = std::move(P);
+    } else {
+        auto P = std::make_unique<AcPlan<float>>();
+        P->dtype = dtype;
+        rc = build_plan<float>(p, g, P.get());
+        pl->impl = std::move(P);
+    }
+    if (rc != ADS_OK) return rc;
+    *out = pl.release();
+    return ADS_OK;
+}
+
+int ads_acoustic_plan_destroy(ads_acoustic_plan *plan)
+{
+    delete plan;
+    return ADS_OK;
+}
+
+int ads_acoustic_plan_sizes(const ads_acoustic_plan *plan, int64_t *srcv_elems,
+                            int64_t *trace_elems, int64_t *srcv_off, int64_t *trace_off)
+{
+    clear_error();
+    if (!plan || !plan->impl) ADS_FAIL(ADS_ERR_ARG, "null plan");
+    const AcPlanBase &b = *plan->impl;
+    if (srcv_elems) *srcv_elems = b.srcv_elems;
+    if (trace_elems) *trace_elems = b.trace_elems;
+    for (int64_t s = 0; s < b.nshots; s++) {
+        if (srcv_off) srcv_off[s] = b.srcv_off_caller[s];
+        if (trace_off) trace_off[s] = b.trace_off_caller[s];
+    }
+    return ADS_OK;
+}
+
+size_t ads_acoustic_plan_workspace_size(const ads_acoustic_plan *plan, int64_t K)
+{
+    if (!plan || !plan->impl) return 0;
+    if (plan->impl->dtype == ADS_F64) return static_cast<AcPlan<double> *>(plan->impl.get())->ws_size(K);
+    return static_cast<AcPlan<float> *>(plan->impl.get())->ws_size(K);
+}
+
+int64_t ads_acoustic_plan_choose_interval(const ads_acoustic_plan *plan, size_t budget)
+{
+    if (!plan || !plan->impl) return 0;
+    int64_t hi = std::max<int64_t>(plan->impl->NMAX - 1, 1), best = 0;
+    for (int64_t K = hi; K >= 1; K = (K > 64 ? K - std::max<int64_t>(K / 64, 1) : K - 1))
+        if (ads_acoustic_plan_workspace_size(plan, K) <= budget) {
+            best = K;
+            break;
+        }
+    return best;
+}
+
+int ads_acoustic_plan_forward(ads_acoustic_plan *plan, const void *model, const void *srcv,
+                              void *traces, void *ws, size_t wsb, int64_t K, ads_stream_t stream)
+{
+    clear_error();
+    if (!model || !srcv) ADS_FAIL(ADS_ERR_ARG, "null model/srcv");
+    return dispatch(plan, [&](auto &P) { return P.forward(model, srcv, traces, ws, wsb, K, S(stream)); });
+}
+
+int ads_acoustic_plan_backward(ads_acoustic_plan *plan, const void *model, const void *srcv,
+                               const void *grad_traces, void *grad_model, void *ws, size_t wsb,
+                               int64_t K, ads_stream_t stream)
+{
+    clear_error();
+    if (!model || !srcv || !grad_traces || !grad_model) ADS_FAIL(ADS_ERR_ARG, "null argument");
+    return dispatch(plan, [&](auto &P) {
+        using T = typename std::remove_reference<decltype(P)>::type;
+        (void)sizeof(T);
+        return P.backward(model, srcv, (decltype(P.k.dt) const *)grad_traces, grad_model, ws, wsb,
+                          K, S(stream));
+    });
+}
+
+int ads_acoustic_plan_gradient(ads_acoustic_plan *plan, const void *model, const void *srcv,
+                               const void *observed, double *loss, void *grad_model, void *traces,
+                               void *ws, size_t wsb, int64_t K, ads_stream_t stream)
+{
+    clear_error();
+    if (!model || !srcv || !observed || !grad_model) ADS_FAIL(ADS_ERR_ARG, "null argument");
+    return dispatch(plan, [&](auto &P) -> int {
+        using R = typename std::remove_const<decltype(P.k.dt)>::type;
+        typename std::remove_reference<decltype(P)>::type::WS w;
+        Carve cv(ws, wsb);
+        P.layout(w, cv, K);
+        if (!ws || !cv.ok())
+            ADS_FAIL(ADS_ERR_WORKSPACE, "workspace too small: need %zu bytes, got %zu", cv.off, wsb);
+        // traces go to the caller's buffer if given, else into the residual area (reused)
+        R *tr = traces ? (R *)traces : w.resid;
+        ADS_TRY(P.forward(model, srcv, tr, ws, wsb, K, S(stream)));
+        ADS_TRY(P.misfit(w, tr, (const R *)observed, S(stream)));
+        if (loss) ADS_HIP(hipMemcpyAsync(loss, w.loss, sizeof(double), hipMemcpyDeviceToDevice, S(stream)));
+        return P.backward(model, srcv, w.resid, grad_model, ws, wsb, K, S(stream));
+    });
+}
+
+int ads_acoustic_plan_forward_history(ads_acoustic_plan *plan, const void *model,
+                                      const void *srcv, void *u_hist, void *phi_hist,
+                                      void *psi_hist, void *scratch, ads_stream_t stream)
+{
+    clear_error();
+    if (!model || !srcv || !u_hist || !scratch) ADS_FAIL(ADS_ERR_ARG, "null argument");
+    return dispatch(plan, [&](auto &P) -> int {
+        if (P.NMAX != P.p.NSTEP)
+            ADS_FAIL(ADS_ERR_SHAPE, "history mode needs nstep == NSTEP for every shot");
+        return P.forward_history(model, srcv, u_hist, phi_hist, psi_hist, scratch, S(stream));
+    });
+}
+
+int ads_acoustic_plan_backward_history(ads_acoustic_plan *plan, const void *model,
+                                       const void *srcv, const void *u_hist,
+                                       const void *grad_u_hist, void *grad_model, void *scratch,
+                                       ads_stream_t stream)
+{
+    clear_error();
+    if (!model || !srcv || !u_hist || !grad_u_hist || !grad_model || !scratch)
+        ADS_FAIL(ADS_ERR_ARG, "null argument");
+    return dispatch(plan, [&](auto &P) -> int {
+        if (P.NMAX != P.p.NSTEP)
+            ADS_FAIL(ADS_ERR_SHAPE, "history mode needs nstep == NSTEP for every shot");
+        return P.backward_history(model, srcv, u_hist, grad_u_hist, grad_model, scratch, S(stream));
+    });
+}
+
+size_t ads_acoustic_plan_history_scratch_size(const ads_acoustic_plan *plan)
+{
+    if (!plan || !plan->impl) return 0;
+    if (plan->impl->dtype == ADS_F64)
+        return static_cast<AcPlan<double> *>(plan->impl.get())->history_scratch();
+    return static_cast<AcPlan<float> *>(plan->impl.get())->history_scratch();
+}
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+// ads_host.cpp -- host-side pieces of the C-ABI: error state, version, PML/CPML coefficient
+// tables and geometry tables.  No device code.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "ads_common.h"
+
+namespace ads {
+
+static thread_local char g_err[1024] = {0};
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+void clear_error() { g_err[0] = 0; }
+
+CellLists build_cell_lists(int64_t nshots, const int64_t *ptr, const std::vector<int64_t> &flat)
+{
+    CellLists L;
+    L.shot_ptr.push_back(0);
+    L.ent_ptr.push_back(0);
+    for (int64_t s = 0; s < nshots; s++) {
+        int64_t b = ptr[s], e = ptr[s + 1];
+        std::vector<int64_t> ord(e - b);
+        std::iota(ord.begin(), ord.end(), 0);
+        // stable sort by cell keeps the original order of entries that share a cell
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](int64_t x, int64_t y) { return flat[b + x] < flat[b + y]; });
+        for (size_t k = 0; k < ord.size();) {
+            int64_t cell = flat[b + ord[k]];
+            size_t q = k;
+            while (q < ord.size() && flat[b + ord[q]] == cell) {
+                L.ent.push_back(ord[q]);
+                q++;
+            }
+            L.cell.push_back(cell);
+            L.ent_ptr.push_back((int64_t)L.ent.size());
+            k = q;
+        }
+        L.shot_ptr.push_back((int64_t)L.cell.size());
+    }
+    return L;
+}
+
+}  // namespace ads
+
+using namespace ads;
+
+extern "C" {
+
+const char *ads_last_error(void) { return g_err; }
+
+int ads_version(void) { return 100; /* 0.1.0 */ }
+
+// pml_helper (src/Core.jl:738-761) on one axis; coordinate k*h + origin, far edge at
+// h*(NXg+1) - L.
+static void acoustic_axis(int64_t len, double h, double origin, int64_t NXg, int64_t npml,
+                          double R, double vp_ref, int use_min, int use_max, double *out)
+{
+    const double L = (double)npml * h;
+    const double xi = vp_ref / L * std::log(1.0 / R);  // damping, Core.jl:625-629
+    const double right = h * (double)(NXg + 1) - L;
+    for (int64_t k = 0; k < len; k++) {
+        double x = (double)k * h + origin;
+        double v = 0.0;
+        if (x < L && use_min) {
+            double d = std::fabs(L - x);
+            v = xi * (d / L - std::sin(2.0 * M_PI * d / L) / (2.0 * M_PI));
+        } else if (x > right && use_max) {
+            double d = std::fabs(x - right);
+            v = xi * (d / L - std::sin(2.0 * M_PI * d / L) / (2.0 * M_PI));
+        }
+        out[k] = v;
+    }
+}
+
+int ads_acoustic_pml_profiles(const ads_acoustic_params *p, double *sigma_x, double *tau_y)
+{
+    clear_error();
+    if (!p || !sigma_x || !tau_y) ADS_FAIL(ADS_ERR_ARG, "ads_acoustic_pml_profiles: null argument");
+    if (p->NX < 1 || p->NY < 1) ADS_FAIL(ADS_ERR_SHAPE, "NX, NY must be >= 1");
+    if (p->NPOINTS_PML < 1 || !(p->Rcoef > 0) || !(p->DELTAX > 0) || !(p->DELTAY > 0))
+        ADS_FAIL(ADS_ERR_ARG, "bad PML parameters (NPOINTS_PML >= 1, Rcoef > 0, h > 0)");
+    int64_t NXg = p->NX_global > 0 ? p->NX_global : p->NX;
+    int64_t NYg = p->NY_global > 0 ? p->NY_global : p->NY;
+    acoustic_axis(p->NX + 2, p->DELTAX, p->origin_x, NXg, p->NPOINTS_PML, p->Rcoef, p->vp_ref,
+                  p->USE_PML_XMIN, p->USE_PML_XMAX, sigma_x);
+    acoustic_axis(p->NY + 2, p->DELTAY, p->origin_y, NYg, p->NPOINTS_PML, p->Rcoef, p->vp_ref,
+                  p->USE_PML_YMIN, p->USE_PML_YMAX, tau_y);
+    return ADS_OK;
+}
+
+int ads_acoustic_check_cfl(const ads_acoustic_params *p, double c_max)
+{
+    clear_error();
+    if (!p) ADS_FAIL(ADS_ERR_ARG, "null params");
+    double r = c_max * p->DELTAT *
+               std::sqrt(1.0 / (p->DELTAX * p->DELTAX) + 1.0 / (p->DELTAY * p->DELTAY));
+    if (!(r <= 1.0))
+        ADS_FAIL(ADS_ERR_CFL, "CFL number %.4f > 1 (c_max=%g, dt=%g): the scheme is unstable", r,
+                 c_max, p->DELTAT);
+    return ADS_OK;
+}
+
+// compute_PML_Params(::ElasticPropagatorParams), src/Core.jl:231-407, one axis.
+static void elastic_axis(int64_t len, double h, double origin, int64_t NXg, int64_t npml,
+                         double npower, double kmax, double amax, double R, double vp_ref,
+                         double dt, int use_min, int use_max, double *a, double *ah, double *b,
+                         double *bh, double *K, double *Kh)
+{
+    const double thick = (double)npml * h;
+    const double d0 = -(npower + 1) * vp_ref * std::log(R) / (2. * thick);
+    const double oleft = thick, oright = ((double)NXg - 0.5) * h - thick;
+    for (int64_t k = 0; k < len; k++) {
+        double x = h * (double)k + origin;
+        double d = 0, dh = 0, kk = 1, kh = 1, al = 0, alh = 0;
+        if (use_min) {
+            double ab = oleft - x;
+            if (ab >= 0.0) {
+                double an = ab / thick;
+                d = d0 * std::pow(an, npower);
+                kk = 1. + (kmax - 1.) * std::pow(an, npower);
+                al = amax * an;
+            }
+            ab = oleft - (x + h / 2.);
+            if (ab >= 0.0) {
+                double an = ab / thick;
+                dh = d0 * std::pow(an, npower);
+                kh = 1. + (kmax - 1.) * std::pow(an, npower);
+                alh = amax * an;
+            }
+        }
+        if (use_max) {
+            double ab = x - oright;
+            if (ab >= 0) {
+                double an = ab / thick;
+                d = d0 * std::pow(an, npower);
+                kk = 1. + (kmax - 1.) * std::pow(an, npower);
+                al = amax * an;
+            }
+            ab = x + h / 2.0 - oright;
+            if (ab >= 0) {
+                double an = ab / thick;
+                dh = d0 * std::pow(an, npower);
+                kh = 1. + (kmax - 1.) * std::pow(an, npower);
+                alh = amax * an;
+            }
+        }
+        if (al < 0) al = 0;
+        if (alh < 0) alh = 0;
+        b[k] = std::exp(-(d / kk + al) * dt);
+        bh[k] = std::exp(-(dh / kh + alh) * dt);
+        a[k] = std::fabs(d) > 1e-6 ? d * (b[k] - 1.) / (kk * (d + kk * al)) : 0.0;
+        ah[k] = std::fabs(dh) > 1e-6 ? dh * (bh[k] - 1.) / (kh * (dh + kh * alh)) : 0.0;
+        K[k] = kk;
+        Kh[k] = kh;
+    }
+}
+
+int ads_elastic_pml_params(const ads_elastic_params *p, double *ax, double *bx, double *kx,
+                           double *ay, double *by, double *ky)
+{
+    clear_error();
+    if (!p || !ax || !bx || !kx || !ay || !by || !ky)
+        ADS_FAIL(ADS_ERR_ARG, "ads_elastic_pml_params: null argument");
+    if (p->NX < 3 || p->NY < 3) ADS_FAIL(ADS_ERR_SHAPE, "NX, NY must be >= 3");
+    if (p->NPOINTS_PML < 1 || !(p->Rcoef > 0))
+        ADS_FAIL(ADS_ERR_ARG, "bad PML parameters (NPOINTS_PML >= 1, Rcoef > 0)");
+    int64_t NXg = p->NX_global > 0 ? p->NX_global : p->NX;
+    int64_t NYg = p->NY_global > 0 ? p->NY_global : p->NY;
+    elastic_axis(p->NX, p->DELTAX, p->origin_x, NXg, p->NPOINTS_PML, p->NPOWER, p->K_MAX_PML,
+                 p->ALPHA_MAX_PML, p->Rcoef, p->vp_ref, p->DELTAT, p->USE_PML_XMIN,
+                 p->USE_PML_XMAX, ax, ax + p->NX, bx, bx + p->NX, kx, kx + p->NX);
+    elastic_axis(p->NY, p->DELTAY, p->origin_y, NYg, p->NPOINTS_PML, p->NPOWER, p->K_MAX_PML,
+                 p->ALPHA_MAX_PML, p->Rcoef, p->vp_ref, p->DELTAT, p->USE_PML_YMIN,
+                 p->USE_PML_YMAX, ay, ay + p->NY, by, by + p->NY, ky, ky + p->NY);
+    return ADS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,300 @@
+/*
+ * adseismic.h -- C-ABI of the MI355X (gfx950) wave-propagation library.
+ *
+ * One shared library, libadseismic_hip.so, is the drop-in boundary for ADSeismic.jl's hot
+ * path.  Plain pointers and sizes only.  Device pointers are HIP device memory; "host" marks
+ * the few small host arrays (geometry).  Every call is stream-ordered on the hipStream_t
+ * passed in (NULL = the null stream); nothing synchronises the device except where noted.
+ * Functions return ADS_OK (0) or an ADS_ERR_* code; ads_last_error() gives the text
+ * (thread-local).
+ *
+ * Index conventions follow the reference at this boundary: source/receiver positions are
+ * 1-based (i, j) on the padded (NX+2) x (NY+2) grid (src/Core.jl:600, 727) and are converted
+ * once to 0-based flat indices (i-1)*(NY+2) + (j-1) inside the library.  Fields are row-major
+ * (NX+2) x (NY+2), j (depth) fastest.
+ */
+#ifndef ADSEISMIC_H
+#define ADSEISMIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *ads_stream_t; /* hipStream_t */
+
+enum {
+    ADS_OK = 0,
+    ADS_ERR_ARG = 1,       /* null pointer / bad enum / bad parameter value */
+    ADS_ERR_SHAPE = 2,     /* sizes inconsistent */
+    ADS_ERR_INDEX = 3,     /* source / receiver index outside the padded grid */
+    ADS_ERR_CFL = 4,       /* time step violates the stability bound */
+    ADS_ERR_HIP = 5,       /* a HIP runtime call or kernel launch failed */
+    ADS_ERR_WORKSPACE = 6, /* workspace too small */
+    ADS_ERR_NONFINITE = 7  /* NaN/Inf in the misfit (blow-up) */
+};
+
+enum { ADS_F32 = 0, ADS_F64 = 1 };
+
+/* Thread-local text of the last error (empty string if none). */
+const char *ads_last_error(void);
+/* Library version: major*10000 + minor*100 + patch. */
+int ads_version(void);
+
+/* ======================================================================================
+ * Drop-in per-step operators.  Each replaces one TensorFlow custom op of the reference
+ * (deps/CustomOps/*) with the same argument meaning; fields are device arrays of
+ * N = (NX+2)*(NY+2) elements; scalar arguments are host values.
+ * ==================================================================================== */
+
+/* AcousticOneStep forward: deps/CustomOps/AcousticOneStep/AcousticOneStep.cpp:14-61 and
+ * AcousticOneStepCpu.h:1-48 (phi/psi from w).  Ring cells of u, phiout, psiout are set to 0. */
+int ads_acoustic_one_step_f64(const double *w, const double *wold, const double *phi,
+                              const double *psi, const double *sigma, const double *tau,
+                              const double *c, double dt, double hx, double hy, int64_t nx,
+                              int64_t ny, double *u, double *phiout, double *psiout,
+                              ads_stream_t stream);
+int ads_acoustic_one_step_f32(const float *w, const float *wold, const float *phi,
+                              const float *psi, const float *sigma, const float *tau,
+                              const float *c, double dt, double hx, double hy, int64_t nx,
+                              int64_t ny, float *u, float *phiout, float *psiout,
+                              ads_stream_t stream);
+
+/* AcousticOneStepGrad: AcousticOneStep.cpp:63-93 (signature), AcousticOneStepCpu.h:51-125
+ * (numerics).  Gather form; outputs are overwritten on every cell, ring cells included, and
+ * equal the CPU op's scatter result.  The reference's grad_sigma/tau/dt/hx/hy are never
+ * written by it and are not produced here. */
+int ads_acoustic_one_step_grad_f64(const double *grad_u, const double *grad_phiout,
+                                   const double *grad_psiout, const double *w, const double *wold,
+                                   const double *phi, const double *psi, const double *sigma,
+                                   const double *tau, const double *c, double dt, double hx,
+                                   double hy, int64_t nx, int64_t ny, double *grad_w,
+                                   double *grad_wold, double *grad_phi, double *grad_psi,
+                                   double *grad_c, ads_stream_t stream);
+int ads_acoustic_one_step_grad_f32(const float *grad_u, const float *grad_phiout,
+                                   const float *grad_psiout, const float *w, const float *wold,
+                                   const float *phi, const float *psi, const float *sigma,
+                                   const float *tau, const float *c, double dt, double hx,
+                                   double hy, int64_t nx, int64_t ny, float *grad_w,
+                                   float *grad_wold, float *grad_phi, float *grad_psi,
+                                   float *grad_c, ads_stream_t stream);
+
+/* MpiAcousticOneStep forward on one subdomain:
+ * deps/CustomOps/MPIAcousticOneStepCpu/MpiAcousticOneStep.h:1-42.  w, wold, phi, psi, sigma,
+ * tau are (n+2)^2 halo'd blocks (the caller fills the halo); c, u, phiout, psiout are n*n. */
+int ads_mpi_acoustic_one_step_f64(const double *w, const double *wold, const double *phi,
+                                  const double *psi, const double *sigma, const double *tau,
+                                  const double *c, double dt, double hx, double hy, int64_t n,
+                                  double *u, double *phiout, double *psiout, ads_stream_t stream);
+int ads_mpi_acoustic_one_step_f32(const float *w, const float *wold, const float *phi,
+                                  const float *psi, const float *sigma, const float *tau,
+                                  const float *c, double dt, double hx, double hy, int64_t n,
+                                  float *u, float *phiout, float *psiout, ads_stream_t stream);
+
+/* GatherOps (GatherOps.h:1-6): out[p] = v[ii[p]-1]; ii is a 1-based device int64 array. */
+int ads_gather_f64(const double *v, int64_t nv, const int64_t *ii, int64_t n, double *out,
+                   ads_stream_t stream);
+int ads_gather_f32(const float *v, int64_t nv, const int64_t *ii, int64_t n, float *out,
+                   ads_stream_t stream);
+/* ScatterAddOps (ScatterAddOps.h:1-7): out = ipt; out[ii[p]-1] += upd[p] (duplicates add,
+ * in index order). */
+int ads_scatter_add_f64(const double *ipt, int64_t d, const int64_t *ii, const double *upd,
+                        int64_t n, double *out, ads_stream_t stream);
+int ads_scatter_add_f32(const float *ipt, int64_t d, const int64_t *ii, const float *upd,
+                        int64_t n, float *out, ads_stream_t stream);
+/* ScatterNdOps (ScatterNdOps.h:1-7): out = zeros(m); out[ii[p]-1] = upd[p] (last one wins). */
+int ads_scatter_nd_f64(const int64_t *ii, const double *upd, int64_t n, int64_t m, double *out,
+                       ads_stream_t stream);
+int ads_scatter_nd_f32(const int64_t *ii, const float *upd, int64_t n, int64_t m, float *out,
+                       ads_stream_t stream);
+
+/* AddSource forward (deps/CustomOps/SourceOps/AddSource.cpp:33-87, CPU semantics: +=):
+ * out_k = in_k for the five fields (order sxx, syy, sxy, vx, vy as the op's inputs,
+ * AddSource.cc:75-123), then field_type[srci-1, srcj-1] += srcv[s] in source order.
+ * srci/srcj/srctype are HOST int64 arrays (validated); srcv is a device array of nsrc. */
+int ads_add_source_f64(const double *sxx, const double *syy, const double *sxy, const double *vx,
+                       const double *vy, const int64_t *srci, const int64_t *srcj,
+                       const int64_t *srctype, int64_t nsrc, int64_t nx, int64_t ny,
+                       const double *srcv, double *sxx_out, double *syy_out, double *sxy_out,
+                       double *vx_out, double *vy_out, ads_stream_t stream);
+/* AddSource backward (AddSource.cpp:91-159): grads pass through; grad_srcv[s] = cotangent of
+ * the source's field at its cell. */
+int ads_add_source_grad_f64(const double *g_sxx, const double *g_syy, const double *g_sxy,
+                            const double *g_vx, const double *g_vy, const int64_t *srci,
+                            const int64_t *srcj, const int64_t *srctype, int64_t nsrc, int64_t nx,
+                            int64_t ny, double *grad_srcv, ads_stream_t stream);
+/* GetReceive forward (deps/CustomOps/ReceiveOps/GetReceive.cpp:10-46):
+ * out[r*nt + k] = field_type_r[k][(rcvi-1)(NY+2) + rcvj-1] for nt = NSTEP+1 history levels.
+ * Histories are device arrays (nt, N); receivers HOST int64 arrays. */
+int ads_get_receive_f64(const double *vx, const double *vy, const double *sxx, const double *syy,
+                        const double *sxy, int64_t nt, const int64_t *rcvi, const int64_t *rcvj,
+                        const int64_t *rcvtype, int64_t nrcv, int64_t nx, int64_t ny, double *out,
+                        ads_stream_t stream);
+
+/* ======================================================================================
+ * Parameters (mirror src/Struct.jl) and host-side coefficient helpers.
+ * ==================================================================================== */
+
+/* AcousticPropagatorParams, src/Struct.jl:82-121 (+ MPIAcoustic.jl:3-49 origin fields). */
+typedef struct {
+    int64_t NX, NY, NSTEP;
+    double DELTAX, DELTAY, DELTAT;
+    int32_t USE_PML_XMIN, USE_PML_XMAX, USE_PML_YMIN, USE_PML_YMAX;
+    int64_t NPOINTS_PML;
+    double Rcoef, vp_ref;
+    /* 0: phi/psi advanced from the new u (default, src/Core.jl:528-549);
+     * 1 or 2: phi/psi advanced from w (AcousticOneStep op, Core.jl:475-525). */
+    int32_t PropagatorKernel;
+    /* 1: the model argument is the velocity and is squared (Core.jl:564), gradients are
+     *    returned w.r.t. the velocity; 0: the model is used as is (MPIAcoustic.jl:336). */
+    int32_t square_velocity;
+    /* x/y coordinate of padded cell 0 (0 for the full grid; subdomain offset otherwise,
+     * MPIAcoustic.jl:177-178) and the global interior sizes for the far PML edge. */
+    double origin_x, origin_y;
+    int64_t NX_global, NY_global; /* 0 = same as NX / NY */
+} ads_acoustic_params;
+
+/* compute_PML_Params!(::AcousticPropagatorParams), src/Core.jl:622-655 + pml_helper
+ * :738-761.  sigma depends on x only and tau on y only: writes the 1-D host arrays
+ * sigma_x[NX+2] and tau_y[NY+2]; the reference's full grids are Sx[i,j] = sigma_x[i],
+ * Sy[i,j] = tau_y[j]. */
+int ads_acoustic_pml_profiles(const ads_acoustic_params *p, double *sigma_x, double *tau_y);
+
+/* Stability bound of the 2nd-order scheme: c_max * dt * sqrt(1/hx^2 + 1/hy^2) <= 1.
+ * Returns ADS_OK or ADS_ERR_CFL. */
+int ads_acoustic_check_cfl(const ads_acoustic_params *p, double c_max);
+
+/* ElasticPropagatorParams, src/Struct.jl:4-32 (+ MPIElastic.jl subdomain fields). */
+typedef struct {
+    int64_t NX, NY, NSTEP;
+    double DELTAX, DELTAY, DELTAT;
+    double f0, vp_ref;
+    int32_t USE_PML_XMIN, USE_PML_XMAX, USE_PML_YMIN, USE_PML_YMAX;
+    int64_t NPOINTS_PML;
+    double NPOWER, K_MAX_PML, ALPHA_MAX_PML, Rcoef;
+    /* 0: ElasticPropagatorSolver semantics (parameter averaging, reduced cell ranges,
+     *    src/Core.jl:96-213); 1: one MPIElastic subdomain (no averaging, every local cell,
+     *    MPIElastic.jl:483-620). */
+    int32_t subdomain;
+    double origin_x, origin_y; /* physical x/y of profile index 0 */
+    int64_t NX_global, NY_global;
+} ads_elastic_params;
+
+/* compute_PML_Params(::ElasticPropagatorParams), src/Core.jl:231-407: writes host arrays
+ * ax, bx, kx [2*NX] and ay, by, ky [2*NY], row 0 = full grid, row 1 = half grid
+ * (Core.jl:397-402). */
+int ads_elastic_pml_params(const ads_elastic_params *p, double *ax, double *bx, double *kx,
+                           double *ay, double *by, double *ky);
+
+/* ======================================================================================
+ * Shot batches (the solver-level hot path).
+ * ==================================================================================== */
+
+/* Geometry of a batch of shots; all HOST arrays, copied at plan creation.
+ *   shot s has sources src_ptr[s] .. src_ptr[s+1]-1 and receivers rcv_ptr[s] .. rcv_ptr[s+1]-1.
+ *   nstep[s] <= NSTEP is the record length of shot s (NULL: NSTEP for every shot).
+ * Device data that goes with it:
+ *   srcv   : concatenation over shots of an (nstep[s] x nsrc_s) row-major block (the
+ *            reference's srcv, Struct.jl:126/54); level n takes row n-1.
+ *   traces : acoustic: per shot (nstep[s]+1) x nrcv_s (Core.jl:728);
+ *            elastic : per shot nrcv_s x (nstep[s]+1) (GetReceive.cc:65).
+ *   offsets: ads_*_plan_offsets() returns the element offset of each shot's block. */
+typedef struct {
+    int64_t nshots;
+    const int64_t *nstep;
+    const int64_t *src_ptr;
+    const int64_t *srci, *srcj;
+    const int64_t *srctype; /* elastic only (Struct.jl:44-48); ignored for acoustic */
+    const int64_t *rcv_ptr;
+    const int64_t *rcvi, *rcvj;
+    const int64_t *rcvtype; /* elastic only */
+} ads_shot_geometry;
+
+typedef struct ads_acoustic_plan ads_acoustic_plan;
+
+/* Creates a plan: validates parameters and geometry (ADS_ERR_INDEX for a position outside the
+ * padded grid), computes PML profiles and uploads index tables.  dtype: ADS_F32 or ADS_F64. */
+int ads_acoustic_plan_create(const ads_acoustic_params *p, const ads_shot_geometry *g, int dtype,
+                             ads_acoustic_plan **out);
+int ads_acoustic_plan_destroy(ads_acoustic_plan *plan);
+/* Sizes in elements of the srcv and trace arrays, and per-shot element offsets (host arrays
+ * of nshots entries, may be NULL). */
+int ads_acoustic_plan_sizes(const ads_acoustic_plan *plan, int64_t *srcv_elems,
+                            int64_t *trace_elems, int64_t *srcv_off, int64_t *trace_off);
+/* Workspace bytes needed for a checkpoint interval K (levels between checkpoints; K >= 1).
+ * K >= NSTEP-1 stores every level (no recomputation). */
+size_t ads_acoustic_plan_workspace_size(const ads_acoustic_plan *plan, int64_t K);
+/* Largest K whose workspace fits in budget_bytes (0 if none). */
+int64_t ads_acoustic_plan_choose_interval(const ads_acoustic_plan *plan, size_t budget_bytes);
+
+/* Forward: traces of every shot (model = velocity or C as square_velocity says).  Keeps the
+ * checkpoints in the workspace for a following ads_acoustic_plan_backward. */
+int ads_acoustic_plan_forward(ads_acoustic_plan *plan, const void *model, const void *srcv,
+                              void *traces, void *workspace, size_t ws_bytes, int64_t K,
+                              ads_stream_t stream);
+/* Adjoint: given d loss / d traces, accumulates nothing else and OVERWRITES grad_model with
+ * d loss / d model (summed over shots).  Needs the workspace of the preceding forward with
+ * the same model, srcv and K. */
+int ads_acoustic_plan_backward(ads_acoustic_plan *plan, const void *model, const void *srcv,
+                               const void *grad_traces, void *grad_model, void *workspace,
+                               size_t ws_bytes, int64_t K, ads_stream_t stream);
+/* FWI gradient in one call: forward, misfit L = sum (traces - observed)^2 (src/Utils.jl:308),
+ * residual, adjoint.  loss: device double[1] (overwritten); traces may be NULL.  Returns
+ * ADS_ERR_NONFINITE (after a device sync) if the misfit is not finite. */
+int ads_acoustic_plan_gradient(ads_acoustic_plan *plan, const void *model, const void *srcv,
+                               const void *observed, double *loss, void *grad_model,
+                               void *traces, void *workspace, size_t ws_bytes, int64_t K,
+                               ads_stream_t stream);
+
+/* Plain drop-in of AcousticPropagatorSolver (Core.jl:562-620) on the plan's shots (every shot
+ * must have nstep == NSTEP): writes the full histories u, phi, psi, each laid out
+ * [shot][level 0..NSTEP][N] (phi/psi may be NULL).  scratch: device buffer of
+ * ads_acoustic_plan_history_scratch_size() bytes. */
+size_t ads_acoustic_plan_history_scratch_size(const ads_acoustic_plan *plan);
+int ads_acoustic_plan_forward_history(ads_acoustic_plan *plan, const void *model,
+                                      const void *srcv, void *u_hist, void *phi_hist,
+                                      void *psi_hist, void *scratch, ads_stream_t stream);
+/* Its pullback: given the dense cotangent of the u history (same layout), OVERWRITES
+ * grad_model with d loss / d model.  Needs u_hist from the forward; phi/psi cotangents are
+ * taken as zero (no caller of the reference differentiates through them). */
+int ads_acoustic_plan_backward_history(ads_acoustic_plan *plan, const void *model,
+                                       const void *srcv, const void *u_hist,
+                                       const void *grad_u_hist, void *grad_model, void *scratch,
+                                       ads_stream_t stream);
+
+/* ---------------------------------- elastic ---------------------------------------- */
+typedef struct ads_elastic_plan ads_elastic_plan;
+
+int ads_elastic_plan_create(const ads_elastic_params *p, const ads_shot_geometry *g, int dtype,
+                            ads_elastic_plan **out);
+int ads_elastic_plan_destroy(ads_elastic_plan *plan);
+int ads_elastic_plan_sizes(const ads_elastic_plan *plan, int64_t *srcv_elems,
+                           int64_t *trace_elems, int64_t *srcv_off, int64_t *trace_off);
+size_t ads_elastic_plan_workspace_size(const ads_elastic_plan *plan, int64_t K);
+int64_t ads_elastic_plan_choose_interval(const ads_elastic_plan *plan, size_t budget_bytes);
+/* Forward: traces of every shot from rho, lambda, mu (device, N each). */
+int ads_elastic_plan_forward(ads_elastic_plan *plan, const void *rho, const void *lam,
+                             const void *mu, const void *srcv, void *traces, void *workspace,
+                             size_t ws_bytes, int64_t K, ads_stream_t stream);
+/* Adjoint: OVERWRITES grad_rho, grad_lam, grad_mu (device, N each) with d loss / d(.). */
+int ads_elastic_plan_backward(ads_elastic_plan *plan, const void *rho, const void *lam,
+                              const void *mu, const void *srcv, const void *grad_traces,
+                              void *grad_rho, void *grad_lam, void *grad_mu, void *workspace,
+                              size_t ws_bytes, int64_t K, ads_stream_t stream);
+int ads_elastic_plan_gradient(ads_elastic_plan *plan, const void *rho, const void *lam,
+                              const void *mu, const void *srcv, const void *observed,
+                              double *loss, void *grad_rho, void *grad_lam, void *grad_mu,
+                              void *traces, void *workspace, size_t ws_bytes, int64_t K,
+                              ads_stream_t stream);
+/* Full 5-field histories (5 x (NSTEP+1) x N, order vx, vy, sxx, syy, sxy) of one shot:
+ * the plain drop-in of ElasticPropagatorSolver (Core.jl:31-94). */
+int ads_elastic_forward_history(ads_elastic_plan *plan, const void *rho, const void *lam,
+                                const void *mu, const void *srcv, void *hist,
+                                ads_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ADSEISMIC_H */

@@ -1,0 +1,244 @@
+"""GPU parity tests of the acoustic HIP path against the FP64 oracle and the golden vectors.
+
+Tolerances: FP64 paths match the oracle to 1e-10 relative (1e-12 for single steps; the
+drop-in per-step op is bit-exact); FP32 paths meet the stated bar of <= 1e-4 relative L2 on
+traces/wavefields and <= 1e-3 on gradients against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def T(x, dt=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=dt, device="cuda")
+
+
+# ------------------------------------------------------------------------------------------
+# drop-in per-step op
+# ------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["10x10", "7x12"])
+def test_one_step_op_bit_exact_f64(ads, golden, cuda, name):
+    g = golden("acoustic_one_step.npz")
+    NX, NY = (int(x) for x in g[f"{name}/dims"])
+    dt, hx, hy = (float(x) for x in g[f"{name}/scalars"])
+    ins = [T(g[f"{name}/{k}"]) for k in ("w", "wold", "phi", "psi", "sigma", "tau", "c")]
+    for t in ins:
+        t.requires_grad_(True)
+    u, po, so = ads.acoustic_one_step(*ins, dt, hx, hy, NX, NY)
+    np.testing.assert_array_equal(u.detach().cpu().numpy(), g[f"{name}/u"])
+    np.testing.assert_array_equal(po.detach().cpu().numpy(), g[f"{name}/phiout"])
+    np.testing.assert_array_equal(so.detach().cpu().numpy(), g[f"{name}/psiout"])
+    gu, gp, gs = (T(g[f"{name}/{k}"]) for k in ("grad_u", "grad_phiout", "grad_psiout"))
+    grads = torch.autograd.grad((u, po, so), [ins[0], ins[1], ins[2], ins[3], ins[6]], (gu, gp, gs))
+    for gr, k in zip(grads, ("grad_w", "grad_wold", "grad_phi", "grad_psi", "grad_c")):
+        np.testing.assert_array_equal(gr.cpu().numpy(), g[f"{name}/{k}"], err_msg=k)
+
+
+def test_one_step_op_f32(ads, golden, cuda):
+    g = golden("acoustic_one_step.npz")
+    name = "10x10"
+    NX, NY = (int(x) for x in g[f"{name}/dims"])
+    dt, hx, hy = (float(x) for x in g[f"{name}/scalars"])
+    ins = [T(g[f"{name}/{k}"], torch.float32) for k in ("w", "wold", "phi", "psi", "sigma", "tau", "c")]
+    u, po, so = ads.acoustic_one_step(*ins, dt, hx, hy, NX, NY)
+    assert rel(u.cpu(), g[f"{name}/u"]) < 1e-6
+    assert rel(po.cpu(), g[f"{name}/phiout"]) < 1e-6
+
+
+def test_mpi_one_step_op_bit_exact(ads, golden, cuda):
+    import ctypes as C
+    import importlib
+    L = importlib.import_module("adseismic_amd._lib")
+    g = golden("mpi_acoustic_one_step.npz")
+    n = int(g["n"])
+    dt, hx, hy = (float(x) for x in g["scalars"])
+    ins = [T(g[k]) for k in ("w", "wold", "phi", "psi", "sigma", "tau", "c")]
+    outs = [torch.empty(n * n, dtype=torch.float64, device="cuda") for _ in range(3)]
+    lib = ads.load_library()
+    L.check(lib.ads_mpi_acoustic_one_step_f64(*[C.c_void_p(t.data_ptr()) for t in ins], dt, hx, hy, n,
+                                              *[C.c_void_p(t.data_ptr()) for t in outs], None))
+    torch.cuda.synchronize()
+    for o, k in zip(outs, ("u", "phiout", "psiout")):
+        np.testing.assert_array_equal(o.cpu().numpy(), g[k], err_msg=k)
+
+
+def test_index_ops(ads, golden, cuda):
+    import ctypes as C
+    import importlib
+    L = importlib.import_module("adseismic_amd._lib")
+    g = golden("index_ops.npz")
+    lib = ads.load_library()
+    v, upd = T(g["v"]), T(g["upd"])
+    ii = torch.as_tensor(g["ii"], device="cuda")
+    P = lambda t: C.c_void_p(t.data_ptr())
+    out = torch.empty(len(g["ii"]), dtype=torch.float64, device="cuda")
+    L.check(lib.ads_gather_f64(P(v), 50, P(ii), len(g["ii"]), P(out), None))
+    sa = torch.empty(50, dtype=torch.float64, device="cuda")
+    L.check(lib.ads_scatter_add_f64(P(v), 50, P(ii), P(upd), len(g["ii"]), P(sa), None))
+    snd = torch.empty(50, dtype=torch.float64, device="cuda")
+    L.check(lib.ads_scatter_nd_f64(P(ii), P(upd), len(g["ii"]), 50, P(snd), None))
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), g["gather"])
+    np.testing.assert_array_equal(sa.cpu().numpy(), g["scatter_add"])
+    np.testing.assert_array_equal(snd.cpu().numpy(), g["scatter_nd"])
+
+
+# ------------------------------------------------------------------------------------------
+# solver level
+# ------------------------------------------------------------------------------------------
+
+def make_case(orc, ads, NX=30, NY=26, NSTEP=80, npml=6, nshots=1, seed=0, uneven=False,
+              kernel=0, h=10.0, dt=1e-3):
+    rng = np.random.default_rng(seed)
+    p = ads.AcousticPropagatorParams(NX=NX, NY=NY, NSTEP=NSTEP, DELTAX=h, DELTAY=h, DELTAT=dt,
+                                     NPOINTS_PML=npml, vp_ref=2000.0, PropagatorKernel=kernel)
+    c = 1800.0 + 600.0 * rng.random((NX + 2, NY + 2))
+    nstep = [NSTEP - (7 * s if uneven else 0) for s in range(nshots)]
+    srcs, rcvs = [], []
+    for s in range(nshots):
+        i0 = 2 + (s * 5) % (NX - 2)
+        w = orc.ricker(nstep[s], 6.0, 12.0 + s, 1e6)
+        if s % 3 == 2:  # two sources, one on the ring, one duplicated
+            srcs.append(ads.AcousticSource([i0, 1, i0], [3, 5, 3],
+                                           np.stack([w, 0.5 * w, -0.25 * w], 1)))
+        else:
+            srcs.append(ads.AcousticSource([i0], [3], w[:, None]))
+        ri = np.arange(1, NX + 3, 2 + s % 2)
+        rj = np.full(len(ri), 2 + s % 2)
+        if s % 2 == 1:  # duplicated receiver and one in the PML corner
+            ri = np.concatenate([ri, [ri[0], 2]])
+            rj = np.concatenate([rj, [rj[0], NY]])
+        rcvs.append(ads.AcousticReceiver(ri, rj))
+    return p, c, srcs, rcvs, nstep
+
+
+def oracle_shot(orc, p, c, src, rcv, nstep, cmod=None, obs=None):
+    """Oracle traces (and gradient if obs is given) for one shot of record length nstep."""
+    NX, NY = p.NX, p.NY
+    Sx, Sy = orc.acoustic_pml(NX, NY, p.DELTAX, p.DELTAY, p.NPOINTS_PML, p.Rcoef, p.vp_ref)
+    k = 0 if p.PropagatorKernel == 0 else 1
+    cc = c if cmod is None else cmod
+    u, ph, ps = orc.acoustic_forward(k, NX, NY, nstep, p.DELTAT, p.DELTAX, p.DELTAY, Sx, Sy, cc ** 2,
+                                     src.srci, src.srcj, src.srcv)
+    tr = orc.acoustic_receive(u, rcv.rcvi, rcv.rcvj, NY)
+    if obs is None:
+        return tr
+    L, g, _ = orc.acoustic_misfit_gradient(k, NX, NY, nstep, p.DELTAT, p.DELTAX, p.DELTAY, Sx, Sy,
+                                           c, src.srci, src.srcj, src.srcv, rcv.rcvi, rcv.rcvj, obs)
+    return tr, L, g
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("K", [7, 1000])
+def test_plan_forward_and_gradient_f64(orc, ads, cuda, kernel, K):
+    p, c, srcs, rcvs, nstep = make_case(orc, ads, nshots=3, kernel=kernel, uneven=True)
+    obs, L_ref, g_ref = [], 0.0, np.zeros_like(c)
+    for s in range(3):
+        o = oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s], cmod=c * 1.03)
+        tr, L, g = oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s], obs=o)
+        obs.append(o)
+        L_ref += L
+        g_ref += g
+    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float64, nstep=nstep)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    ct = T(c)
+    tr = plan.forward(ct, srcv, K)
+    per = plan.unpack_traces(tr)
+    for s in range(3):
+        ref = oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s])
+        assert rel(per[s].cpu(), ref) < 1e-12, (s, rel(per[s].cpu(), ref))
+    loss, g, tr2 = plan.gradient(ct, srcv, plan.pack_traces(obs), K, want_traces=True)
+    torch.cuda.synchronize()
+    assert loss.item() == pytest.approx(L_ref, rel=1e-10)
+    assert rel(g.cpu(), g_ref) < 1e-10, rel(g.cpu(), g_ref)
+    np.testing.assert_array_equal(tr2.cpu().numpy(), tr.cpu().numpy())
+    # backward with an explicit trace cotangent equals gradient()'s residual path
+    tr3 = plan.forward(ct, srcv, K)
+    gb = plan.backward(ct, srcv, 2.0 * (tr3 - plan.pack_traces(obs)), K)
+    assert rel(gb.cpu(), g_ref) < 1e-10
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_plan_f32_against_f64_oracle(orc, ads, cuda, kernel):
+    """FP32 fields vs the FP64 oracle: traces <= 1e-4 rel L2, gradient <= 1e-3 rel L2."""
+    p, c, srcs, rcvs, nstep = make_case(orc, ads, NX=60, NY=50, NSTEP=300, npml=10, nshots=2,
+                                        kernel=kernel, seed=3)
+    obs, g_ref, trs = [], np.zeros_like(c), []
+    for s in range(2):
+        o = oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s], cmod=c * 1.02)
+        tr, L, g = oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s], obs=o)
+        obs.append(o)
+        trs.append(tr)
+        g_ref += g
+    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float32, nstep=nstep)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    loss, g, tr = plan.gradient(T(c, torch.float32), srcv, plan.pack_traces(obs), 40,
+                                want_traces=True)
+    per = plan.unpack_traces(tr)
+    for s in range(2):
+        assert rel(per[s].cpu(), trs[s]) < 1e-4, rel(per[s].cpu(), trs[s])
+    assert rel(g.cpu(), g_ref) < 1e-3, rel(g.cpu(), g_ref)
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_history_dropin_and_autograd(orc, ads, cuda, kernel):
+    """AcousticPropagatorSolver + SimulatedObservation_ + torch autograd (the plain drop-in)."""
+    p, c, srcs, rcvs, nstep = make_case(orc, ads, NX=24, NY=20, NSTEP=60, npml=5, kernel=kernel, seed=5)
+    src, rcv = srcs[0], rcvs[0]
+    Sx, Sy = orc.acoustic_pml(p.NX, p.NY, p.DELTAX, p.DELTAY, p.NPOINTS_PML, p.Rcoef, p.vp_ref)
+    k = 0 if kernel == 0 else 1
+    u_ref, ph_ref, ps_ref = orc.acoustic_forward(k, p.NX, p.NY, p.NSTEP, p.DELTAT, p.DELTAX, p.DELTAY,
+                                                 Sx, Sy, c ** 2, src.srci, src.srcj, src.srcv)
+    ct = T(c).requires_grad_(True)
+    ap = ads.AcousticPropagatorSolver(p, src, ct)
+    assert tuple(ap.u.shape) == (p.NSTEP + 1, p.NX + 2, p.NY + 2)
+    assert rel(ap.u.detach().cpu(), u_ref) < 1e-12
+    assert rel(ap.φ.cpu(), ph_ref) < 1e-12
+    assert rel(ap.ψ.cpu(), ps_ref) < 1e-12
+    obs = orc.acoustic_receive(orc.acoustic_forward(k, p.NX, p.NY, p.NSTEP, p.DELTAT, p.DELTAX,
+                                                    p.DELTAY, Sx, Sy, (1.04 * c) ** 2, src.srci,
+                                                    src.srcj, src.srcv)[0], rcv.rcvi, rcv.rcvj, p.NY)
+    rr = ads.SimulatedObservation_(ap, rcv)
+    loss = torch.sum((rr - T(obs)) ** 2)
+    loss.backward()
+    _, L_ref, g_ref = oracle_shot(orc, p, c, src, rcv, p.NSTEP, obs=obs)
+    assert loss.item() == pytest.approx(L_ref, rel=1e-10)
+    assert rel(ct.grad.cpu(), g_ref) < 1e-10
+
+
+def test_traces_autograd_function(orc, ads, cuda):
+    p, c, srcs, rcvs, nstep = make_case(orc, ads, nshots=2, kernel=0, seed=7)
+    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float64)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    ct = T(c).requires_grad_(True)
+    tr = ads.acoustic_traces(plan, ct, srcv, 9)
+    w = T(np.random.default_rng(1).standard_normal(tr.numel()))
+    (tr * w).sum().backward()
+    # dot-product check: <d tr/dc v, w> vs <v, grad>
+    v = np.random.default_rng(2).standard_normal(c.shape)
+    eps = 1e-3
+    f = lambda cc: float((plan.forward(T(cc), srcv, 9) * w).sum().item())
+    fd = (f(c + eps * v) - f(c - eps * v)) / (2 * eps)
+    assert np.sum(ct.grad.cpu().numpy() * v) == pytest.approx(fd, rel=1e-6)
+
+
+def test_workspace_too_small_is_an_error(orc, ads, cuda):
+    import ctypes as C
+    p, c, srcs, rcvs, nstep = make_case(orc, ads)
+    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float32)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    ws = torch.empty(1024, dtype=torch.uint8, device="cuda")
+    tr = torch.empty(plan.trace_elems, device="cuda")
+    rc = plan.lib.ads_acoustic_plan_forward(plan.h, C.c_void_p(T(c, torch.float32).data_ptr()),
+                                            C.c_void_p(srcv.data_ptr()), C.c_void_p(tr.data_ptr()),
+                                            C.c_void_p(ws.data_ptr()), 1024, 5, None)
+    assert rc == 6
