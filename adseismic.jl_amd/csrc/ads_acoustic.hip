@@ -339,6 +339,7 @@ struct AcPlanBase {
     DevBuf d_us_ptr, d_us_cell, d_us_eptr, d_us_ent, d_nsrc, d_srcv_off;
     DevBuf d_ur_ptr, d_ur_cell, d_ur_eptr, d_ur_ent, d_rcv_ptr, d_rcv_cell, d_trace_off;
     int64_t max_ucells_rcv = 0;
+    double f_pml = 0.0;  // fraction of interior cells inside the PML strips
 };
 
 static int validate_params(const ads_acoustic_params *p)
@@ -368,16 +369,20 @@ template <typename T> struct AcPlan : AcPlanBase {
     {
         if (act <= 0) return ADS_OK;
         dim3 b(BX, BY);
+        const double cells = (double)act * (double)NX * (double)NY;
+        int pr = prof_start(ADS_PROF_AC_FWD, st);
         if (variant == 1)
             ac_fwd_step<T, 1><<<grid_cells(act), b, 0, st>>>(uo, w, wo, phi, psi, phio, psio, C,
                                                              d_sx.as<T>(), d_ty.as<T>(),
                                                              d_nearx.as<uint8_t>(),
                                                              d_neary.as<uint8_t>(), (int)NX, (int)NY, k);
-        else {
+        else
             ac_fwd_step<T, 0><<<grid_cells(act), b, 0, st>>>(uo, w, wo, phi, psi, phio, psio, C,
                                                              d_sx.as<T>(), d_ty.as<T>(),
                                                              d_nearx.as<uint8_t>(),
                                                              d_neary.as<uint8_t>(), (int)NX, (int)NY, k);
+        prof_stop(pr, st, cells * sizeof(T) * (4.0 + (variant == 1 ? 4.0 : 2.0) * f_pml));
+        if (variant == 0) {
             ac_fwd_phipsi<T><<<grid_cells(act), b, 0, st>>>(
                 Lv<const T>{uo.p, uo.stride}, phi, psi, phio, psio, C, d_sx.as<T>(), d_ty.as<T>(),
                 d_pmlx.as<uint8_t>(), d_pmly.as<uint8_t>(), (int)NX, (int)NY, k);
@@ -397,20 +402,25 @@ template <typename T> struct AcPlan : AcPlanBase {
     {
         if (act <= 0) return ADS_OK;
         dim3 b(BX, BY);
+        const double cells = (double)act * (double)NX * (double)NY;
         if (variant == 1) {
+            int pr = prof_start(ADS_PROF_AC_ADJ, st);
             ac_adj_step<T, 1><<<grid_cells(act), b, 0, st>>>(
                 P, P1, F1, G1, F0, G0, um1, dense, gC, C, d_sx.as<T>(), d_ty.as<T>(),
                 d_nearx.as<uint8_t>(), d_neary.as<uint8_t>(), (int)NX, (int)NY, k);
+            prof_stop(pr, st, cells * sizeof(T) * (7.0 + 4.0 * f_pml));
         } else {
             ac_adj_phipsi<T><<<grid_cells(act), b, 0, st>>>(
                 P1, F1, G1, F0, G0, un, gC, n, C, d_sx.as<T>(), d_ty.as<T>(), d_pmlx.as<uint8_t>(),
                 d_pmly.as<uint8_t>(), d_us_ptr.as<int64_t>(), d_us_cell.as<int64_t>(),
                 d_us_eptr.as<int64_t>(), d_us_ent.as<int64_t>(), srcv, d_srcv_off.as<int64_t>(),
                 d_nsrc.as<int64_t>(), (int)NX, (int)NY, k);
+            int pr = prof_start(ADS_PROF_AC_ADJ, st);
             ac_adj_step<T, 0><<<grid_cells(act), b, 0, st>>>(
                 P, P1, Lv<const T>{F0.p, F0.stride}, Lv<const T>{G0.p, G0.stride}, F0, G0, um1,
                 dense, gC, C, d_sx.as<T>(), d_ty.as<T>(), d_nearx.as<uint8_t>(),
                 d_neary.as<uint8_t>(), (int)NX, (int)NY, k);
+            prof_stop(pr, st, cells * sizeof(T) * (7.0 + 2.0 * f_pml));
         }
         if (resid && max_ucells_rcv > 0) {
             dim3 g(cdiv(max_ucells_rcv, 256), act);
@@ -766,6 +776,12 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
     };
     flags(sxT, P->pmlx, P->nearx);
     flags(tyT, P->pmly, P->neary);
+    {
+        int64_t cnt = 0;
+        for (int64_t i = 1; i <= p->NX; i++)
+            for (int64_t j = 1; j <= p->NY; j++) cnt += (P->pmlx[i] | P->pmly[j]) != 0;
+        P->f_pml = (double)cnt / ((double)p->NX * (double)p->NY);
+    }
     ADS_TRY(P->d_sx.upload(sxT));
     ADS_TRY(P->d_ty.upload(tyT));
     ADS_TRY(P->d_pmlx.upload(P->pmlx));

@@ -2,6 +2,7 @@
 // tables and geometry tables.  No device code.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <cstring>
 #include <numeric>
 
@@ -20,6 +21,53 @@ void set_error(const char *fmt, ...)
 }
 
 void clear_error() { g_err[0] = 0; }
+
+// ---- event-based kernel timing ----
+namespace {
+struct ProfRec {
+    int cls;
+    hipEvent_t a, b;
+    double bytes;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfRec> g_recs;
+std::vector<hipEvent_t> g_pool;
+double g_ms[ADS_PROF_COUNT], g_bytes[ADS_PROF_COUNT];
+int64_t g_n[ADS_PROF_COUNT];
+
+hipEvent_t pool_get()
+{
+    if (!g_pool.empty()) {
+        hipEvent_t e = g_pool.back();
+        g_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+}  // namespace
+
+int prof_start(int cls, hipStream_t st)
+{
+    if (!g_prof_on) return -1;
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    ProfRec r{cls, pool_get(), pool_get(), 0.0};
+    if (!r.a || !r.b) return -1;
+    if (hipEventRecord(r.a, st) != hipSuccess) return -1;
+    g_recs.push_back(r);
+    return (int)g_recs.size() - 1;
+}
+
+void prof_stop(int rec, hipStream_t st, double bytes)
+{
+    if (rec < 0) return;
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (rec >= (int)g_recs.size()) return;
+    g_recs[rec].bytes = bytes;
+    (void)hipEventRecord(g_recs[rec].b, st);
+}
 
 CellLists build_cell_lists(int64_t nshots, const int64_t *ptr, const std::vector<int64_t> &flat)
 {
@@ -58,6 +106,50 @@ extern "C" {
 const char *ads_last_error(void) { return g_err; }
 
 int ads_version(void) { return 100; /* 0.1.0 */ }
+
+int ads_profile_begin(void)
+{
+    clear_error();
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (int k = 0; k < ADS_PROF_COUNT; k++) g_ms[k] = g_bytes[k] = 0.0, g_n[k] = 0;
+    g_recs.clear();
+    g_prof_on = true;
+    return ADS_OK;
+}
+
+int ads_profile_end(void)
+{
+    clear_error();
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_on = false;
+    int rc = ADS_OK;
+    for (auto &r : g_recs) {
+        float ms = 0.f;
+        if (hipEventSynchronize(r.b) != hipSuccess || hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) {
+            rc = ADS_ERR_HIP;
+            set_error("event timing failed");
+        } else if (r.cls >= 0 && r.cls < ADS_PROF_COUNT) {
+            g_ms[r.cls] += ms;
+            g_n[r.cls] += 1;
+            g_bytes[r.cls] += r.bytes;
+        }
+        g_pool.push_back(r.a);
+        g_pool.push_back(r.b);
+    }
+    g_recs.clear();
+    return rc;
+}
+
+int ads_profile_query(int cls, double *total_ms, int64_t *launches, double *alg_bytes)
+{
+    clear_error();
+    if (cls < 0 || cls >= ADS_PROF_COUNT) ADS_FAIL(ADS_ERR_ARG, "bad kernel class %d", cls);
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (total_ms) *total_ms = g_ms[cls];
+    if (launches) *launches = g_n[cls];
+    if (alg_bytes) *alg_bytes = g_bytes[cls];
+    return ADS_OK;
+}
 
 // pml_helper (src/Core.jl:738-761) on one axis; coordinate k*h + origin, far edge at
 // h*(NXg+1) - L.

@@ -1,0 +1,226 @@
+"""Benchmark: batched FP32 acoustic FWI gradient (forward + hand-written adjoint) on one MI355X.
+
+Workload (the metric's configuration): 2-D acoustic, 1000 x 1000 interior cells, h = 5 m,
+dt = 0.5 ms, 4000 steps, 40-cell PML, random layered velocity model (8-20 layers,
+1500-4500 m/s increasing with depth, 3 Gaussian anomalies of +-15%, seed 2), 32 Ricker shots
+(8-15 Hz) evenly spaced along the surface, 1000 receivers per shot.  Observed data are simulated
+from the true model; the gradient is taken at a smoothed starting model.  One "step" is one
+complete FWI gradient of all 32 shots: forward with checkpoints, misfit, checkpoint
+recomputation and the adjoint sweep, all in one batched call.
+
+Prints ONE JSON line.  Besides value/ms_per_step it carries:
+  roofline      : the dominant stencil kernel's algorithmic HBM bytes per launch / its average
+                  launch duration (HIP events on the launch stream, over the timed region);
+  cpu_baseline  : the FP64 CPU restatement (oracle/, one thread, like the reference's serial
+                  CPU op) on a bounded sample of the same workload;
+  plain_single_shot : the plain drop-in path (AcousticPropagatorSolver + SimulatedObservation_
+                  + autograd, full histories, one shot) on the same grid.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBPS = 8000.0
+
+
+def velocity_model(NX, NY, seed):
+    """Random layered model (depth = j), 1500-4500 m/s increasing with depth, + 3 anomalies."""
+    rng = np.random.default_rng(seed)
+    nl = int(rng.integers(8, 21))
+    depths = np.sort(rng.choice(np.arange(1, NY + 1), size=nl - 1, replace=False))
+    vels = np.sort(rng.uniform(1500.0, 4500.0, size=nl))
+    c = np.empty((NX + 2, NY + 2))
+    layer = np.searchsorted(depths, np.arange(NY + 2), side="right")
+    c[:] = vels[layer][None, :]
+    ii, jj = np.meshgrid(np.arange(NX + 2), np.arange(NY + 2), indexing="ij")
+    for _ in range(3):
+        ci, cj = rng.uniform(1, NX), rng.uniform(1, NY)
+        sg = rng.uniform(20, 80)
+        amp = rng.choice([-1.0, 1.0]) * 0.15
+        c *= 1.0 + amp * np.exp(-((ii - ci) ** 2 + (jj - cj) ** 2) / (2 * sg * sg))
+    return c
+
+
+def workload(ads, small=False):
+    import torch
+    from scipy.ndimage import gaussian_filter
+
+    if small:
+        NX = NY = 200
+        NSTEP, nshots, npml = 600, 4, 20
+    else:
+        NX = NY = 1000
+        NSTEP, nshots, npml = 4000, 32, 40
+    p = ads.AcousticPropagatorParams(NX=NX, NY=NY, NSTEP=NSTEP, DELTAX=5.0, DELTAY=5.0,
+                                     DELTAT=0.5e-3, NPOINTS_PML=npml, vp_ref=3000.0,
+                                     PropagatorKernel=1)
+    c_true = velocity_model(NX, NY, seed=2)
+    c0 = gaussian_filter(c_true, 10.0, mode="reflect")
+    ads.check_cfl(p, float(c_true.max()))
+    rng = np.random.default_rng(2)
+    xs = np.round(np.linspace(npml + 2, NX + 1 - npml, nshots)).astype(np.int64)
+    f0s = rng.uniform(8.0, 15.0, nshots)
+    srcs, rcvs = [], []
+    for s in range(nshots):
+        w = ads.ricker_freq(f0s[s], p.DELTAT, NSTEP, 1e4)
+        srcs.append(ads.AcousticSource([int(xs[s])], [3], w.reshape(-1, 1)))
+        rcvs.append(ads.AcousticReceiver(np.arange(2, NX + 2), np.full(NX, 3)))
+    return p, c_true, c0, srcs, rcvs
+
+
+def cpu_baseline(p, c0, src, nsteps):
+    """Oracle (FP64, one thread) forward + adjoint of one shot over the first nsteps steps."""
+    import oracle as O
+
+    O.set_num_threads(1)
+    NX, NY = p.NX, p.NY
+    Sx, Sy = O.acoustic_pml(NX, NY, p.DELTAX, p.DELTAY, p.NPOINTS_PML, p.Rcoef, p.vp_ref)
+    srcv = np.asarray(src.srcv)[:nsteps]
+    rcvi, rcvj = np.arange(2, NX + 2), np.full(NX, 3)
+    t0 = time.perf_counter()
+    u, ph, ps = O.acoustic_forward(1, NX, NY, nsteps, p.DELTAT, p.DELTAX, p.DELTAY, Sx, Sy,
+                                   c0 ** 2, src.srci, src.srcj, srcv)
+    tr = O.acoustic_receive(u, rcvi, rcvj, NY)
+    gu = np.zeros((nsteps + 1, (NX + 2) * (NY + 2)))
+    gu[:, O.flat_index(rcvi, rcvj, NY)] = 2.0 * tr
+    O.acoustic_adjoint(1, NX, NY, nsteps, p.DELTAT, p.DELTAX, p.DELTAY, Sx, Sy, c0 ** 2, u, ph,
+                       ps, gu)
+    t = time.perf_counter() - t0
+    cells = NX * NY * (nsteps - 1)
+    return {"value": cells / t / 1e9, "unit": "Gcell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"1 shot, {NX}x{NY} grid, first {nsteps} of {p.NSTEP} steps, FP64 "
+                      f"forward+adjoint (oracle/ads_oracle.c, PropagatorKernel 1), {t:.1f} s",
+            "host_cpu": platform.processor() or platform.machine(),
+            "host_threads_available": len(os.sched_getaffinity(0))}
+
+
+def plain_single_shot(ads, p, c0, src, rcv, obs_shot):
+    """The plain drop-in: history solver + SimulatedObservation_ + torch autograd, one shot."""
+    import torch
+
+    ct = torch.as_tensor(c0, dtype=torch.float32, device="cuda")
+    obs = torch.as_tensor(obs_shot, dtype=torch.float32, device="cuda")
+
+    def once():
+        cc = ct.clone().requires_grad_(True)
+        ap = ads.AcousticPropagatorSolver(p, src, cc)
+        r = ads.SimulatedObservation_(ap, rcv)
+        loss = torch.sum((r - obs) ** 2)
+        loss.backward()
+        return cc.grad
+
+    once()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    once()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    cells = p.NX * p.NY * p.NSTEP
+    return {"value": cells / t / 1e9, "unit": "Gcell-steps/s", "ms": t * 1e3,
+            "path": "AcousticPropagatorSolver + SimulatedObservation_ + autograd (full history, "
+                    "one shot, FP32)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--small", action="store_true", help="reduced grid for quick checks")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--interval", type=int, default=0, help="checkpoint interval (0 = auto)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the "
+              "multi-GPU shard design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__ as g
+
+    ads = g.load_package()
+    import importlib
+    L = importlib.import_module("adseismic_amd._lib")
+    lib = ads.load_library()
+    torch.cuda.init()
+    p, c_true, c0, srcs, rcvs = workload(ads, small=args.small)
+    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float32)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    K = args.interval or plan.choose_interval(int(torch.cuda.mem_get_info()[0] * 0.80))
+    ct = torch.as_tensor(c_true, dtype=torch.float32, device="cuda")
+    c0t = torch.as_tensor(c0, dtype=torch.float32, device="cuda")
+    obs = plan.forward(ct, srcv, K).clone()
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        plan.gradient(c0t, srcv, obs, K)
+    torch.cuda.synchronize()
+    L.check(lib.ads_profile_begin())
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss, grad, _ = plan.gradient(c0t, srcv, obs, K)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    L.check(lib.ads_profile_end())
+    ms = t / args.steps * 1e3
+    cells = float(np.sum(p.NX * p.NY * plan.nstep))
+    value = cells / (ms * 1e-3) / 1e9
+    if not math.isfinite(float(loss.item())):
+        raise RuntimeError("non-finite misfit")
+
+    prof = {}
+    for name, cls in (("ac_fwd_step", 0), ("ac_adj_step", 1)):
+        tm, n, b = C.c_double(), C.c_int64(), C.c_double()
+        L.check(lib.ads_profile_query(cls, C.byref(tm), C.byref(n), C.byref(b)))
+        prof[name] = (tm.value, n.value, b.value)
+    dom = max(prof, key=lambda k: prof[k][0])
+    tm, n, b = prof[dom]
+    achieved = (b / n) / (tm / n * 1e-3) / 1e9 if n else 0.0
+    out = {
+        "metric": "FWI gradient Gcell-steps/s (fwd+adj), batched",
+        "value": round(value, 4), "unit": "Gcell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": (f"acoustic {p.NX}x{p.NY}, h=5 m, dt=0.5 ms, {p.NSTEP} steps, "
+                                f"{p.NPOINTS_PML}-cell PML, {plan.nshots} shots x {p.NX} "
+                                "receivers, FP32 forward + hand-written adjoint, one batched "
+                                "call per gradient"),
+                   "PropagatorKernel": p.PropagatorKernel, "checkpoint_interval": K,
+                   "shots": plan.nshots, "cells": p.NX * p.NY, "nstep": p.NSTEP},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
+                     "traffic": None, "avg_launch_us": round(tm / n * 1e3, 2) if n else None,
+                     "launches": n, "alg_bytes_per_launch": round(b / n) if n else None},
+        "kernels": {k: {"total_ms": round(v[0], 2), "launches": v[1],
+                        "avg_us": round(v[0] / v[1] * 1e3, 2) if v[1] else None,
+                        "alg_GBps": round(v[2] / (v[0] * 1e-3) / 1e9, 1) if v[0] else None}
+                    for k, v in prof.items()},
+        "loss": float(loss.item()),
+    }
+    tr_true = plan.unpack_traces(obs)[0].cpu().numpy()
+    plan._ws = None  # release the batched workspace before the full-history run
+    del grad
+    torch.cuda.empty_cache()
+    if not args.no_plain:
+        out["plain_single_shot"] = plain_single_shot(ads, p, c0, srcs[0], rcvs[0], tr_true)
+        out["plain_single_shot"]["batched_over_plain"] = round(value / out["plain_single_shot"]["value"], 2)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(p, c0, srcs[0], min(args.cpu_steps, p.NSTEP))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

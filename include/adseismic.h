@@ -294,6 +294,29 @@ int ads_elastic_forward_history(ads_elastic_plan *plan, const void *rho, const v
                                 const void *mu, const void *srcv, void *hist,
                                 ads_stream_t stream);
 
+/* ======================================================================================
+ * Kernel timing for benchmarks.  Between ads_profile_begin() and ads_profile_end() every
+ * launch of the main stencil kernels made by the plan entry points is bracketed by HIP events
+ * on its launch stream; ads_profile_end() waits for them and accumulates, per kernel class,
+ * the summed duration, the launch count and the algorithmic HBM bytes of those launches
+ * (compulsory per-cell traffic: acoustic forward w(4 + 4 f_pml), adjoint w(7 + 4 f_pml) for
+ * PropagatorKernel 1; w(4 + 2 f_pml) and w(7 + 2 f_pml) for kernel 0 whose phi/psi updates
+ * run in a separate pass; elastic as documented per class).  Not thread-safe across
+ * concurrent callers; meant for one benchmarking thread.
+ * ==================================================================================== */
+enum {
+    ADS_PROF_AC_FWD = 0,       /* acoustic forward step kernel */
+    ADS_PROF_AC_ADJ = 1,       /* acoustic adjoint step kernel */
+    ADS_PROF_EL_FWD_STRESS = 2,
+    ADS_PROF_EL_FWD_VEL = 3,
+    ADS_PROF_EL_ADJ_VEL = 4,
+    ADS_PROF_EL_ADJ_STRESS = 5,
+    ADS_PROF_COUNT = 6
+};
+int ads_profile_begin(void);
+int ads_profile_end(void);
+int ads_profile_query(int kernel_class, double *total_ms, int64_t *launches, double *alg_bytes);
+
 #ifdef __cplusplus
 }
 #endif
