@@ -220,6 +220,23 @@ class AcousticPlan:
                                                        ws.numel(), int(K), _stream()))
         return loss, g, tr
 
+    def autotune(self, model: torch.Tensor, srcv: torch.Tensor, K: int, reps: int = 3):
+        """Pick the fastest (rows per wavefront, shots per workgroup) for the forward and adjoint
+        kernels on this plan (ads_acoustic_plan_autotune).  Returns the chosen shape."""
+        model = _dev(model, self.dtype)
+        ws = self.workspace(K)
+        out = (C.c_int * 4)()
+        _lib.check(self.lib.ads_acoustic_plan_autotune(self.h, _vp(model), _vp(srcv), _vp(ws),
+                                                       ws.numel(), int(K), int(reps), out,
+                                                       _stream()))
+        self.shape = tuple(out)
+        return self.shape
+
+    def set_shape(self, shape):
+        arr = (C.c_int * 4)(*[int(x) for x in shape])
+        _lib.check(self.lib.ads_acoustic_plan_set_shape(self.h, arr))
+        self.shape = tuple(shape)
+
     def _scratch_buf(self):
         need = int(self.lib.ads_acoustic_plan_history_scratch_size(self.h))
         if self._scratch is None or self._scratch.numel() < need:

@@ -1,5 +1,5 @@
 // ads_acoustic.hip -- acoustic PML time stepper for gfx950: forward, hand-written adjoint,
-// checkpointed multi-shot batches, full-history drop-in.
+// checkpointed multi-shot batches, full-history drop-in, kernel-shape autotuner.
 //
 // Scheme (src/Core.jl:528-549 / AcousticOneStepCpu.h:1-48), interior cell (i, j):
 //   u^n = [(2 - s t dt^2) w + C lap(w) + ex (phi_{i+1} - phi_{i-1}) + ey (psi_{j+1} - psi_{j-1})
@@ -18,7 +18,16 @@
 //   dL/dC += P^n lap(u^{n-1}) + F^n (t - s) fx (v_{i+1} - v_{i-1}) + G^n (s - t) fy (...)
 // where Cp = C (t - s), Cq = C (s - t) and F in the dL/du^n line is F^{n+1} (kernel 1) or F^n
 // (kernel 0).  Receiver residuals R^n are added after the stencil pass by a per-cell gather.
+//
+// Kernel shape.  A workgroup is 4 wavefronts; each wavefront owns 64 consecutive depth cells
+// (j) of RPT consecutive rows (i) and marches down them, keeping rows i-1, i, i+1 in registers
+// (each row is loaded once); j-1/j+1 come from neighbouring lanes by shuffle (edge lanes load
+// them).  A workgroup loops over a group of Sg shots: the model rows it needs stay in registers
+// across the group and, in the adjoint, dL/dC is accumulated in registers and written once per
+// group.  Rows are padded to a 64-element pitch so every wavefront access is 256-byte aligned,
+// and workgroup ids are swizzled so that each of the 8 XCDs walks a contiguous band of tiles.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -53,70 +62,127 @@ template <typename T> struct Lv {
     int64_t stride;
     __device__ __host__ T *at(int s) const { return p + (int64_t)s * stride; }
 };
+template <typename T> inline Lv<const T> cst(Lv<T> a) { return Lv<const T>{a.p, a.stride}; }
 
-constexpr int BX = 64;  // lanes along j (depth, contiguous)
-constexpr int BY = 4;   // rows along i
+constexpr int WAVE = 64;
+constexpr int NWAVES = 4;
+constexpr int NXCD = 8;
 
-// ---------------------------------------------------------------------------------------------
-// forward step: u^n (pre-source) and, for kernel 1, phi^n/psi^n.  One thread per cell.
-// ---------------------------------------------------------------------------------------------
-template <typename T, int V>
-__global__ __launch_bounds__(BX *BY) void ac_fwd_step(
-    Lv<T> uo, Lv<const T> w_, Lv<const T> wold_, Lv<const T> phi_, Lv<const T> psi_, Lv<T> phio,
-    Lv<T> psio, const T *__restrict__ C, const T *__restrict__ sx, const T *__restrict__ ty,
-    const uint8_t *__restrict__ nearx, const uint8_t *__restrict__ neary, int NX, int NY,
-    AcCoef<T> k)
+// tiles: ntj (j) x nstrips (i) x ngroups (shot groups), swizzled over the XCDs
+struct TileMap {
+    int ntj, nstrips, ngroups, nb;
+};
+
+__device__ inline bool decode_tile(const TileMap &m, int &g, int &strip, int &tj)
 {
-    const int j = blockIdx.x * BX + threadIdx.x;
-    const int i = blockIdx.y * BY + threadIdx.y;
-    const int s = blockIdx.z;
-    const int S = NY + 2;
-    if (j >= S || i >= NX + 2) return;
-    const int64_t ij = (int64_t)i * S + j;
-    T *__restrict__ u = uo.at(s);
-    if (i == 0 || i == NX + 1 || j == 0 || j == NY + 1) {
-        u[ij] = T(0);
-        return;
-    }
-    const T *__restrict__ w = w_.at(s);
-    const T wc = w[ij], wn = w[ij - S], ws = w[ij + S], ww = w[ij - 1], we = w[ij + 1];
-    const T Cij = C[ij];
-    const T lap = k.cx * (ws + wn - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
-    const T wo = wold_.at(s)[ij];
-    if (!(nearx[i] | neary[j])) {
-        u[ij] = T(2) * wc - wo + Cij * lap;
-        return;
-    }
-    const T sg = sx[i], tu = ty[j];
-    const T *__restrict__ phi = phi_.at(s);
-    const T *__restrict__ psi = psi_.at(s);
-    T num = (T(2) - sg * tu * k.dt2) * wc + Cij * lap + k.ex * (phi[ij + S] - phi[ij - S]) +
-            k.ey * (psi[ij + 1] - psi[ij - 1]) - (T(1) - (sg + tu) * k.hdt) * wo;
-    u[ij] = num / (T(1) + (sg + tu) * k.hdt);
-    if (V == 1 && (sg != T(0) || tu != T(0))) {
-        phio.at(s)[ij] = (T(1) - k.dt * sg) * phi[ij] + Cij * (tu - sg) * k.fx * (ws - wn);
-        psio.at(s)[ij] = (T(1) - k.dt * tu) * psi[ij] + Cij * (sg - tu) * k.fy * (we - ww);
+    const int nb8 = (m.nb + NXCD - 1) / NXCD;
+    const int b = blockIdx.x;
+    const int L = (b % NXCD) * nb8 + b / NXCD;
+    if (L >= m.nb) return false;
+    tj = L % m.ntj;
+    const int r = L / m.ntj;
+    strip = r % m.nstrips;
+    g = r / m.nstrips;
+    return true;
+}
+
+template <typename T> __device__ inline T ldg(const T *p, int64_t i, bool ok) { return ok ? p[i] : T(0); }
+
+// ---------------------------------------------------------------------------------------------
+// forward step: u^n (pre-source) and, for kernel 1, phi^n/psi^n
+// ---------------------------------------------------------------------------------------------
+template <typename T> struct FwdArgs {
+    Lv<T> u;
+    Lv<const T> w, wold, phi, psi;
+    Lv<T> phio, psio;
+    const T *C, *sx, *ty;
+    const uint8_t *nearx, *neary;
+    int NX, NY, P, act, Sg;
+    TileMap map;
+    AcCoef<T> k;
+};
+
+template <typename T, int V, int RPT>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd2(const FwdArgs<T> a)
+{
+    int g, strip, tj;
+    if (!decode_tile(a.map, g, strip, tj)) return;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int j = tj * WAVE + lane;
+    const int ib = (strip * NWAVES + wv) * RPT;
+    const int NX = a.NX, NY = a.NY, P = a.P;
+    if (ib > NX + 1) return;  // wave-uniform
+    const bool jvalid = j < NY + 2, jin = j >= 1 && j <= NY;
+    const AcCoef<T> k = a.k;
+    T Cr[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; r++) Cr[r] = ldg(a.C, (int64_t)(ib + r) * P + j, jvalid && ib + r <= NX + 1);
+    const T tj_ = jvalid ? a.ty[j] : T(0);
+    const bool nyj = jvalid ? a.neary[j] != 0 : false;
+    const int s1 = min(a.act, (g + 1) * a.Sg);
+    for (int s = g * a.Sg; s < s1; s++) {
+        const T *__restrict__ w = a.w.at(s);
+        T *__restrict__ u = a.u.at(s);
+        T wn = ldg(w, (int64_t)(ib - 1) * P + j, jvalid && ib >= 1);
+        T wc = ldg(w, (int64_t)ib * P + j, jvalid);
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            const int i = ib + r;
+            if (i > NX + 1) break;
+            const int64_t ij = (int64_t)i * P + j;
+            const T ws = ldg(w, ij + P, jvalid && i + 1 <= NX + 1);
+            T ww = __shfl_up(wc, 1), we = __shfl_down(wc, 1);
+            const bool in = jin && i >= 1 && i <= NX;
+            if (lane == 0 && in) ww = w[ij - 1];
+            if (lane == WAVE - 1 && in) we = w[ij + 1];
+            if (jvalid) {
+                if (!in) {
+                    u[ij] = T(0);
+                } else {
+                    const T Cij = Cr[r];
+                    const T lap = k.cx * (ws + wn - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
+                    const T wo = a.wold.at(s)[ij];
+                    if (!(a.nearx[i] | nyj)) {
+                        u[ij] = T(2) * wc - wo + Cij * lap;
+                    } else {
+                        const T sg = a.sx[i], tu = tj_;
+                        const T *__restrict__ phi = a.phi.at(s);
+                        const T *__restrict__ psi = a.psi.at(s);
+                        const T num = (T(2) - sg * tu * k.dt2) * wc + Cij * lap +
+                                      k.ex * (phi[ij + P] - phi[ij - P]) +
+                                      k.ey * (psi[ij + 1] - psi[ij - 1]) -
+                                      (T(1) - (sg + tu) * k.hdt) * wo;
+                        u[ij] = num / (T(1) + (sg + tu) * k.hdt);
+                        if (V == 1 && (sg != T(0) || tu != T(0))) {
+                            a.phio.at(s)[ij] = (T(1) - k.dt * sg) * phi[ij] + Cij * (tu - sg) * k.fx * (ws - wn);
+                            a.psio.at(s)[ij] = (T(1) - k.dt * tu) * psi[ij] + Cij * (sg - tu) * k.fy * (we - ww);
+                        }
+                    }
+                }
+            }
+            wn = wc;
+            wc = ws;
+        }
     }
 }
 
-// kernel 0: phi^n/psi^n from the new u (before sources), PML cells only.
+// kernel 0: phi^n/psi^n from the new u (before sources), PML cells only.  One thread per cell.
 template <typename T>
-__global__ __launch_bounds__(BX *BY) void ac_fwd_phipsi(
+__global__ __launch_bounds__(256) void ac_fwd_phipsi(
     Lv<const T> u_, Lv<const T> phi_, Lv<const T> psi_, Lv<T> phio, Lv<T> psio,
     const T *__restrict__ C, const T *__restrict__ sx, const T *__restrict__ ty,
-    const uint8_t *__restrict__ pmlx, const uint8_t *__restrict__ pmly, int NX, int NY,
+    const uint8_t *__restrict__ pmlx, const uint8_t *__restrict__ pmly, int NX, int NY, int P,
     AcCoef<T> k)
 {
-    const int j = blockIdx.x * BX + threadIdx.x;
-    const int i = blockIdx.y * BY + threadIdx.y;
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int s = blockIdx.z;
-    const int S = NY + 2;
     if (j < 1 || j > NY || i < 1 || i > NX) return;
     if (!(pmlx[i] | pmly[j])) return;
-    const int64_t ij = (int64_t)i * S + j;
+    const int64_t ij = (int64_t)i * P + j;
     const T *__restrict__ u = u_.at(s);
     const T sg = sx[i], tu = ty[j], Cij = C[ij];
-    phio.at(s)[ij] = (T(1) - k.dt * sg) * phi_.at(s)[ij] + Cij * (tu - sg) * k.fx * (u[ij + S] - u[ij - S]);
+    phio.at(s)[ij] = (T(1) - k.dt * sg) * phi_.at(s)[ij] + Cij * (tu - sg) * k.fx * (u[ij + P] - u[ij - P]);
     psio.at(s)[ij] = (T(1) - k.dt * tu) * psi_.at(s)[ij] + Cij * (sg - tu) * k.fy * (u[ij + 1] - u[ij - 1]);
 }
 
@@ -154,114 +220,178 @@ __global__ __launch_bounds__(256) void ac_src_rcv(
 // ---------------------------------------------------------------------------------------------
 // kernel 0 only: F^n, G^n on PML cells and their dL/dC term with v = u^n before sources.
 template <typename T>
-__global__ __launch_bounds__(BX *BY) void ac_adj_phipsi(
+__global__ __launch_bounds__(256) void ac_adj_phipsi(
     Lv<const T> P1_, Lv<const T> F1_, Lv<const T> G1_, Lv<T> F0_, Lv<T> G0_, Lv<const T> un_,
-    Lv<T> gC_, int n, const T *__restrict__ C, const T *__restrict__ sx, const T *__restrict__ ty,
+    Lv<T> gC_, int Sg, int n, const T *__restrict__ sx, const T *__restrict__ ty,
     const uint8_t *__restrict__ pmlx, const uint8_t *__restrict__ pmly,
     const int64_t *__restrict__ us_ptr, const int64_t *__restrict__ us_cell,
     const int64_t *__restrict__ us_eptr, const int64_t *__restrict__ us_ent,
     const T *__restrict__ srcv, const int64_t *__restrict__ srcv_off,
-    const int64_t *__restrict__ nsrc, int NX, int NY, AcCoef<T> k)
+    const int64_t *__restrict__ nsrc, int NX, int NY, int P, AcCoef<T> k)
 {
-    const int j = blockIdx.x * BX + threadIdx.x;
-    const int i = blockIdx.y * BY + threadIdx.y;
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int s = blockIdx.z;
-    const int S = NY + 2;
     if (j < 1 || j > NY || i < 1 || i > NX) return;
     if (!(pmlx[i] | pmly[j])) return;
-    const int64_t ij = (int64_t)i * S + j;
+    const int64_t ij = (int64_t)i * P + j;
     const T *__restrict__ P1 = P1_.at(s);
     const T sg = sx[i], tu = ty[j];
-    const T F0 = (T(1) - k.dt * sg) * F1_.at(s)[ij] + k.ex * (P1[ij - S] - P1[ij + S]);
+    const T F0 = (T(1) - k.dt * sg) * F1_.at(s)[ij] + k.ex * (P1[ij - P] - P1[ij + P]);
     const T G0 = (T(1) - k.dt * tu) * G1_.at(s)[ij] + k.ey * (P1[ij - 1] - P1[ij + 1]);
     F0_.at(s)[ij] = F0;
     G0_.at(s)[ij] = G0;
     const T *__restrict__ u = un_.at(s);
-    T vn = u[ij - S], vs = u[ij + S], vw = u[ij - 1], ve = u[ij + 1];
+    T vn = u[ij - P], vs = u[ij + P], vw = u[ij - 1], ve = u[ij + 1];
     // remove this level's sources from the neighbours (v is u^n before injection)
     const int64_t ns = nsrc[s];
     const T *__restrict__ row = srcv + srcv_off[s] + (int64_t)(n - 1) * ns;
     for (int64_t c = us_ptr[s]; c < us_ptr[s + 1]; c++) {
         const int64_t cell = us_cell[c];
-        if (cell != ij - S && cell != ij + S && cell != ij - 1 && cell != ij + 1) continue;
+        if (cell != ij - P && cell != ij + P && cell != ij - 1 && cell != ij + 1) continue;
         T add = T(0);
         for (int64_t e = us_eptr[c]; e < us_eptr[c + 1]; e++) add += row[us_ent[e]] * k.dt2;
-        if (cell == ij - S) vn -= add;
-        if (cell == ij + S) vs -= add;
+        if (cell == ij - P) vn -= add;
+        if (cell == ij + P) vs -= add;
         if (cell == ij - 1) vw -= add;
         if (cell == ij + 1) ve -= add;
     }
-    const T Cij = C[ij];
-    (void)Cij;
-    gC_.at(s)[ij] += F0 * (tu - sg) * k.fx * (vs - vn) + G0 * (sg - tu) * k.fy * (ve - vw);
+    gC_.at(s / Sg)[ij] += F0 * (tu - sg) * k.fx * (vs - vn) + G0 * (sg - tu) * k.fy * (ve - vw);
 }
 
-template <typename T, int V>
-__global__ __launch_bounds__(BX *BY) void ac_adj_step(
-    Lv<T> P_, Lv<const T> P1_, Lv<const T> Fn_, Lv<const T> Gn_, Lv<T> F0_, Lv<T> G0_,
-    Lv<const T> um1_, Lv<const T> dense_, Lv<T> gC_, const T *__restrict__ C,
-    const T *__restrict__ sx, const T *__restrict__ ty, const uint8_t *__restrict__ nearx,
-    const uint8_t *__restrict__ neary, int NX, int NY, AcCoef<T> k)
+template <typename T> struct AdjArgs {
+    Lv<T> Pn;                 // in: P^{n+2}, out: P^n
+    Lv<const T> P1, Fn, Gn;   // P^{n+1}; F/G read at neighbours (F^{n+1} kernel 1, F^n kernel 0)
+    Lv<T> F0, G0;             // kernel 1: F^n, G^n
+    Lv<const T> um1, dense;   // u^{n-1}; optional dense cotangent of u^n
+    Lv<T> gC;                 // per shot group
+    const T *C, *sx, *ty;
+    const uint8_t *nearx, *neary;
+    int NX, NY, P, act, Sg;
+    TileMap map;
+    AcCoef<T> k;
+};
+
+template <typename T, int V, int RPT>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_adj2(const AdjArgs<T> a)
 {
-    const int j = blockIdx.x * BX + threadIdx.x;
-    const int i = blockIdx.y * BY + threadIdx.y;
-    const int s = blockIdx.z;
-    const int S = NY + 2;
-    if (j < 1 || j > NY || i < 1 || i > NX) return;
-    const int64_t ij = (int64_t)i * S + j;
-    const T *__restrict__ P1 = P1_.at(s);
-    T *__restrict__ P = P_.at(s);
-    const T p1c = P1[ij], p1n = P1[ij - S], p1s = P1[ij + S], p1w = P1[ij - 1], p1e = P1[ij + 1];
-    const T Cc = C[ij], Cn = C[ij - S], Cs = C[ij + S], Cw = C[ij - 1], Ce = C[ij + 1];
-    const T qc = Cc * p1c;
-    const T lapq = k.cx * (Cs * p1s + Cn * p1n - T(2) * qc) + k.cy * (Ce * p1e + Cw * p1w - T(2) * qc);
-    const T p2 = P[ij];
-    const bool near = nearx[i] | neary[j];
-    T sg = T(0), tu = T(0);
-    T U;
-    if (!near) {
-        U = T(2) * p1c + lapq - p2;
-    } else {
-        sg = sx[i];
-        tu = ty[j];
-        const T *__restrict__ Fn = Fn_.at(s);
-        const T *__restrict__ Gn = Gn_.at(s);
-        U = (T(2) - sg * tu * k.dt2) * p1c + lapq - (T(1) - (sg + tu) * k.hdt) * p2 +
-            k.fx * (Cn * (tu - sx[i - 1]) * Fn[ij - S] - Cs * (tu - sx[i + 1]) * Fn[ij + S]) +
-            k.fy * (Cw * (sg - ty[j - 1]) * Gn[ij - 1] - Ce * (sg - ty[j + 1]) * Gn[ij + 1]);
+    int g, strip, tj;
+    if (!decode_tile(a.map, g, strip, tj)) return;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int j = tj * WAVE + lane;
+    const int ib = (strip * NWAVES + wv) * RPT;
+    const int NX = a.NX, NY = a.NY, P = a.P;
+    if (ib > NX + 1) return;
+    const bool jvalid = j < NY + 2, jin = j >= 1 && j <= NY;
+    const AcCoef<T> k = a.k;
+    // model rows ib-1 .. ib+RPT stay in registers across the shot group
+    T Cr[RPT + 2];
+#pragma unroll
+    for (int r = 0; r < RPT + 2; r++) {
+        const int i = ib - 1 + r;
+        Cr[r] = ldg(a.C, (int64_t)i * P + j, jvalid && i >= 0 && i <= NX + 1);
     }
-    if (dense_.p) U += dense_.at(s)[ij];
-    const T Pn = near ? U / (T(1) + (sg + tu) * k.hdt) : U;
-    P[ij] = Pn;
-    const T *__restrict__ um = um1_.at(s);
-    const T uc = um[ij], un = um[ij - S], us = um[ij + S], uw = um[ij - 1], ue = um[ij + 1];
-    T g = Pn * (k.cx * (us + un - T(2) * uc) + k.cy * (ue + uw - T(2) * uc));
-    if (V == 1 && near && (sg != T(0) || tu != T(0))) {
-        const T F0 = (T(1) - k.dt * sg) * Fn_.at(s)[ij] + k.ex * (p1n - p1s);
-        const T G0 = (T(1) - k.dt * tu) * Gn_.at(s)[ij] + k.ey * (p1w - p1e);
-        F0_.at(s)[ij] = F0;
-        G0_.at(s)[ij] = G0;
-        g += F0 * (tu - sg) * k.fx * (us - un) + G0 * (sg - tu) * k.fy * (ue - uw);
+    T gacc[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; r++) gacc[r] = T(0);
+    const T tu = jvalid ? a.ty[j] : T(0);
+    const bool nyj = jvalid ? a.neary[j] != 0 : false;
+    const int s1 = min(a.act, (g + 1) * a.Sg);
+    for (int s = g * a.Sg; s < s1; s++) {
+        const T *__restrict__ P1 = a.P1.at(s);
+        const T *__restrict__ um = a.um1.at(s);
+        T *__restrict__ Pout = a.Pn.at(s);
+        T p1n = ldg(P1, (int64_t)(ib - 1) * P + j, jvalid && ib >= 1);
+        T p1c = ldg(P1, (int64_t)ib * P + j, jvalid);
+        T un = ldg(um, (int64_t)(ib - 1) * P + j, jvalid && ib >= 1);
+        T uc = ldg(um, (int64_t)ib * P + j, jvalid);
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            const int i = ib + r;
+            if (i > NX + 1) break;
+            const int64_t ij = (int64_t)i * P + j;
+            const bool nxt = jvalid && i + 1 <= NX + 1;
+            const T p1s = ldg(P1, ij + P, nxt);
+            const T us = ldg(um, ij + P, nxt);
+            const T Cc = Cr[r + 1];
+            const T qc = Cc * p1c;
+            T qw = __shfl_up(qc, 1), qe = __shfl_down(qc, 1);
+            T p1w = __shfl_up(p1c, 1), p1e = __shfl_down(p1c, 1);
+            T uw = __shfl_up(uc, 1), ue = __shfl_down(uc, 1);
+            T Cw = __shfl_up(Cc, 1), Ce = __shfl_down(Cc, 1);
+            const bool in = jin && i >= 1 && i <= NX;
+            if (lane == 0 && in) {
+                p1w = P1[ij - 1];
+                Cw = a.C[ij - 1];
+                qw = Cw * p1w;
+                uw = um[ij - 1];
+            }
+            if (lane == WAVE - 1 && in) {
+                p1e = P1[ij + 1];
+                Ce = a.C[ij + 1];
+                qe = Ce * p1e;
+                ue = um[ij + 1];
+            }
+            if (in) {
+                const T lapq = k.cx * (Cr[r + 2] * p1s + Cr[r] * p1n - T(2) * qc) +
+                               k.cy * (qe + qw - T(2) * qc);
+                const T p2 = Pout[ij];
+                const bool near = a.nearx[i] | nyj;
+                T sg = T(0), U;
+                if (!near) {
+                    U = T(2) * p1c + lapq - p2;
+                } else {
+                    sg = a.sx[i];
+                    const T *__restrict__ Fn = a.Fn.at(s);
+                    const T *__restrict__ Gn = a.Gn.at(s);
+                    U = (T(2) - sg * tu * k.dt2) * p1c + lapq - (T(1) - (sg + tu) * k.hdt) * p2 +
+                        k.fx * (Cr[r] * (tu - a.sx[i - 1]) * Fn[ij - P] -
+                                Cr[r + 2] * (tu - a.sx[i + 1]) * Fn[ij + P]) +
+                        k.fy * (Cw * (sg - a.ty[j - 1]) * Gn[ij - 1] - Ce * (sg - a.ty[j + 1]) * Gn[ij + 1]);
+                }
+                if (a.dense.p) U += a.dense.at(s)[ij];
+                const T Pv = near ? U / (T(1) + (sg + tu) * k.hdt) : U;
+                Pout[ij] = Pv;
+                T gg = Pv * (k.cx * (us + un - T(2) * uc) + k.cy * (ue + uw - T(2) * uc));
+                if (V == 1 && near && (sg != T(0) || tu != T(0))) {
+                    const T F0 = (T(1) - k.dt * sg) * a.Fn.at(s)[ij] + k.ex * (p1n - p1s);
+                    const T G0 = (T(1) - k.dt * tu) * a.Gn.at(s)[ij] + k.ey * (p1w - p1e);
+                    a.F0.at(s)[ij] = F0;
+                    a.G0.at(s)[ij] = G0;
+                    gg += F0 * (tu - sg) * k.fx * (us - un) + G0 * (sg - tu) * k.fy * (ue - uw);
+                }
+                gacc[r] += gg;
+            }
+            p1n = p1c;
+            p1c = p1s;
+            un = uc;
+            uc = us;
+        }
     }
-    gC_.at(s)[ij] += g;
+    if (!jin) return;
+    T *__restrict__ gC = a.gC.at(g);
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+        const int i = ib + r;
+        if (i >= 1 && i <= NX) gC[(int64_t)i * P + j] += gacc[r];
+    }
 }
 
 // Receiver residual injection at level n after the stencil pass: for each distinct interior
 // receiver cell, R = sum of its receivers' residuals; P += R/d; dL/dC += (R/d) lap(u^{n-1}).
 template <typename T>
 __global__ __launch_bounds__(256) void ac_adj_inject(
-    Lv<T> P_, Lv<const T> um1_, Lv<T> gC_, int n, const int64_t *__restrict__ ur_ptr,
+    Lv<T> P_, Lv<const T> um1_, Lv<T> gC_, int Sg, int n, const int64_t *__restrict__ ur_ptr,
     const int64_t *__restrict__ ur_cell, const int64_t *__restrict__ ur_eptr,
     const int64_t *__restrict__ ur_ent, const int64_t *__restrict__ rcv_ptr,
     const T *__restrict__ resid, const int64_t *__restrict__ trace_off, const T *__restrict__ sx,
-    const T *__restrict__ ty, int NX, int NY, AcCoef<T> k)
+    const T *__restrict__ ty, int NX, int NY, int P, AcCoef<T> k)
 {
     const int s = blockIdx.y;
     const int64_t c = ur_ptr[s] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ur_ptr[s + 1]) return;
-    const int S = NY + 2;
     const int64_t cell = ur_cell[c];
-    const int i = (int)(cell / S), j = (int)(cell % S);
+    const int i = (int)(cell / P), j = (int)(cell % P);
     if (i < 1 || i > NX || j < 1 || j > NY) return;  // ring cells carry no gradient
     const int64_t nr = rcv_ptr[s + 1] - rcv_ptr[s];
     const T *__restrict__ row = resid + trace_off[s] + (int64_t)n * nr;
@@ -271,27 +401,31 @@ __global__ __launch_bounds__(256) void ac_adj_inject(
     P_.at(s)[cell] += add;
     const T *__restrict__ um = um1_.at(s);
     const T uc = um[cell];
-    gC_.at(s)[cell] += add * (k.cx * (um[cell + S] + um[cell - S] - T(2) * uc) +
-                              k.cy * (um[cell + 1] + um[cell - 1] - T(2) * uc));
+    gC_.at(s / Sg)[cell] += add * (k.cx * (um[cell + P] + um[cell - P] - T(2) * uc) +
+                                   k.cy * (um[cell + 1] + um[cell - 1] - T(2) * uc));
 }
 
-// misc elementwise kernels
+// model (caller layout, (NX+2) x (NY+2)) -> C (pitched), squared if asked
 template <typename T>
-__global__ void k_square(const T *__restrict__ m, T *__restrict__ C, int64_t n, int sq)
+__global__ void k_model_in(const T *__restrict__ m, T *__restrict__ C, int NX, int NY, int P, int sq)
 {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) C[q] = sq ? m[q] * m[q] : m[q];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= NY + 2) return;
+    const T v = m[(int64_t)i * (NY + 2) + j];
+    C[(int64_t)i * P + j] = sq ? v * v : v;
 }
 
+// sum of the shot-group accumulators (pitched) -> gradient in caller layout (x 2m if squared)
 template <typename T>
-__global__ void k_reduce_grad(const T *__restrict__ gC, int64_t stride, int nshots,
-                              const T *__restrict__ m, T *__restrict__ g, int64_t n, int sq)
+__global__ void k_grad_out(const T *__restrict__ gC, int64_t stride, int ngroups,
+                           const T *__restrict__ m, T *__restrict__ g, int NX, int NY, int P, int sq)
 {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= NY + 2) return;
+    const int64_t q = (int64_t)i * P + j, o = (int64_t)i * (NY + 2) + j;
     T acc = T(0);
-    for (int s = 0; s < nshots; s++) acc += gC[(int64_t)s * stride + q];
-    g[q] = sq ? T(2) * m[q] * acc : acc;
+    for (int s = 0; s < ngroups; s++) acc += gC[(int64_t)s * stride + q];
+    g[o] = sq ? T(2) * m[o] * acc : acc;
 }
 
 // resid = 2 (traces - observed); loss += sum (traces - observed)^2 (double accumulation).
@@ -320,26 +454,32 @@ __global__ __launch_bounds__(256) void k_misfit(const T *__restrict__ tr, const 
 // ---------------------------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------------------------
+struct Shape {
+    int rpt = 4, sg = 8;
+};
+
 struct AcPlanBase {
     virtual ~AcPlanBase() = default;
     ads_acoustic_params p{};
     int dtype = ADS_F32;
     int variant = 0;  // 0 or 1
-    int64_t NX = 0, NY = 0, S = 0, Npad = 0, NMAX = 0, nshots = 0;
-    std::vector<int64_t> order;        // plan position -> caller shot
-    std::vector<int64_t> nstep_sorted; // per plan position
+    int64_t NX = 0, NY = 0, Pitch = 0, Npad = 0, Ndense = 0, NMAX = 0, nshots = 0;
+    std::vector<int64_t> order;         // plan position -> caller shot
+    std::vector<int64_t> nstep_sorted;  // per plan position
     std::vector<int64_t> srcv_off_caller, trace_off_caller;  // per caller shot
     int64_t srcv_elems = 0, trace_elems = 0;
     std::vector<int> active;  // active[n] = #shots with nstep >= n
     std::vector<double> sx, ty;
     std::vector<uint8_t> pmlx, pmly, nearx, neary;
-    CellLists usrc, urcv;
-    // device tables
-    DevBuf d_sx, d_ty, d_pmlx, d_pmly, d_nearx, d_neary;
-    DevBuf d_us_ptr, d_us_cell, d_us_eptr, d_us_ent, d_nsrc, d_srcv_off;
-    DevBuf d_ur_ptr, d_ur_cell, d_ur_eptr, d_ur_ent, d_rcv_ptr, d_rcv_cell, d_trace_off;
-    int64_t max_ucells_rcv = 0;
     double f_pml = 0.0;  // fraction of interior cells inside the PML strips
+    Shape fwd_shape, adj_shape;
+    int tuned = 0;
+    // device tables; cell tables exist for the pitched (plan) and dense (history) layouts
+    DevBuf d_sx, d_ty, d_pmlx, d_pmly, d_nearx, d_neary;
+    DevBuf d_us_ptr, d_us_eptr, d_us_ent, d_nsrc, d_srcv_off;
+    DevBuf d_ur_ptr, d_ur_eptr, d_ur_ent, d_rcv_ptr, d_trace_off;
+    DevBuf d_us_cell[2], d_ur_cell[2], d_rcv_cell[2];  // [0] pitched, [1] dense
+    int64_t max_ucells_rcv = 0;
 };
 
 static int validate_params(const ads_acoustic_params *p)
@@ -348,7 +488,7 @@ static int validate_params(const ads_acoustic_params *p)
     if (p->NX < 1 || p->NY < 1 || p->NSTEP < 1)
         ADS_FAIL(ADS_ERR_SHAPE, "NX, NY, NSTEP must be >= 1 (got %lld, %lld, %lld)",
                  (long long)p->NX, (long long)p->NY, (long long)p->NSTEP);
-    if ((p->NX + 2) * (p->NY + 2) >= (int64_t)1 << 31)
+    if ((p->NX + 2) * (((p->NY + 2 + 63) / 64) * 64) >= (int64_t)1 << 31 || p->NX + 2 > (1 << 24))
         ADS_FAIL(ADS_ERR_SHAPE, "grid too large for 32-bit cell indexing within a shot");
     if (!(p->DELTAT > 0) || !(p->DELTAX > 0) || !(p->DELTAY > 0))
         ADS_FAIL(ADS_ERR_ARG, "DELTAT, DELTAX, DELTAY must be > 0");
@@ -360,79 +500,115 @@ static int validate_params(const ads_acoustic_params *p)
 template <typename T> struct AcPlan : AcPlanBase {
     AcCoef<T> k;
 
-    // ---- launch helpers ----
-    dim3 grid_cells(int act) const { return dim3(cdiv(S, BX), cdiv(NX + 2, BY), act); }
+    struct Layout {  // pitched (plan) or dense (history) addressing
+        int P;
+        int li;  // index into the cell tables
+    };
+    Layout plan_layout() const { return Layout{(int)Pitch, 0}; }
+    Layout dense_layout() const { return Layout{(int)(NY + 2), 1}; }
 
-    int fwd_step(int n, int act, Lv<T> uo, Lv<const T> w, Lv<const T> wo, Lv<const T> phi,
-                 Lv<const T> psi, Lv<T> phio, Lv<T> psio, const T *C, const T *srcv, T *traces,
-                 int record, hipStream_t st)
+    TileMap tiles(int P, int rpt, int act, int sg) const
+    {
+        TileMap m;
+        m.ntj = (int)cdiv(P, WAVE);
+        m.nstrips = (int)cdiv(NX + 2, NWAVES * rpt);
+        m.ngroups = (int)cdiv(act, sg);
+        m.nb = m.ntj * m.nstrips * m.ngroups;
+        return m;
+    }
+    static unsigned grid_of(const TileMap &m) { return cdiv(m.nb, NXCD) * NXCD; }
+
+    template <int V, int RPT> void launch_fwd(FwdArgs<T> &a, hipStream_t st)
+    {
+        a.map = tiles(a.P, RPT, a.act, a.Sg);
+        ac_fwd2<T, V, RPT><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a);
+    }
+    template <int V> void launch_fwd_rpt(FwdArgs<T> &a, int rpt, hipStream_t st)
+    {
+        if (rpt == 2) launch_fwd<V, 2>(a, st);
+        else if (rpt == 8) launch_fwd<V, 8>(a, st);
+        else launch_fwd<V, 4>(a, st);
+    }
+    template <int V, int RPT> void launch_adj(AdjArgs<T> &a, hipStream_t st)
+    {
+        a.map = tiles(a.P, RPT, a.act, a.Sg);
+        ac_adj2<T, V, RPT><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a);
+    }
+    template <int V> void launch_adj_rpt(AdjArgs<T> &a, int rpt, hipStream_t st)
+    {
+        if (rpt == 2) launch_adj<V, 2>(a, st);
+        else if (rpt == 8) launch_adj<V, 8>(a, st);
+        else launch_adj<V, 4>(a, st);
+    }
+
+    int fwd_step(const Layout &L, int n, int act, Lv<T> uo, Lv<const T> w, Lv<const T> wo,
+                 Lv<const T> phi, Lv<const T> psi, Lv<T> phio, Lv<T> psio, const T *C,
+                 const T *srcv, T *traces, int record, hipStream_t st, const Shape *sh = nullptr)
     {
         if (act <= 0) return ADS_OK;
-        dim3 b(BX, BY);
+        const Shape &shape = sh ? *sh : fwd_shape;
         const double cells = (double)act * (double)NX * (double)NY;
+        FwdArgs<T> a{uo, w, wo, phi, psi, phio, psio, C, d_sx.as<T>(), d_ty.as<T>(),
+                     d_nearx.as<uint8_t>(), d_neary.as<uint8_t>(), (int)NX, (int)NY, L.P, act,
+                     std::min(shape.sg, act), TileMap{}, k};
         int pr = prof_start(ADS_PROF_AC_FWD, st);
-        if (variant == 1)
-            ac_fwd_step<T, 1><<<grid_cells(act), b, 0, st>>>(uo, w, wo, phi, psi, phio, psio, C,
-                                                             d_sx.as<T>(), d_ty.as<T>(),
-                                                             d_nearx.as<uint8_t>(),
-                                                             d_neary.as<uint8_t>(), (int)NX, (int)NY, k);
-        else
-            ac_fwd_step<T, 0><<<grid_cells(act), b, 0, st>>>(uo, w, wo, phi, psi, phio, psio, C,
-                                                             d_sx.as<T>(), d_ty.as<T>(),
-                                                             d_nearx.as<uint8_t>(),
-                                                             d_neary.as<uint8_t>(), (int)NX, (int)NY, k);
+        if (variant == 1) launch_fwd_rpt<1>(a, shape.rpt, st);
+        else launch_fwd_rpt<0>(a, shape.rpt, st);
         prof_stop(pr, st, cells * sizeof(T) * (4.0 + (variant == 1 ? 4.0 : 2.0) * f_pml));
         if (variant == 0) {
-            ac_fwd_phipsi<T><<<grid_cells(act), b, 0, st>>>(
-                Lv<const T>{uo.p, uo.stride}, phi, psi, phio, psio, C, d_sx.as<T>(), d_ty.as<T>(),
-                d_pmlx.as<uint8_t>(), d_pmly.as<uint8_t>(), (int)NX, (int)NY, k);
+            dim3 g(cdiv(NY + 2, 64), cdiv(NX + 2, 4), act);
+            ac_fwd_phipsi<T><<<g, 256, 0, st>>>(cst(uo), phi, psi, phio, psio, C, d_sx.as<T>(),
+                                                d_ty.as<T>(), d_pmlx.as<uint8_t>(),
+                                                d_pmly.as<uint8_t>(), (int)NX, (int)NY, L.P, k);
         }
-        ac_src_rcv<T><<<act, 256, 0, st>>>(uo, n, d_us_ptr.as<int64_t>(), d_us_cell.as<int64_t>(),
+        ac_src_rcv<T><<<act, 256, 0, st>>>(uo, n, d_us_ptr.as<int64_t>(), d_us_cell[L.li].template as<int64_t>(),
                                            d_us_eptr.as<int64_t>(), d_us_ent.as<int64_t>(), srcv,
                                            d_srcv_off.as<int64_t>(), d_nsrc.as<int64_t>(), k.dt2,
-                                           d_rcv_ptr.as<int64_t>(), d_rcv_cell.as<int64_t>(),
+                                           d_rcv_ptr.as<int64_t>(), d_rcv_cell[L.li].template as<int64_t>(),
                                            traces, d_trace_off.as<int64_t>(), record);
         ADS_LAUNCH_CHECK();
         return ADS_OK;
     }
 
-    int adj_level(int n, int act, Lv<T> P, Lv<const T> P1, Lv<T> F0, Lv<T> G0, Lv<const T> F1,
-                  Lv<const T> G1, Lv<const T> um1, Lv<const T> un, Lv<const T> dense, Lv<T> gC,
-                  const T *C, const T *srcv, const T *resid, hipStream_t st)
+    int adj_level(const Layout &L, int n, int act, Lv<T> P, Lv<const T> P1, Lv<T> F0, Lv<T> G0,
+                  Lv<const T> F1, Lv<const T> G1, Lv<const T> um1, Lv<const T> un,
+                  Lv<const T> dense, Lv<T> gC, const T *C, const T *srcv, const T *resid,
+                  hipStream_t st, const Shape *sh = nullptr)
     {
         if (act <= 0) return ADS_OK;
-        dim3 b(BX, BY);
+        const Shape &shape = sh ? *sh : adj_shape;
+        const int Sg = adj_shape.sg;  // gC grouping is fixed per plan (see group_arrays())
         const double cells = (double)act * (double)NX * (double)NY;
-        if (variant == 1) {
-            int pr = prof_start(ADS_PROF_AC_ADJ, st);
-            ac_adj_step<T, 1><<<grid_cells(act), b, 0, st>>>(
-                P, P1, F1, G1, F0, G0, um1, dense, gC, C, d_sx.as<T>(), d_ty.as<T>(),
-                d_nearx.as<uint8_t>(), d_neary.as<uint8_t>(), (int)NX, (int)NY, k);
-            prof_stop(pr, st, cells * sizeof(T) * (7.0 + 4.0 * f_pml));
-        } else {
-            ac_adj_phipsi<T><<<grid_cells(act), b, 0, st>>>(
-                P1, F1, G1, F0, G0, un, gC, n, C, d_sx.as<T>(), d_ty.as<T>(), d_pmlx.as<uint8_t>(),
-                d_pmly.as<uint8_t>(), d_us_ptr.as<int64_t>(), d_us_cell.as<int64_t>(),
+        if (variant == 0) {
+            dim3 g(cdiv(NY + 2, 64), cdiv(NX + 2, 4), act);
+            ac_adj_phipsi<T><<<g, 256, 0, st>>>(
+                P1, F1, G1, F0, G0, un, gC, Sg, n, d_sx.as<T>(), d_ty.as<T>(), d_pmlx.as<uint8_t>(),
+                d_pmly.as<uint8_t>(), d_us_ptr.as<int64_t>(), d_us_cell[L.li].template as<int64_t>(),
                 d_us_eptr.as<int64_t>(), d_us_ent.as<int64_t>(), srcv, d_srcv_off.as<int64_t>(),
-                d_nsrc.as<int64_t>(), (int)NX, (int)NY, k);
-            int pr = prof_start(ADS_PROF_AC_ADJ, st);
-            ac_adj_step<T, 0><<<grid_cells(act), b, 0, st>>>(
-                P, P1, Lv<const T>{F0.p, F0.stride}, Lv<const T>{G0.p, G0.stride}, F0, G0, um1,
-                dense, gC, C, d_sx.as<T>(), d_ty.as<T>(), d_nearx.as<uint8_t>(),
-                d_neary.as<uint8_t>(), (int)NX, (int)NY, k);
-            prof_stop(pr, st, cells * sizeof(T) * (7.0 + 2.0 * f_pml));
+                d_nsrc.as<int64_t>(), (int)NX, (int)NY, L.P, k);
         }
+        AdjArgs<T> a{P, P1, variant == 1 ? F1 : cst(F0), variant == 1 ? G1 : cst(G0), F0, G0,
+                     um1, dense, gC, C, d_sx.as<T>(), d_ty.as<T>(), d_nearx.as<uint8_t>(),
+                     d_neary.as<uint8_t>(), (int)NX, (int)NY, L.P, act, std::min(Sg, act),
+                     TileMap{}, k};
+        (void)shape;
+        int pr = prof_start(ADS_PROF_AC_ADJ, st);
+        if (variant == 1) launch_adj_rpt<1>(a, shape.rpt, st);
+        else launch_adj_rpt<0>(a, shape.rpt, st);
+        prof_stop(pr, st, cells * sizeof(T) * (7.0 + (variant == 1 ? 4.0 : 2.0) * f_pml));
         if (resid && max_ucells_rcv > 0) {
             dim3 g(cdiv(max_ucells_rcv, 256), act);
-            ac_adj_inject<T><<<g, 256, 0, st>>>(P, um1, gC, n, d_ur_ptr.as<int64_t>(),
-                                                d_ur_cell.as<int64_t>(), d_ur_eptr.as<int64_t>(),
+            ac_adj_inject<T><<<g, 256, 0, st>>>(P, um1, gC, Sg, n, d_ur_ptr.as<int64_t>(),
+                                                d_ur_cell[L.li].template as<int64_t>(), d_ur_eptr.as<int64_t>(),
                                                 d_ur_ent.as<int64_t>(), d_rcv_ptr.as<int64_t>(),
                                                 resid, d_trace_off.as<int64_t>(), d_sx.as<T>(),
-                                                d_ty.as<T>(), (int)NX, (int)NY, k);
+                                                d_ty.as<T>(), (int)NX, (int)NY, L.P, k);
         }
         ADS_LAUNCH_CHECK();
         return ADS_OK;
     }
+
+    int64_t group_arrays() const { return (int64_t)cdiv(nshots, adj_shape.sg); }
 
     // ---- workspace layout for checkpoint interval K ----
     struct WS {
@@ -457,7 +633,7 @@ template <typename T> struct AcPlan : AcPlanBase {
             w.F[q] = (T *)cv.take(F);
             w.G[q] = (T *)cv.take(F);
         }
-        w.gC = (T *)cv.take(F);
+        w.gC = (T *)cv.take((size_t)nshots * Npad * sizeof(T));  // up to one array per shot
         w.ckpt = (T *)cv.take(F * 4 * (size_t)std::max<int64_t>(w.M - 1, 0));
         w.C = (T *)cv.take((size_t)Npad * sizeof(T));
         w.resid = (T *)cv.take((size_t)std::max<int64_t>(trace_elems, 1) * sizeof(T));
@@ -475,25 +651,26 @@ template <typename T> struct AcPlan : AcPlanBase {
     {
         return Lv<T>{w.seg + (size_t)(n % w.NS) * nshots * Npad, Npad};
     }
-    static Lv<const T> cst(Lv<T> a) { return Lv<const T>{a.p, a.stride}; }
 
-    int prepare_C(const WS &w, const void *model, hipStream_t st)
+    int prepare_C(T *C, const void *model, int P, hipStream_t st)
     {
-        k_square<T><<<cdiv(Npad, 256), 256, 0, st>>>((const T *)model, w.C, Npad, p.square_velocity);
+        k_model_in<T><<<dim3(cdiv(NY + 2, 256), NX + 2), 256, 0, st>>>((const T *)model, C, (int)NX,
+                                                                      (int)NY, P, p.square_velocity);
         ADS_LAUNCH_CHECK();
         return ADS_OK;
     }
 
     // forward over levels lo+1..hi from the state held in slots/phi (record traces if asked)
     int run_forward(const WS &w, int64_t lo, int64_t hi, const T *srcv, T *traces, int record,
-                    hipStream_t st)
+                    hipStream_t st, const Shape *sh = nullptr)
     {
+        const Layout L = plan_layout();
         for (int64_t n = lo + 1; n <= hi; n++) {
             int act = active[n];
             Lv<T> pi{w.phi[(n - 1) & 1], Npad}, si{w.psi[(n - 1) & 1], Npad};
             Lv<T> po{w.phi[n & 1], Npad}, so{w.psi[n & 1], Npad};
-            ADS_TRY(fwd_step((int)n, act, slot(w, n), cst(slot(w, n - 1)), cst(slot(w, n - 2)),
-                             cst(pi), cst(si), po, so, w.C, srcv, traces, record, st));
+            ADS_TRY(fwd_step(L, (int)n, act, slot(w, n), cst(slot(w, n - 1)), cst(slot(w, n - 2)),
+                             cst(pi), cst(si), po, so, w.C, srcv, traces, record, st, sh));
         }
         return ADS_OK;
     }
@@ -532,16 +709,22 @@ template <typename T> struct AcPlan : AcPlanBase {
         return ADS_OK;
     }
 
-    int forward(const void *model, const void *srcv, void *traces, void *ws, size_t wsb,
-                int64_t K, hipStream_t st)
+    int check_ws(WS &w, void *ws, size_t wsb, int64_t K)
     {
-        WS w;
         Carve cv(ws, wsb);
         layout(w, cv, K);
         if (!ws || !cv.ok())
             ADS_FAIL(ADS_ERR_WORKSPACE, "workspace too small: need %zu bytes, got %zu", cv.off, wsb);
+        return ADS_OK;
+    }
+
+    int forward(const void *model, const void *srcv, void *traces, void *ws, size_t wsb,
+                int64_t K, hipStream_t st)
+    {
+        WS w;
+        ADS_TRY(check_ws(w, ws, wsb, K));
         const size_t F = (size_t)nshots * Npad * sizeof(T);
-        ADS_TRY(prepare_C(w, model, st));
+        ADS_TRY(prepare_C(w.C, model, (int)Pitch, st));
         ADS_HIP(hipMemsetAsync(slot(w, 0).p, 0, F, st));
         ADS_HIP(hipMemsetAsync(slot(w, 1).p, 0, F, st));
         for (int q = 0; q < 2; q++) {
@@ -557,21 +740,35 @@ template <typename T> struct AcPlan : AcPlanBase {
         return ADS_OK;
     }
 
-    int backward(const void *model, const void *srcv, const T *resid, void *grad, void *ws,
-                 size_t wsb, int64_t K, hipStream_t st)
+    int zero_adjoint_state(const WS &w, hipStream_t st)
     {
-        WS w;
-        Carve cv(ws, wsb);
-        layout(w, cv, K);
-        if (!ws || !cv.ok())
-            ADS_FAIL(ADS_ERR_WORKSPACE, "workspace too small: need %zu bytes, got %zu", cv.off, wsb);
         const size_t F = (size_t)nshots * Npad * sizeof(T);
         for (int q = 0; q < 2; q++) {
             ADS_HIP(hipMemsetAsync(w.P[q], 0, F, st));
             ADS_HIP(hipMemsetAsync(w.F[q], 0, F, st));
             ADS_HIP(hipMemsetAsync(w.G[q], 0, F, st));
         }
-        ADS_HIP(hipMemsetAsync(w.gC, 0, F, st));
+        ADS_HIP(hipMemsetAsync(w.gC, 0, (size_t)group_arrays() * Npad * sizeof(T), st));
+        return ADS_OK;
+    }
+
+    int adj_at(const WS &w, int64_t n, const T *srcv, const T *resid, hipStream_t st,
+               const Shape *sh = nullptr)
+    {
+        Lv<T> P{w.P[n & 1], Npad}, P1{w.P[(n + 1) & 1], Npad};
+        Lv<T> F0{w.F[n & 1], Npad}, G0{w.G[n & 1], Npad};
+        Lv<T> F1{w.F[(n + 1) & 1], Npad}, G1{w.G[(n + 1) & 1], Npad};
+        return adj_level(plan_layout(), (int)n, active[n], P, cst(P1), F0, G0, cst(F1), cst(G1),
+                         cst(slot(w, n - 1)), cst(slot(w, n)), Lv<const T>{nullptr, 0},
+                         Lv<T>{w.gC, Npad}, w.C, srcv, resid, st, sh);
+    }
+
+    int backward(const void *model, const void *srcv, const T *resid, void *grad, void *ws,
+                 size_t wsb, int64_t K, hipStream_t st)
+    {
+        WS w;
+        ADS_TRY(check_ws(w, ws, wsb, K));
+        ADS_TRY(zero_adjoint_state(w, st));
         const T *sv = (const T *)srcv;
         for (int64_t m = w.M - 1; m >= 0; m--) {
             int64_t a = 1 + m * w.K, b = std::min<int64_t>(a + w.K, NMAX);
@@ -579,20 +776,11 @@ template <typename T> struct AcPlan : AcPlanBase {
                 ADS_TRY(restore_ckpt(w, m, a, st));
                 ADS_TRY(run_forward(w, a, b, sv, nullptr, 0, st));
             }
-            for (int64_t n = b; n > a; n--) {
-                int act = active[n];
-                Lv<T> P{w.P[n & 1], Npad};
-                Lv<T> P1{w.P[(n + 1) & 1], Npad};
-                Lv<T> F0{w.F[n & 1], Npad}, G0{w.G[n & 1], Npad};
-                Lv<T> F1{w.F[(n + 1) & 1], Npad}, G1{w.G[(n + 1) & 1], Npad};
-                ADS_TRY(adj_level((int)n, act, P, cst(P1), F0, G0, cst(F1), cst(G1),
-                                  cst(slot(w, n - 1)), cst(slot(w, n)), Lv<const T>{nullptr, 0},
-                                  Lv<T>{w.gC, Npad}, w.C, sv, resid, st));
-            }
+            for (int64_t n = b; n > a; n--) ADS_TRY(adj_at(w, n, sv, resid, st));
         }
-        k_reduce_grad<T><<<cdiv(Npad, 256), 256, 0, st>>>(w.gC, Npad, (int)nshots,
-                                                          (const T *)model, (T *)grad, Npad,
-                                                          p.square_velocity);
+        k_grad_out<T><<<dim3(cdiv(NY + 2, 256), NX + 2), 256, 0, st>>>(
+            w.gC, Npad, (int)group_arrays(), (const T *)model, (T *)grad, (int)NX, (int)NY,
+            (int)Pitch, p.square_velocity);
         ADS_LAUNCH_CHECK();
         return ADS_OK;
     }
@@ -607,33 +795,98 @@ template <typename T> struct AcPlan : AcPlanBase {
         return ADS_OK;
     }
 
-    // ---- full-history drop-in (every shot has NSTEP levels) ----
+    // ---- autotuner: time candidate (RPT, Sg) shapes on the plan's own workload ----
+    int autotune(const void *model, const void *srcv, void *ws, size_t wsb, int64_t K, int reps,
+                 hipStream_t st)
+    {
+        WS w;
+        ADS_TRY(check_ws(w, ws, wsb, K));
+        if (NMAX < 3) {  // nothing worth timing
+            tuned = 1;
+            return ADS_OK;
+        }
+        const T *sv = (const T *)srcv;
+        ADS_TRY(prepare_C(w.C, model, (int)Pitch, st));
+        const size_t F = (size_t)nshots * Npad * sizeof(T);
+        for (int64_t q = 0; q < std::min<int64_t>(w.NS, 4); q++) ADS_HIP(hipMemsetAsync(slot(w, q).p, 0, F, st));
+        for (int q = 0; q < 2; q++) {
+            ADS_HIP(hipMemsetAsync(w.phi[q], 0, F, st));
+            ADS_HIP(hipMemsetAsync(w.psi[q], 0, F, st));
+        }
+        const int rpts[3] = {2, 4, 8};
+        std::vector<int> sgs;
+        for (int sg = 1; sg <= std::max<int64_t>(nshots, 1) && sg <= 32; sg *= 2) sgs.push_back(sg);
+        if (reps < 1) reps = 3;
+        hipEvent_t e0, e1;
+        ADS_HIP(hipEventCreate(&e0));
+        ADS_HIP(hipEventCreate(&e1));
+        auto timeit = [&](auto &&body) -> float {
+            body();  // warm
+            (void)hipEventRecord(e0, st);
+            for (int r = 0; r < reps; r++) body();
+            (void)hipEventRecord(e1, st);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            return ms;
+        };
+        int64_t n0 = std::min<int64_t>(2, NMAX);
+        float best = 1e30f;
+        Shape bf = fwd_shape;
+        for (int rp : rpts)
+            for (int sg : sgs) {
+                Shape sh{rp, sg};
+                float ms = timeit([&] { (void)run_forward(w, n0 - 1, n0 + 1 <= NMAX ? n0 + 1 : n0, sv, nullptr, 0, st, &sh); });
+                if (ms < best) best = ms, bf = sh;
+            }
+        fwd_shape = bf;
+        ADS_TRY(zero_adjoint_state(w, st));
+        best = 1e30f;
+        Shape ba = adj_shape;
+        const Shape keep = adj_shape;
+        for (int rp : rpts)
+            for (int sg : sgs) {
+                Shape sh{rp, sg};
+                adj_shape = sh;  // the gC grouping follows the shape being timed
+                float ms = timeit([&] { (void)adj_at(w, std::max<int64_t>(n0, 2), sv, nullptr, st, &sh); });
+                if (ms < best) best = ms, ba = sh;
+            }
+        adj_shape = nshots > 0 ? ba : keep;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        tuned = 1;
+        ADS_LAUNCH_CHECK();
+        return ADS_OK;
+    }
+
+    // ---- full-history drop-in (every shot has NSTEP levels, dense caller layout) ----
     int forward_history(const void *model, const void *srcv, void *u_hist, void *phi_hist,
                         void *psi_hist, void *scratch, hipStream_t st)
     {
-        const int64_t L = NMAX + 1;
-        const size_t H = (size_t)nshots * L * Npad * sizeof(T);
+        const Layout L = dense_layout();
+        const int64_t Lv_ = NMAX + 1;
+        const size_t H = (size_t)nshots * Lv_ * Ndense * sizeof(T);
         T *C = (T *)scratch;
-        T *pp[2] = {(T *)scratch + Npad, (T *)scratch + Npad + (size_t)nshots * Npad};
-        T *qq[2] = {(T *)scratch + Npad + 2 * (size_t)nshots * Npad,
-                    (T *)scratch + Npad + 3 * (size_t)nshots * Npad};
-        k_square<T><<<cdiv(Npad, 256), 256, 0, st>>>((const T *)model, C, Npad, p.square_velocity);
+        const size_t F = (size_t)nshots * Ndense;
+        T *pp[2] = {(T *)scratch + Ndense, (T *)scratch + Ndense + F};
+        T *qq[2] = {(T *)scratch + Ndense + 2 * F, (T *)scratch + Ndense + 3 * F};
+        ADS_TRY(prepare_C(C, model, L.P, st));
         ADS_HIP(hipMemsetAsync(u_hist, 0, H, st));
         if (phi_hist) ADS_HIP(hipMemsetAsync(phi_hist, 0, H, st));
         if (psi_hist) ADS_HIP(hipMemsetAsync(psi_hist, 0, H, st));
         if (!phi_hist || !psi_hist)
             for (int q = 0; q < 2; q++) {
-                ADS_HIP(hipMemsetAsync(pp[q], 0, (size_t)nshots * Npad * sizeof(T), st));
-                ADS_HIP(hipMemsetAsync(qq[q], 0, (size_t)nshots * Npad * sizeof(T), st));
+                ADS_HIP(hipMemsetAsync(pp[q], 0, F * sizeof(T), st));
+                ADS_HIP(hipMemsetAsync(qq[q], 0, F * sizeof(T), st));
             }
-        const int64_t hs = L * Npad;
-        auto lvl = [&](void *base, int64_t n) { return Lv<T>{(T *)base + n * Npad, hs}; };
+        const int64_t hs = Lv_ * Ndense;
+        auto lvl = [&](void *base, int64_t n) { return Lv<T>{(T *)base + n * Ndense, hs}; };
         for (int64_t n = 2; n <= NMAX; n++) {
-            Lv<T> pi = phi_hist ? lvl(phi_hist, n - 1) : Lv<T>{pp[(n - 1) & 1], Npad};
-            Lv<T> po = phi_hist ? lvl(phi_hist, n) : Lv<T>{pp[n & 1], Npad};
-            Lv<T> si = psi_hist ? lvl(psi_hist, n - 1) : Lv<T>{qq[(n - 1) & 1], Npad};
-            Lv<T> so = psi_hist ? lvl(psi_hist, n) : Lv<T>{qq[n & 1], Npad};
-            ADS_TRY(fwd_step((int)n, (int)nshots, lvl(u_hist, n), cst(lvl(u_hist, n - 1)),
+            Lv<T> pi = phi_hist ? lvl(phi_hist, n - 1) : Lv<T>{pp[(n - 1) & 1], Ndense};
+            Lv<T> po = phi_hist ? lvl(phi_hist, n) : Lv<T>{pp[n & 1], Ndense};
+            Lv<T> si = psi_hist ? lvl(psi_hist, n - 1) : Lv<T>{qq[(n - 1) & 1], Ndense};
+            Lv<T> so = psi_hist ? lvl(psi_hist, n) : Lv<T>{qq[n & 1], Ndense};
+            ADS_TRY(fwd_step(L, (int)n, (int)nshots, lvl(u_hist, n), cst(lvl(u_hist, n - 1)),
                              cst(lvl(u_hist, n - 2)), cst(pi), cst(si), po, so, C,
                              (const T *)srcv, nullptr, 0, st));
         }
@@ -643,32 +896,34 @@ template <typename T> struct AcPlan : AcPlanBase {
     int backward_history(const void *model, const void *srcv, const void *u_hist,
                          const void *g_hist, void *grad, void *scratch, hipStream_t st)
     {
-        const int64_t L = NMAX + 1;
-        const size_t F = (size_t)nshots * Npad;
+        const Layout L = dense_layout();
+        const int64_t Lv_ = NMAX + 1;
+        const size_t F = (size_t)nshots * Ndense;
         T *C = (T *)scratch;
-        T *b = (T *)scratch + Npad;
+        T *b = (T *)scratch + Ndense;
         T *P[2] = {b, b + F}, *Fa[2] = {b + 2 * F, b + 3 * F}, *Ga[2] = {b + 4 * F, b + 5 * F};
         T *gC = b + 6 * F;
-        k_square<T><<<cdiv(Npad, 256), 256, 0, st>>>((const T *)model, C, Npad, p.square_velocity);
+        ADS_TRY(prepare_C(C, model, L.P, st));
         ADS_HIP(hipMemsetAsync(b, 0, 7 * F * sizeof(T), st));
-        const int64_t hs = L * Npad;
+        const int64_t hs = Lv_ * Ndense;
         auto lvl = [&](const void *base, int64_t n) {
-            return Lv<const T>{(const T *)base + n * Npad, hs};
+            return Lv<const T>{(const T *)base + n * Ndense, hs};
         };
         for (int64_t n = NMAX; n >= 2; n--) {
-            ADS_TRY(adj_level((int)n, (int)nshots, Lv<T>{P[n & 1], Npad},
-                              Lv<const T>{P[(n + 1) & 1], Npad}, Lv<T>{Fa[n & 1], Npad},
-                              Lv<T>{Ga[n & 1], Npad}, Lv<const T>{Fa[(n + 1) & 1], Npad},
-                              Lv<const T>{Ga[(n + 1) & 1], Npad}, lvl(u_hist, n - 1),
-                              lvl(u_hist, n), lvl(g_hist, n), Lv<T>{gC, Npad}, C,
+            ADS_TRY(adj_level(L, (int)n, (int)nshots, Lv<T>{P[n & 1], Ndense},
+                              Lv<const T>{P[(n + 1) & 1], Ndense}, Lv<T>{Fa[n & 1], Ndense},
+                              Lv<T>{Ga[n & 1], Ndense}, Lv<const T>{Fa[(n + 1) & 1], Ndense},
+                              Lv<const T>{Ga[(n + 1) & 1], Ndense}, lvl(u_hist, n - 1),
+                              lvl(u_hist, n), lvl(g_hist, n), Lv<T>{gC, Ndense}, C,
                               (const T *)srcv, nullptr, st));
         }
-        k_reduce_grad<T><<<cdiv(Npad, 256), 256, 0, st>>>(gC, Npad, (int)nshots, (const T *)model,
-                                                          (T *)grad, Npad, p.square_velocity);
+        k_grad_out<T><<<dim3(cdiv(NY + 2, 256), NX + 2), 256, 0, st>>>(
+            gC, Ndense, (int)group_arrays(), (const T *)model, (T *)grad, (int)NX, (int)NY, L.P,
+            p.square_velocity);
         ADS_LAUNCH_CHECK();
         return ADS_OK;
     }
-    size_t history_scratch() const { return ((size_t)Npad + 7 * (size_t)nshots * Npad) * sizeof(T); }
+    size_t history_scratch() const { return ((size_t)Ndense + 7 * (size_t)nshots * Ndense) * sizeof(T); }
 };
 
 template <typename T>
@@ -681,10 +936,13 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
     P->variant = p->PropagatorKernel == 0 ? 0 : 1;
     P->NX = p->NX;
     P->NY = p->NY;
-    P->S = p->NY + 2;
-    P->Npad = (p->NX + 2) * (p->NY + 2);
+    P->Pitch = ((p->NY + 2 + WAVE - 1) / WAVE) * WAVE;
+    P->Npad = (p->NX + 2) * P->Pitch;
+    P->Ndense = (p->NX + 2) * (p->NY + 2);
     P->nshots = g->nshots;
     P->k = make_coef<T>(p->DELTAT, p->DELTAX, p->DELTAY);
+    P->fwd_shape = Shape{4, (int)std::min<int64_t>(8, g->nshots)};
+    P->adj_shape = Shape{4, (int)std::min<int64_t>(8, g->nshots)};
     const int64_t ns = g->nshots;
     if (g->src_ptr[0] != 0 || g->rcv_ptr[0] != 0) ADS_FAIL(ADS_ERR_ARG, "src_ptr/rcv_ptr must start at 0");
     std::vector<int64_t> nstep(ns);
@@ -722,18 +980,16 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
     }
     P->srcv_elems = so;
     P->trace_elems = to;
-    // flat indices, validated, in plan order
-    auto flat = [&](const int64_t *I, const int64_t *J, int64_t e, const char *what,
-                    int64_t *out) -> int {
+    // validated 1-based (i, j) in plan order
+    auto check = [&](const int64_t *I, const int64_t *J, int64_t e, const char *what) -> int {
         int64_t i = I[e], j = J[e];
         if (i < 1 || i > p->NX + 2 || j < 1 || j > p->NY + 2)
             ADS_FAIL(ADS_ERR_INDEX, "%s %lld at (%lld, %lld) is outside the padded grid [1, %lld] x [1, %lld]",
                      what, (long long)e, (long long)i, (long long)j, (long long)(p->NX + 2),
                      (long long)(p->NY + 2));
-        *out = (i - 1) * (p->NY + 2) + (j - 1);
         return ADS_OK;
     };
-    std::vector<int64_t> sp(ns + 1, 0), rp(ns + 1, 0), sflat, rflat, nsrc(ns), soff(ns), toff(ns);
+    std::vector<int64_t> sp(ns + 1, 0), rp(ns + 1, 0), si, sj, ri, rj, nsrc(ns), soff(ns), toff(ns);
     for (int64_t q = 0; q < ns; q++) {
         int64_t s = P->order[q];
         if (g->src_ptr[s + 1] > g->src_ptr[s] && (!g->srci || !g->srcj))
@@ -741,26 +997,47 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
         if (g->rcv_ptr[s + 1] > g->rcv_ptr[s] && (!g->rcvi || !g->rcvj))
             ADS_FAIL(ADS_ERR_ARG, "rcvi/rcvj are null");
         for (int64_t e = g->src_ptr[s]; e < g->src_ptr[s + 1]; e++) {
-            int64_t f;
-            ADS_TRY(flat(g->srci, g->srcj, e, "source", &f));
-            sflat.push_back(f);
+            ADS_TRY(check(g->srci, g->srcj, e, "source"));
+            si.push_back(g->srci[e] - 1);
+            sj.push_back(g->srcj[e] - 1);
         }
         for (int64_t e = g->rcv_ptr[s]; e < g->rcv_ptr[s + 1]; e++) {
-            int64_t f;
-            ADS_TRY(flat(g->rcvi, g->rcvj, e, "receiver", &f));
-            rflat.push_back(f);
+            ADS_TRY(check(g->rcvi, g->rcvj, e, "receiver"));
+            ri.push_back(g->rcvi[e] - 1);
+            rj.push_back(g->rcvj[e] - 1);
         }
-        sp[q + 1] = (int64_t)sflat.size();
-        rp[q + 1] = (int64_t)rflat.size();
+        sp[q + 1] = (int64_t)si.size();
+        rp[q + 1] = (int64_t)ri.size();
         nsrc[q] = g->src_ptr[s + 1] - g->src_ptr[s];
         soff[q] = P->srcv_off_caller[s];
         toff[q] = P->trace_off_caller[s];
     }
-    P->usrc = build_cell_lists(ns, sp.data(), sflat);
-    P->urcv = build_cell_lists(ns, rp.data(), rflat);
-    P->max_ucells_rcv = 0;
-    for (int64_t q = 0; q < ns; q++)
-        P->max_ucells_rcv = std::max(P->max_ucells_rcv, P->urcv.shot_ptr[q + 1] - P->urcv.shot_ptr[q]);
+    auto flat = [](const std::vector<int64_t> &I, const std::vector<int64_t> &J, int64_t pitch) {
+        std::vector<int64_t> f(I.size());
+        for (size_t e = 0; e < I.size(); e++) f[e] = I[e] * pitch + J[e];
+        return f;
+    };
+    const int64_t pitches[2] = {P->Pitch, p->NY + 2};
+    for (int L = 0; L < 2; L++) {
+        std::vector<int64_t> sf = flat(si, sj, pitches[L]), rf = flat(ri, rj, pitches[L]);
+        CellLists us = build_cell_lists(ns, sp.data(), sf), ur = build_cell_lists(ns, rp.data(), rf);
+        // cell lists are identical in structure for both pitches (the order of cells is the
+        // same because (i, j) -> i*pitch + j is monotone for j < pitch)
+        if (L == 0) {
+            ADS_TRY(P->d_us_ptr.upload(us.shot_ptr));
+            ADS_TRY(P->d_us_eptr.upload(us.ent_ptr));
+            ADS_TRY(P->d_us_ent.upload(us.ent));
+            ADS_TRY(P->d_ur_ptr.upload(ur.shot_ptr));
+            ADS_TRY(P->d_ur_eptr.upload(ur.ent_ptr));
+            ADS_TRY(P->d_ur_ent.upload(ur.ent));
+            P->max_ucells_rcv = 0;
+            for (int64_t q = 0; q < ns; q++)
+                P->max_ucells_rcv = std::max(P->max_ucells_rcv, ur.shot_ptr[q + 1] - ur.shot_ptr[q]);
+        }
+        ADS_TRY(P->d_us_cell[L].upload(us.cell));
+        ADS_TRY(P->d_ur_cell[L].upload(ur.cell));
+        ADS_TRY(P->d_rcv_cell[L].upload(rf));
+    }
     // PML profiles and flags
     P->sx.resize(p->NX + 2);
     P->ty.resize(p->NY + 2);
@@ -788,18 +1065,9 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
     ADS_TRY(P->d_pmly.upload(P->pmly));
     ADS_TRY(P->d_nearx.upload(P->nearx));
     ADS_TRY(P->d_neary.upload(P->neary));
-    ADS_TRY(P->d_us_ptr.upload(P->usrc.shot_ptr));
-    ADS_TRY(P->d_us_cell.upload(P->usrc.cell));
-    ADS_TRY(P->d_us_eptr.upload(P->usrc.ent_ptr));
-    ADS_TRY(P->d_us_ent.upload(P->usrc.ent));
     ADS_TRY(P->d_nsrc.upload(nsrc));
     ADS_TRY(P->d_srcv_off.upload(soff));
-    ADS_TRY(P->d_ur_ptr.upload(P->urcv.shot_ptr));
-    ADS_TRY(P->d_ur_cell.upload(P->urcv.cell));
-    ADS_TRY(P->d_ur_eptr.upload(P->urcv.ent_ptr));
-    ADS_TRY(P->d_ur_ent.upload(P->urcv.ent));
     ADS_TRY(P->d_rcv_ptr.upload(rp));
-    ADS_TRY(P->d_rcv_cell.upload(rflat));
     ADS_TRY(P->d_trace_off.upload(toff));
     return ADS_OK;
 }
@@ -901,10 +1169,8 @@ int ads_acoustic_plan_backward(ads_acoustic_plan *plan, const void *model, const
     clear_error();
     if (!model || !srcv || !grad_traces || !grad_model) ADS_FAIL(ADS_ERR_ARG, "null argument");
     return dispatch(plan, [&](auto &P) {
-        using T = typename std::remove_reference<decltype(P)>::type;
-        (void)sizeof(T);
-        return P.backward(model, srcv, (decltype(P.k.dt) const *)grad_traces, grad_model, ws, wsb,
-                          K, S(stream));
+        using R = typename std::remove_const<decltype(P.k.dt)>::type;
+        return P.backward(model, srcv, (const R *)grad_traces, grad_model, ws, wsb, K, S(stream));
     });
 }
 
@@ -917,16 +1183,46 @@ int ads_acoustic_plan_gradient(ads_acoustic_plan *plan, const void *model, const
     return dispatch(plan, [&](auto &P) -> int {
         using R = typename std::remove_const<decltype(P.k.dt)>::type;
         typename std::remove_reference<decltype(P)>::type::WS w;
-        Carve cv(ws, wsb);
-        P.layout(w, cv, K);
-        if (!ws || !cv.ok())
-            ADS_FAIL(ADS_ERR_WORKSPACE, "workspace too small: need %zu bytes, got %zu", cv.off, wsb);
+        ADS_TRY(P.check_ws(w, ws, wsb, K));
         // traces go to the caller's buffer if given, else into the residual area (reused)
         R *tr = traces ? (R *)traces : w.resid;
         ADS_TRY(P.forward(model, srcv, tr, ws, wsb, K, S(stream)));
         ADS_TRY(P.misfit(w, tr, (const R *)observed, S(stream)));
         if (loss) ADS_HIP(hipMemcpyAsync(loss, w.loss, sizeof(double), hipMemcpyDeviceToDevice, S(stream)));
         return P.backward(model, srcv, w.resid, grad_model, ws, wsb, K, S(stream));
+    });
+}
+
+int ads_acoustic_plan_autotune(ads_acoustic_plan *plan, const void *model, const void *srcv,
+                               void *ws, size_t wsb, int64_t K, int reps, int *shape_out,
+                               ads_stream_t stream)
+{
+    clear_error();
+    if (!model || !srcv) ADS_FAIL(ADS_ERR_ARG, "null argument");
+    return dispatch(plan, [&](auto &P) -> int {
+        ADS_TRY(P.autotune(model, srcv, ws, wsb, K, reps, S(stream)));
+        if (shape_out) {
+            shape_out[0] = P.fwd_shape.rpt;
+            shape_out[1] = P.fwd_shape.sg;
+            shape_out[2] = P.adj_shape.rpt;
+            shape_out[3] = P.adj_shape.sg;
+        }
+        return ADS_OK;
+    });
+}
+
+int ads_acoustic_plan_set_shape(ads_acoustic_plan *plan, const int *shape)
+{
+    clear_error();
+    if (!shape) ADS_FAIL(ADS_ERR_ARG, "null shape");
+    for (int q = 0; q < 4; q++)
+        if (shape[q] < 1) ADS_FAIL(ADS_ERR_ARG, "shape entries must be >= 1");
+    if ((shape[0] != 2 && shape[0] != 4 && shape[0] != 8) || (shape[2] != 2 && shape[2] != 4 && shape[2] != 8))
+        ADS_FAIL(ADS_ERR_ARG, "rows per thread must be 2, 4 or 8");
+    return dispatch(plan, [&](auto &P) -> int {
+        P.fwd_shape = Shape{shape[0], shape[1]};
+        P.adj_shape = Shape{shape[2], shape[3]};
+        return ADS_OK;
     });
 }
 

@@ -248,6 +248,16 @@ int ads_acoustic_plan_gradient(ads_acoustic_plan *plan, const void *model, const
                                void *traces, void *workspace, size_t ws_bytes, int64_t K,
                                ads_stream_t stream);
 
+/* Kernel-shape autotuner: times every candidate shape (rows per wavefront 2/4/8 x shots per
+ * workgroup 1..32) of the forward and adjoint stencil kernels on this plan's own grid and
+ * shots for `reps` steps each, keeps the fastest in the plan, and writes it to shape_out[4] =
+ * {fwd rows, fwd shots, adj rows, adj shots} (may be NULL).  Synchronises the stream; leaves
+ * the workspace contents undefined.  ads_acoustic_plan_set_shape() sets a shape directly. */
+int ads_acoustic_plan_autotune(ads_acoustic_plan *plan, const void *model, const void *srcv,
+                               void *workspace, size_t ws_bytes, int64_t K, int reps,
+                               int *shape_out, ads_stream_t stream);
+int ads_acoustic_plan_set_shape(ads_acoustic_plan *plan, const int *shape);
+
 /* Plain drop-in of AcousticPropagatorSolver (Core.jl:562-620) on the plan's shots (every shot
  * must have nstep == NSTEP): writes the full histories u, phi, psi, each laid out
  * [shot][level 0..NSTEP][N] (phi/psi may be NULL).  scratch: device buffer of

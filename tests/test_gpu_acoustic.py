@@ -242,3 +242,27 @@ def test_workspace_too_small_is_an_error(orc, ads, cuda):
                                             C.c_void_p(srcv.data_ptr()), C.c_void_p(tr.data_ptr()),
                                             C.c_void_p(ws.data_ptr()), 1024, 5, None)
     assert rc == 6
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_kernel_shapes_and_autotune_agree(orc, ads, cuda, kernel):
+    """Every (rows per wavefront, shots per workgroup) shape, and the autotuned one, gives the
+    oracle's gradient (shot groups change only the order of the dL/dC summation)."""
+    p, c, srcs, rcvs, nstep = make_case(orc, ads, NX=37, NY=70, NSTEP=50, nshots=5, kernel=kernel,
+                                        seed=11, uneven=True)
+    obs, g_ref = [], np.zeros_like(c)
+    for s in range(5):
+        o = oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s], cmod=c * 0.97)
+        g_ref += oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s], obs=o)[2]
+        obs.append(o)
+    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float64, nstep=nstep)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    ct, ob = T(c), plan.pack_traces(obs)
+    for shape in [(2, 1, 2, 1), (4, 2, 8, 3), (8, 5, 4, 2), (2, 32, 2, 32)]:
+        plan.set_shape(shape)
+        _, g, _ = plan.gradient(ct, srcv, ob, 6)
+        assert rel(g.cpu(), g_ref) < 1e-10, (shape, rel(g.cpu(), g_ref))
+    shape = plan.autotune(ct, srcv, 6, reps=1)
+    assert len(shape) == 4 and shape[0] in (2, 4, 8) and shape[2] in (2, 4, 8)
+    _, g, _ = plan.gradient(ct, srcv, ob, 6)
+    assert rel(g.cpu(), g_ref) < 1e-10
