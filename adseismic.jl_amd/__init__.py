@@ -12,6 +12,9 @@ from .params import (AcousticPropagatorParams, AcousticReceiver, AcousticSource,
 from .acoustic import (AcousticPlan, AcousticPropagator, AcousticPropagatorSolver,  # noqa: F401
                        SimulatedObservation_, acoustic_one_step, acoustic_one_step_customop,
                        acoustic_traces, check_cfl, compute_PML_Params_, getid2)
+from .elastic import (ElasticPlan, ElasticPropagator, ElasticPropagatorSolver,  # noqa: F401
+                      SimulatedObservation_elastic, elastic_traces)
+from .elastic import compute_PML_Params as compute_PML_Params_elastic  # noqa: F401
 from .utils import (Gauss, Ricker, compute_lame_parameters, compute_properties,  # noqa: F401
                     ricker_freq)
 

@@ -100,7 +100,9 @@ _SIGS = {
     "ads_elastic_plan_forward": (C.c_int, [vp] * 6 + [vp, C.c_size_t, i64, vp]),
     "ads_elastic_plan_backward": (C.c_int, [vp] * 9 + [vp, C.c_size_t, i64, vp]),
     "ads_elastic_plan_gradient": (C.c_int, [vp] * 11 + [C.c_size_t, i64, vp]),
-    "ads_elastic_forward_history": (C.c_int, [vp] * 7),
+    "ads_elastic_plan_backward_dense": (C.c_int, [vp] * 9 + [vp, C.c_size_t, i64, vp]),
+    "ads_elastic_plan_history_scratch_size": (C.c_size_t, [vp]),
+    "ads_elastic_forward_history": (C.c_int, [vp] * 8),
     "ads_profile_begin": (C.c_int, []),
     "ads_profile_end": (C.c_int, []),
     "ads_profile_query": (C.c_int, [C.c_int, C.POINTER(f64), i64p, C.POINTER(f64)]),

@@ -219,10 +219,11 @@ __global__ __launch_bounds__(256) void ac_src_rcv(
 // adjoint
 // ---------------------------------------------------------------------------------------------
 // kernel 0 only: F^n, G^n on PML cells and their dL/dC term with v = u^n before sources.
+// One thread per cell and shot group; the group's shots are summed in order into one write.
 template <typename T>
 __global__ __launch_bounds__(256) void ac_adj_phipsi(
     Lv<const T> P1_, Lv<const T> F1_, Lv<const T> G1_, Lv<T> F0_, Lv<T> G0_, Lv<const T> un_,
-    Lv<T> gC_, int Sg, int n, const T *__restrict__ sx, const T *__restrict__ ty,
+    Lv<T> gC_, int Sg, int act, int n, const T *__restrict__ sx, const T *__restrict__ ty,
     const uint8_t *__restrict__ pmlx, const uint8_t *__restrict__ pmly,
     const int64_t *__restrict__ us_ptr, const int64_t *__restrict__ us_cell,
     const int64_t *__restrict__ us_eptr, const int64_t *__restrict__ us_ent,
@@ -231,32 +232,37 @@ __global__ __launch_bounds__(256) void ac_adj_phipsi(
 {
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
     const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int s = blockIdx.z;
+    const int g = blockIdx.z;
     if (j < 1 || j > NY || i < 1 || i > NX) return;
     if (!(pmlx[i] | pmly[j])) return;
     const int64_t ij = (int64_t)i * P + j;
-    const T *__restrict__ P1 = P1_.at(s);
     const T sg = sx[i], tu = ty[j];
-    const T F0 = (T(1) - k.dt * sg) * F1_.at(s)[ij] + k.ex * (P1[ij - P] - P1[ij + P]);
-    const T G0 = (T(1) - k.dt * tu) * G1_.at(s)[ij] + k.ey * (P1[ij - 1] - P1[ij + 1]);
-    F0_.at(s)[ij] = F0;
-    G0_.at(s)[ij] = G0;
-    const T *__restrict__ u = un_.at(s);
-    T vn = u[ij - P], vs = u[ij + P], vw = u[ij - 1], ve = u[ij + 1];
-    // remove this level's sources from the neighbours (v is u^n before injection)
-    const int64_t ns = nsrc[s];
-    const T *__restrict__ row = srcv + srcv_off[s] + (int64_t)(n - 1) * ns;
-    for (int64_t c = us_ptr[s]; c < us_ptr[s + 1]; c++) {
-        const int64_t cell = us_cell[c];
-        if (cell != ij - P && cell != ij + P && cell != ij - 1 && cell != ij + 1) continue;
-        T add = T(0);
-        for (int64_t e = us_eptr[c]; e < us_eptr[c + 1]; e++) add += row[us_ent[e]] * k.dt2;
-        if (cell == ij - P) vn -= add;
-        if (cell == ij + P) vs -= add;
-        if (cell == ij - 1) vw -= add;
-        if (cell == ij + 1) ve -= add;
+    T acc = T(0);
+    const int s1 = min(act, (g + 1) * Sg);
+    for (int s = g * Sg; s < s1; s++) {
+        const T *__restrict__ P1 = P1_.at(s);
+        const T F0 = (T(1) - k.dt * sg) * F1_.at(s)[ij] + k.ex * (P1[ij - P] - P1[ij + P]);
+        const T G0 = (T(1) - k.dt * tu) * G1_.at(s)[ij] + k.ey * (P1[ij - 1] - P1[ij + 1]);
+        F0_.at(s)[ij] = F0;
+        G0_.at(s)[ij] = G0;
+        const T *__restrict__ u = un_.at(s);
+        T vn = u[ij - P], vs = u[ij + P], vw = u[ij - 1], ve = u[ij + 1];
+        // remove this level's sources from the neighbours (v is u^n before injection)
+        const int64_t ns = nsrc[s];
+        const T *__restrict__ row = srcv + srcv_off[s] + (int64_t)(n - 1) * ns;
+        for (int64_t c = us_ptr[s]; c < us_ptr[s + 1]; c++) {
+            const int64_t cell = us_cell[c];
+            if (cell != ij - P && cell != ij + P && cell != ij - 1 && cell != ij + 1) continue;
+            T add = T(0);
+            for (int64_t e = us_eptr[c]; e < us_eptr[c + 1]; e++) add += row[us_ent[e]] * k.dt2;
+            if (cell == ij - P) vn -= add;
+            if (cell == ij + P) vs -= add;
+            if (cell == ij - 1) vw -= add;
+            if (cell == ij + 1) ve -= add;
+        }
+        acc += F0 * (tu - sg) * k.fx * (vs - vn) + G0 * (sg - tu) * k.fy * (ve - vw);
     }
-    gC_.at(s / Sg)[ij] += F0 * (tu - sg) * k.fx * (vs - vn) + G0 * (sg - tu) * k.fy * (ve - vw);
+    gC_.at(g)[ij] += acc;
 }
 
 template <typename T> struct AdjArgs {
@@ -377,32 +383,46 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_adj2(const AdjArgs<T> a)
     }
 }
 
-// Receiver residual injection at level n after the stencil pass: for each distinct interior
-// receiver cell, R = sum of its receivers' residuals; P += R/d; dL/dC += (R/d) lap(u^{n-1}).
+// Receiver residual injection at level n after the stencil pass.  One thread owns each
+// distinct interior receiver cell of a shot group; for every active shot of the group with
+// receivers there, R = sum of those receivers' residuals (in receiver order), P_s += R/d and
+// the group's dL/dC gets (R/d) lap(u_s^{n-1}), summed over the shots in order (no atomics).
 template <typename T>
 __global__ __launch_bounds__(256) void ac_adj_inject(
-    Lv<T> P_, Lv<const T> um1_, Lv<T> gC_, int Sg, int n, const int64_t *__restrict__ ur_ptr,
-    const int64_t *__restrict__ ur_cell, const int64_t *__restrict__ ur_eptr,
-    const int64_t *__restrict__ ur_ent, const int64_t *__restrict__ rcv_ptr,
-    const T *__restrict__ resid, const int64_t *__restrict__ trace_off, const T *__restrict__ sx,
-    const T *__restrict__ ty, int NX, int NY, int P, AcCoef<T> k)
+    Lv<T> P_, Lv<const T> um1_, Lv<T> gC_, int act, int n, const int64_t *__restrict__ gi_ptr,
+    const int64_t *__restrict__ gi_cell, const int64_t *__restrict__ gi_eptr,
+    const int32_t *__restrict__ gi_shot, const int64_t *__restrict__ gi_rcv,
+    const int64_t *__restrict__ rcv_ptr, const T *__restrict__ resid,
+    const int64_t *__restrict__ trace_off, const T *__restrict__ sx, const T *__restrict__ ty,
+    int NX, int NY, int P, AcCoef<T> k)
 {
-    const int s = blockIdx.y;
-    const int64_t c = ur_ptr[s] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ur_ptr[s + 1]) return;
-    const int64_t cell = ur_cell[c];
+    const int g = blockIdx.y;
+    const int64_t c = gi_ptr[g] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= gi_ptr[g + 1]) return;
+    const int64_t cell = gi_cell[c];
     const int i = (int)(cell / P), j = (int)(cell % P);
     if (i < 1 || i > NX || j < 1 || j > NY) return;  // ring cells carry no gradient
-    const int64_t nr = rcv_ptr[s + 1] - rcv_ptr[s];
-    const T *__restrict__ row = resid + trace_off[s] + (int64_t)n * nr;
-    T R = T(0);
-    for (int64_t e = ur_eptr[c]; e < ur_eptr[c + 1]; e++) R += row[ur_ent[e]];
-    const T add = R / (T(1) + (sx[i] + ty[j]) * k.hdt);
-    P_.at(s)[cell] += add;
-    const T *__restrict__ um = um1_.at(s);
-    const T uc = um[cell];
-    gC_.at(s / Sg)[cell] += add * (k.cx * (um[cell + P] + um[cell - P] - T(2) * uc) +
-                                   k.cy * (um[cell + 1] + um[cell - 1] - T(2) * uc));
+    const T den = T(1) + (sx[i] + ty[j]) * k.hdt;
+    T acc = T(0);
+    int64_t e = gi_eptr[c];
+    const int64_t e1 = gi_eptr[c + 1];
+    while (e < e1) {
+        const int s = gi_shot[e];
+        T R = T(0);
+        const int64_t nr = rcv_ptr[s + 1] - rcv_ptr[s];
+        const T *__restrict__ row = resid + trace_off[s] + (int64_t)n * nr;
+        const bool on = s < act;
+        for (; e < e1 && gi_shot[e] == s; e++)
+            if (on) R += row[gi_rcv[e]];
+        if (!on) continue;
+        const T add = R / den;
+        P_.at(s)[cell] += add;
+        const T *__restrict__ um = um1_.at(s);
+        const T uc = um[cell];
+        acc += add * (k.cx * (um[cell + P] + um[cell - P] - T(2) * uc) +
+                      k.cy * (um[cell + 1] + um[cell - 1] - T(2) * uc));
+    }
+    gC_.at(g)[cell] += acc;
 }
 
 // model (caller layout, (NX+2) x (NY+2)) -> C (pitched), squared if asked
@@ -477,9 +497,13 @@ struct AcPlanBase {
     // device tables; cell tables exist for the pitched (plan) and dense (history) layouts
     DevBuf d_sx, d_ty, d_pmlx, d_pmly, d_nearx, d_neary;
     DevBuf d_us_ptr, d_us_eptr, d_us_ent, d_nsrc, d_srcv_off;
-    DevBuf d_ur_ptr, d_ur_eptr, d_ur_ent, d_rcv_ptr, d_trace_off;
-    DevBuf d_us_cell[2], d_ur_cell[2], d_rcv_cell[2];  // [0] pitched, [1] dense
-    int64_t max_ucells_rcv = 0;
+    DevBuf d_rcv_ptr, d_trace_off;
+    DevBuf d_us_cell[2], d_rcv_cell[2];  // [0] pitched, [1] dense
+    // receiver injection tables for the current adjoint shot grouping (inj_sg)
+    std::vector<int64_t> rcv_i0, rcv_j0, rcv_ptr_h;  // 0-based receiver (i, j), plan order
+    DevBuf d_gi_ptr, d_gi_eptr, d_gi_shot, d_gi_rcv, d_gi_cell[2];
+    int inj_sg = 0;
+    int64_t max_gi_cells = 0;
 };
 
 static int validate_params(const ads_acoustic_params *p)
@@ -580,9 +604,9 @@ template <typename T> struct AcPlan : AcPlanBase {
         const int Sg = adj_shape.sg;  // gC grouping is fixed per plan (see group_arrays())
         const double cells = (double)act * (double)NX * (double)NY;
         if (variant == 0) {
-            dim3 g(cdiv(NY + 2, 64), cdiv(NX + 2, 4), act);
+            dim3 g(cdiv(NY + 2, 64), cdiv(NX + 2, 4), cdiv(act, std::min(Sg, act)));
             ac_adj_phipsi<T><<<g, 256, 0, st>>>(
-                P1, F1, G1, F0, G0, un, gC, Sg, n, d_sx.as<T>(), d_ty.as<T>(), d_pmlx.as<uint8_t>(),
+                P1, F1, G1, F0, G0, un, gC, std::min(Sg, act), act, n, d_sx.as<T>(), d_ty.as<T>(), d_pmlx.as<uint8_t>(),
                 d_pmly.as<uint8_t>(), d_us_ptr.as<int64_t>(), d_us_cell[L.li].template as<int64_t>(),
                 d_us_eptr.as<int64_t>(), d_us_ent.as<int64_t>(), srcv, d_srcv_off.as<int64_t>(),
                 d_nsrc.as<int64_t>(), (int)NX, (int)NY, L.P, k);
@@ -596,11 +620,13 @@ template <typename T> struct AcPlan : AcPlanBase {
         if (variant == 1) launch_adj_rpt<1>(a, shape.rpt, st);
         else launch_adj_rpt<0>(a, shape.rpt, st);
         prof_stop(pr, st, cells * sizeof(T) * (7.0 + (variant == 1 ? 4.0 : 2.0) * f_pml));
-        if (resid && max_ucells_rcv > 0) {
-            dim3 g(cdiv(max_ucells_rcv, 256), act);
-            ac_adj_inject<T><<<g, 256, 0, st>>>(P, um1, gC, Sg, n, d_ur_ptr.as<int64_t>(),
-                                                d_ur_cell[L.li].template as<int64_t>(), d_ur_eptr.as<int64_t>(),
-                                                d_ur_ent.as<int64_t>(), d_rcv_ptr.as<int64_t>(),
+        if (resid && max_gi_cells > 0) {
+            if (inj_sg != Sg) ADS_TRY(build_inject_tables(Sg));
+            dim3 g(cdiv(max_gi_cells, 256), cdiv(act, std::min(Sg, act)));
+            ac_adj_inject<T><<<g, 256, 0, st>>>(P, um1, gC, act, n, d_gi_ptr.as<int64_t>(),
+                                                d_gi_cell[L.li].template as<int64_t>(),
+                                                d_gi_eptr.as<int64_t>(), d_gi_shot.as<int32_t>(),
+                                                d_gi_rcv.as<int64_t>(), d_rcv_ptr.as<int64_t>(),
                                                 resid, d_trace_off.as<int64_t>(), d_sx.as<T>(),
                                                 d_ty.as<T>(), (int)NX, (int)NY, L.P, k);
         }
@@ -609,6 +635,54 @@ template <typename T> struct AcPlan : AcPlanBase {
     }
 
     int64_t group_arrays() const { return (int64_t)cdiv(nshots, adj_shape.sg); }
+
+    // Union of receiver cells per shot group of size Sg (plan order); per cell, entries sorted
+    // by (shot, receiver order).  Note: the Sg used at a level with act < Sg shots is act,
+    // which puts every active shot in group 0 -- the same as these tables (shots < Sg).
+    int build_inject_tables(int Sg)
+    {
+        if (Sg < 1) Sg = 1;
+        const int64_t ng = (int64_t)cdiv(nshots, Sg);
+        std::vector<int64_t> gptr(1, 0), eptr(1, 0), rcv, cells[2];
+        std::vector<int32_t> shot;
+        max_gi_cells = 0;
+        for (int64_t g = 0; g < ng; g++) {
+            struct E { int64_t i, j; int32_t s; int64_t r; };
+            std::vector<E> es;
+            for (int64_t s = g * Sg; s < std::min<int64_t>(nshots, (g + 1) * Sg); s++)
+                for (int64_t e = rcv_ptr_h[s]; e < rcv_ptr_h[s + 1]; e++)
+                    es.push_back(E{rcv_i0[e], rcv_j0[e], (int32_t)s, e - rcv_ptr_h[s]});
+            std::stable_sort(es.begin(), es.end(), [](const E &a, const E &b) {
+                if (a.i != b.i) return a.i < b.i;
+                if (a.j != b.j) return a.j < b.j;
+                return a.s < b.s;  // stable: receiver order kept within a shot
+            });
+            int64_t ncell = 0;
+            for (size_t q = 0; q < es.size();) {
+                size_t z = q;
+                while (z < es.size() && es[z].i == es[q].i && es[z].j == es[q].j) {
+                    shot.push_back(es[z].s);
+                    rcv.push_back(es[z].r);
+                    z++;
+                }
+                cells[0].push_back(es[q].i * Pitch + es[q].j);
+                cells[1].push_back(es[q].i * (NY + 2) + es[q].j);
+                eptr.push_back((int64_t)shot.size());
+                ncell++;
+                q = z;
+            }
+            gptr.push_back(gptr.back() + ncell);
+            max_gi_cells = std::max(max_gi_cells, ncell);
+        }
+        ADS_TRY(d_gi_ptr.upload(gptr));
+        ADS_TRY(d_gi_eptr.upload(eptr));
+        ADS_TRY(d_gi_shot.upload(shot));
+        ADS_TRY(d_gi_rcv.upload(rcv));
+        ADS_TRY(d_gi_cell[0].upload(cells[0]));
+        ADS_TRY(d_gi_cell[1].upload(cells[1]));
+        inj_sg = Sg;
+        return ADS_OK;
+    }
 
     // ---- workspace layout for checkpoint interval K ----
     struct WS {
@@ -1020,22 +1094,15 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
     const int64_t pitches[2] = {P->Pitch, p->NY + 2};
     for (int L = 0; L < 2; L++) {
         std::vector<int64_t> sf = flat(si, sj, pitches[L]), rf = flat(ri, rj, pitches[L]);
-        CellLists us = build_cell_lists(ns, sp.data(), sf), ur = build_cell_lists(ns, rp.data(), rf);
+        CellLists us = build_cell_lists(ns, sp.data(), sf);
         // cell lists are identical in structure for both pitches (the order of cells is the
         // same because (i, j) -> i*pitch + j is monotone for j < pitch)
         if (L == 0) {
             ADS_TRY(P->d_us_ptr.upload(us.shot_ptr));
             ADS_TRY(P->d_us_eptr.upload(us.ent_ptr));
             ADS_TRY(P->d_us_ent.upload(us.ent));
-            ADS_TRY(P->d_ur_ptr.upload(ur.shot_ptr));
-            ADS_TRY(P->d_ur_eptr.upload(ur.ent_ptr));
-            ADS_TRY(P->d_ur_ent.upload(ur.ent));
-            P->max_ucells_rcv = 0;
-            for (int64_t q = 0; q < ns; q++)
-                P->max_ucells_rcv = std::max(P->max_ucells_rcv, ur.shot_ptr[q + 1] - ur.shot_ptr[q]);
         }
         ADS_TRY(P->d_us_cell[L].upload(us.cell));
-        ADS_TRY(P->d_ur_cell[L].upload(ur.cell));
         ADS_TRY(P->d_rcv_cell[L].upload(rf));
     }
     // PML profiles and flags
@@ -1069,6 +1136,10 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
     ADS_TRY(P->d_srcv_off.upload(soff));
     ADS_TRY(P->d_rcv_ptr.upload(rp));
     ADS_TRY(P->d_trace_off.upload(toff));
+    P->rcv_i0 = ri;
+    P->rcv_j0 = rj;
+    P->rcv_ptr_h = rp;
+    ADS_TRY(P->build_inject_tables(P->adj_shape.sg));
     return ADS_OK;
 }
 

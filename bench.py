@@ -143,6 +143,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--interval", type=int, default=0, help="checkpoint interval (0 = auto)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the kernel-shape autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the "
@@ -165,6 +166,7 @@ def main():
     c0t = torch.as_tensor(c0, dtype=torch.float32, device="cuda")
     obs = plan.forward(ct, srcv, K).clone()
     torch.cuda.synchronize()
+    shape = plan.autotune(c0t, srcv, K, reps=3) if not args.no_tune else None
     for _ in range(args.warmup):
         plan.gradient(c0t, srcv, obs, K)
     torch.cuda.synchronize()
@@ -199,6 +201,8 @@ def main():
                                 "receivers, FP32 forward + hand-written adjoint, one batched "
                                 "call per gradient"),
                    "PropagatorKernel": p.PropagatorKernel, "checkpoint_interval": K,
+                   "kernel_shape": {"fwd_rows": shape[0], "fwd_shots": shape[1],
+                                    "adj_rows": shape[2], "adj_shots": shape[3]} if shape else None,
                    "shots": plan.nshots, "cells": p.NX * p.NY, "nstep": p.NSTEP},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),

@@ -298,10 +298,18 @@ int ads_elastic_plan_gradient(ads_elastic_plan *plan, const void *rho, const voi
                               double *loss, void *grad_rho, void *grad_lam, void *grad_mu,
                               void *traces, void *workspace, size_t ws_bytes, int64_t K,
                               ads_stream_t stream);
-/* Full 5-field histories (5 x (NSTEP+1) x N, order vx, vy, sxx, syy, sxy) of one shot:
- * the plain drop-in of ElasticPropagatorSolver (Core.jl:31-94). */
+/* Pullback of the full-history drop-in: given dense cotangents of the 5 field histories
+ * (layout below), OVERWRITES the three gradients.  Reruns the checkpointed forward internally. */
+int ads_elastic_plan_backward_dense(ads_elastic_plan *plan, const void *rho, const void *lam,
+                                    const void *mu, const void *srcv, const void *grad_hist,
+                                    void *grad_rho, void *grad_lam, void *grad_mu, void *workspace,
+                                    size_t ws_bytes, int64_t K, ads_stream_t stream);
+/* Plain drop-in of ElasticPropagatorSolver (Core.jl:31-94): the full histories of every shot
+ * (nstep == NSTEP), dense layout [shot][field vx, vy, sxx, syy, sxy][level 0..NSTEP][N].
+ * scratch: ads_elastic_plan_history_scratch_size() bytes (CPML memory variables). */
+size_t ads_elastic_plan_history_scratch_size(const ads_elastic_plan *plan);
 int ads_elastic_forward_history(ads_elastic_plan *plan, const void *rho, const void *lam,
-                                const void *mu, const void *srcv, void *hist,
+                                const void *mu, const void *srcv, void *hist, void *scratch,
                                 ads_stream_t stream);
 
 /* ======================================================================================
