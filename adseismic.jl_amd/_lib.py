@@ -99,7 +99,7 @@ _SIGS = {
     "ads_elastic_plan_choose_interval": (i64, [vp, C.c_size_t]),
     "ads_elastic_plan_forward": (C.c_int, [vp] * 6 + [vp, C.c_size_t, i64, vp]),
     "ads_elastic_plan_backward": (C.c_int, [vp] * 9 + [vp, C.c_size_t, i64, vp]),
-    "ads_elastic_plan_gradient": (C.c_int, [vp] * 11 + [C.c_size_t, i64, vp]),
+    "ads_elastic_plan_gradient": (C.c_int, [vp] * 12 + [C.c_size_t, i64, vp]),
     "ads_elastic_plan_backward_dense": (C.c_int, [vp] * 9 + [vp, C.c_size_t, i64, vp]),
     "ads_elastic_plan_history_scratch_size": (C.c_size_t, [vp]),
     "ads_elastic_forward_history": (C.c_int, [vp] * 8),

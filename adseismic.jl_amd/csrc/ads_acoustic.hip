@@ -102,66 +102,122 @@ template <typename T> struct FwdArgs {
     AcCoef<T> k;
 };
 
-template <typename T, int V, int RPT>
-__global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd2(const FwdArgs<T> a)
+// Vector access of VEC consecutive depth cells per lane (128-bit loads in the pitched layout).
+template <typename T, int VEC> struct VecT;
+template <> struct VecT<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct VecT<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
+template <typename T, int VEC> __device__ inline void vld(T (&a)[VEC], const T *p)
+{
+    if constexpr (VEC == 1) {
+        a[0] = *p;
+    } else {
+        const typename VecT<T, VEC>::type x = *reinterpret_cast<const typename VecT<T, VEC>::type *>(p);
+#pragma unroll
+        for (int e = 0; e < VEC; e++) a[e] = x[e];
+    }
+}
+template <typename T, int VEC> __device__ inline void vst(T *p, const T (&a)[VEC])
+{
+    if constexpr (VEC == 1) {
+        *p = a[0];
+    } else {
+        typename VecT<T, VEC>::type x;
+#pragma unroll
+        for (int e = 0; e < VEC; e++) x[e] = a[e];
+        *reinterpret_cast<typename VecT<T, VEC>::type *>(p) = x;
+    }
+}
+template <typename T, int VEC> __device__ inline void vzero(T (&a)[VEC])
+{
+#pragma unroll
+    for (int e = 0; e < VEC; e++) a[e] = T(0);
+}
+
+// All rows a wavefront needs for one shot are loaded before any store is issued, so the
+// loads of a shot overlap (the stores may alias the inputs as far as the compiler knows).
+template <typename T, int V, int RPT, int VEC>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd3(const FwdArgs<T> a)
 {
     int g, strip, tj;
     if (!decode_tile(a.map, g, strip, tj)) return;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
-    const int j = tj * WAVE + lane;
+    const int j0 = (tj * WAVE + lane) * VEC;
     const int ib = (strip * NWAVES + wv) * RPT;
     const int NX = a.NX, NY = a.NY, P = a.P;
     if (ib > NX + 1) return;  // wave-uniform
-    const bool jvalid = j < NY + 2, jin = j >= 1 && j <= NY;
+    // VEC > 1 only in the pitched layout, where the whole vector lies inside the row
+    const bool vin = VEC == 1 ? j0 < NY + 2 : j0 < P;
     const AcCoef<T> k = a.k;
-    T Cr[RPT];
+    T Cr[RPT][VEC], tyv[VEC];
+    bool nyv[VEC];
 #pragma unroll
-    for (int r = 0; r < RPT; r++) Cr[r] = ldg(a.C, (int64_t)(ib + r) * P + j, jvalid && ib + r <= NX + 1);
-    const T tj_ = jvalid ? a.ty[j] : T(0);
-    const bool nyj = jvalid ? a.neary[j] != 0 : false;
+    for (int r = 0; r < RPT; r++) {
+        if (vin && ib + r <= NX + 1) vld(Cr[r], a.C + (int64_t)(ib + r) * P + j0);
+        else vzero(Cr[r]);
+    }
+#pragma unroll
+    for (int e = 0; e < VEC; e++) {
+        const bool ok = j0 + e < NY + 2;
+        tyv[e] = ok ? a.ty[j0 + e] : T(0);
+        nyv[e] = ok ? a.neary[j0 + e] != 0 : false;
+    }
     const int s1 = min(a.act, (g + 1) * a.Sg);
     for (int s = g * a.Sg; s < s1; s++) {
         const T *__restrict__ w = a.w.at(s);
+        const T *__restrict__ wo = a.wold.at(s);
         T *__restrict__ u = a.u.at(s);
-        T wn = ldg(w, (int64_t)(ib - 1) * P + j, jvalid && ib >= 1);
-        T wc = ldg(w, (int64_t)ib * P + j, jvalid);
+        T W[RPT + 2][VEC], WO[RPT][VEC];
+#pragma unroll
+        for (int q = 0; q < RPT + 2; q++) {
+            const int i = ib - 1 + q;
+            if (vin && i >= 0 && i <= NX + 1) vld(W[q], w + (int64_t)i * P + j0);
+            else vzero(W[q]);
+        }
 #pragma unroll
         for (int r = 0; r < RPT; r++) {
             const int i = ib + r;
-            if (i > NX + 1) break;
-            const int64_t ij = (int64_t)i * P + j;
-            const T ws = ldg(w, ij + P, jvalid && i + 1 <= NX + 1);
-            T ww = __shfl_up(wc, 1), we = __shfl_down(wc, 1);
-            const bool in = jin && i >= 1 && i <= NX;
-            if (lane == 0 && in) ww = w[ij - 1];
-            if (lane == WAVE - 1 && in) we = w[ij + 1];
-            if (jvalid) {
-                if (!in) {
-                    u[ij] = T(0);
+            if (vin && i >= 1 && i <= NX) vld(WO[r], wo + (int64_t)i * P + j0);
+            else vzero(WO[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            const int i = ib + r;
+            if (i > NX + 1) break;  // wave-uniform
+            const int64_t row = (int64_t)i * P;
+            const bool rowin = i >= 1 && i <= NX;
+            T wl = __shfl_up(W[r + 1][VEC - 1], 1), wr = __shfl_down(W[r + 1][0], 1);
+            if (lane == 0 && rowin && j0 >= 1 && j0 <= NY + 1) wl = w[row + j0 - 1];
+            if (lane == WAVE - 1 && rowin && j0 + VEC <= NY + 1) wr = w[row + j0 + VEC];
+            const bool nx = a.nearx[i] != 0;
+            T out[VEC];
+#pragma unroll
+            for (int e = 0; e < VEC; e++) {
+                const int j = j0 + e;
+                const T wc = W[r + 1][e], wn = W[r][e], ws = W[r + 2][e];
+                const T ww = e == 0 ? wl : W[r + 1][e - 1];
+                const T we = e == VEC - 1 ? wr : W[r + 1][e + 1];
+                out[e] = T(0);
+                if (!(rowin && j >= 1 && j <= NY)) continue;
+                const T Cij = Cr[r][e];
+                const T lap = k.cx * (ws + wn - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
+                if (!(nx | nyv[e])) {
+                    out[e] = T(2) * wc - WO[r][e] + Cij * lap;
                 } else {
-                    const T Cij = Cr[r];
-                    const T lap = k.cx * (ws + wn - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
-                    const T wo = a.wold.at(s)[ij];
-                    if (!(a.nearx[i] | nyj)) {
-                        u[ij] = T(2) * wc - wo + Cij * lap;
-                    } else {
-                        const T sg = a.sx[i], tu = tj_;
-                        const T *__restrict__ phi = a.phi.at(s);
-                        const T *__restrict__ psi = a.psi.at(s);
-                        const T num = (T(2) - sg * tu * k.dt2) * wc + Cij * lap +
-                                      k.ex * (phi[ij + P] - phi[ij - P]) +
-                                      k.ey * (psi[ij + 1] - psi[ij - 1]) -
-                                      (T(1) - (sg + tu) * k.hdt) * wo;
-                        u[ij] = num / (T(1) + (sg + tu) * k.hdt);
-                        if (V == 1 && (sg != T(0) || tu != T(0))) {
-                            a.phio.at(s)[ij] = (T(1) - k.dt * sg) * phi[ij] + Cij * (tu - sg) * k.fx * (ws - wn);
-                            a.psio.at(s)[ij] = (T(1) - k.dt * tu) * psi[ij] + Cij * (sg - tu) * k.fy * (we - ww);
-                        }
+                    const int64_t ij = row + j;
+                    const T sg = a.sx[i], tu = tyv[e];
+                    const T *__restrict__ phi = a.phi.at(s);
+                    const T *__restrict__ psi = a.psi.at(s);
+                    const T num = (T(2) - sg * tu * k.dt2) * wc + Cij * lap +
+                                  k.ex * (phi[ij + P] - phi[ij - P]) +
+                                  k.ey * (psi[ij + 1] - psi[ij - 1]) - (T(1) - (sg + tu) * k.hdt) * WO[r][e];
+                    out[e] = num / (T(1) + (sg + tu) * k.hdt);
+                    if (V == 1 && (sg != T(0) || tu != T(0))) {
+                        a.phio.at(s)[ij] = (T(1) - k.dt * sg) * phi[ij] + Cij * (tu - sg) * k.fx * (ws - wn);
+                        a.psio.at(s)[ij] = (T(1) - k.dt * tu) * psi[ij] + Cij * (sg - tu) * k.fy * (we - ww);
                     }
                 }
             }
-            wn = wc;
-            wc = ws;
+            if (vin) vst(u + row + j0, out);
         }
     }
 }
@@ -278,72 +334,109 @@ template <typename T> struct AdjArgs {
     AcCoef<T> k;
 };
 
-template <typename T, int V, int RPT>
-__global__ __launch_bounds__(WAVE *NWAVES) void ac_adj2(const AdjArgs<T> a)
+template <typename T, int V, int RPT, int VEC>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_adj3(const AdjArgs<T> a)
 {
     int g, strip, tj;
     if (!decode_tile(a.map, g, strip, tj)) return;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
-    const int j = tj * WAVE + lane;
+    const int j0 = (tj * WAVE + lane) * VEC;
     const int ib = (strip * NWAVES + wv) * RPT;
     const int NX = a.NX, NY = a.NY, P = a.P;
     if (ib > NX + 1) return;
-    const bool jvalid = j < NY + 2, jin = j >= 1 && j <= NY;
+    const bool vin = VEC == 1 ? j0 < NY + 2 : j0 < P;
     const AcCoef<T> k = a.k;
     // model rows ib-1 .. ib+RPT stay in registers across the shot group
-    T Cr[RPT + 2];
+    T Cr[RPT + 2][VEC], tyv[VEC];
+    bool nyv[VEC];
 #pragma unroll
-    for (int r = 0; r < RPT + 2; r++) {
-        const int i = ib - 1 + r;
-        Cr[r] = ldg(a.C, (int64_t)i * P + j, jvalid && i >= 0 && i <= NX + 1);
+    for (int q = 0; q < RPT + 2; q++) {
+        const int i = ib - 1 + q;
+        if (vin && i >= 0 && i <= NX + 1) vld(Cr[q], a.C + (int64_t)i * P + j0);
+        else vzero(Cr[q]);
     }
-    T gacc[RPT];
 #pragma unroll
-    for (int r = 0; r < RPT; r++) gacc[r] = T(0);
-    const T tu = jvalid ? a.ty[j] : T(0);
-    const bool nyj = jvalid ? a.neary[j] != 0 : false;
+    for (int e = 0; e < VEC; e++) {
+        const bool ok = j0 + e < NY + 2;
+        tyv[e] = ok ? a.ty[j0 + e] : T(0);
+        nyv[e] = ok ? a.neary[j0 + e] != 0 : false;
+    }
+    // model values one cell west/east of the vector, per row
+    T cl[RPT], cr[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+        const int i = ib + r;
+        const bool rowin = i >= 1 && i <= NX;
+        cl[r] = __shfl_up(Cr[r + 1][VEC - 1], 1);
+        cr[r] = __shfl_down(Cr[r + 1][0], 1);
+        if (lane == 0 && rowin && j0 >= 1 && j0 <= NY + 1) cl[r] = a.C[(int64_t)i * P + j0 - 1];
+        if (lane == WAVE - 1 && rowin && j0 + VEC <= NY + 1) cr[r] = a.C[(int64_t)i * P + j0 + VEC];
+    }
+    T gacc[RPT][VEC];
+#pragma unroll
+    for (int r = 0; r < RPT; r++) vzero(gacc[r]);
     const int s1 = min(a.act, (g + 1) * a.Sg);
     for (int s = g * a.Sg; s < s1; s++) {
         const T *__restrict__ P1 = a.P1.at(s);
         const T *__restrict__ um = a.um1.at(s);
         T *__restrict__ Pout = a.Pn.at(s);
-        T p1n = ldg(P1, (int64_t)(ib - 1) * P + j, jvalid && ib >= 1);
-        T p1c = ldg(P1, (int64_t)ib * P + j, jvalid);
-        T un = ldg(um, (int64_t)(ib - 1) * P + j, jvalid && ib >= 1);
-        T uc = ldg(um, (int64_t)ib * P + j, jvalid);
+        T P1r[RPT + 2][VEC], Ur[RPT + 2][VEC], P2r[RPT][VEC];
+#pragma unroll
+        for (int q = 0; q < RPT + 2; q++) {
+            const int i = ib - 1 + q;
+            if (vin && i >= 0 && i <= NX + 1) {
+                vld(P1r[q], P1 + (int64_t)i * P + j0);
+                vld(Ur[q], um + (int64_t)i * P + j0);
+            } else {
+                vzero(P1r[q]);
+                vzero(Ur[q]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            const int i = ib + r;
+            if (vin && i >= 1 && i <= NX) vld(P2r[r], Pout + (int64_t)i * P + j0);
+            else vzero(P2r[r]);
+        }
 #pragma unroll
         for (int r = 0; r < RPT; r++) {
             const int i = ib + r;
             if (i > NX + 1) break;
-            const int64_t ij = (int64_t)i * P + j;
-            const bool nxt = jvalid && i + 1 <= NX + 1;
-            const T p1s = ldg(P1, ij + P, nxt);
-            const T us = ldg(um, ij + P, nxt);
-            const T Cc = Cr[r + 1];
-            const T qc = Cc * p1c;
-            T qw = __shfl_up(qc, 1), qe = __shfl_down(qc, 1);
-            T p1w = __shfl_up(p1c, 1), p1e = __shfl_down(p1c, 1);
-            T uw = __shfl_up(uc, 1), ue = __shfl_down(uc, 1);
-            T Cw = __shfl_up(Cc, 1), Ce = __shfl_down(Cc, 1);
-            const bool in = jin && i >= 1 && i <= NX;
-            if (lane == 0 && in) {
-                p1w = P1[ij - 1];
-                Cw = a.C[ij - 1];
-                qw = Cw * p1w;
-                uw = um[ij - 1];
+            const int64_t row = (int64_t)i * P;
+            const bool rowin = i >= 1 && i <= NX;
+            T pl = __shfl_up(P1r[r + 1][VEC - 1], 1), pr = __shfl_down(P1r[r + 1][0], 1);
+            T ul = __shfl_up(Ur[r + 1][VEC - 1], 1), ur = __shfl_down(Ur[r + 1][0], 1);
+            if (lane == 0 && rowin && j0 >= 1 && j0 <= NY + 1) {
+                pl = P1[row + j0 - 1];
+                ul = um[row + j0 - 1];
             }
-            if (lane == WAVE - 1 && in) {
-                p1e = P1[ij + 1];
-                Ce = a.C[ij + 1];
-                qe = Ce * p1e;
-                ue = um[ij + 1];
+            if (lane == WAVE - 1 && rowin && j0 + VEC <= NY + 1) {
+                pr = P1[row + j0 + VEC];
+                ur = um[row + j0 + VEC];
             }
-            if (in) {
-                const T lapq = k.cx * (Cr[r + 2] * p1s + Cr[r] * p1n - T(2) * qc) +
-                               k.cy * (qe + qw - T(2) * qc);
-                const T p2 = Pout[ij];
-                const bool near = a.nearx[i] | nyj;
+            const bool nx = a.nearx[i] != 0;
+            T out[VEC];
+#pragma unroll
+            for (int e = 0; e < VEC; e++) {
+                const int j = j0 + e;
+                out[e] = T(0);
+                if (!(rowin && j >= 1 && j <= NY)) continue;
+                const T p1c = P1r[r + 1][e], p1n = P1r[r][e], p1s = P1r[r + 2][e];
+                const T p1w = e == 0 ? pl : P1r[r + 1][e - 1];
+                const T p1e = e == VEC - 1 ? pr : P1r[r + 1][e + 1];
+                const T Cc = Cr[r + 1][e], Cn = Cr[r][e], Cs = Cr[r + 2][e];
+                const T Cw = e == 0 ? cl[r] : Cr[r + 1][e - 1];
+                const T Ce = e == VEC - 1 ? cr[r] : Cr[r + 1][e + 1];
+                const T uc = Ur[r + 1][e], un = Ur[r][e], us = Ur[r + 2][e];
+                const T uw = e == 0 ? ul : Ur[r + 1][e - 1];
+                const T ue = e == VEC - 1 ? ur : Ur[r + 1][e + 1];
+                const T qc = Cc * p1c;
+                const T lapq = k.cx * (Cs * p1s + Cn * p1n - T(2) * qc) + k.cy * (Ce * p1e + Cw * p1w - T(2) * qc);
+                const T p2 = P2r[r][e];
+                const bool near = nx | nyv[e];
+                const T tu = tyv[e];
                 T sg = T(0), U;
+                const int64_t ij = row + j;
                 if (!near) {
                     U = T(2) * p1c + lapq - p2;
                 } else {
@@ -351,13 +444,12 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_adj2(const AdjArgs<T> a)
                     const T *__restrict__ Fn = a.Fn.at(s);
                     const T *__restrict__ Gn = a.Gn.at(s);
                     U = (T(2) - sg * tu * k.dt2) * p1c + lapq - (T(1) - (sg + tu) * k.hdt) * p2 +
-                        k.fx * (Cr[r] * (tu - a.sx[i - 1]) * Fn[ij - P] -
-                                Cr[r + 2] * (tu - a.sx[i + 1]) * Fn[ij + P]) +
+                        k.fx * (Cn * (tu - a.sx[i - 1]) * Fn[ij - P] - Cs * (tu - a.sx[i + 1]) * Fn[ij + P]) +
                         k.fy * (Cw * (sg - a.ty[j - 1]) * Gn[ij - 1] - Ce * (sg - a.ty[j + 1]) * Gn[ij + 1]);
                 }
                 if (a.dense.p) U += a.dense.at(s)[ij];
                 const T Pv = near ? U / (T(1) + (sg + tu) * k.hdt) : U;
-                Pout[ij] = Pv;
+                out[e] = Pv;
                 T gg = Pv * (k.cx * (us + un - T(2) * uc) + k.cy * (ue + uw - T(2) * uc));
                 if (V == 1 && near && (sg != T(0) || tu != T(0))) {
                     const T F0 = (T(1) - k.dt * sg) * a.Fn.at(s)[ij] + k.ex * (p1n - p1s);
@@ -366,20 +458,22 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_adj2(const AdjArgs<T> a)
                     a.G0.at(s)[ij] = G0;
                     gg += F0 * (tu - sg) * k.fx * (us - un) + G0 * (sg - tu) * k.fy * (ue - uw);
                 }
-                gacc[r] += gg;
+                gacc[r][e] += gg;
             }
-            p1n = p1c;
-            p1c = p1s;
-            un = uc;
-            uc = us;
+            if (vin) vst(Pout + row + j0, out);
         }
     }
-    if (!jin) return;
+    if (!vin) return;
     T *__restrict__ gC = a.gC.at(g);
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
         const int i = ib + r;
-        if (i >= 1 && i <= NX) gC[(int64_t)i * P + j] += gacc[r];
+        if (i < 1 || i > NX) continue;
+        T cur[VEC];
+        vld(cur, gC + (int64_t)i * P + j0);
+#pragma unroll
+        for (int e = 0; e < VEC; e++) cur[e] += gacc[r][e];  // zero outside the interior
+        vst(gC + (int64_t)i * P + j0, cur);
     }
 }
 
@@ -475,7 +569,7 @@ __global__ __launch_bounds__(256) void k_misfit(const T *__restrict__ tr, const 
 // plan
 // ---------------------------------------------------------------------------------------------
 struct Shape {
-    int rpt = 4, sg = 8;
+    int rpt = 2, sg = 8;
 };
 
 struct AcPlanBase {
@@ -531,38 +625,49 @@ template <typename T> struct AcPlan : AcPlanBase {
     Layout plan_layout() const { return Layout{(int)Pitch, 0}; }
     Layout dense_layout() const { return Layout{(int)(NY + 2), 1}; }
 
-    TileMap tiles(int P, int rpt, int act, int sg) const
+    static unsigned grid_of(const TileMap &m) { return cdiv(m.nb, NXCD) * NXCD; }
+
+    static constexpr int VECP = sizeof(T) == 4 ? 4 : 2;  // pitched layout: 16-byte vectors
+    TileMap tiles(int P, int vec, int rpt, int act, int sg) const
     {
         TileMap m;
-        m.ntj = (int)cdiv(P, WAVE);
+        m.ntj = (int)cdiv(P, WAVE * vec);
         m.nstrips = (int)cdiv(NX + 2, NWAVES * rpt);
         m.ngroups = (int)cdiv(act, sg);
         m.nb = m.ntj * m.nstrips * m.ngroups;
         return m;
     }
-    static unsigned grid_of(const TileMap &m) { return cdiv(m.nb, NXCD) * NXCD; }
-
-    template <int V, int RPT> void launch_fwd(FwdArgs<T> &a, hipStream_t st)
+    template <int V, int RPT, int VEC> void launch_fwd(FwdArgs<T> &a, hipStream_t st)
     {
-        a.map = tiles(a.P, RPT, a.act, a.Sg);
-        ac_fwd2<T, V, RPT><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a);
+        a.map = tiles(a.P, VEC, RPT, a.act, a.Sg);
+        ac_fwd3<T, V, RPT, VEC><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a);
     }
-    template <int V> void launch_fwd_rpt(FwdArgs<T> &a, int rpt, hipStream_t st)
+    template <int V, int VEC> void launch_fwd_rpt(FwdArgs<T> &a, int rpt, hipStream_t st)
     {
-        if (rpt == 2) launch_fwd<V, 2>(a, st);
-        else if (rpt == 8) launch_fwd<V, 8>(a, st);
-        else launch_fwd<V, 4>(a, st);
+        if (rpt == 1) launch_fwd<V, 1, VEC>(a, st);
+        else if (rpt == 4) launch_fwd<V, 4, VEC>(a, st);
+        else launch_fwd<V, 2, VEC>(a, st);
     }
-    template <int V, int RPT> void launch_adj(AdjArgs<T> &a, hipStream_t st)
+    template <int V> void launch_fwd_any(FwdArgs<T> &a, int rpt, bool pitched, hipStream_t st)
     {
-        a.map = tiles(a.P, RPT, a.act, a.Sg);
-        ac_adj2<T, V, RPT><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a);
+        if (pitched) launch_fwd_rpt<V, VECP>(a, rpt, st);
+        else launch_fwd_rpt<V, 1>(a, rpt, st);
     }
-    template <int V> void launch_adj_rpt(AdjArgs<T> &a, int rpt, hipStream_t st)
+    template <int V, int RPT, int VEC> void launch_adj(AdjArgs<T> &a, hipStream_t st)
     {
-        if (rpt == 2) launch_adj<V, 2>(a, st);
-        else if (rpt == 8) launch_adj<V, 8>(a, st);
-        else launch_adj<V, 4>(a, st);
+        a.map = tiles(a.P, VEC, RPT, a.act, a.Sg);
+        ac_adj3<T, V, RPT, VEC><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a);
+    }
+    template <int V, int VEC> void launch_adj_rpt(AdjArgs<T> &a, int rpt, hipStream_t st)
+    {
+        if (rpt == 1) launch_adj<V, 1, VEC>(a, st);
+        else if (rpt == 4) launch_adj<V, 4, VEC>(a, st);
+        else launch_adj<V, 2, VEC>(a, st);
+    }
+    template <int V> void launch_adj_any(AdjArgs<T> &a, int rpt, bool pitched, hipStream_t st)
+    {
+        if (pitched) launch_adj_rpt<V, VECP>(a, rpt, st);
+        else launch_adj_rpt<V, 1>(a, rpt, st);
     }
 
     int fwd_step(const Layout &L, int n, int act, Lv<T> uo, Lv<const T> w, Lv<const T> wo,
@@ -576,8 +681,8 @@ template <typename T> struct AcPlan : AcPlanBase {
                      d_nearx.as<uint8_t>(), d_neary.as<uint8_t>(), (int)NX, (int)NY, L.P, act,
                      std::min(shape.sg, act), TileMap{}, k};
         int pr = prof_start(ADS_PROF_AC_FWD, st);
-        if (variant == 1) launch_fwd_rpt<1>(a, shape.rpt, st);
-        else launch_fwd_rpt<0>(a, shape.rpt, st);
+        if (variant == 1) launch_fwd_any<1>(a, shape.rpt, L.li == 0, st);
+        else launch_fwd_any<0>(a, shape.rpt, L.li == 0, st);
         prof_stop(pr, st, cells * sizeof(T) * (4.0 + (variant == 1 ? 4.0 : 2.0) * f_pml));
         if (variant == 0) {
             dim3 g(cdiv(NY + 2, 64), cdiv(NX + 2, 4), act);
@@ -617,8 +722,8 @@ template <typename T> struct AcPlan : AcPlanBase {
                      TileMap{}, k};
         (void)shape;
         int pr = prof_start(ADS_PROF_AC_ADJ, st);
-        if (variant == 1) launch_adj_rpt<1>(a, shape.rpt, st);
-        else launch_adj_rpt<0>(a, shape.rpt, st);
+        if (variant == 1) launch_adj_any<1>(a, shape.rpt, L.li == 0, st);
+        else launch_adj_any<0>(a, shape.rpt, L.li == 0, st);
         prof_stop(pr, st, cells * sizeof(T) * (7.0 + (variant == 1 ? 4.0 : 2.0) * f_pml));
         if (resid && max_gi_cells > 0) {
             if (inj_sg != Sg) ADS_TRY(build_inject_tables(Sg));
@@ -887,7 +992,7 @@ template <typename T> struct AcPlan : AcPlanBase {
             ADS_HIP(hipMemsetAsync(w.phi[q], 0, F, st));
             ADS_HIP(hipMemsetAsync(w.psi[q], 0, F, st));
         }
-        const int rpts[3] = {2, 4, 8};
+        const int rpts[3] = {1, 2, 4};
         std::vector<int> sgs;
         for (int sg = 1; sg <= std::max<int64_t>(nshots, 1) && sg <= 32; sg *= 2) sgs.push_back(sg);
         if (reps < 1) reps = 3;
@@ -1015,8 +1120,8 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
     P->Ndense = (p->NX + 2) * (p->NY + 2);
     P->nshots = g->nshots;
     P->k = make_coef<T>(p->DELTAT, p->DELTAX, p->DELTAY);
-    P->fwd_shape = Shape{4, (int)std::min<int64_t>(8, g->nshots)};
-    P->adj_shape = Shape{4, (int)std::min<int64_t>(8, g->nshots)};
+    P->fwd_shape = Shape{2, (int)std::min<int64_t>(4, g->nshots)};
+    P->adj_shape = Shape{2, (int)std::min<int64_t>(8, g->nshots)};
     const int64_t ns = g->nshots;
     if (g->src_ptr[0] != 0 || g->rcv_ptr[0] != 0) ADS_FAIL(ADS_ERR_ARG, "src_ptr/rcv_ptr must start at 0");
     std::vector<int64_t> nstep(ns);
@@ -1288,8 +1393,8 @@ int ads_acoustic_plan_set_shape(ads_acoustic_plan *plan, const int *shape)
     if (!shape) ADS_FAIL(ADS_ERR_ARG, "null shape");
     for (int q = 0; q < 4; q++)
         if (shape[q] < 1) ADS_FAIL(ADS_ERR_ARG, "shape entries must be >= 1");
-    if ((shape[0] != 2 && shape[0] != 4 && shape[0] != 8) || (shape[2] != 2 && shape[2] != 4 && shape[2] != 8))
-        ADS_FAIL(ADS_ERR_ARG, "rows per thread must be 2, 4 or 8");
+    if ((shape[0] != 1 && shape[0] != 2 && shape[0] != 4) || (shape[2] != 1 && shape[2] != 2 && shape[2] != 4))
+        ADS_FAIL(ADS_ERR_ARG, "rows per wavefront must be 1, 2 or 4");
     return dispatch(plan, [&](auto &P) -> int {
         P.fwd_shape = Shape{shape[0], shape[1]};
         P.adj_shape = Shape{shape[2], shape[3]};

@@ -53,7 +53,7 @@ def velocity_model(NX, NY, seed):
     return c
 
 
-def workload(ads, small=False):
+def workload(ads, small=False, nstep=0):
     import torch
     from scipy.ndimage import gaussian_filter
 
@@ -63,6 +63,8 @@ def workload(ads, small=False):
     else:
         NX = NY = 1000
         NSTEP, nshots, npml = 4000, 32, 40
+    if nstep:
+        NSTEP = nstep
     p = ads.AcousticPropagatorParams(NX=NX, NY=NY, NSTEP=NSTEP, DELTAX=5.0, DELTAY=5.0,
                                      DELTAT=0.5e-3, NPOINTS_PML=npml, vp_ref=3000.0,
                                      PropagatorKernel=1)
@@ -144,6 +146,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--interval", type=int, default=0, help="checkpoint interval (0 = auto)")
     ap.add_argument("--no-tune", action="store_true", help="skip the kernel-shape autotuner")
+    ap.add_argument("--nstep", type=int, default=0,
+                    help="override the record length (profiling runs only; not the metric)")
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the "
@@ -158,7 +164,14 @@ def main():
     L = importlib.import_module("adseismic_amd._lib")
     lib = ads.load_library()
     torch.cuda.init()
-    p, c_true, c0, srcs, rcvs = workload(ads, small=args.small)
+    p, c_true, c0, srcs, rcvs = workload(ads, small=args.small, nstep=args.nstep)
+    if args.calib_copy:
+        n = args.calib_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float32)
     srcv = plan.pack_srcv([s.srcv for s in srcs])
     K = args.interval or plan.choose_interval(int(torch.cuda.mem_get_info()[0] * 0.80))

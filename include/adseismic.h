@@ -248,7 +248,7 @@ int ads_acoustic_plan_gradient(ads_acoustic_plan *plan, const void *model, const
                                void *traces, void *workspace, size_t ws_bytes, int64_t K,
                                ads_stream_t stream);
 
-/* Kernel-shape autotuner: times every candidate shape (rows per wavefront 2/4/8 x shots per
+/* Kernel-shape autotuner: times every candidate shape (rows per wavefront 1/2/4 x shots per
  * workgroup 1..32) of the forward and adjoint stencil kernels on this plan's own grid and
  * shots for `reps` steps each, keeps the fastest in the plan, and writes it to shape_out[4] =
  * {fwd rows, fwd shots, adj rows, adj shots} (may be NULL).  Synchronises the stream; leaves

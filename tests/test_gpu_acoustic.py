@@ -258,11 +258,11 @@ def test_kernel_shapes_and_autotune_agree(orc, ads, cuda, kernel):
     plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float64, nstep=nstep)
     srcv = plan.pack_srcv([s.srcv for s in srcs])
     ct, ob = T(c), plan.pack_traces(obs)
-    for shape in [(2, 1, 2, 1), (4, 2, 8, 3), (8, 5, 4, 2), (2, 32, 2, 32)]:
+    for shape in [(2, 1, 2, 1), (4, 2, 1, 3), (1, 5, 4, 2), (2, 32, 2, 32)]:
         plan.set_shape(shape)
         _, g, _ = plan.gradient(ct, srcv, ob, 6)
         assert rel(g.cpu(), g_ref) < 1e-10, (shape, rel(g.cpu(), g_ref))
     shape = plan.autotune(ct, srcv, 6, reps=1)
-    assert len(shape) == 4 and shape[0] in (2, 4, 8) and shape[2] in (2, 4, 8)
+    assert len(shape) == 4 and shape[0] in (1, 2, 4) and shape[2] in (1, 2, 4)
     _, g, _ = plan.gradient(ct, srcv, ob, 6)
     assert rel(g.cpu(), g_ref) < 1e-10
