@@ -225,7 +225,7 @@ class AcousticPlan:
         kernels on this plan (ads_acoustic_plan_autotune).  Returns the chosen shape."""
         model = _dev(model, self.dtype)
         ws = self.workspace(K)
-        out = (C.c_int * 4)()
+        out = (C.c_int * 6)()
         _lib.check(self.lib.ads_acoustic_plan_autotune(self.h, _vp(model), _vp(srcv), _vp(ws),
                                                        ws.numel(), int(K), int(reps), out,
                                                        _stream()))
@@ -233,7 +233,9 @@ class AcousticPlan:
         return self.shape
 
     def set_shape(self, shape):
-        arr = (C.c_int * 4)(*[int(x) for x in shape])
+        if len(shape) != 6:
+            raise ValueError("shape is (fwd rows, fwd shots, fwd batch, adj rows, adj shots, adj batch)")
+        arr = (C.c_int * 6)(*[int(x) for x in shape])
         _lib.check(self.lib.ads_acoustic_plan_set_shape(self.h, arr))
         self.shape = tuple(shape)
 

@@ -19,13 +19,15 @@
 // where Cp = C (t - s), Cq = C (s - t) and F in the dL/du^n line is F^{n+1} (kernel 1) or F^n
 // (kernel 0).  Receiver residuals R^n are added after the stencil pass by a per-cell gather.
 //
-// Kernel shape.  A workgroup is 4 wavefronts; each wavefront owns 64 consecutive depth cells
-// (j) of RPT consecutive rows (i) and marches down them, keeping rows i-1, i, i+1 in registers
-// (each row is loaded once); j-1/j+1 come from neighbouring lanes by shuffle (edge lanes load
-// them).  A workgroup loops over a group of Sg shots: the model rows it needs stay in registers
-// across the group and, in the adjoint, dL/dC is accumulated in registers and written once per
-// group.  Rows are padded to a 64-element pitch so every wavefront access is 256-byte aligned,
-// and workgroup ids are swizzled so that each of the 8 XCDs walks a contiguous band of tiles.
+// Kernel shape.  A workgroup is 4 wavefronts; each lane owns VEC consecutive depth cells (j)
+// (one 16-byte vector per row in the 64-element-pitched plan layout) of RPT consecutive rows
+// (i).  j-1/j+1 come from neighbouring lanes by shuffle; the two edge lanes load theirs.  A
+// workgroup loops over a group of Sg shots, SB at a time, issuing every load of those SB shots
+// before the first store; the model rows stay in registers across the group and, in the
+// adjoint, dL/dC is accumulated in registers and written once per group.  PML work is
+// lane-masked: only lanes with a cell near the PML read phi/psi (F/G), with vector loads issued
+// together with the others, and only wavefronts with such a lane evaluate the PML formula.
+// Workgroup ids are swizzled so that each of the 8 XCDs walks a contiguous band of tiles.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -133,91 +135,181 @@ template <typename T, int VEC> __device__ inline void vzero(T (&a)[VEC])
     for (int e = 0; e < VEC; e++) a[e] = T(0);
 }
 
-// All rows a wavefront needs for one shot are loaded before any store is issued, so the
-// loads of a shot overlap (the stores may alias the inputs as far as the compiler knows).
-template <typename T, int V, int RPT, int VEC>
-__global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd3(const FwdArgs<T> a)
+// The grid is split once per plan into a rectangle of interior cells none of which needs the
+// PML formula (rows [iL, iR) x columns [jL, jR); jL, jR multiples of the vector width) and the
+// frame around it (PML strips, the cells next to them and the padding ring).  The rectangle
+// runs the plain scheme with nothing but u in registers; the frame runs the full PML scheme.
+struct Rect {
+    int iL, iR, jL, jR;
+};
+// Frame work item of one wavefront: rows ib .. ib+nr-1, columns js .. je-1 (<= 64 columns).
+constexpr int FRPT = 2;  // rows per frame item
+
+// west/east neighbours of a lane's vector: from the adjacent lanes or, on the segment's edge
+// lanes, from their own loads (el: lane 0; er: the last lane of the segment)
+template <typename T, int VEC>
+__device__ inline void lr_of(const T (&x)[VEC], T el, T er, int lane, bool re, T &l, T &r)
+{
+    l = __shfl_up(x[VEC - 1], 1);
+    r = __shfl_down(x[0], 1);
+    if (lane == 0) l = el;
+    if (re) r = er;
+}
+
+// Rectangle forward: u^n = 2 w - wold + C lap(w).  A wavefront owns RPT rows x (64 lanes x VEC
+// depth cells); a workgroup loops over a group of Sg shots, SB at a time, issuing every load of
+// those SB shots before the first store, so each wavefront keeps SB shots' rows in flight.
+template <typename T, int RPT, int VEC, int SB>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_rect(const FwdArgs<T> a, const Rect R)
 {
     int g, strip, tj;
     if (!decode_tile(a.map, g, strip, tj)) return;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
-    const int j0 = (tj * WAVE + lane) * VEC;
-    const int ib = (strip * NWAVES + wv) * RPT;
-    const int NX = a.NX, NY = a.NY, P = a.P;
-    if (ib > NX + 1) return;  // wave-uniform
-    // VEC > 1 only in the pitched layout, where the whole vector lies inside the row
-    const bool vin = VEC == 1 ? j0 < NY + 2 : j0 < P;
+    const int j0 = R.jL + (tj * WAVE + lane) * VEC;
+    const int ib = R.iL + (strip * NWAVES + wv) * RPT;
+    if (ib >= R.iR) return;  // wave-uniform
+    const int P = a.P, nr = min(RPT, R.iR - ib);
+    const bool act = j0 < R.jR;
+    const bool le = lane == 0 && act;                                // reads j0-1 >= jL-1 >= 0
+    const bool re = act && (lane == WAVE - 1 || j0 + VEC == R.jR);  // reads j0+VEC <= jR <= NY+1
     const AcCoef<T> k = a.k;
-    T Cr[RPT][VEC], tyv[VEC];
-    bool nyv[VEC];
+    T Cr[RPT][VEC];
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
-        if (vin && ib + r <= NX + 1) vld(Cr[r], a.C + (int64_t)(ib + r) * P + j0);
+        if (act && r < nr) vld(Cr[r], a.C + (int64_t)(ib + r) * P + j0);
         else vzero(Cr[r]);
     }
+    const int s1 = min(a.act, (g + 1) * a.Sg);
+    for (int s = g * a.Sg; s < s1; s += SB) {
+        T W[SB][RPT + 2][VEC], WO[SB][RPT][VEC], EL[SB][RPT], ER[SB][RPT];
 #pragma unroll
-    for (int e = 0; e < VEC; e++) {
-        const bool ok = j0 + e < NY + 2;
-        tyv[e] = ok ? a.ty[j0 + e] : T(0);
-        nyv[e] = ok ? a.neary[j0 + e] != 0 : false;
+        for (int b = 0; b < SB; b++) {
+            if (s + b >= s1) break;  // uniform
+            const T *__restrict__ w = a.w.at(s + b);
+            const T *__restrict__ wo = a.wold.at(s + b);
+#pragma unroll
+            for (int q = 0; q < RPT + 2; q++) {  // rows iL-1 .. iR lie inside [0, NX+1]
+                if (act && q <= nr + 1) vld(W[b][q], w + (int64_t)(ib - 1 + q) * P + j0);
+                else vzero(W[b][q]);
+            }
+#pragma unroll
+            for (int r = 0; r < RPT; r++) {
+                const int64_t row = (int64_t)(ib + r) * P;
+                if (act && r < nr) vld(WO[b][r], wo + row + j0);
+                else vzero(WO[b][r]);
+                EL[b][r] = le && r < nr ? w[row + j0 - 1] : T(0);
+                ER[b][r] = re && r < nr ? w[row + j0 + VEC] : T(0);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < SB; b++) {
+            if (s + b >= s1) break;
+            T *__restrict__ u = a.u.at(s + b);
+#pragma unroll
+            for (int r = 0; r < RPT; r++) {
+                if (r >= nr) break;  // uniform
+                T wl, wr;
+                lr_of(W[b][r + 1], EL[b][r], ER[b][r], lane, re, wl, wr);
+                T out[VEC];
+#pragma unroll
+                for (int e = 0; e < VEC; e++) {
+                    const T wc = W[b][r + 1][e];
+                    const T ww = e == 0 ? wl : W[b][r + 1][e - 1];
+                    const T we = e == VEC - 1 ? wr : W[b][r + 1][e + 1];
+                    const T lap = k.cx * (W[b][r + 2][e] + W[b][r][e] - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
+                    out[e] = T(2) * wc - WO[b][r][e] + Cr[r][e] * lap;
+                }
+                if (act) vst(u + (int64_t)(ib + r) * P + j0, out);
+            }
+        }
     }
+}
+
+// Frame forward, one depth cell per lane.  Only lanes with a cell near the PML read phi/psi;
+// cells of the padding ring get u = 0 (sources are added afterwards).
+template <typename T, int V>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_frame(const FwdArgs<T> a,
+                                                              const int4 *__restrict__ items,
+                                                              int nitems)
+{
+    int g, blk, tj;
+    if (!decode_tile(a.map, g, blk, tj)) return;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int it = blk * NWAVES + wv;
+    if (it >= nitems) return;  // wave-uniform
+    const int4 m = items[it];
+    const int ib = m.x, nr = m.y, j = m.z + lane;
+    const int NX = a.NX, NY = a.NY, P = a.P;
+    const bool act = j < m.w;
+    const bool le = lane == 0 && act && j >= 1;
+    const bool re = act && (lane == WAVE - 1 || j + 1 == m.w);
+    const bool reok = re && j + 1 <= NY + 1;
+    const AcCoef<T> k = a.k;
+    const bool jin = j >= 1 && j <= NY;
+    const T tu = act ? a.ty[j] : T(0);
+    const bool ny = act && a.neary[j] != 0;
+    bool rin[FRPT], nx[FRPT];
+    T sg[FRPT], Cr[FRPT];
+    bool lp = false;
+#pragma unroll
+    for (int r = 0; r < FRPT; r++) {
+        const int i = ib + r;
+        const bool ok = r < nr;
+        rin[r] = ok && i >= 1 && i <= NX;
+        nx[r] = ok && a.nearx[i] != 0;
+        sg[r] = ok ? a.sx[i] : T(0);
+        Cr[r] = act && rin[r] ? a.C[(int64_t)i * P + j] : T(0);
+        lp |= rin[r] && jin && (nx[r] || ny);
+    }
+    lp = lp && act;
     const int s1 = min(a.act, (g + 1) * a.Sg);
     for (int s = g * a.Sg; s < s1; s++) {
         const T *__restrict__ w = a.w.at(s);
         const T *__restrict__ wo = a.wold.at(s);
-        T *__restrict__ u = a.u.at(s);
-        T W[RPT + 2][VEC], WO[RPT][VEC];
+        const T *__restrict__ phi = a.phi.at(s);
+        const T *__restrict__ psi = a.psi.at(s);
+        T W[FRPT + 2], WO[FRPT], PH[FRPT + 2], PS[FRPT], EL[FRPT], ER[FRPT], SL[FRPT], SR[FRPT];
 #pragma unroll
-        for (int q = 0; q < RPT + 2; q++) {
+        for (int q = 0; q < FRPT + 2; q++) {
             const int i = ib - 1 + q;
-            if (vin && i >= 0 && i <= NX + 1) vld(W[q], w + (int64_t)i * P + j0);
-            else vzero(W[q]);
+            const bool ok = act && q <= nr + 1 && i >= 0 && i <= NX + 1;
+            W[q] = ok ? w[(int64_t)i * P + j] : T(0);
+            PH[q] = ok && lp ? phi[(int64_t)i * P + j] : T(0);
         }
 #pragma unroll
-        for (int r = 0; r < RPT; r++) {
-            const int i = ib + r;
-            if (vin && i >= 1 && i <= NX) vld(WO[r], wo + (int64_t)i * P + j0);
-            else vzero(WO[r]);
+        for (int r = 0; r < FRPT; r++) {
+            const int64_t row = (int64_t)(ib + r) * P;
+            WO[r] = act && rin[r] ? wo[row + j] : T(0);
+            PS[r] = lp && rin[r] ? psi[row + j] : T(0);
+            EL[r] = le && rin[r] ? w[row + j - 1] : T(0);
+            ER[r] = reok && rin[r] ? w[row + j + 1] : T(0);
+            SL[r] = le && lp && rin[r] ? psi[row + j - 1] : T(0);
+            SR[r] = reok && lp && rin[r] ? psi[row + j + 1] : T(0);
         }
+        T *__restrict__ u = a.u.at(s);
 #pragma unroll
-        for (int r = 0; r < RPT; r++) {
-            const int i = ib + r;
-            if (i > NX + 1) break;  // wave-uniform
-            const int64_t row = (int64_t)i * P;
-            const bool rowin = i >= 1 && i <= NX;
-            T wl = __shfl_up(W[r + 1][VEC - 1], 1), wr = __shfl_down(W[r + 1][0], 1);
-            if (lane == 0 && rowin && j0 >= 1 && j0 <= NY + 1) wl = w[row + j0 - 1];
-            if (lane == WAVE - 1 && rowin && j0 + VEC <= NY + 1) wr = w[row + j0 + VEC];
-            const bool nx = a.nearx[i] != 0;
-            T out[VEC];
-#pragma unroll
-            for (int e = 0; e < VEC; e++) {
-                const int j = j0 + e;
-                const T wc = W[r + 1][e], wn = W[r][e], ws = W[r + 2][e];
-                const T ww = e == 0 ? wl : W[r + 1][e - 1];
-                const T we = e == VEC - 1 ? wr : W[r + 1][e + 1];
-                out[e] = T(0);
-                if (!(rowin && j >= 1 && j <= NY)) continue;
-                const T Cij = Cr[r][e];
-                const T lap = k.cx * (ws + wn - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
-                if (!(nx | nyv[e])) {
-                    out[e] = T(2) * wc - WO[r][e] + Cij * lap;
-                } else {
-                    const int64_t ij = row + j;
-                    const T sg = a.sx[i], tu = tyv[e];
-                    const T *__restrict__ phi = a.phi.at(s);
-                    const T *__restrict__ psi = a.psi.at(s);
-                    const T num = (T(2) - sg * tu * k.dt2) * wc + Cij * lap +
-                                  k.ex * (phi[ij + P] - phi[ij - P]) +
-                                  k.ey * (psi[ij + 1] - psi[ij - 1]) - (T(1) - (sg + tu) * k.hdt) * WO[r][e];
-                    out[e] = num / (T(1) + (sg + tu) * k.hdt);
-                    if (V == 1 && (sg != T(0) || tu != T(0))) {
-                        a.phio.at(s)[ij] = (T(1) - k.dt * sg) * phi[ij] + Cij * (tu - sg) * k.fx * (ws - wn);
-                        a.psio.at(s)[ij] = (T(1) - k.dt * tu) * psi[ij] + Cij * (sg - tu) * k.fy * (we - ww);
-                    }
-                }
+        for (int r = 0; r < FRPT; r++) {
+            if (r >= nr) break;  // uniform
+            const int64_t row = (int64_t)(ib + r) * P;
+            T wv1[1] = {W[r + 1]}, pv1[1] = {PS[r]};
+            T ww, we, pw, pe;
+            lr_of(wv1, EL[r], ER[r], lane, re, ww, we);
+            lr_of(pv1, SL[r], SR[r], lane, re, pw, pe);
+            const T wc = W[r + 1], wn = W[r], ws = W[r + 2];
+            const T lap = k.cx * (ws + wn - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
+            const T s_ = sg[r], Cij = Cr[r];
+            const bool in = rin[r] && jin;
+            const bool near = nx[r] || ny;
+            const T plain = T(2) * wc - WO[r] + Cij * lap;
+            const T num = (T(2) - s_ * tu * k.dt2) * wc + Cij * lap + k.ex * (PH[r + 2] - PH[r]) +
+                          k.ey * (pe - pw) - (T(1) - (s_ + tu) * k.hdt) * WO[r];
+            const T out = in ? (near ? num / (T(1) + (s_ + tu) * k.hdt) : plain) : T(0);
+            if (act) u[row + j] = out;
+            if (V == 1 && lp) {
+                const bool pml = in && (s_ != T(0) || tu != T(0));
+                a.phio.at(s)[row + j] = pml ? (T(1) - k.dt * s_) * PH[r + 1] + Cij * (tu - s_) * k.fx * (ws - wn) : T(0);
+                a.psio.at(s)[row + j] = pml ? (T(1) - k.dt * tu) * PS[r] + Cij * (s_ - tu) * k.fy * (we - ww) : T(0);
             }
-            if (vin) vst(u + row + j0, out);
         }
     }
 }
@@ -334,146 +426,259 @@ template <typename T> struct AdjArgs {
     AcCoef<T> k;
 };
 
-template <typename T, int V, int RPT, int VEC>
-__global__ __launch_bounds__(WAVE *NWAVES) void ac_adj3(const AdjArgs<T> a)
+// Rectangle adjoint: P^n = 2 P^{n+1} + lap(C P^{n+1}) - P^{n+2} (+ the dense cotangent of u^n
+// in the full-history mode), dL/dC += P^n lap(u^{n-1}); no PML term reaches these cells.
+template <typename T, int RPT, int VEC, int SB>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_rect(const AdjArgs<T> a, const Rect R)
 {
+    constexpr bool DENSE = VEC == 1;  // the dense cotangent only exists in the history layout
     int g, strip, tj;
     if (!decode_tile(a.map, g, strip, tj)) return;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
-    const int j0 = (tj * WAVE + lane) * VEC;
-    const int ib = (strip * NWAVES + wv) * RPT;
-    const int NX = a.NX, NY = a.NY, P = a.P;
-    if (ib > NX + 1) return;
-    const bool vin = VEC == 1 ? j0 < NY + 2 : j0 < P;
+    const int j0 = R.jL + (tj * WAVE + lane) * VEC;
+    const int ib = R.iL + (strip * NWAVES + wv) * RPT;
+    if (ib >= R.iR) return;  // wave-uniform
+    const int P = a.P, nr = min(RPT, R.iR - ib);
+    const bool act = j0 < R.jR;
+    const bool le = lane == 0 && act;
+    const bool re = act && (lane == WAVE - 1 || j0 + VEC == R.jR);
+    const bool dense = DENSE && a.dense.p != nullptr;
     const AcCoef<T> k = a.k;
-    // model rows ib-1 .. ib+RPT stay in registers across the shot group
-    T Cr[RPT + 2][VEC], tyv[VEC];
-    bool nyv[VEC];
+    T Cr[RPT + 2][VEC], cl[RPT], cr[RPT];
 #pragma unroll
     for (int q = 0; q < RPT + 2; q++) {
-        const int i = ib - 1 + q;
-        if (vin && i >= 0 && i <= NX + 1) vld(Cr[q], a.C + (int64_t)i * P + j0);
+        if (act && q <= nr + 1) vld(Cr[q], a.C + (int64_t)(ib - 1 + q) * P + j0);
         else vzero(Cr[q]);
     }
 #pragma unroll
-    for (int e = 0; e < VEC; e++) {
-        const bool ok = j0 + e < NY + 2;
-        tyv[e] = ok ? a.ty[j0 + e] : T(0);
-        nyv[e] = ok ? a.neary[j0 + e] != 0 : false;
-    }
-    // model values one cell west/east of the vector, per row
-    T cl[RPT], cr[RPT];
-#pragma unroll
     for (int r = 0; r < RPT; r++) {
-        const int i = ib + r;
-        const bool rowin = i >= 1 && i <= NX;
-        cl[r] = __shfl_up(Cr[r + 1][VEC - 1], 1);
-        cr[r] = __shfl_down(Cr[r + 1][0], 1);
-        if (lane == 0 && rowin && j0 >= 1 && j0 <= NY + 1) cl[r] = a.C[(int64_t)i * P + j0 - 1];
-        if (lane == WAVE - 1 && rowin && j0 + VEC <= NY + 1) cr[r] = a.C[(int64_t)i * P + j0 + VEC];
+        const int64_t row = (int64_t)(ib + r) * P;
+        const T el = le && r < nr ? a.C[row + j0 - 1] : T(0);
+        const T er = re && r < nr ? a.C[row + j0 + VEC] : T(0);
+        lr_of(Cr[r + 1], el, er, lane, re, cl[r], cr[r]);
     }
     T gacc[RPT][VEC];
 #pragma unroll
     for (int r = 0; r < RPT; r++) vzero(gacc[r]);
     const int s1 = min(a.act, (g + 1) * a.Sg);
-    for (int s = g * a.Sg; s < s1; s++) {
-        const T *__restrict__ P1 = a.P1.at(s);
-        const T *__restrict__ um = a.um1.at(s);
-        T *__restrict__ Pout = a.Pn.at(s);
-        T P1r[RPT + 2][VEC], Ur[RPT + 2][VEC], P2r[RPT][VEC];
+    for (int s = g * a.Sg; s < s1; s += SB) {
+        T P1[SB][RPT + 2][VEC], U[SB][RPT + 2][VEC], P2[SB][RPT][VEC], DN[SB][RPT][VEC];
+        T PL[SB][RPT], PR[SB][RPT], UL[SB][RPT], UR[SB][RPT];
 #pragma unroll
-        for (int q = 0; q < RPT + 2; q++) {
-            const int i = ib - 1 + q;
-            if (vin && i >= 0 && i <= NX + 1) {
-                vld(P1r[q], P1 + (int64_t)i * P + j0);
-                vld(Ur[q], um + (int64_t)i * P + j0);
-            } else {
-                vzero(P1r[q]);
-                vzero(Ur[q]);
-            }
-        }
+        for (int b = 0; b < SB; b++) {
+            if (s + b >= s1) break;  // uniform
+            const T *__restrict__ p1 = a.P1.at(s + b);
+            const T *__restrict__ um = a.um1.at(s + b);
+            const T *__restrict__ p2 = a.Pn.at(s + b);
 #pragma unroll
-        for (int r = 0; r < RPT; r++) {
-            const int i = ib + r;
-            if (vin && i >= 1 && i <= NX) vld(P2r[r], Pout + (int64_t)i * P + j0);
-            else vzero(P2r[r]);
-        }
-#pragma unroll
-        for (int r = 0; r < RPT; r++) {
-            const int i = ib + r;
-            if (i > NX + 1) break;
-            const int64_t row = (int64_t)i * P;
-            const bool rowin = i >= 1 && i <= NX;
-            T pl = __shfl_up(P1r[r + 1][VEC - 1], 1), pr = __shfl_down(P1r[r + 1][0], 1);
-            T ul = __shfl_up(Ur[r + 1][VEC - 1], 1), ur = __shfl_down(Ur[r + 1][0], 1);
-            if (lane == 0 && rowin && j0 >= 1 && j0 <= NY + 1) {
-                pl = P1[row + j0 - 1];
-                ul = um[row + j0 - 1];
-            }
-            if (lane == WAVE - 1 && rowin && j0 + VEC <= NY + 1) {
-                pr = P1[row + j0 + VEC];
-                ur = um[row + j0 + VEC];
-            }
-            const bool nx = a.nearx[i] != 0;
-            T out[VEC];
-#pragma unroll
-            for (int e = 0; e < VEC; e++) {
-                const int j = j0 + e;
-                out[e] = T(0);
-                if (!(rowin && j >= 1 && j <= NY)) continue;
-                const T p1c = P1r[r + 1][e], p1n = P1r[r][e], p1s = P1r[r + 2][e];
-                const T p1w = e == 0 ? pl : P1r[r + 1][e - 1];
-                const T p1e = e == VEC - 1 ? pr : P1r[r + 1][e + 1];
-                const T Cc = Cr[r + 1][e], Cn = Cr[r][e], Cs = Cr[r + 2][e];
-                const T Cw = e == 0 ? cl[r] : Cr[r + 1][e - 1];
-                const T Ce = e == VEC - 1 ? cr[r] : Cr[r + 1][e + 1];
-                const T uc = Ur[r + 1][e], un = Ur[r][e], us = Ur[r + 2][e];
-                const T uw = e == 0 ? ul : Ur[r + 1][e - 1];
-                const T ue = e == VEC - 1 ? ur : Ur[r + 1][e + 1];
-                const T qc = Cc * p1c;
-                const T lapq = k.cx * (Cs * p1s + Cn * p1n - T(2) * qc) + k.cy * (Ce * p1e + Cw * p1w - T(2) * qc);
-                const T p2 = P2r[r][e];
-                const bool near = nx | nyv[e];
-                const T tu = tyv[e];
-                T sg = T(0), U;
-                const int64_t ij = row + j;
-                if (!near) {
-                    U = T(2) * p1c + lapq - p2;
+            for (int q = 0; q < RPT + 2; q++) {
+                const int64_t row = (int64_t)(ib - 1 + q) * P;
+                if (act && q <= nr + 1) {
+                    vld(P1[b][q], p1 + row + j0);
+                    vld(U[b][q], um + row + j0);
                 } else {
-                    sg = a.sx[i];
-                    const T *__restrict__ Fn = a.Fn.at(s);
-                    const T *__restrict__ Gn = a.Gn.at(s);
-                    U = (T(2) - sg * tu * k.dt2) * p1c + lapq - (T(1) - (sg + tu) * k.hdt) * p2 +
-                        k.fx * (Cn * (tu - a.sx[i - 1]) * Fn[ij - P] - Cs * (tu - a.sx[i + 1]) * Fn[ij + P]) +
-                        k.fy * (Cw * (sg - a.ty[j - 1]) * Gn[ij - 1] - Ce * (sg - a.ty[j + 1]) * Gn[ij + 1]);
+                    vzero(P1[b][q]);
+                    vzero(U[b][q]);
                 }
-                if (a.dense.p) U += a.dense.at(s)[ij];
-                const T Pv = near ? U / (T(1) + (sg + tu) * k.hdt) : U;
-                out[e] = Pv;
-                T gg = Pv * (k.cx * (us + un - T(2) * uc) + k.cy * (ue + uw - T(2) * uc));
-                if (V == 1 && near && (sg != T(0) || tu != T(0))) {
-                    const T F0 = (T(1) - k.dt * sg) * a.Fn.at(s)[ij] + k.ex * (p1n - p1s);
-                    const T G0 = (T(1) - k.dt * tu) * a.Gn.at(s)[ij] + k.ey * (p1w - p1e);
-                    a.F0.at(s)[ij] = F0;
-                    a.G0.at(s)[ij] = G0;
-                    gg += F0 * (tu - sg) * k.fx * (us - un) + G0 * (sg - tu) * k.fy * (ue - uw);
-                }
-                gacc[r][e] += gg;
             }
-            if (vin) vst(Pout + row + j0, out);
+#pragma unroll
+            for (int r = 0; r < RPT; r++) {
+                const int64_t row = (int64_t)(ib + r) * P;
+                const bool ok = act && r < nr;
+                if (ok) vld(P2[b][r], p2 + row + j0);
+                else vzero(P2[b][r]);
+                if (dense) {
+                    if (ok) vld(DN[b][r], a.dense.at(s + b) + row + j0);
+                    else vzero(DN[b][r]);
+                }
+                PL[b][r] = le && r < nr ? p1[row + j0 - 1] : T(0);
+                PR[b][r] = re && r < nr ? p1[row + j0 + VEC] : T(0);
+                UL[b][r] = le && r < nr ? um[row + j0 - 1] : T(0);
+                UR[b][r] = re && r < nr ? um[row + j0 + VEC] : T(0);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < SB; b++) {
+            if (s + b >= s1) break;
+            T *__restrict__ Pout = a.Pn.at(s + b);
+#pragma unroll
+            for (int r = 0; r < RPT; r++) {
+                if (r >= nr) break;  // uniform
+                T pl, pr, ul, ur;
+                lr_of(P1[b][r + 1], PL[b][r], PR[b][r], lane, re, pl, pr);
+                lr_of(U[b][r + 1], UL[b][r], UR[b][r], lane, re, ul, ur);
+                T out[VEC];
+#pragma unroll
+                for (int e = 0; e < VEC; e++) {
+                    const T p1c = P1[b][r + 1][e];
+                    const T p1w = e == 0 ? pl : P1[b][r + 1][e - 1];
+                    const T p1e = e == VEC - 1 ? pr : P1[b][r + 1][e + 1];
+                    const T Cw = e == 0 ? cl[r] : Cr[r + 1][e - 1];
+                    const T Ce = e == VEC - 1 ? cr[r] : Cr[r + 1][e + 1];
+                    const T qc = Cr[r + 1][e] * p1c;
+                    const T lapq = k.cx * (Cr[r + 2][e] * P1[b][r + 2][e] + Cr[r][e] * P1[b][r][e] - T(2) * qc) +
+                                   k.cy * (Ce * p1e + Cw * p1w - T(2) * qc);
+                    T Pv = T(2) * p1c + lapq - P2[b][r][e];
+                    if (dense) Pv += DN[b][r][e];
+                    out[e] = Pv;
+                    const T uc = U[b][r + 1][e];
+                    const T uw = e == 0 ? ul : U[b][r + 1][e - 1];
+                    const T ue = e == VEC - 1 ? ur : U[b][r + 1][e + 1];
+                    gacc[r][e] += Pv * (k.cx * (U[b][r + 2][e] + U[b][r][e] - T(2) * uc) + k.cy * (ue + uw - T(2) * uc));
+                }
+                if (act) vst(Pout + (int64_t)(ib + r) * P + j0, out);
+            }
         }
     }
-    if (!vin) return;
+    if (!act) return;
     T *__restrict__ gC = a.gC.at(g);
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
-        const int i = ib + r;
-        if (i < 1 || i > NX) continue;
+        if (r >= nr) break;
+        T *p = gC + (int64_t)(ib + r) * P + j0;
         T cur[VEC];
-        vld(cur, gC + (int64_t)i * P + j0);
+        vld(cur, p);
 #pragma unroll
-        for (int e = 0; e < VEC; e++) cur[e] += gacc[r][e];  // zero outside the interior
-        vst(gC + (int64_t)i * P + j0, cur);
+        for (int e = 0; e < VEC; e++) cur[e] += gacc[r][e];
+        vst(p, cur);
+    }
+}
+
+// Frame adjoint, one depth cell per lane: the full PML transpose (F/G read at neighbours,
+// F^n/G^n written for kernel 1 on PML lanes), zero on the padding ring.
+template <typename T, int V>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_frame(const AdjArgs<T> a,
+                                                              const int4 *__restrict__ items,
+                                                              int nitems)
+{
+    int g, blk, tj;
+    if (!decode_tile(a.map, g, blk, tj)) return;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int it = blk * NWAVES + wv;
+    if (it >= nitems) return;  // wave-uniform
+    const int4 m = items[it];
+    const int ib = m.x, nr = m.y, j = m.z + lane;
+    const int NX = a.NX, NY = a.NY, P = a.P;
+    const bool act = j < m.w;
+    const bool le = lane == 0 && act && j >= 1;
+    const bool re = act && (lane == WAVE - 1 || j + 1 == m.w);
+    const bool reok = re && j + 1 <= NY + 1;
+    const AcCoef<T> k = a.k;
+    const bool jin = j >= 1 && j <= NY;
+    const T tu = act ? a.ty[j] : T(0);
+    const T tw = act && j >= 1 ? a.ty[j - 1] : T(0);
+    const T te = act && j + 1 <= NY + 1 ? a.ty[j + 1] : T(0);
+    const bool ny = act && a.neary[j] != 0;
+    bool rin[FRPT], nx[FRPT];
+    T sg[FRPT], sn[FRPT], ss[FRPT], Cr[FRPT + 2], cl[FRPT], cr[FRPT];
+    bool lp = false;
+#pragma unroll
+    for (int q = 0; q < FRPT + 2; q++) {
+        const int i = ib - 1 + q;
+        Cr[q] = act && q <= nr + 1 && i >= 0 && i <= NX + 1 ? a.C[(int64_t)i * P + j] : T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < FRPT; r++) {
+        const int i = ib + r;
+        const bool ok = r < nr;
+        rin[r] = ok && i >= 1 && i <= NX;
+        nx[r] = ok && a.nearx[i] != 0;
+        sg[r] = ok ? a.sx[i] : T(0);
+        sn[r] = rin[r] ? a.sx[i - 1] : T(0);
+        ss[r] = rin[r] ? a.sx[i + 1] : T(0);
+        lp |= rin[r] && jin && (nx[r] || ny);
+        const T el = le && rin[r] ? a.C[(int64_t)i * P + j - 1] : T(0);
+        const T er = reok && rin[r] ? a.C[(int64_t)i * P + j + 1] : T(0);
+        T c1[1] = {Cr[r + 1]};
+        lr_of(c1, el, er, lane, re, cl[r], cr[r]);
+    }
+    lp = lp && act;
+    const bool dense = a.dense.p != nullptr;
+    T gacc[FRPT] = {};
+    const int s1 = min(a.act, (g + 1) * a.Sg);
+    for (int s = g * a.Sg; s < s1; s++) {
+        const T *__restrict__ p1 = a.P1.at(s);
+        const T *__restrict__ um = a.um1.at(s);
+        const T *__restrict__ Fn = a.Fn.at(s);
+        const T *__restrict__ Gn = a.Gn.at(s);
+        T *__restrict__ Pout = a.Pn.at(s);
+        T P1[FRPT + 2], U[FRPT + 2], F[FRPT + 2], P2[FRPT], DN[FRPT], G[FRPT];
+        T PL[FRPT], PR[FRPT], UL[FRPT], UR[FRPT], GL[FRPT], GR[FRPT];
+#pragma unroll
+        for (int q = 0; q < FRPT + 2; q++) {
+            const int i = ib - 1 + q;
+            const bool ok = act && q <= nr + 1 && i >= 0 && i <= NX + 1;
+            const int64_t ij = (int64_t)i * P + j;
+            P1[q] = ok ? p1[ij] : T(0);
+            U[q] = ok ? um[ij] : T(0);
+            F[q] = ok && lp ? Fn[ij] : T(0);
+        }
+#pragma unroll
+        for (int r = 0; r < FRPT; r++) {
+            const int64_t ij = (int64_t)(ib + r) * P + j;
+            const bool ok = act && r < nr;
+            P2[r] = ok ? Pout[ij] : T(0);
+            DN[r] = ok && dense ? a.dense.at(s)[ij] : T(0);
+            G[r] = lp && rin[r] ? Gn[ij] : T(0);
+            PL[r] = le && rin[r] ? p1[ij - 1] : T(0);
+            PR[r] = reok && rin[r] ? p1[ij + 1] : T(0);
+            UL[r] = le && rin[r] ? um[ij - 1] : T(0);
+            UR[r] = reok && rin[r] ? um[ij + 1] : T(0);
+            GL[r] = le && lp && rin[r] ? Gn[ij - 1] : T(0);
+            GR[r] = reok && lp && rin[r] ? Gn[ij + 1] : T(0);
+        }
+#pragma unroll
+        for (int r = 0; r < FRPT; r++) {
+            if (r >= nr) break;  // uniform
+            const int64_t ij = (int64_t)(ib + r) * P + j;
+            T v1[1] = {P1[r + 1]}, v2[1] = {U[r + 1]}, v3[1] = {G[r]};
+            T p1w, p1e, uw, ue, gw, ge;
+            lr_of(v1, PL[r], PR[r], lane, re, p1w, p1e);
+            lr_of(v2, UL[r], UR[r], lane, re, uw, ue);
+            lr_of(v3, GL[r], GR[r], lane, re, gw, ge);
+            const T p1c = P1[r + 1], p1n = P1[r], p1s = P1[r + 2];
+            const T Cc = Cr[r + 1], Cn = Cr[r], Cs = Cr[r + 2], Cw = cl[r], Ce = cr[r];
+            const T uc = U[r + 1], un = U[r], us = U[r + 2];
+            const T qc = Cc * p1c;
+            const T lapq = k.cx * (Cs * p1s + Cn * p1n - T(2) * qc) + k.cy * (Ce * p1e + Cw * p1w - T(2) * qc);
+            const T s_ = sg[r], p2 = P2[r];
+            const bool in = rin[r] && jin;
+            const bool near = nx[r] || ny;
+            T Pv;
+            if (near) {
+                T U_ = (T(2) - s_ * tu * k.dt2) * p1c + lapq - (T(1) - (s_ + tu) * k.hdt) * p2 +
+                       k.fx * (Cn * (tu - sn[r]) * F[r] - Cs * (tu - ss[r]) * F[r + 2]) +
+                       k.fy * (Cw * (s_ - tw) * gw - Ce * (s_ - te) * ge);
+                if (dense) U_ += DN[r];
+                Pv = U_ / (T(1) + (s_ + tu) * k.hdt);
+            } else {
+                Pv = T(2) * p1c + lapq - p2;
+                if (dense) Pv += DN[r];
+            }
+            Pv = in ? Pv : T(0);
+            T gg = Pv * (k.cx * (us + un - T(2) * uc) + k.cy * (ue + uw - T(2) * uc));
+            const bool pml = in && (s_ != T(0) || tu != T(0));
+            T fo = T(0), go = T(0);
+            if (V == 1 && pml) {
+                fo = (T(1) - k.dt * s_) * F[r + 1] + k.ex * (p1n - p1s);
+                go = (T(1) - k.dt * tu) * G[r] + k.ey * (p1w - p1e);
+                gg += fo * (tu - s_) * k.fx * (us - un) + go * (s_ - tu) * k.fy * (ue - uw);
+            }
+            if (act) Pout[ij] = Pv;
+            if (V == 1 && lp) {
+                a.F0.at(s)[ij] = fo;
+                a.G0.at(s)[ij] = go;
+            }
+            gacc[r] += in ? gg : T(0);
+        }
+    }
+    if (!act) return;
+    T *__restrict__ gC = a.gC.at(g);
+#pragma unroll
+    for (int r = 0; r < FRPT; r++) {
+        if (r >= nr) break;
+        if (!rin[r] || !jin) continue;
+        gC[(int64_t)(ib + r) * P + j] += gacc[r];
     }
 }
 
@@ -569,7 +774,7 @@ __global__ __launch_bounds__(256) void k_misfit(const T *__restrict__ tr, const 
 // plan
 // ---------------------------------------------------------------------------------------------
 struct Shape {
-    int rpt = 2, sg = 8;
+    int rpt = 2, sg = 8, sb = 1;  // rows per wavefront, shots per workgroup, shots per batch
 };
 
 struct AcPlanBase {
@@ -598,7 +803,55 @@ struct AcPlanBase {
     DevBuf d_gi_ptr, d_gi_eptr, d_gi_shot, d_gi_rcv, d_gi_cell[2];
     int inj_sg = 0;
     int64_t max_gi_cells = 0;
+    // interior rectangle and frame work items, per layout ([0] pitched, [1] dense)
+    Rect rect[2]{};
+    DevBuf d_items[2];
+    int nitems[2] = {0, 0};
 };
+
+// Interior rectangle: the longest run of rows in [1, NX] and of columns in [1, NY] with no near
+// flag, columns rounded inwards to multiples of vec.  Frame items cover every other cell of
+// [0, NX+1] x [0, NY+1] (the padding columns of a pitched row are never touched), FRPT rows by
+// at most 64 columns each, in row order within each band.
+static void split_grid(const std::vector<uint8_t> &nearx, const std::vector<uint8_t> &neary,
+                       int64_t NX, int64_t NY, int vec, Rect &R, std::vector<int4> &items)
+{
+    auto run = [](const std::vector<uint8_t> &f, int64_t n, int64_t &a, int64_t &b) {
+        a = b = 1;
+        for (int64_t q = 1; q <= n;) {
+            if (f[q]) {
+                q++;
+                continue;
+            }
+            int64_t e = q;
+            while (e <= n && !f[e]) e++;
+            if (e - q > b - a) a = q, b = e;
+            q = e;
+        }
+    };
+    int64_t iL, iR, jL, jR;
+    run(nearx, NX, iL, iR);
+    run(neary, NY, jL, jR);
+    jL = (jL + vec - 1) / vec * vec;
+    jR = jR / vec * vec;
+    if (iR <= iL || jR <= jL) iL = iR = jL = jR = 1;
+    R = Rect{(int)iL, (int)iR, (int)jL, (int)jR};
+    items.clear();
+    auto add = [&](int64_t r0, int64_t r1, int64_t c0, int64_t c1) {
+        for (int64_t i = r0; i < r1; i += FRPT)
+            for (int64_t j = c0; j < c1; j += WAVE)
+                items.push_back(make_int4((int)i, (int)std::min<int64_t>(FRPT, r1 - i), (int)j,
+                                          (int)std::min<int64_t>(j + WAVE, c1)));
+    };
+    if (iR <= iL) {
+        add(0, NX + 2, 0, NY + 2);
+        return;
+    }
+    add(0, iL, 0, NY + 2);
+    add(iL, iR, 0, jL);
+    add(iL, iR, jR, NY + 2);
+    add(iR, NX + 2, 0, NY + 2);
+}
 
 static int validate_params(const ads_acoustic_params *p)
 {
@@ -628,46 +881,74 @@ template <typename T> struct AcPlan : AcPlanBase {
     static unsigned grid_of(const TileMap &m) { return cdiv(m.nb, NXCD) * NXCD; }
 
     static constexpr int VECP = sizeof(T) == 4 ? 4 : 2;  // pitched layout: 16-byte vectors
-    TileMap tiles(int P, int vec, int rpt, int act, int sg) const
+    static TileMap rect_tiles(const Rect &R, int vec, int rpt, int act, int sg)
     {
         TileMap m;
-        m.ntj = (int)cdiv(P, WAVE * vec);
-        m.nstrips = (int)cdiv(NX + 2, NWAVES * rpt);
+        m.ntj = (int)cdiv(R.jR - R.jL, WAVE * vec);
+        m.nstrips = (int)cdiv(R.iR - R.iL, NWAVES * rpt);
         m.ngroups = (int)cdiv(act, sg);
         m.nb = m.ntj * m.nstrips * m.ngroups;
         return m;
     }
-    template <int V, int RPT, int VEC> void launch_fwd(FwdArgs<T> &a, hipStream_t st)
+    TileMap frame_tiles(int li, int act, int sg) const
     {
-        a.map = tiles(a.P, VEC, RPT, a.act, a.Sg);
-        ac_fwd3<T, V, RPT, VEC><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a);
+        TileMap m;
+        m.ntj = 1;
+        m.nstrips = (int)cdiv(nitems[li], NWAVES);
+        m.ngroups = (int)cdiv(act, sg);
+        m.nb = m.ntj * m.nstrips * m.ngroups;
+        return m;
     }
-    template <int V, int VEC> void launch_fwd_rpt(FwdArgs<T> &a, int rpt, hipStream_t st)
+    static bool empty(const Rect &R) { return R.iR <= R.iL || R.jR <= R.jL; }
+    // rectangle kernel instances: rows per wavefront 1/2/4, shots per batch 1/2 (2 with 1-2 rows)
+    template <int RPT, int VEC, int SB> void launch_fwd_rect(FwdArgs<T> a, const Rect &R, hipStream_t st)
     {
-        if (rpt == 1) launch_fwd<V, 1, VEC>(a, st);
-        else if (rpt == 4) launch_fwd<V, 4, VEC>(a, st);
-        else launch_fwd<V, 2, VEC>(a, st);
+        a.map = rect_tiles(R, VEC, RPT, a.act, a.Sg);
+        ac_fwd_rect<T, RPT, VEC, SB><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, R);
     }
-    template <int V> void launch_fwd_any(FwdArgs<T> &a, int rpt, bool pitched, hipStream_t st)
+    template <int VEC> void launch_fwd_shape(const FwdArgs<T> &a, const Rect &R, const Shape &sh, hipStream_t st)
     {
-        if (pitched) launch_fwd_rpt<V, VECP>(a, rpt, st);
-        else launch_fwd_rpt<V, 1>(a, rpt, st);
+        if (empty(R)) return;
+        if (sh.sb == 2 && sh.rpt == 1) launch_fwd_rect<1, VEC, 2>(a, R, st);
+        else if (sh.sb == 2 && sh.rpt == 2) launch_fwd_rect<2, VEC, 2>(a, R, st);
+        else if (sh.rpt == 1) launch_fwd_rect<1, VEC, 1>(a, R, st);
+        else if (sh.rpt == 4) launch_fwd_rect<4, VEC, 1>(a, R, st);
+        else launch_fwd_rect<2, VEC, 1>(a, R, st);
     }
-    template <int V, int RPT, int VEC> void launch_adj(AdjArgs<T> &a, hipStream_t st)
+    template <int RPT, int VEC, int SB> void launch_adj_rect(AdjArgs<T> a, const Rect &R, hipStream_t st)
     {
-        a.map = tiles(a.P, VEC, RPT, a.act, a.Sg);
-        ac_adj3<T, V, RPT, VEC><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a);
+        a.map = rect_tiles(R, VEC, RPT, a.act, a.Sg);
+        ac_adj_rect<T, RPT, VEC, SB><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, R);
     }
-    template <int V, int VEC> void launch_adj_rpt(AdjArgs<T> &a, int rpt, hipStream_t st)
+    template <int VEC> void launch_adj_shape(const AdjArgs<T> &a, const Rect &R, const Shape &sh, hipStream_t st)
     {
-        if (rpt == 1) launch_adj<V, 1, VEC>(a, st);
-        else if (rpt == 4) launch_adj<V, 4, VEC>(a, st);
-        else launch_adj<V, 2, VEC>(a, st);
+        if (empty(R)) return;
+        if (sh.sb == 2 && sh.rpt == 1) launch_adj_rect<1, VEC, 2>(a, R, st);
+        else if (sh.sb == 2 && sh.rpt == 2) launch_adj_rect<2, VEC, 2>(a, R, st);
+        else if (sh.rpt == 1) launch_adj_rect<1, VEC, 1>(a, R, st);
+        else if (sh.rpt == 4) launch_adj_rect<4, VEC, 1>(a, R, st);
+        else launch_adj_rect<2, VEC, 1>(a, R, st);
     }
-    template <int V> void launch_adj_any(AdjArgs<T> &a, int rpt, bool pitched, hipStream_t st)
+    // the rectangle, then the frame (disjoint cells, same inputs)
+    void launch_fwd_all(FwdArgs<T> a, const Layout &L, const Shape &sh, hipStream_t st)
     {
-        if (pitched) launch_adj_rpt<V, VECP>(a, rpt, st);
-        else launch_adj_rpt<V, 1>(a, rpt, st);
+        if (L.li == 0) launch_fwd_shape<VECP>(a, rect[0], sh, st);
+        else launch_fwd_shape<1>(a, rect[1], sh, st);
+        if (nitems[L.li] == 0) return;
+        a.map = frame_tiles(L.li, a.act, a.Sg);
+        const int4 *it = d_items[L.li].template as<int4>();
+        if (variant == 1) ac_fwd_frame<T, 1><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, it, nitems[L.li]);
+        else ac_fwd_frame<T, 0><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, it, nitems[L.li]);
+    }
+    void launch_adj_all(AdjArgs<T> a, const Layout &L, const Shape &sh, hipStream_t st)
+    {
+        if (L.li == 0) launch_adj_shape<VECP>(a, rect[0], sh, st);
+        else launch_adj_shape<1>(a, rect[1], sh, st);
+        if (nitems[L.li] == 0) return;
+        a.map = frame_tiles(L.li, a.act, a.Sg);
+        const int4 *it = d_items[L.li].template as<int4>();
+        if (variant == 1) ac_adj_frame<T, 1><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, it, nitems[L.li]);
+        else ac_adj_frame<T, 0><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, it, nitems[L.li]);
     }
 
     int fwd_step(const Layout &L, int n, int act, Lv<T> uo, Lv<const T> w, Lv<const T> wo,
@@ -681,8 +962,7 @@ template <typename T> struct AcPlan : AcPlanBase {
                      d_nearx.as<uint8_t>(), d_neary.as<uint8_t>(), (int)NX, (int)NY, L.P, act,
                      std::min(shape.sg, act), TileMap{}, k};
         int pr = prof_start(ADS_PROF_AC_FWD, st);
-        if (variant == 1) launch_fwd_any<1>(a, shape.rpt, L.li == 0, st);
-        else launch_fwd_any<0>(a, shape.rpt, L.li == 0, st);
+        launch_fwd_all(a, L, shape, st);
         prof_stop(pr, st, cells * sizeof(T) * (4.0 + (variant == 1 ? 4.0 : 2.0) * f_pml));
         if (variant == 0) {
             dim3 g(cdiv(NY + 2, 64), cdiv(NX + 2, 4), act);
@@ -720,10 +1000,8 @@ template <typename T> struct AcPlan : AcPlanBase {
                      um1, dense, gC, C, d_sx.as<T>(), d_ty.as<T>(), d_nearx.as<uint8_t>(),
                      d_neary.as<uint8_t>(), (int)NX, (int)NY, L.P, act, std::min(Sg, act),
                      TileMap{}, k};
-        (void)shape;
         int pr = prof_start(ADS_PROF_AC_ADJ, st);
-        if (variant == 1) launch_adj_any<1>(a, shape.rpt, L.li == 0, st);
-        else launch_adj_any<0>(a, shape.rpt, L.li == 0, st);
+        launch_adj_all(a, L, shape, st);
         prof_stop(pr, st, cells * sizeof(T) * (7.0 + (variant == 1 ? 4.0 : 2.0) * f_pml));
         if (resid && max_gi_cells > 0) {
             if (inj_sg != Sg) ADS_TRY(build_inject_tables(Sg));
@@ -974,7 +1252,7 @@ template <typename T> struct AcPlan : AcPlanBase {
         return ADS_OK;
     }
 
-    // ---- autotuner: time candidate (RPT, Sg) shapes on the plan's own workload ----
+    // ---- autotuner: time candidate (RPT, Sg, SB) shapes on the plan's own workload ----
     int autotune(const void *model, const void *srcv, void *ws, size_t wsb, int64_t K, int reps,
                  hipStream_t st)
     {
@@ -1013,23 +1291,27 @@ template <typename T> struct AcPlan : AcPlanBase {
         float best = 1e30f;
         Shape bf = fwd_shape;
         for (int rp : rpts)
-            for (int sg : sgs) {
-                Shape sh{rp, sg};
-                float ms = timeit([&] { (void)run_forward(w, n0 - 1, n0 + 1 <= NMAX ? n0 + 1 : n0, sv, nullptr, 0, st, &sh); });
-                if (ms < best) best = ms, bf = sh;
-            }
+            for (int sg : sgs)
+                for (int sb = 1; sb <= std::min(2, sg); sb++) {
+                    if (sb == 2 && rp == 4) continue;
+                    Shape sh{rp, sg, sb};
+                    float ms = timeit([&] { (void)run_forward(w, n0 - 1, n0 + 1 <= NMAX ? n0 + 1 : n0, sv, nullptr, 0, st, &sh); });
+                    if (ms < best) best = ms, bf = sh;
+                }
         fwd_shape = bf;
         ADS_TRY(zero_adjoint_state(w, st));
         best = 1e30f;
         Shape ba = adj_shape;
         const Shape keep = adj_shape;
         for (int rp : rpts)
-            for (int sg : sgs) {
-                Shape sh{rp, sg};
-                adj_shape = sh;  // the gC grouping follows the shape being timed
-                float ms = timeit([&] { (void)adj_at(w, std::max<int64_t>(n0, 2), sv, nullptr, st, &sh); });
-                if (ms < best) best = ms, ba = sh;
-            }
+            for (int sg : sgs)
+                for (int sb = 1; sb <= std::min(2, sg); sb++) {
+                    if (sb == 2 && rp == 4) continue;
+                    Shape sh{rp, sg, sb};
+                    adj_shape = sh;  // the gC grouping follows the shape being timed
+                    float ms = timeit([&] { (void)adj_at(w, std::max<int64_t>(n0, 2), sv, nullptr, st, &sh); });
+                    if (ms < best) best = ms, ba = sh;
+                }
         adj_shape = nshots > 0 ? ba : keep;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -1120,8 +1402,8 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
     P->Ndense = (p->NX + 2) * (p->NY + 2);
     P->nshots = g->nshots;
     P->k = make_coef<T>(p->DELTAT, p->DELTAX, p->DELTAY);
-    P->fwd_shape = Shape{2, (int)std::min<int64_t>(4, g->nshots)};
-    P->adj_shape = Shape{2, (int)std::min<int64_t>(8, g->nshots)};
+    P->fwd_shape = Shape{1, (int)std::min<int64_t>(8, g->nshots), 1};
+    P->adj_shape = Shape{1, (int)std::min<int64_t>(8, g->nshots), 1};
     const int64_t ns = g->nshots;
     if (g->src_ptr[0] != 0 || g->rcv_ptr[0] != 0) ADS_FAIL(ADS_ERR_ARG, "src_ptr/rcv_ptr must start at 0");
     std::vector<int64_t> nstep(ns);
@@ -1233,6 +1515,12 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
     }
     ADS_TRY(P->d_sx.upload(sxT));
     ADS_TRY(P->d_ty.upload(tyT));
+    for (int li = 0; li < 2; li++) {
+        std::vector<int4> items;
+        split_grid(P->nearx, P->neary, p->NX, p->NY, li == 0 ? AcPlan<T>::VECP : 1, P->rect[li], items);
+        P->nitems[li] = (int)items.size();
+        ADS_TRY(P->d_items[li].upload(items));
+    }
     ADS_TRY(P->d_pmlx.upload(P->pmlx));
     ADS_TRY(P->d_pmly.upload(P->pmly));
     ADS_TRY(P->d_nearx.upload(P->nearx));
@@ -1378,10 +1666,12 @@ int ads_acoustic_plan_autotune(ads_acoustic_plan *plan, const void *model, const
     return dispatch(plan, [&](auto &P) -> int {
         ADS_TRY(P.autotune(model, srcv, ws, wsb, K, reps, S(stream)));
         if (shape_out) {
-            shape_out[0] = P.fwd_shape.rpt;
-            shape_out[1] = P.fwd_shape.sg;
-            shape_out[2] = P.adj_shape.rpt;
-            shape_out[3] = P.adj_shape.sg;
+            const Shape *sh[2] = {&P.fwd_shape, &P.adj_shape};
+            for (int q = 0; q < 2; q++) {
+                shape_out[3 * q] = sh[q]->rpt;
+                shape_out[3 * q + 1] = sh[q]->sg;
+                shape_out[3 * q + 2] = sh[q]->sb;
+            }
         }
         return ADS_OK;
     });
@@ -1391,13 +1681,16 @@ int ads_acoustic_plan_set_shape(ads_acoustic_plan *plan, const int *shape)
 {
     clear_error();
     if (!shape) ADS_FAIL(ADS_ERR_ARG, "null shape");
-    for (int q = 0; q < 4; q++)
-        if (shape[q] < 1) ADS_FAIL(ADS_ERR_ARG, "shape entries must be >= 1");
-    if ((shape[0] != 1 && shape[0] != 2 && shape[0] != 4) || (shape[2] != 1 && shape[2] != 2 && shape[2] != 4))
-        ADS_FAIL(ADS_ERR_ARG, "rows per wavefront must be 1, 2 or 4");
+    for (int q = 0; q < 2; q++) {
+        const int rpt = shape[3 * q], sg = shape[3 * q + 1], sb = shape[3 * q + 2];
+        if (rpt != 1 && rpt != 2 && rpt != 4) ADS_FAIL(ADS_ERR_ARG, "rows per wavefront must be 1, 2 or 4");
+        if (sg < 1) ADS_FAIL(ADS_ERR_ARG, "shots per workgroup must be >= 1");
+        if (sb != 1 && sb != 2) ADS_FAIL(ADS_ERR_ARG, "shots per batch must be 1 or 2");
+        if (sb == 2 && rpt == 4) ADS_FAIL(ADS_ERR_ARG, "shots per batch 2 needs 1 or 2 rows per wavefront");
+    }
     return dispatch(plan, [&](auto &P) -> int {
-        P.fwd_shape = Shape{shape[0], shape[1]};
-        P.adj_shape = Shape{shape[2], shape[3]};
+        P.fwd_shape = Shape{shape[0], shape[1], shape[2]};
+        P.adj_shape = Shape{shape[3], shape[4], shape[5]};
         return ADS_OK;
     });
 }

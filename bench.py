@@ -146,6 +146,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--interval", type=int, default=0, help="checkpoint interval (0 = auto)")
     ap.add_argument("--no-tune", action="store_true", help="skip the kernel-shape autotuner")
+    ap.add_argument("--shape", default="",
+                    help="fixed kernel shape 'fr,fs,fb,ar,as,ab' (rows, shots per workgroup, shots "
+                         "per batch of the forward and adjoint kernels) instead of the autotuner")
     ap.add_argument("--nstep", type=int, default=0,
                     help="override the record length (profiling runs only; not the metric)")
     ap.add_argument("--calib-copy", type=int, default=0,
@@ -179,7 +182,11 @@ def main():
     c0t = torch.as_tensor(c0, dtype=torch.float32, device="cuda")
     obs = plan.forward(ct, srcv, K).clone()
     torch.cuda.synchronize()
-    shape = plan.autotune(c0t, srcv, K, reps=3) if not args.no_tune else None
+    if args.shape:
+        shape = tuple(int(x) for x in args.shape.split(","))
+        plan.set_shape(shape)
+    else:
+        shape = plan.autotune(c0t, srcv, K, reps=3) if not args.no_tune else None
     for _ in range(args.warmup):
         plan.gradient(c0t, srcv, obs, K)
     torch.cuda.synchronize()
@@ -215,7 +222,8 @@ def main():
                                 "call per gradient"),
                    "PropagatorKernel": p.PropagatorKernel, "checkpoint_interval": K,
                    "kernel_shape": {"fwd_rows": shape[0], "fwd_shots": shape[1],
-                                    "adj_rows": shape[2], "adj_shots": shape[3]} if shape else None,
+                                    "fwd_batch": shape[2], "adj_rows": shape[3],
+                                    "adj_shots": shape[4], "adj_batch": shape[5]} if shape else None,
                    "shots": plan.nshots, "cells": p.NX * p.NY, "nstep": p.NSTEP},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),

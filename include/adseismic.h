@@ -249,10 +249,11 @@ int ads_acoustic_plan_gradient(ads_acoustic_plan *plan, const void *model, const
                                ads_stream_t stream);
 
 /* Kernel-shape autotuner: times every candidate shape (rows per wavefront 1/2/4 x shots per
- * workgroup 1..32) of the forward and adjoint stencil kernels on this plan's own grid and
- * shots for `reps` steps each, keeps the fastest in the plan, and writes it to shape_out[4] =
- * {fwd rows, fwd shots, adj rows, adj shots} (may be NULL).  Synchronises the stream; leaves
- * the workspace contents undefined.  ads_acoustic_plan_set_shape() sets a shape directly. */
+ * workgroup 1..32 x shots per batch 1/2, batch 2 only with 1-2 rows) of the forward and adjoint
+ * stencil kernels on this plan's own grid and shots for `reps` steps each, keeps the fastest in
+ * the plan, and writes it to shape_out[6] = {fwd rows, fwd shots, fwd batch, adj rows, adj
+ * shots, adj batch} (may be NULL).  Synchronises the stream; leaves the workspace contents
+ * undefined.  ads_acoustic_plan_set_shape() sets a shape directly (same 6-entry layout). */
 int ads_acoustic_plan_autotune(ads_acoustic_plan *plan, const void *model, const void *srcv,
                                void *workspace, size_t ws_bytes, int64_t K, int reps,
                                int *shape_out, ads_stream_t stream);

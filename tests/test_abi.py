@@ -74,6 +74,20 @@ def test_plan_rejects_bad_shapes_and_params(ads):
     assert e.value.code == 2
 
 
+def test_set_shape_validates_before_touching_the_plan(ads):
+    """Shapes are checked on the host: rows 1/2/4, shots >= 1, batch 1/2 (2 only with 1-2 rows)."""
+    import ctypes as C
+    import importlib
+    L = importlib.import_module("adseismic_amd._lib")
+    lib = L.load()
+    for bad in [(3, 1, 1, 1, 1, 1), (1, 0, 1, 1, 1, 1), (1, 1, 3, 1, 1, 1), (1, 1, 1, 4, 2, 2)]:
+        rc = lib.ads_acoustic_plan_set_shape(None, (C.c_int * 6)(*bad))
+        assert rc == 1, bad
+        assert b"must" in lib.ads_last_error() or b"needs" in lib.ads_last_error()
+    rc = lib.ads_acoustic_plan_set_shape(None, (C.c_int * 6)(1, 8, 2, 2, 8, 2))
+    assert rc == 1 and b"null plan" in lib.ads_last_error()  # a valid shape still needs a plan
+
+
 def test_cfl_check(ads):
     p = _params(ads, DELTAT=1e-3, DELTAX=10.0, DELTAY=10.0)
     ads.check_cfl(p, 2000.0)  # 0.28

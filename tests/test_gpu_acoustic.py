@@ -124,7 +124,8 @@ def make_case(orc, ads, NX=30, NY=26, NSTEP=80, npml=6, nshots=1, seed=0, uneven
 def oracle_shot(orc, p, c, src, rcv, nstep, cmod=None, obs=None):
     """Oracle traces (and gradient if obs is given) for one shot of record length nstep."""
     NX, NY = p.NX, p.NY
-    Sx, Sy = orc.acoustic_pml(NX, NY, p.DELTAX, p.DELTAY, p.NPOINTS_PML, p.Rcoef, p.vp_ref)
+    Sx, Sy = orc.acoustic_pml(NX, NY, p.DELTAX, p.DELTAY, p.NPOINTS_PML, p.Rcoef, p.vp_ref,
+                              use=(p.USE_PML_XMIN, p.USE_PML_XMAX, p.USE_PML_YMIN, p.USE_PML_YMAX))
     k = 0 if p.PropagatorKernel == 0 else 1
     cc = c if cmod is None else cmod
     u, ph, ps = orc.acoustic_forward(k, NX, NY, nstep, p.DELTAT, p.DELTAX, p.DELTAY, Sx, Sy, cc ** 2,
@@ -165,6 +166,53 @@ def test_plan_forward_and_gradient_f64(orc, ads, cuda, kernel, K):
     tr3 = plan.forward(ct, srcv, K)
     gb = plan.backward(ct, srcv, 2.0 * (tr3 - plan.pack_traces(obs)), K)
     assert rel(gb.cpu(), g_ref) < 1e-10
+
+
+# interior rectangle / PML frame splits: partial PML sides, odd sizes, an interior narrower than
+# one vector, no PML at all, and a PML that covers the whole grid (empty rectangle)
+SPLITS = [
+    dict(NX=29, NY=33, npml=5, use=(False, True, True, False)),
+    dict(NX=21, NY=19, npml=4, use=(True, True, False, False)),
+    dict(NX=18, NY=15, npml=6, use=(True, True, True, True)),
+    dict(NX=17, NY=23, npml=4, use=(False, False, False, False)),
+    dict(NX=12, NY=12, npml=7, use=(True, True, True, True)),
+]
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("split", range(len(SPLITS)))
+def test_plan_grid_splits_f64(orc, ads, cuda, kernel, split):
+    cfg = SPLITS[split]
+    p, c, srcs, rcvs, nstep = make_case(orc, ads, NX=cfg["NX"], NY=cfg["NY"], NSTEP=60,
+                                        npml=cfg["npml"], nshots=3, kernel=kernel, seed=20 + split,
+                                        uneven=True)
+    p.USE_PML_XMIN, p.USE_PML_XMAX, p.USE_PML_YMIN, p.USE_PML_YMAX = cfg["use"]
+    obs, L_ref, g_ref = [], 0.0, np.zeros_like(c)
+    for s in range(3):
+        o = oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s], cmod=c * 1.03)
+        tr, L, g = oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s], obs=o)
+        obs.append(o)
+        L_ref += L
+        g_ref += g
+    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float64, nstep=nstep)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    for shape in [(1, 2, 1, 1, 2, 1), (2, 3, 2, 4, 1, 1)]:
+        plan.set_shape(shape)
+        loss, g, tr = plan.gradient(T(c), srcv, plan.pack_traces(obs), 8, want_traces=True)
+        per = plan.unpack_traces(tr)
+        for s in range(3):
+            ref = oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s])
+            assert rel(per[s].cpu(), ref) < 1e-12, (shape, s)
+        assert loss.item() == pytest.approx(L_ref, rel=1e-10)
+        assert rel(g.cpu(), g_ref) < 1e-10, (shape, rel(g.cpu(), g_ref))
+    # the full-history (dense layout) drop-in on the same split
+    ct = T(c).requires_grad_(True)
+    ap = ads.AcousticPropagatorSolver(p, srcs[0], ct)
+    rr = ads.SimulatedObservation_(ap, rcvs[0])
+    o0 = oracle_shot(orc, p, c, srcs[0], rcvs[0], p.NSTEP, cmod=c * 1.03)
+    torch.sum((rr - T(o0)) ** 2).backward()
+    _, _, g0 = oracle_shot(orc, p, c, srcs[0], rcvs[0], p.NSTEP, obs=o0)
+    assert rel(ct.grad.cpu(), g0) < 1e-10
 
 
 @pytest.mark.parametrize("kernel", [0, 1])
@@ -246,8 +294,8 @@ def test_workspace_too_small_is_an_error(orc, ads, cuda):
 
 @pytest.mark.parametrize("kernel", [0, 1])
 def test_kernel_shapes_and_autotune_agree(orc, ads, cuda, kernel):
-    """Every (rows per wavefront, shots per workgroup) shape, and the autotuned one, gives the
-    oracle's gradient (shot groups change only the order of the dL/dC summation)."""
+    """Every (rows per wavefront, shots per workgroup, shots per batch) shape, and the autotuned
+    one, gives the oracle's gradient (shot groups change only the order of the dL/dC sum)."""
     p, c, srcs, rcvs, nstep = make_case(orc, ads, NX=37, NY=70, NSTEP=50, nshots=5, kernel=kernel,
                                         seed=11, uneven=True)
     obs, g_ref = [], np.zeros_like(c)
@@ -258,11 +306,13 @@ def test_kernel_shapes_and_autotune_agree(orc, ads, cuda, kernel):
     plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float64, nstep=nstep)
     srcv = plan.pack_srcv([s.srcv for s in srcs])
     ct, ob = T(c), plan.pack_traces(obs)
-    for shape in [(2, 1, 2, 1), (4, 2, 1, 3), (1, 5, 4, 2), (2, 32, 2, 32)]:
+    for shape in [(2, 1, 1, 2, 1, 1), (4, 2, 1, 1, 3, 2), (1, 5, 2, 4, 2, 1), (2, 32, 2, 2, 32, 2),
+                  (1, 3, 2, 1, 5, 2)]:
         plan.set_shape(shape)
         _, g, _ = plan.gradient(ct, srcv, ob, 6)
         assert rel(g.cpu(), g_ref) < 1e-10, (shape, rel(g.cpu(), g_ref))
     shape = plan.autotune(ct, srcv, 6, reps=1)
-    assert len(shape) == 4 and shape[0] in (1, 2, 4) and shape[2] in (1, 2, 4)
+    assert len(shape) == 6 and shape[0] in (1, 2, 4) and shape[3] in (1, 2, 4)
+    assert shape[2] in (1, 2) and shape[5] in (1, 2)
     _, g, _ = plan.gradient(ct, srcv, ob, 6)
     assert rel(g.cpu(), g_ref) < 1e-10
