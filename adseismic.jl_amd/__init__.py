@@ -8,11 +8,14 @@ The directory name is not a Python identifier: load it with ``__graft_entry__.lo
 """
 from ._lib import ADSError, load as load_library  # noqa: F401
 from .params import (AcousticPropagatorParams, AcousticReceiver, AcousticSource,  # noqa: F401
-                     ElasticPropagatorParams, ElasticReceiver, ElasticSource, Geometry)
+                     ElasticPropagatorParams, ElasticReceiver, ElasticSource, Geometry,
+                     MPIElasticPropagatorParams, MPIElasticReceiver, MPIElasticSource,
+                     compute_PML_Params_mpi)
 from .acoustic import (AcousticPlan, AcousticPropagator, AcousticPropagatorSolver,  # noqa: F401
                        SimulatedObservation_, acoustic_one_step, acoustic_one_step_customop,
                        acoustic_traces, check_cfl, compute_PML_Params_, getid2)
 from .elastic import (ElasticPlan, ElasticPropagator, ElasticPropagatorSolver,  # noqa: F401
+                      MPIElasticPropagator, MPIElasticPropagatorSolver, MPISimulatedObservation_,
                       SimulatedObservation_elastic, elastic_traces)
 from .elastic import compute_PML_Params as compute_PML_Params_elastic  # noqa: F401
 from .utils import (Gauss, Ricker, compute_lame_parameters, compute_properties,  # noqa: F401

@@ -239,6 +239,10 @@ class AcousticPlan:
         _lib.check(self.lib.ads_acoustic_plan_set_shape(self.h, arr))
         self.shape = tuple(shape)
 
+    def set_graph(self, enable: bool = True):
+        """Replay checkpoint segments as captured hipGraphs (ads_acoustic_plan_set_graph)."""
+        _lib.check(self.lib.ads_acoustic_plan_set_graph(self.h, int(bool(enable))))
+
     def _scratch_buf(self):
         need = int(self.lib.ads_acoustic_plan_history_scratch_size(self.h))
         if self._scratch is None or self._scratch.numel() < need:

@@ -32,10 +32,13 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <tuple>
 #include <vector>
 
 #include "ads_common.h"
+#include "ads_stencil.h"
 
 namespace ads {
 
@@ -66,28 +69,6 @@ template <typename T> struct Lv {
 };
 template <typename T> inline Lv<const T> cst(Lv<T> a) { return Lv<const T>{a.p, a.stride}; }
 
-constexpr int WAVE = 64;
-constexpr int NWAVES = 4;
-constexpr int NXCD = 8;
-
-// tiles: ntj (j) x nstrips (i) x ngroups (shot groups), swizzled over the XCDs
-struct TileMap {
-    int ntj, nstrips, ngroups, nb;
-};
-
-__device__ inline bool decode_tile(const TileMap &m, int &g, int &strip, int &tj)
-{
-    const int nb8 = (m.nb + NXCD - 1) / NXCD;
-    const int b = blockIdx.x;
-    const int L = (b % NXCD) * nb8 + b / NXCD;
-    if (L >= m.nb) return false;
-    tj = L % m.ntj;
-    const int r = L / m.ntj;
-    strip = r % m.nstrips;
-    g = r / m.nstrips;
-    return true;
-}
-
 template <typename T> __device__ inline T ldg(const T *p, int64_t i, bool ok) { return ok ? p[i] : T(0); }
 
 // ---------------------------------------------------------------------------------------------
@@ -103,58 +84,6 @@ template <typename T> struct FwdArgs {
     TileMap map;
     AcCoef<T> k;
 };
-
-// Vector access of VEC consecutive depth cells per lane (128-bit loads in the pitched layout).
-template <typename T, int VEC> struct VecT;
-template <> struct VecT<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
-template <> struct VecT<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
-template <typename T, int VEC> __device__ inline void vld(T (&a)[VEC], const T *p)
-{
-    if constexpr (VEC == 1) {
-        a[0] = *p;
-    } else {
-        const typename VecT<T, VEC>::type x = *reinterpret_cast<const typename VecT<T, VEC>::type *>(p);
-#pragma unroll
-        for (int e = 0; e < VEC; e++) a[e] = x[e];
-    }
-}
-template <typename T, int VEC> __device__ inline void vst(T *p, const T (&a)[VEC])
-{
-    if constexpr (VEC == 1) {
-        *p = a[0];
-    } else {
-        typename VecT<T, VEC>::type x;
-#pragma unroll
-        for (int e = 0; e < VEC; e++) x[e] = a[e];
-        *reinterpret_cast<typename VecT<T, VEC>::type *>(p) = x;
-    }
-}
-template <typename T, int VEC> __device__ inline void vzero(T (&a)[VEC])
-{
-#pragma unroll
-    for (int e = 0; e < VEC; e++) a[e] = T(0);
-}
-
-// The grid is split once per plan into a rectangle of interior cells none of which needs the
-// PML formula (rows [iL, iR) x columns [jL, jR); jL, jR multiples of the vector width) and the
-// frame around it (PML strips, the cells next to them and the padding ring).  The rectangle
-// runs the plain scheme with nothing but u in registers; the frame runs the full PML scheme.
-struct Rect {
-    int iL, iR, jL, jR;
-};
-// Frame work item of one wavefront: rows ib .. ib+nr-1, columns js .. je-1 (<= 64 columns).
-constexpr int FRPT = 2;  // rows per frame item
-
-// west/east neighbours of a lane's vector: from the adjacent lanes or, on the segment's edge
-// lanes, from their own loads (el: lane 0; er: the last lane of the segment)
-template <typename T, int VEC>
-__device__ inline void lr_of(const T (&x)[VEC], T el, T er, int lane, bool re, T &l, T &r)
-{
-    l = __shfl_up(x[VEC - 1], 1);
-    r = __shfl_down(x[0], 1);
-    if (lane == 0) l = el;
-    if (re) r = er;
-}
 
 // Rectangle forward: u^n = 2 w - wold + C lap(w).  A wavefront owns RPT rows x (64 lanes x VEC
 // depth cells); a workgroup loops over a group of Sg shots, SB at a time, issuing every load of
@@ -225,8 +154,10 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_rect(const FwdArgs<T> a, 
     }
 }
 
-// Frame forward, one depth cell per lane.  Only lanes with a cell near the PML read phi/psi;
-// cells of the padding ring get u = 0 (sources are added afterwards).
+// Frame forward, one depth cell per lane.  A workgroup takes one frame item and its four
+// wavefronts take every fourth shot of the group, so the thin PML strips still fill the GPU.
+// Only lanes with a cell near the PML read phi/psi; cells of the padding ring get u = 0
+// (sources are added afterwards).
 template <typename T, int V>
 __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_frame(const FwdArgs<T> a,
                                                               const int4 *__restrict__ items,
@@ -235,8 +166,8 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_frame(const FwdArgs<T> a,
     int g, blk, tj;
     if (!decode_tile(a.map, g, blk, tj)) return;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
-    const int it = blk * NWAVES + wv;
-    if (it >= nitems) return;  // wave-uniform
+    const int it = blk;  // one item per workgroup; its 4 wavefronts take every 4th shot
+    if (it >= nitems) return;  // workgroup-uniform
     const int4 m = items[it];
     const int ib = m.x, nr = m.y, j = m.z + lane;
     const int NX = a.NX, NY = a.NY, P = a.P;
@@ -263,7 +194,7 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_frame(const FwdArgs<T> a,
     }
     lp = lp && act;
     const int s1 = min(a.act, (g + 1) * a.Sg);
-    for (int s = g * a.Sg; s < s1; s++) {
+    for (int s = g * a.Sg + wv; s < s1; s += NWAVES) {
         const T *__restrict__ w = a.w.at(s);
         const T *__restrict__ wo = a.wold.at(s);
         const T *__restrict__ phi = a.phi.at(s);
@@ -545,7 +476,8 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_rect(const AdjArgs<T> a, 
 }
 
 // Frame adjoint, one depth cell per lane: the full PML transpose (F/G read at neighbours,
-// F^n/G^n written for kernel 1 on PML lanes), zero on the padding ring.
+// F^n/G^n written for kernel 1 on PML lanes), zero on the padding ring.  Work split as in the
+// frame forward; the wavefronts' dL/dC partial sums are combined through LDS.
 template <typename T, int V>
 __global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_frame(const AdjArgs<T> a,
                                                               const int4 *__restrict__ items,
@@ -554,8 +486,8 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_frame(const AdjArgs<T> a,
     int g, blk, tj;
     if (!decode_tile(a.map, g, blk, tj)) return;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
-    const int it = blk * NWAVES + wv;
-    if (it >= nitems) return;  // wave-uniform
+    const int it = blk;  // one item per workgroup; its 4 wavefronts take every 4th shot
+    if (it >= nitems) return;  // workgroup-uniform
     const int4 m = items[it];
     const int ib = m.x, nr = m.y, j = m.z + lane;
     const int NX = a.NX, NY = a.NY, P = a.P;
@@ -596,7 +528,7 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_frame(const AdjArgs<T> a,
     const bool dense = a.dense.p != nullptr;
     T gacc[FRPT] = {};
     const int s1 = min(a.act, (g + 1) * a.Sg);
-    for (int s = g * a.Sg; s < s1; s++) {
+    for (int s = g * a.Sg + wv; s < s1; s += NWAVES) {
         const T *__restrict__ p1 = a.P1.at(s);
         const T *__restrict__ um = a.um1.at(s);
         const T *__restrict__ Fn = a.Fn.at(s);
@@ -672,48 +604,54 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_frame(const AdjArgs<T> a,
             gacc[r] += in ? gg : T(0);
         }
     }
-    if (!act) return;
+    // the group's dL/dC: the four wavefronts' partial sums, added in wavefront order
+    __shared__ T red[NWAVES][FRPT][WAVE];
+#pragma unroll
+    for (int r = 0; r < FRPT; r++) red[wv][r][lane] = gacc[r];
+    __syncthreads();
+    if (wv != 0 || !act) return;
     T *__restrict__ gC = a.gC.at(g);
 #pragma unroll
     for (int r = 0; r < FRPT; r++) {
         if (r >= nr) break;
         if (!rin[r] || !jin) continue;
-        gC[(int64_t)(ib + r) * P + j] += gacc[r];
+        T sum = red[0][r][lane];
+#pragma unroll
+        for (int q = 1; q < NWAVES; q++) sum += red[q][r][lane];
+        gC[(int64_t)(ib + r) * P + j] += sum;
     }
 }
 
-// Receiver residual injection at level n after the stencil pass.  One thread owns each
-// distinct interior receiver cell of a shot group; for every active shot of the group with
-// receivers there, R = sum of those receivers' residuals (in receiver order), P_s += R/d and
-// the group's dL/dC gets (R/d) lap(u_s^{n-1}), summed over the shots in order (no atomics).
+// Receiver residual injection at level n after the stencil pass.  One wavefront owns each
+// distinct interior receiver cell of a shot group; its lanes take the group's shots with
+// receivers there (one segment of entries per shot): R = sum of that shot's residuals at the
+// cell (in receiver order), P_s += R/d, and the group's dL/dC gets the sum over the shots of
+// (R/d) lap(u_s^{n-1}), reduced across the lanes in a fixed order (no atomics).
 template <typename T>
 __global__ __launch_bounds__(256) void ac_adj_inject(
     Lv<T> P_, Lv<const T> um1_, Lv<T> gC_, int act, int n, const int64_t *__restrict__ gi_ptr,
-    const int64_t *__restrict__ gi_cell, const int64_t *__restrict__ gi_eptr,
-    const int32_t *__restrict__ gi_shot, const int64_t *__restrict__ gi_rcv,
-    const int64_t *__restrict__ rcv_ptr, const T *__restrict__ resid,
-    const int64_t *__restrict__ trace_off, const T *__restrict__ sx, const T *__restrict__ ty,
-    int NX, int NY, int P, AcCoef<T> k)
+    const int64_t *__restrict__ gi_cell, const int64_t *__restrict__ gi_sptr,
+    const int32_t *__restrict__ gi_sshot, const int64_t *__restrict__ gi_seg,
+    const int64_t *__restrict__ gi_rcv, const int64_t *__restrict__ rcv_ptr,
+    const T *__restrict__ resid, const int64_t *__restrict__ trace_off,
+    const T *__restrict__ sx, const T *__restrict__ ty, int NX, int NY, int P, AcCoef<T> k)
 {
     const int g = blockIdx.y;
-    const int64_t c = gi_ptr[g] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= gi_ptr[g + 1]) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t c = gi_ptr[g] + (int64_t)blockIdx.x * NWAVES + threadIdx.x / WAVE;
+    if (c >= gi_ptr[g + 1]) return;  // wave-uniform
     const int64_t cell = gi_cell[c];
     const int i = (int)(cell / P), j = (int)(cell % P);
     if (i < 1 || i > NX || j < 1 || j > NY) return;  // ring cells carry no gradient
     const T den = T(1) + (sx[i] + ty[j]) * k.hdt;
     T acc = T(0);
-    int64_t e = gi_eptr[c];
-    const int64_t e1 = gi_eptr[c + 1];
-    while (e < e1) {
-        const int s = gi_shot[e];
-        T R = T(0);
+    for (int64_t q = gi_sptr[c] + lane; q < gi_sptr[c + 1]; q += WAVE) {
+        const int s = gi_sshot[q];
+        if (s >= act) continue;
         const int64_t nr = rcv_ptr[s + 1] - rcv_ptr[s];
         const T *__restrict__ row = resid + trace_off[s] + (int64_t)n * nr;
-        const bool on = s < act;
-        for (; e < e1 && gi_shot[e] == s; e++)
-            if (on) R += row[gi_rcv[e]];
-        if (!on) continue;
+        T R = T(0);
+        for (int64_t e = gi_seg[q]; e < gi_seg[q + 1]; e++) R += row[gi_rcv[e]];
         const T add = R / den;
         P_.at(s)[cell] += add;
         const T *__restrict__ um = um1_.at(s);
@@ -721,7 +659,9 @@ __global__ __launch_bounds__(256) void ac_adj_inject(
         acc += add * (k.cx * (um[cell + P] + um[cell - P] - T(2) * uc) +
                       k.cy * (um[cell + 1] + um[cell - 1] - T(2) * uc));
     }
-    gC_.at(g)[cell] += acc;
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) gC_.at(g)[cell] += acc;
 }
 
 // model (caller layout, (NX+2) x (NY+2)) -> C (pitched), squared if asked
@@ -747,28 +687,6 @@ __global__ void k_grad_out(const T *__restrict__ gC, int64_t stride, int ngroups
     g[o] = sq ? T(2) * m[o] * acc : acc;
 }
 
-// resid = 2 (traces - observed); loss += sum (traces - observed)^2 (double accumulation).
-template <typename T>
-__global__ __launch_bounds__(256) void k_misfit(const T *__restrict__ tr, const T *__restrict__ obs,
-                                                T *__restrict__ resid, int64_t n,
-                                                double *__restrict__ loss)
-{
-    __shared__ double red[256];
-    double acc = 0.0;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        T d = tr[q] - obs[q];
-        resid[q] = T(2) * d;
-        acc += (double)d * (double)d;
-    }
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
-}
 
 // ---------------------------------------------------------------------------------------------
 // plan
@@ -778,7 +696,6 @@ struct Shape {
 };
 
 struct AcPlanBase {
-    virtual ~AcPlanBase() = default;
     ads_acoustic_params p{};
     int dtype = ADS_F32;
     int variant = 0;  // 0 or 1
@@ -800,58 +717,41 @@ struct AcPlanBase {
     DevBuf d_us_cell[2], d_rcv_cell[2];  // [0] pitched, [1] dense
     // receiver injection tables for the current adjoint shot grouping (inj_sg)
     std::vector<int64_t> rcv_i0, rcv_j0, rcv_ptr_h;  // 0-based receiver (i, j), plan order
-    DevBuf d_gi_ptr, d_gi_eptr, d_gi_shot, d_gi_rcv, d_gi_cell[2];
+    // per union cell: segments [gi_sptr[c], gi_sptr[c+1]) of (shot gi_sshot, entries
+    // [gi_seg[q], gi_seg[q+1]) into gi_rcv)
+    DevBuf d_gi_ptr, d_gi_sptr, d_gi_sshot, d_gi_seg, d_gi_rcv, d_gi_cell[2];
     int inj_sg = 0;
     int64_t max_gi_cells = 0;
     // interior rectangle and frame work items, per layout ([0] pitched, [1] dense)
     Rect rect[2]{};
     DevBuf d_items[2];
     int nitems[2] = {0, 0};
-};
-
-// Interior rectangle: the longest run of rows in [1, NX] and of columns in [1, NY] with no near
-// flag, columns rounded inwards to multiples of vec.  Frame items cover every other cell of
-// [0, NX+1] x [0, NY+1] (the padding columns of a pitched row are never touched), FRPT rows by
-// at most 64 columns each, in row order within each band.
-static void split_grid(const std::vector<uint8_t> &nearx, const std::vector<uint8_t> &neary,
-                       int64_t NX, int64_t NY, int vec, Rect &R, std::vector<int4> &items)
-{
-    auto run = [](const std::vector<uint8_t> &f, int64_t n, int64_t &a, int64_t &b) {
-        a = b = 1;
-        for (int64_t q = 1; q <= n;) {
-            if (f[q]) {
-                q++;
-                continue;
-            }
-            int64_t e = q;
-            while (e <= n && !f[e]) e++;
-            if (e - q > b - a) a = q, b = e;
-            q = e;
+    // hipGraph replay of whole checkpoint segments (forward sweep, recomputation, adjoint
+    // sweep), keyed by everything the captured launches depend on
+    struct GraphKey {
+        int kind, record;
+        int64_t lo, hi, ns;  // ns: slots of the workspace layout (fixes every slot address)
+        const void *ws, *srcv, *traces, *resid;
+        bool operator<(const GraphKey &o) const
+        {
+            return std::tie(kind, record, lo, hi, ns, ws, srcv, traces, resid) <
+                   std::tie(o.kind, o.record, o.lo, o.hi, o.ns, o.ws, o.srcv, o.traces, o.resid);
         }
     };
-    int64_t iL, iR, jL, jR;
-    run(nearx, NX, iL, iR);
-    run(neary, NY, jL, jR);
-    jL = (jL + vec - 1) / vec * vec;
-    jR = jR / vec * vec;
-    if (iR <= iL || jR <= jL) iL = iR = jL = jR = 1;
-    R = Rect{(int)iL, (int)iR, (int)jL, (int)jR};
-    items.clear();
-    auto add = [&](int64_t r0, int64_t r1, int64_t c0, int64_t c1) {
-        for (int64_t i = r0; i < r1; i += FRPT)
-            for (int64_t j = c0; j < c1; j += WAVE)
-                items.push_back(make_int4((int)i, (int)std::min<int64_t>(FRPT, r1 - i), (int)j,
-                                          (int)std::min<int64_t>(j + WAVE, c1)));
-    };
-    if (iR <= iL) {
-        add(0, NX + 2, 0, NY + 2);
-        return;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    hipStream_t cap_stream = nullptr;  // capture needs a stream other than the null stream
+    int use_graph = 0;
+    void drop_graphs()
+    {
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
     }
-    add(0, iL, 0, NY + 2);
-    add(iL, iR, 0, jL);
-    add(iL, iR, jR, NY + 2);
-    add(iR, NX + 2, 0, NY + 2);
-}
+    virtual ~AcPlanBase()
+    {
+        drop_graphs();
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    }
+};
 
 static int validate_params(const ads_acoustic_params *p)
 {
@@ -890,11 +790,20 @@ template <typename T> struct AcPlan : AcPlanBase {
         m.nb = m.ntj * m.nstrips * m.ngroups;
         return m;
     }
+    static TileMap frame_tiles_n(int n, int act, int sg)
+    {
+        TileMap m;
+        m.ntj = 1;
+        m.nstrips = n;
+        m.ngroups = (int)cdiv(act, sg);
+        m.nb = m.ntj * m.nstrips * m.ngroups;
+        return m;
+    }
     TileMap frame_tiles(int li, int act, int sg) const
     {
         TileMap m;
         m.ntj = 1;
-        m.nstrips = (int)cdiv(nitems[li], NWAVES);
+        m.nstrips = nitems[li];
         m.ngroups = (int)cdiv(act, sg);
         m.nb = m.ntj * m.nstrips * m.ngroups;
         return m;
@@ -1005,10 +914,11 @@ template <typename T> struct AcPlan : AcPlanBase {
         prof_stop(pr, st, cells * sizeof(T) * (7.0 + (variant == 1 ? 4.0 : 2.0) * f_pml));
         if (resid && max_gi_cells > 0) {
             if (inj_sg != Sg) ADS_TRY(build_inject_tables(Sg));
-            dim3 g(cdiv(max_gi_cells, 256), cdiv(act, std::min(Sg, act)));
-            ac_adj_inject<T><<<g, 256, 0, st>>>(P, um1, gC, act, n, d_gi_ptr.as<int64_t>(),
+            dim3 g(cdiv(max_gi_cells, NWAVES), cdiv(act, std::min(Sg, act)));
+            ac_adj_inject<T><<<g, WAVE * NWAVES, 0, st>>>(P, um1, gC, act, n, d_gi_ptr.as<int64_t>(),
                                                 d_gi_cell[L.li].template as<int64_t>(),
-                                                d_gi_eptr.as<int64_t>(), d_gi_shot.as<int32_t>(),
+                                                d_gi_sptr.as<int64_t>(), d_gi_sshot.as<int32_t>(),
+                                                d_gi_seg.as<int64_t>(),
                                                 d_gi_rcv.as<int64_t>(), d_rcv_ptr.as<int64_t>(),
                                                 resid, d_trace_off.as<int64_t>(), d_sx.as<T>(),
                                                 d_ty.as<T>(), (int)NX, (int)NY, L.P, k);
@@ -1026,8 +936,8 @@ template <typename T> struct AcPlan : AcPlanBase {
     {
         if (Sg < 1) Sg = 1;
         const int64_t ng = (int64_t)cdiv(nshots, Sg);
-        std::vector<int64_t> gptr(1, 0), eptr(1, 0), rcv, cells[2];
-        std::vector<int32_t> shot;
+        std::vector<int64_t> gptr(1, 0), sptr(1, 0), seg(1, 0), rcv, cells[2];
+        std::vector<int32_t> sshot;
         max_gi_cells = 0;
         for (int64_t g = 0; g < ng; g++) {
             struct E { int64_t i, j; int32_t s; int64_t r; };
@@ -1044,13 +954,17 @@ template <typename T> struct AcPlan : AcPlanBase {
             for (size_t q = 0; q < es.size();) {
                 size_t z = q;
                 while (z < es.size() && es[z].i == es[q].i && es[z].j == es[q].j) {
-                    shot.push_back(es[z].s);
+                    if (z == q || es[z].s != es[z - 1].s) {  // a new shot segment
+                        if (z != q) seg.push_back((int64_t)rcv.size());
+                        sshot.push_back(es[z].s);
+                    }
                     rcv.push_back(es[z].r);
                     z++;
                 }
+                seg.push_back((int64_t)rcv.size());
+                sptr.push_back((int64_t)sshot.size());
                 cells[0].push_back(es[q].i * Pitch + es[q].j);
                 cells[1].push_back(es[q].i * (NY + 2) + es[q].j);
-                eptr.push_back((int64_t)shot.size());
                 ncell++;
                 q = z;
             }
@@ -1058,8 +972,9 @@ template <typename T> struct AcPlan : AcPlanBase {
             max_gi_cells = std::max(max_gi_cells, ncell);
         }
         ADS_TRY(d_gi_ptr.upload(gptr));
-        ADS_TRY(d_gi_eptr.upload(eptr));
-        ADS_TRY(d_gi_shot.upload(shot));
+        ADS_TRY(d_gi_sptr.upload(sptr));
+        ADS_TRY(d_gi_sshot.upload(sshot));
+        ADS_TRY(d_gi_seg.upload(seg));
         ADS_TRY(d_gi_rcv.upload(rcv));
         ADS_TRY(d_gi_cell[0].upload(cells[0]));
         ADS_TRY(d_gi_cell[1].upload(cells[1]));
@@ -1071,7 +986,7 @@ template <typename T> struct AcPlan : AcPlanBase {
     struct WS {
         int64_t K, NS, M;  // interval, slots, segments
         T *seg, *phi[2], *psi[2], *P[2], *F[2], *G[2], *gC, *ckpt, *C, *resid;
-        double *loss;
+        double *loss, *part;
     };
     int64_t seg_count(int64_t K) const { return NMAX >= 2 ? (NMAX - 1 + K - 1) / K : 0; }
     void layout(WS &w, Carve &cv, int64_t K) const
@@ -1095,6 +1010,7 @@ template <typename T> struct AcPlan : AcPlanBase {
         w.C = (T *)cv.take((size_t)Npad * sizeof(T));
         w.resid = (T *)cv.take((size_t)std::max<int64_t>(trace_elems, 1) * sizeof(T));
         w.loss = (double *)cv.take(64);
+        w.part = (double *)cv.take(MISFIT_BLOCKS * sizeof(double));
     }
     size_t ws_size(int64_t K) const
     {
@@ -1118,8 +1034,44 @@ template <typename T> struct AcPlan : AcPlanBase {
     }
 
     // forward over levels lo+1..hi from the state held in slots/phi (record traces if asked)
+    // Runs body(stream) directly, or -- with graphs on and no event profiling -- captures it
+    // once on a private stream into a graph and replays that graph on st.
+    template <class F> int replay(const GraphKey &key, hipStream_t st, F &&body)
+    {
+        if (!use_graph || prof_enabled()) return body(st);
+        auto it = graphs.find(key);
+        if (it == graphs.end()) {
+            if (!cap_stream) ADS_HIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+            ADS_HIP(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));
+            const int rc = body(cap_stream);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(cap_stream, &g);
+            if (rc != ADS_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            ADS_HIP(e);
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            ADS_HIP(ei);
+            it = graphs.emplace(key, ex).first;
+        }
+        ADS_HIP(hipGraphLaunch(it->second, st));
+        return ADS_OK;
+    }
+
     int run_forward(const WS &w, int64_t lo, int64_t hi, const T *srcv, T *traces, int record,
                     hipStream_t st, const Shape *sh = nullptr)
+    {
+        if (!sh) {
+            GraphKey key{0, record, lo, hi, w.NS, w.seg, srcv, traces, nullptr};
+            return replay(key, st, [&](hipStream_t s) { return run_forward_levels(w, lo, hi, srcv, traces, record, s); });
+        }
+        return run_forward_levels(w, lo, hi, srcv, traces, record, st, sh);
+    }
+    int run_forward_levels(const WS &w, int64_t lo, int64_t hi, const T *srcv, T *traces, int record,
+                           hipStream_t st, const Shape *sh = nullptr)
     {
         const Layout L = plan_layout();
         for (int64_t n = lo + 1; n <= hi; n++) {
@@ -1233,7 +1185,13 @@ template <typename T> struct AcPlan : AcPlanBase {
                 ADS_TRY(restore_ckpt(w, m, a, st));
                 ADS_TRY(run_forward(w, a, b, sv, nullptr, 0, st));
             }
-            for (int64_t n = b; n > a; n--) ADS_TRY(adj_at(w, n, sv, resid, st));
+            if (resid && max_gi_cells > 0 && inj_sg != adj_shape.sg)
+                ADS_TRY(build_inject_tables(adj_shape.sg));  // host uploads: not inside a capture
+            GraphKey key{1, 0, a, b, w.NS, w.seg, sv, nullptr, resid};
+            ADS_TRY(replay(key, st, [&](hipStream_t s) -> int {
+                for (int64_t n = b; n > a; n--) ADS_TRY(adj_at(w, n, sv, resid, s));
+                return ADS_OK;
+            }));
         }
         k_grad_out<T><<<dim3(cdiv(NY + 2, 256), NX + 2), 256, 0, st>>>(
             w.gC, Npad, (int)group_arrays(), (const T *)model, (T *)grad, (int)NX, (int)NY,
@@ -1244,10 +1202,7 @@ template <typename T> struct AcPlan : AcPlanBase {
 
     int misfit(const WS &w, const T *traces, const T *obs, hipStream_t st)
     {
-        ADS_HIP(hipMemsetAsync(w.loss, 0, sizeof(double), st));
-        unsigned nb = std::min<unsigned>(cdiv(trace_elems, 256), 2048u);
-        if (nb == 0) nb = 1;
-        k_misfit<T><<<nb, 256, 0, st>>>(traces, obs, w.resid, trace_elems, w.loss);
+        launch_misfit<T>(traces, obs, w.resid, trace_elems, w.part, w.loss, st);
         ADS_LAUNCH_CHECK();
         return ADS_OK;
     }
@@ -1256,6 +1211,7 @@ template <typename T> struct AcPlan : AcPlanBase {
     int autotune(const void *model, const void *srcv, void *ws, size_t wsb, int64_t K, int reps,
                  hipStream_t st)
     {
+        drop_graphs();  // captured segments embed the kernel shapes
         WS w;
         ADS_TRY(check_ws(w, ws, wsb, K));
         if (NMAX < 3) {  // nothing worth timing
@@ -1300,6 +1256,8 @@ template <typename T> struct AcPlan : AcPlanBase {
                 }
         fwd_shape = bf;
         ADS_TRY(zero_adjoint_state(w, st));
+        // the receiver injection's cost depends on the shot grouping: time it with the stencils
+        if (trace_elems) ADS_HIP(hipMemsetAsync(w.resid, 0, (size_t)trace_elems * sizeof(T), st));
         best = 1e30f;
         Shape ba = adj_shape;
         const Shape keep = adj_shape;
@@ -1309,7 +1267,7 @@ template <typename T> struct AcPlan : AcPlanBase {
                     if (sb == 2 && rp == 4) continue;
                     Shape sh{rp, sg, sb};
                     adj_shape = sh;  // the gC grouping follows the shape being timed
-                    float ms = timeit([&] { (void)adj_at(w, std::max<int64_t>(n0, 2), sv, nullptr, st, &sh); });
+                    float ms = timeit([&] { (void)adj_at(w, std::max<int64_t>(n0, 2), sv, w.resid, st, &sh); });
                     if (ms < best) best = ms, ba = sh;
                 }
         adj_shape = nshots > 0 ? ba : keep;
@@ -1517,7 +1475,11 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
     ADS_TRY(P->d_ty.upload(tyT));
     for (int li = 0; li < 2; li++) {
         std::vector<int4> items;
-        split_grid(P->nearx, P->neary, p->NX, p->NY, li == 0 ? AcPlan<T>::VECP : 1, P->rect[li], items);
+        // plain rows/columns: interior and no PML formula needed
+        std::vector<uint8_t> px(p->NX + 2, 0), py(p->NY + 2, 0);
+        for (int64_t i = 1; i <= p->NX; i++) px[i] = !P->nearx[i];
+        for (int64_t j = 1; j <= p->NY; j++) py[j] = !P->neary[j];
+        split_grid(px, py, p->NX, p->NY, li == 0 ? AcPlan<T>::VECP : 1, P->rect[li], items);
         P->nitems[li] = (int)items.size();
         ADS_TRY(P->d_items[li].upload(items));
     }
@@ -1691,6 +1653,17 @@ int ads_acoustic_plan_set_shape(ads_acoustic_plan *plan, const int *shape)
     return dispatch(plan, [&](auto &P) -> int {
         P.fwd_shape = Shape{shape[0], shape[1], shape[2]};
         P.adj_shape = Shape{shape[3], shape[4], shape[5]};
+        P.drop_graphs();
+        return ADS_OK;
+    });
+}
+
+int ads_acoustic_plan_set_graph(ads_acoustic_plan *plan, int enable)
+{
+    clear_error();
+    return dispatch(plan, [&](auto &P) -> int {
+        P.use_graph = enable ? 1 : 0;
+        if (!enable) P.drop_graphs();
         return ADS_OK;
     });
 }

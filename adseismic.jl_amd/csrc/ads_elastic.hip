@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "ads_common.h"
+#include "ads_stencil.h"
 
 namespace ads {
 
@@ -57,42 +58,63 @@ template <typename T> struct ElCoef {
     T dt, ihx, ihy;  // 1/(24 hx), 1/(24 hy)
     const T *axf, *axh, *bxf, *bxh, *ayf, *ayh, *byf, *byh;
     int NX, NY, P;
+    // sub = 1: one MPIElastic subdomain (MPIElastic.jl:483-620): a block of (NX-2) x (NY-2)
+    // cells at [2, NX-1] x [2, NY-1] inside a 2-cell zero halo; every fw updates the whole
+    // block, parameters are not averaged and the profile index is i-2 / j-2.
+    int sub;
 };
 
-// ------------------------------- forward -------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void el_stress_fwd(ElC<T> I, El<T> O, const T *__restrict__ lam,
-                                                     const T *__restrict__ mu, ElCoef<T> c)
+// update range of fw1..fw4 (Core.jl:100-107, 132-139, 162-169, 190-197), or the block
+template <typename T> __device__ inline bool fw_in(const ElCoef<T> &c, int fw, int i, int j)
 {
-    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int s = blockIdx.z;
+    if (c.sub) return i >= 2 && i <= c.NX - 1 && j >= 2 && j <= c.NY - 1;
+    switch (fw) {
+    case 1: return i >= 1 && i <= c.NX - 1 && j >= 2 && j <= c.NY;
+    case 2: return i >= 2 && i <= c.NX && j >= 1 && j <= c.NY - 1;
+    case 3: return i >= 2 && i <= c.NX && j >= 2 && j <= c.NY;
+    default: return i >= 1 && i <= c.NX - 1 && j >= 1 && j <= c.NY - 1;
+    }
+}
+
+// ------------------------------- forward -------------------------------------------------
+// The padded grid is split once per plan into an interior rectangle, where every update range
+// holds and the CPML coefficients are the identity (a = 0, b = 1, so the memory variables stay
+// 0 and are neither read nor written), and the frame around it, which runs the full per-cell
+// scheme below.  Memory variables of rectangle cells are 0 in every slot (cleared once per
+// forward sweep).
+
+// one cell of fw1/fw2 (level n-1 -> n), memory variables included
+template <typename T>
+__device__ inline void el_stress_cell(const ElC<T> &I, const El<T> &O, const T *__restrict__ lam,
+                                      const T *__restrict__ mu, const ElCoef<T> &c, int s, int i,
+                                      int j)
+{
     const int NX = c.NX, NY = c.NY, P = c.P;
-    if (j >= NY + 2 || i >= NX + 2) return;
     const int64_t ij = (int64_t)i * P + j;
     const T *__restrict__ vx = I.f(VX, s);
     const T *__restrict__ vy = I.f(VY, s);
     T sxx = I.f(SXX, s)[ij], syy = I.f(SYY, s)[ij], sxy = I.f(SXY, s)[ij];
     T m1 = 0, m2 = 0, m3 = 0, m4 = 0;
-    if (i >= 1 && i <= NX - 1 && j >= 2 && j <= NY) {
-        const T lb = T(0.5) * (lam[ij + P] + lam[ij]);
-        const T mb = T(0.5) * (mu[ij + P] + mu[ij]);
+    const int pi = i - 1 - c.sub, pj = j - 1 - c.sub;  // profile index
+    if (fw_in(c, 1, i, j)) {
+        const T lb = c.sub ? lam[ij] : T(0.5) * (lam[ij + P] + lam[ij]);
+        const T mb = c.sub ? mu[ij] : T(0.5) * (mu[ij + P] + mu[ij]);
         const T lm = lb + T(2) * mb;
         T dx = (T(27) * vx[ij + P] - T(27) * vx[ij] - vx[ij + 2 * P] + vx[ij - P]) * c.ihx;
         T dy = (T(27) * vy[ij] - T(27) * vy[ij - 1] - vy[ij + 1] + vy[ij - 2]) * c.ihy;
-        m1 = c.bxh[i - 1] * I.f(M1, s)[ij] + c.axh[i - 1] * dx;
-        m2 = c.byf[j - 1] * I.f(M2, s)[ij] + c.ayf[j - 1] * dy;
+        m1 = c.bxh[pi] * I.f(M1, s)[ij] + c.axh[pi] * dx;
+        m2 = c.byf[pj] * I.f(M2, s)[ij] + c.ayf[pj] * dy;
         dx += m1;
         dy += m2;
         sxx += (lm * dx + lb * dy) * c.dt;
         syy += (lm * dy + lb * dx) * c.dt;
     }
-    if (i >= 2 && i <= NX && j >= 1 && j <= NY - 1) {
-        const T mb = T(0.5) * (mu[ij] + mu[ij + 1]);
+    if (fw_in(c, 2, i, j)) {
+        const T mb = c.sub ? mu[ij] : T(0.5) * (mu[ij] + mu[ij + 1]);
         T dx = (T(27) * vy[ij] - T(27) * vy[ij - P] - vy[ij + P] + vy[ij - 2 * P]) * c.ihx;
         T dy = (T(27) * vx[ij + 1] - T(27) * vx[ij] - vx[ij + 2] + vx[ij - 1]) * c.ihy;
-        m3 = c.bxf[i - 1] * I.f(M3, s)[ij] + c.axf[i - 1] * dx;
-        m4 = c.byh[j - 1] * I.f(M4, s)[ij] + c.ayh[j - 1] * dy;
+        m3 = c.bxf[pi] * I.f(M3, s)[ij] + c.axf[pi] * dx;
+        m4 = c.byh[pj] * I.f(M4, s)[ij] + c.ayh[pj] * dy;
         sxy += mb * (dx + m3 + dy + m4) * c.dt;
     }
     O.f(SXX, s)[ij] = sxx;
@@ -104,34 +126,32 @@ __global__ __launch_bounds__(256) void el_stress_fwd(ElC<T> I, El<T> O, const T 
     O.f(M4, s)[ij] = m4;
 }
 
+// one cell of fw3/fw4 (reads the new stresses of level n)
 template <typename T>
-__global__ __launch_bounds__(256) void el_vel_fwd(ElC<T> I, El<T> O, const T *__restrict__ rho,
-                                                  ElCoef<T> c)
+__device__ inline void el_vel_cell(const ElC<T> &I, const El<T> &O, const T *__restrict__ rho,
+                                   const ElCoef<T> &c, int s, int i, int j)
 {
-    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int s = blockIdx.z;
     const int NX = c.NX, NY = c.NY, P = c.P;
-    if (j >= NY + 2 || i >= NX + 2) return;
     const int64_t ij = (int64_t)i * P + j;
     const T *__restrict__ sxx = O.f(SXX, s);
     const T *__restrict__ syy = O.f(SYY, s);
     const T *__restrict__ sxy = O.f(SXY, s);
     T vx = I.f(VX, s)[ij], vy = I.f(VY, s)[ij];
     T m5 = 0, m6 = 0, m7 = 0, m8 = 0;
-    if (i >= 2 && i <= NX && j >= 2 && j <= NY) {
+    const int pi = i - 1 - c.sub, pj = j - 1 - c.sub;  // profile index
+    if (fw_in(c, 3, i, j)) {
         T dx = (T(27) * sxx[ij] - T(27) * sxx[ij - P] - sxx[ij + P] + sxx[ij - 2 * P]) * c.ihx;
         T dy = (T(27) * sxy[ij] - T(27) * sxy[ij - 1] - sxy[ij + 1] + sxy[ij - 2]) * c.ihy;
-        m5 = c.bxf[i - 1] * I.f(M5, s)[ij] + c.axf[i - 1] * dx;
-        m6 = c.byf[j - 1] * I.f(M6, s)[ij] + c.ayf[j - 1] * dy;
+        m5 = c.bxf[pi] * I.f(M5, s)[ij] + c.axf[pi] * dx;
+        m6 = c.byf[pj] * I.f(M6, s)[ij] + c.ayf[pj] * dy;
         vx += (dx + m5 + dy + m6) * c.dt / rho[ij];
     }
-    if (i >= 1 && i <= NX - 1 && j >= 1 && j <= NY - 1) {
-        const T rb = T(0.25) * (rho[ij] + rho[ij + P] + rho[ij + P + 1] + rho[ij + 1]);
+    if (fw_in(c, 4, i, j)) {
+        const T rb = c.sub ? rho[ij] : T(0.25) * (rho[ij] + rho[ij + P] + rho[ij + P + 1] + rho[ij + 1]);
         T dx = (T(27) * sxy[ij + P] - T(27) * sxy[ij] - sxy[ij + 2 * P] + sxy[ij - P]) * c.ihx;
         T dy = (T(27) * syy[ij + 1] - T(27) * syy[ij] - syy[ij + 2] + syy[ij - 1]) * c.ihy;
-        m7 = c.bxh[i - 1] * I.f(M7, s)[ij] + c.axh[i - 1] * dx;
-        m8 = c.byh[j - 1] * I.f(M8, s)[ij] + c.ayh[j - 1] * dy;
+        m7 = c.bxh[pi] * I.f(M7, s)[ij] + c.axh[pi] * dx;
+        m8 = c.byh[pj] * I.f(M8, s)[ij] + c.ayh[pj] * dy;
         vy += (dx + m7 + dy + m8) * c.dt / rb;
     }
     O.f(VX, s)[ij] = vx;
@@ -140,6 +160,249 @@ __global__ __launch_bounds__(256) void el_vel_fwd(ElC<T> I, El<T> O, const T *__
     O.f(M6, s)[ij] = m6;
     O.f(M7, s)[ij] = m7;
     O.f(M8, s)[ij] = m8;
+}
+
+// frame: one column per lane, FRPT rows per wavefront item, one shot per grid z
+template <typename T>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_stress_frame(ElC<T> I, El<T> O, const T *__restrict__ lam,
+                                                                const T *__restrict__ mu, ElCoef<T> c,
+                                                                const int4 *__restrict__ items, int nitems)
+{
+    const int it = blockIdx.x * NWAVES + threadIdx.x / WAVE;
+    if (it >= nitems) return;
+    const int4 m = items[it];
+    const int j = m.z + (threadIdx.x & (WAVE - 1));
+    if (j >= m.w) return;
+    for (int r = 0; r < m.y; r++) el_stress_cell(I, O, lam, mu, c, blockIdx.z, m.x + r, j);
+}
+
+template <typename T>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_vel_frame(ElC<T> I, El<T> O, const T *__restrict__ rho,
+                                                             ElCoef<T> c, const int4 *__restrict__ items,
+                                                             int nitems)
+{
+    const int it = blockIdx.x * NWAVES + threadIdx.x / WAVE;
+    if (it >= nitems) return;
+    const int4 m = items[it];
+    const int j = m.z + (threadIdx.x & (WAVE - 1));
+    if (j >= m.w) return;
+    for (int r = 0; r < m.y; r++) el_vel_cell(I, O, rho, c, blockIdx.z, m.x + r, j);
+}
+
+// Row neighbours at column offsets -2, -1, VEC, VEC+1 of a lane's vector: from the lanes that
+// own them or, where that lane is outside [0, last], from the lane's own load (issued with the
+// other loads of the shot).
+template <int VEC> __device__ constexpr int hoff(int q) { return q < 2 ? q - 2 : VEC + q - 2; }
+template <int VEC> __device__ constexpr int hlane(int q)
+{
+    return hoff<VEC>(q) >= 0 ? hoff<VEC>(q) / VEC : -((-hoff<VEC>(q) + VEC - 1) / VEC);
+}
+template <typename T, int VEC>
+__device__ inline void halo_load(T (&h)[4], const T *__restrict__ row, int j0, int lane, int last, bool ok)
+{
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int src = lane + hlane<VEC>(q);
+        h[q] = ok && (src < 0 || src > last) ? row[j0 + hoff<VEC>(q)] : T(0);
+    }
+}
+// ext[k + 2] = value at column j0 + k, k = -2 .. VEC+1
+template <typename T, int VEC>
+__device__ inline void halo_fill(T (&ext)[VEC + 4], const T (&x)[VEC], const T (&h)[4], int lane, int last)
+{
+#pragma unroll
+    for (int e = 0; e < VEC; e++) ext[e + 2] = x[e];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int d = hlane<VEC>(q);
+        const int el = hoff<VEC>(q) - d * VEC;
+        const T v = d < 0 ? __shfl_up(x[el], -d) : __shfl_down(x[el], d);
+        const int src = lane + d;
+        ext[hoff<VEC>(q) + 2] = (src < 0 || src > last) ? h[q] : v;
+    }
+}
+
+// Rectangle stress update (fw1 + fw2 without memory variables).  A wavefront owns RPT rows x
+// (64 lanes x VEC depth cells); a workgroup loops over a group of Sg shots with lambda and mu
+// in registers.
+template <typename T, int RPT, int VEC>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_stress_rect(ElC<T> I, El<T> O, const T *__restrict__ lam,
+                                                               const T *__restrict__ mu, ElCoef<T> c,
+                                                               Rect R, TileMap map, int act, int Sg)
+{
+    int g, strip, tj;
+    if (!decode_tile(map, g, strip, tj)) return;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int jb = R.jL + tj * WAVE * VEC, j0 = jb + lane * VEC;
+    const int ib = R.iL + (strip * NWAVES + wv) * RPT;
+    if (ib >= R.iR) return;  // wave-uniform
+    const int P = c.P, nr = min(RPT, R.iR - ib);
+    const bool on = j0 < R.jR;
+    const int last = min(WAVE - 1, (R.jR - jb) / VEC - 1);
+    T La[RPT + 1][VEC], Mu[RPT + 1][VEC], MuE[RPT];
+#pragma unroll
+    for (int q = 0; q <= RPT; q++) {
+        if (on && q <= nr) {
+            vld(La[q], lam + (int64_t)(ib + q) * P + j0);
+            vld(Mu[q], mu + (int64_t)(ib + q) * P + j0);
+        } else {
+            vzero(La[q]);
+            vzero(Mu[q]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+        T h[4], ext[VEC + 4];
+        halo_load<T, VEC>(h, mu + (int64_t)(ib + r) * P, j0, lane, last, on && r < nr);
+        halo_fill<T, VEC>(ext, Mu[r], h, lane, last);
+        MuE[r] = ext[VEC + 2];
+    }
+    const int s1 = min(act, (g + 1) * Sg);
+    for (int s = g * Sg; s < s1; s++) {
+        const T *__restrict__ vx = I.f(VX, s);
+        const T *__restrict__ vy = I.f(VY, s);
+        T X[RPT + 3][VEC], Y[RPT + 3][VEC], HX[RPT][4], HY[RPT][4];
+        T S0[RPT][VEC], S1[RPT][VEC], S2[RPT][VEC];
+#pragma unroll
+        for (int q = 0; q < RPT + 3; q++) {  // vx rows ib-1 .., vy rows ib-2 .. (inside [0, NX+1])
+            if (on && q <= nr + 2) {
+                vld(X[q], vx + (int64_t)(ib - 1 + q) * P + j0);
+                vld(Y[q], vy + (int64_t)(ib - 2 + q) * P + j0);
+            } else {
+                vzero(X[q]);
+                vzero(Y[q]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            const int64_t row = (int64_t)(ib + r) * P;
+            const bool ok = on && r < nr;
+            halo_load<T, VEC>(HX[r], vx + row, j0, lane, last, ok);
+            halo_load<T, VEC>(HY[r], vy + row, j0, lane, last, ok);
+            if (ok) {
+                vld(S0[r], I.f(SXX, s) + row + j0);
+                vld(S1[r], I.f(SYY, s) + row + j0);
+                vld(S2[r], I.f(SXY, s) + row + j0);
+            } else {
+                vzero(S0[r]);
+                vzero(S1[r]);
+                vzero(S2[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            if (r >= nr) break;  // uniform
+            T xe[VEC + 4], ye[VEC + 4];
+            halo_fill<T, VEC>(xe, X[r + 1], HX[r], lane, last);
+            halo_fill<T, VEC>(ye, Y[r + 2], HY[r], lane, last);
+#pragma unroll
+            for (int e = 0; e < VEC; e++) {
+                const T lb = c.sub ? La[r][e] : T(0.5) * (La[r + 1][e] + La[r][e]);
+                const T mb = c.sub ? Mu[r][e] : T(0.5) * (Mu[r + 1][e] + Mu[r][e]);
+                const T lm = lb + T(2) * mb;
+                const T dx = (T(27) * X[r + 2][e] - T(27) * X[r + 1][e] - X[r + 3][e] + X[r][e]) * c.ihx;
+                const T dy = (T(27) * ye[e + 2] - T(27) * ye[e + 1] - ye[e + 3] + ye[e]) * c.ihy;
+                S0[r][e] += (lm * dx + lb * dy) * c.dt;
+                S1[r][e] += (lm * dy + lb * dx) * c.dt;
+                const T mb2 = c.sub ? Mu[r][e]
+                                    : T(0.5) * (Mu[r][e] + (e + 1 < VEC ? Mu[r][e + 1 < VEC ? e + 1 : 0] : MuE[r]));
+                const T dx2 = (T(27) * Y[r + 2][e] - T(27) * Y[r + 1][e] - Y[r + 3][e] + Y[r][e]) * c.ihx;
+                const T dy2 = (T(27) * xe[e + 3] - T(27) * xe[e + 2] - xe[e + 4] + xe[e + 1]) * c.ihy;
+                S2[r][e] += mb2 * (dx2 + dy2) * c.dt;
+            }
+            if (on) {
+                const int64_t row = (int64_t)(ib + r) * P;
+                vst(O.f(SXX, s) + row + j0, S0[r]);
+                vst(O.f(SYY, s) + row + j0, S1[r]);
+                vst(O.f(SXY, s) + row + j0, S2[r]);
+            }
+        }
+    }
+}
+
+// Rectangle velocity update (fw3 + fw4 without memory variables), reading the new stresses.
+template <typename T, int RPT, int VEC>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_vel_rect(ElC<T> I, El<T> O, const T *__restrict__ rho,
+                                                            ElCoef<T> c, Rect R, TileMap map, int act,
+                                                            int Sg)
+{
+    int g, strip, tj;
+    if (!decode_tile(map, g, strip, tj)) return;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int jb = R.jL + tj * WAVE * VEC, j0 = jb + lane * VEC;
+    const int ib = R.iL + (strip * NWAVES + wv) * RPT;
+    if (ib >= R.iR) return;  // wave-uniform
+    const int P = c.P, nr = min(RPT, R.iR - ib);
+    const bool on = j0 < R.jR;
+    const int last = min(WAVE - 1, (R.jR - jb) / VEC - 1);
+    T Rh[RPT + 1][VEC], RhE[RPT + 1];
+#pragma unroll
+    for (int q = 0; q <= RPT; q++) {
+        if (on && q <= nr) vld(Rh[q], rho + (int64_t)(ib + q) * P + j0);
+        else vzero(Rh[q]);
+        T h[4], ext[VEC + 4];
+        halo_load<T, VEC>(h, rho + (int64_t)(ib + q) * P, j0, lane, last, on && q <= nr);
+        halo_fill<T, VEC>(ext, Rh[q], h, lane, last);
+        RhE[q] = ext[VEC + 2];
+    }
+    const int s1 = min(act, (g + 1) * Sg);
+    for (int s = g * Sg; s < s1; s++) {
+        const T *__restrict__ sxx = O.f(SXX, s);
+        const T *__restrict__ syy = O.f(SYY, s);
+        const T *__restrict__ sxy = O.f(SXY, s);
+        T SX[RPT + 3][VEC], SY[RPT + 3][VEC], YY[RPT][VEC], HS[RPT][4], HY[RPT][4];
+        T VXr[RPT][VEC], VYr[RPT][VEC];
+#pragma unroll
+        for (int q = 0; q < RPT + 3; q++) {  // sxx rows ib-2 .., sxy rows ib-1 ..
+            if (on && q <= nr + 2) {
+                vld(SX[q], sxx + (int64_t)(ib - 2 + q) * P + j0);
+                vld(SY[q], sxy + (int64_t)(ib - 1 + q) * P + j0);
+            } else {
+                vzero(SX[q]);
+                vzero(SY[q]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            const int64_t row = (int64_t)(ib + r) * P;
+            const bool ok = on && r < nr;
+            halo_load<T, VEC>(HS[r], sxy + row, j0, lane, last, ok);
+            halo_load<T, VEC>(HY[r], syy + row, j0, lane, last, ok);
+            if (ok) {
+                vld(YY[r], syy + row + j0);
+                vld(VXr[r], I.f(VX, s) + row + j0);
+                vld(VYr[r], I.f(VY, s) + row + j0);
+            } else {
+                vzero(YY[r]);
+                vzero(VXr[r]);
+                vzero(VYr[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            if (r >= nr) break;  // uniform
+            T se[VEC + 4], ye[VEC + 4];
+            halo_fill<T, VEC>(se, SY[r + 1], HS[r], lane, last);
+            halo_fill<T, VEC>(ye, YY[r], HY[r], lane, last);
+#pragma unroll
+            for (int e = 0; e < VEC; e++) {
+                const T dx = (T(27) * SX[r + 2][e] - T(27) * SX[r + 1][e] - SX[r + 3][e] + SX[r][e]) * c.ihx;
+                const T dy = (T(27) * se[e + 2] - T(27) * se[e + 1] - se[e + 3] + se[e]) * c.ihy;
+                VXr[r][e] += (dx + dy) * c.dt / Rh[r][e];
+                const T re0 = e + 1 < VEC ? Rh[r][e + 1 < VEC ? e + 1 : 0] : RhE[r];
+                const T re1 = e + 1 < VEC ? Rh[r + 1][e + 1 < VEC ? e + 1 : 0] : RhE[r + 1];
+                const T rb = c.sub ? Rh[r][e] : T(0.25) * (Rh[r][e] + Rh[r + 1][e] + re1 + re0);
+                const T dx2 = (T(27) * SY[r + 2][e] - T(27) * SY[r + 1][e] - SY[r + 3][e] + SY[r][e]) * c.ihx;
+                const T dy2 = (T(27) * ye[e + 3] - T(27) * ye[e + 2] - ye[e + 4] + ye[e + 1]) * c.ihy;
+                VYr[r][e] += (dx2 + dy2) * c.dt / rb;
+            }
+            if (on) {
+                const int64_t row = (int64_t)(ib + r) * P;
+                vst(O.f(VX, s) + row + j0, VXr[r]);
+                vst(O.f(VY, s) + row + j0, VYr[r]);
+            }
+        }
+    }
 }
 
 // Sources at level n (unique (cell, type) keys, entries in source order), then receivers:
@@ -179,6 +442,487 @@ __global__ __launch_bounds__(256) void el_src_rcv(
 }
 
 // ------------------------------- adjoint -------------------------------------------------
+// Per level, each phase (velocity, then stress) runs three kernels over the forward's split:
+//   frame_a : on frame cells, the cotangents t0..t3 of the four derivative terms (to Tb),
+//             including the CPML memory recursion, and the parameter terms;
+//   rect    : on rectangle cells, the parameter terms and the transposed-stencil gathers into
+//             the stress (velocity) cotangents.  There the CPML is the identity, so t0..t3
+//             are recomputed from the field cotangents (t0 = t1, t2 = t3) instead of stored;
+//   frame_b : the same gathers on frame cells.
+// Parameter terms are accumulated per shot group at the cell where the forward formula uses
+// the averaged parameter (rho at fw3, rho-bar at fw4, lambda-bar and mu-bar at fw1, mu-bar at
+// fw2) and mapped onto (rho, lambda, mu) once after the sweep (el_grad_map).
+enum { G_RHO = 0, G_RB, G_LB, G_MB1, G_MB2, G_COUNT };
+
+template <typename T> struct ElAdj {
+    El<T> A;          // cotangents of the 13 state arrays
+    ElC<T> Sn, Sm;    // forward state of levels n (this level's sources removed) and n-1
+    T *Tb;            // [4][shot][Npad]: t0..t3 on frame cells
+    T *G;             // [G_COUNT][group][Npad]
+    int64_t npad, tstride, gstride;  // Npad, nshots * Npad, ngroups * Npad
+    const T *rho, *lam, *mu, *rb;    // rb: rho-bar of fw4 (Core.jl:201)
+    Rect R;
+    int act, Sg;
+};
+
+template <typename T> __device__ inline bool in_rect(const ElAdj<T> &a, int i, int j)
+{
+    return i >= a.R.iL && i < a.R.iR && j >= a.R.jL && j < a.R.jR;
+}
+template <typename T> __device__ inline T tb_at(const ElAdj<T> &a, int k, int s, int64_t ij)
+{
+    return a.Tb[(int64_t)k * a.tstride + (int64_t)s * a.npad + ij];
+}
+template <typename T> __device__ inline T &g_at(const ElAdj<T> &a, int k, int g, int64_t ij)
+{
+    return a.G[(int64_t)k * a.gstride + (int64_t)g * a.npad + ij];
+}
+
+// t_k of the velocity phase at (i, j), shot s; 0 outside the interior (Core.jl:215-228 ranges)
+template <typename T>
+__device__ inline T tv_vel(const ElAdj<T> &a, const ElCoef<T> &c, int k, int s, int i, int j)
+{
+    if (i < 1 || i > c.NX || j < 1 || j > c.NY) return T(0);
+    const int64_t ij = (int64_t)i * c.P + j;
+    if (in_rect(a, i, j))
+        return k < 2 ? a.A.f(VX, s)[ij] * c.dt / a.rho[ij] : a.A.f(VY, s)[ij] * c.dt / a.rb[ij];
+    return tb_at(a, k, s, ij);
+}
+// t_k of the stress phase
+template <typename T>
+__device__ inline T tv_str(const ElAdj<T> &a, const ElCoef<T> &c, int k, int s, int i, int j)
+{
+    if (i < 1 || i > c.NX || j < 1 || j > c.NY) return T(0);
+    const int64_t ij = (int64_t)i * c.P + j;
+    if (in_rect(a, i, j)) {
+        const int P = c.P;
+        if (k < 2) {
+            const T lb = c.sub ? a.lam[ij] : T(0.5) * (a.lam[ij + P] + a.lam[ij]);
+            const T mb = c.sub ? a.mu[ij] : T(0.5) * (a.mu[ij + P] + a.mu[ij]);
+            const T lm = lb + T(2) * mb;
+            const T gx = a.A.f(SXX, s)[ij] * c.dt, gy = a.A.f(SYY, s)[ij] * c.dt;
+            return k == 0 ? lm * gx + lb * gy : lb * gx + lm * gy;
+        }
+        const T mb = c.sub ? a.mu[ij] : T(0.5) * (a.mu[ij] + a.mu[ij + 1]);
+        return a.A.f(SXY, s)[ij] * mb * c.dt;
+    }
+    return tb_at(a, k, s, ij);
+}
+
+// velocity phase, frame_a: fw3/fw4 transposed through the memory recursion (m5..m8)
+template <typename T>
+__device__ inline void adjv_a_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s, int i, int j,
+                                   T &gR, T &gB)
+{
+    const int NX = c.NX, NY = c.NY, P = c.P;
+    const int64_t ij = (int64_t)i * P + j;
+    T t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+    T *a5 = a.A.f(M5, s), *a6 = a.A.f(M6, s), *a7 = a.A.f(M7, s), *a8 = a.A.f(M8, s);
+    T n5 = 0, n6 = 0, n7 = 0, n8 = 0;
+    const T *__restrict__ sxx = a.Sn.f(SXX, s);
+    const T *__restrict__ syy = a.Sn.f(SYY, s);
+    const T *__restrict__ sxy = a.Sn.f(SXY, s);
+    const int pi = i - 1 - c.sub, pj = j - 1 - c.sub;  // profile index
+    if (fw_in(c, 3, i, j)) {
+        const T dx = (T(27) * sxx[ij] - T(27) * sxx[ij - P] - sxx[ij + P] + sxx[ij - 2 * P]) * c.ihx;
+        const T dy = (T(27) * sxy[ij] - T(27) * sxy[ij - 1] - sxy[ij + 1] + sxy[ij - 2]) * c.ihy;
+        const T S3 = dx + a.Sn.f(M5, s)[ij] + dy + a.Sn.f(M6, s)[ij];
+        const T g = a.A.f(VX, s)[ij], r = a.rho[ij];
+        gR += -g * S3 * c.dt / (r * r);
+        const T e = g * c.dt / r;
+        const T g5 = a5[ij] + e, g6 = a6[ij] + e;
+        t0 = e + c.axf[pi] * g5;
+        t1 = e + c.ayf[pj] * g6;
+        n5 = c.bxf[pi] * g5;
+        n6 = c.byf[pj] * g6;
+    }
+    if (fw_in(c, 4, i, j)) {
+        const T rb = a.rb[ij];
+        const T dx = (T(27) * sxy[ij + P] - T(27) * sxy[ij] - sxy[ij + 2 * P] + sxy[ij - P]) * c.ihx;
+        const T dy = (T(27) * syy[ij + 1] - T(27) * syy[ij] - syy[ij + 2] + syy[ij - 1]) * c.ihy;
+        const T S4 = dx + a.Sn.f(M7, s)[ij] + dy + a.Sn.f(M8, s)[ij];
+        const T g = a.A.f(VY, s)[ij];
+        gB += -g * S4 * c.dt / (rb * rb);
+        const T e = g * c.dt / rb;
+        const T g7 = a7[ij] + e, g8 = a8[ij] + e;
+        t2 = e + c.axh[pi] * g7;
+        t3 = e + c.ayh[pj] * g8;
+        n7 = c.bxh[pi] * g7;
+        n8 = c.byh[pj] * g8;
+    }
+    a5[ij] = n5;
+    a6[ij] = n6;
+    a7[ij] = n7;
+    a8[ij] = n8;
+    T *tb = a.Tb + (int64_t)s * a.npad + ij;
+    tb[0] = t0;
+    tb[a.tstride] = t1;
+    tb[2 * a.tstride] = t2;
+    tb[3 * a.tstride] = t3;
+}
+
+// velocity phase gathers at (i, j): sxx <- (D-x)^T t0 ; sxy <- (D-y)^T t1 + (D+x)^T t2 ;
+// syy <- (D+y)^T t3
+template <typename T>
+__device__ inline void adjv_b_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s, int i, int j)
+{
+    const int64_t ij = (int64_t)i * c.P + j;
+    auto t = [&](int k, int ii, int jj) { return tv_vel(a, c, k, s, ii, jj); };
+    const T gxx = (T(27) * t(0, i, j) - T(27) * t(0, i + 1, j) - t(0, i - 1, j) + t(0, i + 2, j)) * c.ihx;
+    const T gxy = (T(27) * t(1, i, j) - T(27) * t(1, i, j + 1) - t(1, i, j - 1) + t(1, i, j + 2)) * c.ihy +
+                  (T(27) * t(2, i - 1, j) - T(27) * t(2, i, j) - t(2, i - 2, j) + t(2, i + 1, j)) * c.ihx;
+    const T gyy = (T(27) * t(3, i, j - 1) - T(27) * t(3, i, j) - t(3, i, j - 2) + t(3, i, j + 1)) * c.ihy;
+    a.A.f(SXX, s)[ij] += gxx;
+    a.A.f(SXY, s)[ij] += gxy;
+    a.A.f(SYY, s)[ij] += gyy;
+}
+
+// velocity phase parameter terms of a rectangle cell (no memory variables)
+template <typename T>
+__device__ inline void adjv_rect_params(const ElAdj<T> &a, const ElCoef<T> &c, int s, int i, int j,
+                                        T &gR, T &gB)
+{
+    const int P = c.P;
+    const int64_t ij = (int64_t)i * P + j;
+    const T *__restrict__ sxx = a.Sn.f(SXX, s);
+    const T *__restrict__ syy = a.Sn.f(SYY, s);
+    const T *__restrict__ sxy = a.Sn.f(SXY, s);
+    const T dx = (T(27) * sxx[ij] - T(27) * sxx[ij - P] - sxx[ij + P] + sxx[ij - 2 * P]) * c.ihx;
+    const T dy = (T(27) * sxy[ij] - T(27) * sxy[ij - 1] - sxy[ij + 1] + sxy[ij - 2]) * c.ihy;
+    const T r = a.rho[ij];
+    gR += -a.A.f(VX, s)[ij] * (dx + dy) * c.dt / (r * r);
+    const T dx2 = (T(27) * sxy[ij + P] - T(27) * sxy[ij] - sxy[ij + 2 * P] + sxy[ij - P]) * c.ihx;
+    const T dy2 = (T(27) * syy[ij + 1] - T(27) * syy[ij] - syy[ij + 2] + syy[ij - 1]) * c.ihy;
+    const T rb = a.rb[ij];
+    gB += -a.A.f(VY, s)[ij] * (dx2 + dy2) * c.dt / (rb * rb);
+}
+
+// stress phase, frame_a: fw1/fw2 transposed through m1..m4
+template <typename T>
+__device__ inline void adjs_a_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s, int i, int j,
+                                   T &gL, T &gM1, T &gM2)
+{
+    const int NX = c.NX, NY = c.NY, P = c.P;
+    const int64_t ij = (int64_t)i * P + j;
+    const T *__restrict__ vx = a.Sm.f(VX, s);
+    const T *__restrict__ vy = a.Sm.f(VY, s);
+    T *a1 = a.A.f(M1, s), *a2 = a.A.f(M2, s), *a3 = a.A.f(M3, s), *a4 = a.A.f(M4, s);
+    T t0 = 0, t1 = 0, t2 = 0, t3 = 0, n1 = 0, n2 = 0, n3 = 0, n4 = 0;
+    const int pi = i - 1 - c.sub, pj = j - 1 - c.sub;  // profile index
+    if (fw_in(c, 1, i, j)) {
+        const T lb = c.sub ? a.lam[ij] : T(0.5) * (a.lam[ij + P] + a.lam[ij]);
+        const T mb = c.sub ? a.mu[ij] : T(0.5) * (a.mu[ij + P] + a.mu[ij]);
+        const T lm = lb + T(2) * mb;
+        const T dx = (T(27) * vx[ij + P] - T(27) * vx[ij] - vx[ij + 2 * P] + vx[ij - P]) * c.ihx;
+        const T dy = (T(27) * vy[ij] - T(27) * vy[ij - 1] - vy[ij + 1] + vy[ij - 2]) * c.ihy;
+        const T X = dx + a.Sn.f(M1, s)[ij], Y = dy + a.Sn.f(M2, s)[ij];
+        const T gx = a.A.f(SXX, s)[ij] * c.dt, gy = a.A.f(SYY, s)[ij] * c.dt;
+        const T glm = X * gx + Y * gy;
+        gL += Y * gx + X * gy + glm;  // d/d lambda-bar
+        gM1 += T(2) * glm;            // d/d mu-bar (fw1)
+        const T gX = lm * gx + lb * gy, gY = lb * gx + lm * gy;
+        const T g1 = a1[ij] + gX, g2 = a2[ij] + gY;
+        t0 = gX + c.axh[pi] * g1;
+        t1 = gY + c.ayf[pj] * g2;
+        n1 = c.bxh[pi] * g1;
+        n2 = c.byf[pj] * g2;
+    }
+    if (fw_in(c, 2, i, j)) {
+        const T mb = c.sub ? a.mu[ij] : T(0.5) * (a.mu[ij] + a.mu[ij + 1]);
+        const T dx = (T(27) * vy[ij] - T(27) * vy[ij - P] - vy[ij + P] + vy[ij - 2 * P]) * c.ihx;
+        const T dy = (T(27) * vx[ij + 1] - T(27) * vx[ij] - vx[ij + 2] + vx[ij - 1]) * c.ihy;
+        const T g = a.A.f(SXY, s)[ij];
+        gM2 += g * (dx + a.Sn.f(M3, s)[ij] + dy + a.Sn.f(M4, s)[ij]) * c.dt;  // d/d mu-bar (fw2)
+        const T gs = g * mb * c.dt;
+        const T g3 = a3[ij] + gs, g4 = a4[ij] + gs;
+        t2 = gs + c.axf[pi] * g3;
+        t3 = gs + c.ayh[pj] * g4;
+        n3 = c.bxf[pi] * g3;
+        n4 = c.byh[pj] * g4;
+    }
+    a1[ij] = n1;
+    a2[ij] = n2;
+    a3[ij] = n3;
+    a4[ij] = n4;
+    T *tb = a.Tb + (int64_t)s * a.npad + ij;
+    tb[0] = t0;
+    tb[a.tstride] = t1;
+    tb[2 * a.tstride] = t2;
+    tb[3 * a.tstride] = t3;
+}
+
+// stress phase gathers: vx <- (D+x)^T t0 + (D+y)^T t3 ; vy <- (D-y)^T t1 + (D-x)^T t2
+template <typename T>
+__device__ inline void adjs_b_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s, int i, int j)
+{
+    const int64_t ij = (int64_t)i * c.P + j;
+    auto t = [&](int k, int ii, int jj) { return tv_str(a, c, k, s, ii, jj); };
+    const T gvx = (T(27) * t(0, i - 1, j) - T(27) * t(0, i, j) - t(0, i - 2, j) + t(0, i + 1, j)) * c.ihx +
+                  (T(27) * t(3, i, j - 1) - T(27) * t(3, i, j) - t(3, i, j - 2) + t(3, i, j + 1)) * c.ihy;
+    const T gvy = (T(27) * t(1, i, j) - T(27) * t(1, i, j + 1) - t(1, i, j - 1) + t(1, i, j + 2)) * c.ihy +
+                  (T(27) * t(2, i, j) - T(27) * t(2, i + 1, j) - t(2, i - 1, j) + t(2, i + 2, j)) * c.ihx;
+    a.A.f(VX, s)[ij] += gvx;
+    a.A.f(VY, s)[ij] += gvy;
+}
+
+// stress phase parameter terms of a rectangle cell
+template <typename T>
+__device__ inline void adjs_rect_params(const ElAdj<T> &a, const ElCoef<T> &c, int s, int i, int j,
+                                        T &gL, T &gM1, T &gM2)
+{
+    const int P = c.P;
+    const int64_t ij = (int64_t)i * P + j;
+    const T *__restrict__ vx = a.Sm.f(VX, s);
+    const T *__restrict__ vy = a.Sm.f(VY, s);
+    const T X = (T(27) * vx[ij + P] - T(27) * vx[ij] - vx[ij + 2 * P] + vx[ij - P]) * c.ihx;
+    const T Y = (T(27) * vy[ij] - T(27) * vy[ij - 1] - vy[ij + 1] + vy[ij - 2]) * c.ihy;
+    const T gx = a.A.f(SXX, s)[ij] * c.dt, gy = a.A.f(SYY, s)[ij] * c.dt;
+    const T glm = X * gx + Y * gy;
+    gL += Y * gx + X * gy + glm;
+    gM1 += T(2) * glm;
+    const T dx = (T(27) * vy[ij] - T(27) * vy[ij - P] - vy[ij + P] + vy[ij - 2 * P]) * c.ihx;
+    const T dy = (T(27) * vx[ij + 1] - T(27) * vx[ij] - vx[ij + 2] + vx[ij - 1]) * c.ihy;
+    gM2 += a.A.f(SXY, s)[ij] * (dx + dy) * c.dt;
+}
+
+// frame kernels: one column per lane, FRPT rows per wavefront item, grid y = shot group
+template <typename T>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_adjv_frame_a(ElAdj<T> a, ElCoef<T> c,
+                                                                const int4 *__restrict__ items, int nitems)
+{
+    const int it = blockIdx.x * NWAVES + threadIdx.x / WAVE;
+    if (it >= nitems) return;
+    const int4 m = items[it];
+    const int j = m.z + (threadIdx.x & (WAVE - 1));
+    if (j >= m.w) return;
+    const int g = blockIdx.y, s1 = min(a.act, (g + 1) * a.Sg);
+    for (int r = 0; r < m.y; r++) {
+        const int i = m.x + r;
+        T gR = 0, gB = 0;
+        for (int s = g * a.Sg; s < s1; s++) adjv_a_cell(a, c, s, i, j, gR, gB);
+        const int64_t ij = (int64_t)i * c.P + j;
+        g_at(a, G_RHO, g, ij) += gR;
+        g_at(a, G_RB, g, ij) += gB;
+    }
+}
+template <typename T>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_adjv_frame_b(ElAdj<T> a, ElCoef<T> c,
+                                                                const int4 *__restrict__ items, int nitems)
+{
+    const int it = blockIdx.x * NWAVES + threadIdx.x / WAVE;
+    if (it >= nitems) return;
+    const int4 m = items[it];
+    const int j = m.z + (threadIdx.x & (WAVE - 1));
+    if (j >= m.w) return;
+    const int g = blockIdx.y, s1 = min(a.act, (g + 1) * a.Sg);
+    for (int r = 0; r < m.y; r++)
+        for (int s = g * a.Sg; s < s1; s++) adjv_b_cell(a, c, s, m.x + r, j);
+}
+template <typename T>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_adjs_frame_a(ElAdj<T> a, ElCoef<T> c,
+                                                                const int4 *__restrict__ items, int nitems)
+{
+    const int it = blockIdx.x * NWAVES + threadIdx.x / WAVE;
+    if (it >= nitems) return;
+    const int4 m = items[it];
+    const int j = m.z + (threadIdx.x & (WAVE - 1));
+    if (j >= m.w) return;
+    const int g = blockIdx.y, s1 = min(a.act, (g + 1) * a.Sg);
+    for (int r = 0; r < m.y; r++) {
+        const int i = m.x + r;
+        T gL = 0, gM1 = 0, gM2 = 0;
+        for (int s = g * a.Sg; s < s1; s++) adjs_a_cell(a, c, s, i, j, gL, gM1, gM2);
+        const int64_t ij = (int64_t)i * c.P + j;
+        g_at(a, G_LB, g, ij) += gL;
+        g_at(a, G_MB1, g, ij) += gM1;
+        g_at(a, G_MB2, g, ij) += gM2;
+    }
+}
+template <typename T>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_adjs_frame_b(ElAdj<T> a, ElCoef<T> c,
+                                                                const int4 *__restrict__ items, int nitems)
+{
+    const int it = blockIdx.x * NWAVES + threadIdx.x / WAVE;
+    if (it >= nitems) return;
+    const int4 m = items[it];
+    const int j = m.z + (threadIdx.x & (WAVE - 1));
+    if (j >= m.w) return;
+    const int g = blockIdx.y, s1 = min(a.act, (g + 1) * a.Sg);
+    for (int r = 0; r < m.y; r++)
+        for (int s = g * a.Sg; s < s1; s++) adjs_b_cell(a, c, s, m.x + r, j);
+}
+
+// Rectangle kernels: a workgroup owns an LT_I x LT_J tile and loops over its shot group.  Per
+// shot it stages t0..t3 of the tile and a 2-cell halo in LDS (recomputed from the field
+// cotangents on rectangle cells, read from the temporaries on frame cells), then each thread
+// gathers four cells of one column from LDS.  The parameter terms use the cell's own loads.
+constexpr int LT_I = 16, LT_J = 64, LH = 2;
+constexpr int LS_I = LT_I + 2 * LH, LS_J = LT_J + 2 * LH;
+
+template <typename T>
+__global__ __launch_bounds__(256) void el_adjv_rect(ElAdj<T> a, ElCoef<T> c, TileMap map)
+{
+    __shared__ T S[4][LS_I][LS_J];
+    int g, strip, tj;
+    if (!decode_tile(map, g, strip, tj)) return;
+    const int i0 = a.R.iL + strip * LT_I, j0 = a.R.jL + tj * LT_J;
+    const int tc = threadIdx.x % LT_J, tr = threadIdx.x / LT_J;
+    const int j = j0 + tc;
+    const int s1 = min(a.act, (g + 1) * a.Sg);
+    T gR[4] = {}, gB[4] = {};
+    for (int s = g * a.Sg; s < s1; s++) {
+        for (int q = threadIdx.x; q < LS_I * LS_J; q += 256) {
+            const int li = q / LS_J, lj = q % LS_J;
+            const int i = i0 - LH + li, jj = j0 - LH + lj;
+            T t[4] = {T(0), T(0), T(0), T(0)};
+            if (i >= 1 && i <= c.NX && jj >= 1 && jj <= c.NY) {
+                const int64_t ij = (int64_t)i * c.P + jj;
+                if (in_rect(a, i, jj)) {
+                    t[0] = t[1] = a.A.f(VX, s)[ij] * c.dt / a.rho[ij];
+                    t[2] = t[3] = a.A.f(VY, s)[ij] * c.dt / a.rb[ij];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) t[k] = tb_at(a, k, s, ij);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) S[k][li][lj] = t[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int li = tr + 4 * r, i = i0 + li;
+            if (i >= a.R.iR || j >= a.R.jR) continue;
+            adjv_rect_params(a, c, s, i, j, gR[r], gB[r]);
+            const int Li = li + LH, Lj = tc + LH;
+            // sxx <- (D-x)^T t0 ; sxy <- (D-y)^T t1 + (D+x)^T t2 ; syy <- (D+y)^T t3
+            const T gxx = (T(27) * S[0][Li][Lj] - T(27) * S[0][Li + 1][Lj] - S[0][Li - 1][Lj] + S[0][Li + 2][Lj]) * c.ihx;
+            const T gxy = (T(27) * S[1][Li][Lj] - T(27) * S[1][Li][Lj + 1] - S[1][Li][Lj - 1] + S[1][Li][Lj + 2]) * c.ihy +
+                          (T(27) * S[2][Li - 1][Lj] - T(27) * S[2][Li][Lj] - S[2][Li - 2][Lj] + S[2][Li + 1][Lj]) * c.ihx;
+            const T gyy = (T(27) * S[3][Li][Lj - 1] - T(27) * S[3][Li][Lj] - S[3][Li][Lj - 2] + S[3][Li][Lj + 1]) * c.ihy;
+            const int64_t ij = (int64_t)i * c.P + j;
+            a.A.f(SXX, s)[ij] += gxx;
+            a.A.f(SXY, s)[ij] += gxy;
+            a.A.f(SYY, s)[ij] += gyy;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int i = i0 + tr + 4 * r;
+        if (i >= a.R.iR || j >= a.R.jR) continue;
+        const int64_t ij = (int64_t)i * c.P + j;
+        g_at(a, G_RHO, g, ij) += gR[r];
+        g_at(a, G_RB, g, ij) += gB[r];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void el_adjs_rect(ElAdj<T> a, ElCoef<T> c, TileMap map)
+{
+    __shared__ T S[4][LS_I][LS_J];
+    int g, strip, tj;
+    if (!decode_tile(map, g, strip, tj)) return;
+    const int i0 = a.R.iL + strip * LT_I, j0 = a.R.jL + tj * LT_J;
+    const int tc = threadIdx.x % LT_J, tr = threadIdx.x / LT_J;
+    const int j = j0 + tc;
+    const int s1 = min(a.act, (g + 1) * a.Sg);
+    T gL[4] = {}, gM1[4] = {}, gM2[4] = {};
+    for (int s = g * a.Sg; s < s1; s++) {
+        for (int q = threadIdx.x; q < LS_I * LS_J; q += 256) {
+            const int li = q / LS_J, lj = q % LS_J;
+            const int i = i0 - LH + li, jj = j0 - LH + lj;
+            T t[4] = {T(0), T(0), T(0), T(0)};
+            if (i >= 1 && i <= c.NX && jj >= 1 && jj <= c.NY) {
+                const int64_t ij = (int64_t)i * c.P + jj;
+                if (in_rect(a, i, jj)) {
+#pragma unroll
+                    for (int k = 0; k < 3; k++) t[k] = tv_str(a, c, k, s, i, jj);
+                    t[3] = t[2];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) t[k] = tb_at(a, k, s, ij);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) S[k][li][lj] = t[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int li = tr + 4 * r, i = i0 + li;
+            if (i >= a.R.iR || j >= a.R.jR) continue;
+            adjs_rect_params(a, c, s, i, j, gL[r], gM1[r], gM2[r]);
+            const int Li = li + LH, Lj = tc + LH;
+            // vx <- (D+x)^T t0 + (D+y)^T t3 ; vy <- (D-y)^T t1 + (D-x)^T t2
+            const T gvx = (T(27) * S[0][Li - 1][Lj] - T(27) * S[0][Li][Lj] - S[0][Li - 2][Lj] + S[0][Li + 1][Lj]) * c.ihx +
+                          (T(27) * S[3][Li][Lj - 1] - T(27) * S[3][Li][Lj] - S[3][Li][Lj - 2] + S[3][Li][Lj + 1]) * c.ihy;
+            const T gvy = (T(27) * S[1][Li][Lj] - T(27) * S[1][Li][Lj + 1] - S[1][Li][Lj - 1] + S[1][Li][Lj + 2]) * c.ihy +
+                          (T(27) * S[2][Li][Lj] - T(27) * S[2][Li + 1][Lj] - S[2][Li - 1][Lj] + S[2][Li + 2][Lj]) * c.ihx;
+            const int64_t ij = (int64_t)i * c.P + j;
+            a.A.f(VX, s)[ij] += gvx;
+            a.A.f(VY, s)[ij] += gvy;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int i = i0 + tr + 4 * r;
+        if (i >= a.R.iR || j >= a.R.jR) continue;
+        const int64_t ij = (int64_t)i * c.P + j;
+        g_at(a, G_LB, g, ij) += gL[r];
+        g_at(a, G_MB1, g, ij) += gM1[r];
+        g_at(a, G_MB2, g, ij) += gM2[r];
+    }
+}
+
+// rho-bar of fw4 at every cell of [1, NX-1] x [1, NY-1] (Core.jl:201), 0 elsewhere
+template <typename T>
+__global__ void el_rho_bar(const T *__restrict__ rho, T *__restrict__ rb, int NX, int NY, int P, int sub)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= P) return;
+    const int64_t ij = (int64_t)i * P + j;
+    if (sub) rb[ij] = (i >= 2 && i <= NX - 1 && j >= 2 && j <= NY - 1) ? rho[ij] : T(0);
+    else
+        rb[ij] = (i >= 1 && i <= NX - 1 && j >= 1 && j <= NY - 1)
+                     ? T(0.25) * (rho[ij] + rho[ij + P] + rho[ij + P + 1] + rho[ij + 1])
+                     : T(0);
+}
+
+// (rho, lambda, mu) gradients from the per-group accumulators, dense caller layout:
+//   rho += G_RHO + (G_RB at (i,j), (i-1,j), (i-1,j-1), (i,j-1)) / 4
+//   lambda += (G_LB at (i,j), (i-1,j)) / 2 ; mu += (G_MB1 at (i,j), (i-1,j)) / 2 + (G_MB2 at (i,j), (i,j-1)) / 2
+template <typename T>
+__global__ void el_grad_map(const T *__restrict__ G, int64_t npad, int64_t gstride, int ngroups,
+                            T *__restrict__ grho, T *__restrict__ glam, T *__restrict__ gmu,
+                            int NX, int NY, int P, int sub)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= NY + 2) return;
+    const int64_t ij = (int64_t)i * P + j;
+    auto at = [&](int k, int g, int ii, int jj) -> T {
+        return ii >= 0 && jj >= 0 ? G[(int64_t)k * gstride + (int64_t)g * npad + (int64_t)ii * P + jj] : T(0);
+    };
+    T r = 0, l = 0, m = 0;
+    for (int g = 0; g < ngroups; g++) {
+        if (sub) {  // no averaging: each term belongs to its own cell
+            r += at(G_RHO, g, i, j) + at(G_RB, g, i, j);
+            l += at(G_LB, g, i, j);
+            m += at(G_MB1, g, i, j) + at(G_MB2, g, i, j);
+            continue;
+        }
+        r += at(G_RHO, g, i, j) +
+             T(0.25) * (at(G_RB, g, i, j) + at(G_RB, g, i - 1, j) + at(G_RB, g, i - 1, j - 1) + at(G_RB, g, i, j - 1));
+        l += T(0.5) * (at(G_LB, g, i, j) + at(G_LB, g, i - 1, j));
+        m += T(0.5) * (at(G_MB1, g, i, j) + at(G_MB1, g, i - 1, j)) + T(0.5) * (at(G_MB2, g, i, j) + at(G_MB2, g, i, j - 1));
+    }
+    const int64_t d = (int64_t)i * (NY + 2) + j;
+    grho[d] = r;
+    glam[d] = l;
+    gmu[d] = m;
+}
+
 // cotangent injection of the trace residual at level n (unique (cell, type) per shot)
 template <typename T>
 __global__ __launch_bounds__(256) void el_adj_inject(
@@ -209,197 +953,6 @@ __global__ void el_adj_dense(El<T> A, ElC<T> D, int NX, int NY, int P)
     for (int q = 0; q < 5; q++) A.f(q, s)[ij] += D.f(q, s)[d];
 }
 
-// velocity phase, pass a: per-cell cotangents of the fw3/fw4 derivatives (T0..T3), the
-// rho-bar cotangent of fw4 (T4), memory cotangents m5..m8, and fw3's d/drho.
-template <typename T>
-__global__ __launch_bounds__(256) void el_adj_vel_a(El<T> A, ElC<T> Sn, T *__restrict__ Tb,
-                                                    int64_t tstride, int64_t shot_tstride,
-                                                    T *__restrict__ grho, int64_t gstride,
-                                                    const T *__restrict__ rho, ElCoef<T> c)
-{
-    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int s = blockIdx.z;
-    const int NX = c.NX, NY = c.NY, P = c.P;
-    if (j >= NY + 2 || i >= NX + 2) return;
-    const int64_t ij = (int64_t)i * P + j;
-    T *tb = Tb + (int64_t)s * shot_tstride;
-    T t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0;
-    T *a5 = A.f(M5, s), *a6 = A.f(M6, s), *a7 = A.f(M7, s), *a8 = A.f(M8, s);
-    T n5 = 0, n6 = 0, n7 = 0, n8 = 0;
-    const T *__restrict__ sxx = Sn.f(SXX, s);
-    const T *__restrict__ syy = Sn.f(SYY, s);
-    const T *__restrict__ sxy = Sn.f(SXY, s);
-    if (i >= 2 && i <= NX && j >= 2 && j <= NY) {
-        const T dx = (T(27) * sxx[ij] - T(27) * sxx[ij - P] - sxx[ij + P] + sxx[ij - 2 * P]) * c.ihx;
-        const T dy = (T(27) * sxy[ij] - T(27) * sxy[ij - 1] - sxy[ij + 1] + sxy[ij - 2]) * c.ihy;
-        const T S3 = dx + Sn.f(M5, s)[ij] + dy + Sn.f(M6, s)[ij];
-        const T g = A.f(VX, s)[ij], r = rho[ij];
-        grho[(int64_t)s * gstride + ij] += -g * S3 * c.dt / (r * r);
-        const T e = g * c.dt / r;
-        const T g5 = a5[ij] + e, g6 = a6[ij] + e;
-        t0 = e + c.axf[i - 1] * g5;
-        t1 = e + c.ayf[j - 1] * g6;
-        n5 = c.bxf[i - 1] * g5;
-        n6 = c.byf[j - 1] * g6;
-    }
-    if (i >= 1 && i <= NX - 1 && j >= 1 && j <= NY - 1) {
-        const T rb = T(0.25) * (rho[ij] + rho[ij + P] + rho[ij + P + 1] + rho[ij + 1]);
-        const T dx = (T(27) * sxy[ij + P] - T(27) * sxy[ij] - sxy[ij + 2 * P] + sxy[ij - P]) * c.ihx;
-        const T dy = (T(27) * syy[ij + 1] - T(27) * syy[ij] - syy[ij + 2] + syy[ij - 1]) * c.ihy;
-        const T S4 = dx + Sn.f(M7, s)[ij] + dy + Sn.f(M8, s)[ij];
-        const T g = A.f(VY, s)[ij];
-        t4 = -g * S4 * c.dt / (rb * rb);
-        const T e = g * c.dt / rb;
-        const T g7 = a7[ij] + e, g8 = a8[ij] + e;
-        t2 = e + c.axh[i - 1] * g7;
-        t3 = e + c.ayh[j - 1] * g8;
-        n7 = c.bxh[i - 1] * g7;
-        n8 = c.byh[j - 1] * g8;
-    }
-    a5[ij] = n5;
-    a6[ij] = n6;
-    a7[ij] = n7;
-    a8[ij] = n8;
-    tb[ij] = t0;
-    tb[tstride + ij] = t1;
-    tb[2 * tstride + ij] = t2;
-    tb[3 * tstride + ij] = t3;
-    tb[4 * tstride + ij] = t4;
-}
-
-template <typename T> __device__ inline T tg(const T *t, int i, int j, int NX, int NY, int P)
-{
-    return (i >= 1 && i <= NX && j >= 1 && j <= NY) ? t[(int64_t)i * P + j] : T(0);
-}
-
-// velocity phase, pass b: transposed-stencil gathers into the stress cotangents and rho.
-template <typename T>
-__global__ __launch_bounds__(256) void el_adj_vel_b(El<T> A, const T *__restrict__ Tb, int64_t tstride,
-                                                    int64_t shot_tstride, T *__restrict__ grho,
-                                                    int64_t gstride, ElCoef<T> c)
-{
-    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int s = blockIdx.z;
-    const int NX = c.NX, NY = c.NY, P = c.P;
-    if (j >= NY + 2 || i >= NX + 2) return;
-    const int64_t ij = (int64_t)i * P + j;
-    const T *t0 = Tb + (int64_t)s * shot_tstride, *t1 = t0 + tstride, *t2 = t0 + 2 * tstride,
-            *t3 = t0 + 3 * tstride, *t4 = t0 + 4 * tstride;
-    // sxx <- (D-x)^T t0 ; sxy <- (D-y)^T t1 + (D+x)^T t2 ; syy <- (D+y)^T t3
-    const T gxx = (T(27) * tg(t0, i, j, NX, NY, P) - T(27) * tg(t0, i + 1, j, NX, NY, P) -
-                   tg(t0, i - 1, j, NX, NY, P) + tg(t0, i + 2, j, NX, NY, P)) * c.ihx;
-    const T gxy = (T(27) * tg(t1, i, j, NX, NY, P) - T(27) * tg(t1, i, j + 1, NX, NY, P) -
-                   tg(t1, i, j - 1, NX, NY, P) + tg(t1, i, j + 2, NX, NY, P)) * c.ihy +
-                  (T(27) * tg(t2, i - 1, j, NX, NY, P) - T(27) * tg(t2, i, j, NX, NY, P) -
-                   tg(t2, i - 2, j, NX, NY, P) + tg(t2, i + 1, j, NX, NY, P)) * c.ihx;
-    const T gyy = (T(27) * tg(t3, i, j - 1, NX, NY, P) - T(27) * tg(t3, i, j, NX, NY, P) -
-                   tg(t3, i, j - 2, NX, NY, P) + tg(t3, i, j + 1, NX, NY, P)) * c.ihy;
-    A.f(SXX, s)[ij] += gxx;
-    A.f(SXY, s)[ij] += gxy;
-    A.f(SYY, s)[ij] += gyy;
-    const T gr = T(0.25) * (tg(t4, i, j, NX, NY, P) + tg(t4, i - 1, j, NX, NY, P) +
-                            tg(t4, i - 1, j - 1, NX, NY, P) + tg(t4, i, j - 1, NX, NY, P));
-    grho[(int64_t)s * gstride + ij] += gr;
-}
-
-// stress phase, pass a
-template <typename T>
-__global__ __launch_bounds__(256) void el_adj_stress_a(El<T> A, ElC<T> Vm, ElC<T> Mn,
-                                                       T *__restrict__ Tb, int64_t tstride,
-                                                       int64_t shot_tstride,
-                                                       const T *__restrict__ lam,
-                                                       const T *__restrict__ mu, ElCoef<T> c)
-{
-    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int s = blockIdx.z;
-    const int NX = c.NX, NY = c.NY, P = c.P;
-    if (j >= NY + 2 || i >= NX + 2) return;
-    const int64_t ij = (int64_t)i * P + j;
-    T *tb = Tb + (int64_t)s * shot_tstride;
-    const T *__restrict__ vx = Vm.f(VX, s);
-    const T *__restrict__ vy = Vm.f(VY, s);
-    T *a1 = A.f(M1, s), *a2 = A.f(M2, s), *a3 = A.f(M3, s), *a4 = A.f(M4, s);
-    T t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0, t5 = 0, t6 = 0, n1 = 0, n2 = 0, n3 = 0, n4 = 0;
-    if (i >= 1 && i <= NX - 1 && j >= 2 && j <= NY) {
-        const T lb = T(0.5) * (lam[ij + P] + lam[ij]);
-        const T mb = T(0.5) * (mu[ij + P] + mu[ij]);
-        const T lm = lb + T(2) * mb;
-        const T dx = (T(27) * vx[ij + P] - T(27) * vx[ij] - vx[ij + 2 * P] + vx[ij - P]) * c.ihx;
-        const T dy = (T(27) * vy[ij] - T(27) * vy[ij - 1] - vy[ij + 1] + vy[ij - 2]) * c.ihy;
-        const T X = dx + Mn.f(M1, s)[ij], Y = dy + Mn.f(M2, s)[ij];
-        const T gx = A.f(SXX, s)[ij] * c.dt, gy = A.f(SYY, s)[ij] * c.dt;
-        const T glm = X * gx + Y * gy;
-        t4 = Y * gx + X * gy + glm;  // d/d lambda-bar
-        t5 = T(2) * glm;             // d/d mu-bar (fw1)
-        const T gX = lm * gx + lb * gy, gY = lb * gx + lm * gy;
-        const T g1 = a1[ij] + gX, g2 = a2[ij] + gY;
-        t0 = gX + c.axh[i - 1] * g1;
-        t1 = gY + c.ayf[j - 1] * g2;
-        n1 = c.bxh[i - 1] * g1;
-        n2 = c.byf[j - 1] * g2;
-    }
-    if (i >= 2 && i <= NX && j >= 1 && j <= NY - 1) {
-        const T mb = T(0.5) * (mu[ij] + mu[ij + 1]);
-        const T dx = (T(27) * vy[ij] - T(27) * vy[ij - P] - vy[ij + P] + vy[ij - 2 * P]) * c.ihx;
-        const T dy = (T(27) * vx[ij + 1] - T(27) * vx[ij] - vx[ij + 2] + vx[ij - 1]) * c.ihy;
-        const T g = A.f(SXY, s)[ij];
-        t6 = g * (dx + Mn.f(M3, s)[ij] + dy + Mn.f(M4, s)[ij]) * c.dt;  // d/d mu-bar (fw2)
-        const T gs = g * mb * c.dt;
-        const T g3 = a3[ij] + gs, g4 = a4[ij] + gs;
-        t2 = gs + c.axf[i - 1] * g3;
-        t3 = gs + c.ayh[j - 1] * g4;
-        n3 = c.bxf[i - 1] * g3;
-        n4 = c.byh[j - 1] * g4;
-    }
-    a1[ij] = n1;
-    a2[ij] = n2;
-    a3[ij] = n3;
-    a4[ij] = n4;
-    tb[ij] = t0;
-    tb[tstride + ij] = t1;
-    tb[2 * tstride + ij] = t2;
-    tb[3 * tstride + ij] = t3;
-    tb[4 * tstride + ij] = t4;
-    tb[5 * tstride + ij] = t5;
-    tb[6 * tstride + ij] = t6;
-}
-
-// stress phase, pass b: gathers into the velocity cotangents, lambda and mu.
-template <typename T>
-__global__ __launch_bounds__(256) void el_adj_stress_b(El<T> A, const T *__restrict__ Tb,
-                                                       int64_t tstride, int64_t shot_tstride,
-                                                       T *__restrict__ glam, T *__restrict__ gmu,
-                                                       int64_t gstride, ElCoef<T> c)
-{
-    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int s = blockIdx.z;
-    const int NX = c.NX, NY = c.NY, P = c.P;
-    if (j >= NY + 2 || i >= NX + 2) return;
-    const int64_t ij = (int64_t)i * P + j;
-    const T *t0 = Tb + (int64_t)s * shot_tstride, *t1 = t0 + tstride, *t2 = t0 + 2 * tstride,
-            *t3 = t0 + 3 * tstride, *t4 = t0 + 4 * tstride, *t5 = t0 + 5 * tstride,
-            *t6 = t0 + 6 * tstride;
-    // vx <- (D+x)^T t0 + (D+y)^T t3 ; vy <- (D-y)^T t1 + (D-x)^T t2
-    const T gvx = (T(27) * tg(t0, i - 1, j, NX, NY, P) - T(27) * tg(t0, i, j, NX, NY, P) -
-                   tg(t0, i - 2, j, NX, NY, P) + tg(t0, i + 1, j, NX, NY, P)) * c.ihx +
-                  (T(27) * tg(t3, i, j - 1, NX, NY, P) - T(27) * tg(t3, i, j, NX, NY, P) -
-                   tg(t3, i, j - 2, NX, NY, P) + tg(t3, i, j + 1, NX, NY, P)) * c.ihy;
-    const T gvy = (T(27) * tg(t1, i, j, NX, NY, P) - T(27) * tg(t1, i, j + 1, NX, NY, P) -
-                   tg(t1, i, j - 1, NX, NY, P) + tg(t1, i, j + 2, NX, NY, P)) * c.ihy +
-                  (T(27) * tg(t2, i, j, NX, NY, P) - T(27) * tg(t2, i + 1, j, NX, NY, P) -
-                   tg(t2, i - 1, j, NX, NY, P) + tg(t2, i + 2, j, NX, NY, P)) * c.ihx;
-    A.f(VX, s)[ij] += gvx;
-    A.f(VY, s)[ij] += gvy;
-    const int64_t g = (int64_t)s * gstride + ij;
-    glam[g] += T(0.5) * (tg(t4, i, j, NX, NY, P) + tg(t4, i - 1, j, NX, NY, P));
-    gmu[g] += T(0.5) * (tg(t5, i, j, NX, NY, P) + tg(t5, i - 1, j, NX, NY, P)) +
-              T(0.5) * (tg(t6, i, j, NX, NY, P) + tg(t6, i, j - 1, NX, NY, P));
-}
-
 template <typename T>
 __global__ void el_model_in(const T *__restrict__ m, T *__restrict__ o, int NX, int NY, int P)
 {
@@ -408,39 +961,6 @@ __global__ void el_model_in(const T *__restrict__ m, T *__restrict__ o, int NX, 
     o[(int64_t)i * P + j] = m[(int64_t)i * (NY + 2) + j];
 }
 
-template <typename T>
-__global__ void el_grad_out(const T *__restrict__ g, int64_t stride, int nshots, T *__restrict__ out,
-                            int NX, int NY, int P)
-{
-    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
-    if (j >= NY + 2) return;
-    const int64_t q = (int64_t)i * P + j;
-    T acc = T(0);
-    for (int s = 0; s < nshots; s++) acc += g[(int64_t)s * stride + q];
-    out[(int64_t)i * (NY + 2) + j] = acc;
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void el_misfit(const T *__restrict__ tr, const T *__restrict__ obs,
-                                                 T *__restrict__ resid, int64_t n,
-                                                 double *__restrict__ loss)
-{
-    __shared__ double red[256];
-    double acc = 0.0;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        T d = tr[q] - obs[q];
-        resid[q] = T(2) * d;
-        acc += (double)d * (double)d;
-    }
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
-}
 
 // ------------------------------- plan ---------------------------------------------------
 struct ElPlanBase {
@@ -457,6 +977,13 @@ struct ElPlanBase {
     DevBuf d_rcv_ptr, d_trace_off, d_nstep;
     DevBuf d_us_key[2], d_ur_key[2], d_rcv_key[2];  // [0] pitched, [1] dense; key = cell*8+type
     int64_t max_ur = 0;
+    // interior rectangle (identity CPML, every update range) and frame items, per layout
+    Rect rect[2]{};
+    DevBuf d_items[2];
+    int nitems[2] = {0, 0};
+    int rpt = 1, sg = 4;  // rectangle kernels: rows per wavefront, shots per workgroup
+    int sub = 0;          // one MPIElastic subdomain (see ElCoef)
+    int64_t PX = 0, PY = 0;  // CPML profile lengths (NX, NY; the block size in the subdomain form)
 };
 
 template <typename T> struct ElPlan : ElPlanBase {
@@ -470,26 +997,28 @@ template <typename T> struct ElPlan : ElPlanBase {
         c.ihx = ihx;
         c.ihy = ihy;
         c.axf = b;
-        c.axh = b + NX;
-        c.bxf = b + 2 * NX;
-        c.bxh = b + 3 * NX;
-        c.ayf = b + 4 * NX;
-        c.ayh = b + 4 * NX + NY;
-        c.byf = b + 4 * NX + 2 * NY;
-        c.byh = b + 4 * NX + 3 * NY;
+        c.axh = b + PX;
+        c.bxf = b + 2 * PX;
+        c.bxh = b + 3 * PX;
+        c.ayf = b + 4 * PX;
+        c.ayh = b + 4 * PX + PY;
+        c.byf = b + 4 * PX + 2 * PY;
+        c.byh = b + 4 * PX + 3 * PY;
         c.NX = (int)NX;
         c.NY = (int)NY;
         c.P = P;
+        c.sub = sub;
         return c;
     }
     dim3 grid(int act) const { return dim3(cdiv(NY + 2, 64), cdiv(NX + 2, 4), act); }
 
     struct WS {
         int64_t K, NS, M;
-        T *seg, *ckpt, *adj, *tmp, *grad, *rho, *lam, *mu, *resid;
-        double *loss;
+        T *seg, *ckpt, *adj, *tmp, *grad, *rho, *lam, *mu, *rb, *resid;
+        double *loss, *part;
     };
     int64_t seg_count(int64_t K) const { return NMAX >= 1 ? (NMAX + K - 1) / K : 0; }
+    int64_t adj_groups() const { return (int64_t)cdiv(nshots, std::max(sg, 1)); }
     void layout(WS &w, Carve &cv, int64_t K) const
     {
         if (K < 1) K = 1;
@@ -501,13 +1030,15 @@ template <typename T> struct ElPlan : ElPlanBase {
         w.seg = (T *)cv.take(F * NF * w.NS);
         w.ckpt = (T *)cv.take(F * NF * (size_t)std::max<int64_t>(w.M - 1, 0));
         w.adj = (T *)cv.take(F * NF);
-        w.tmp = (T *)cv.take(F * 7);
-        w.grad = (T *)cv.take(F * 3);
+        w.tmp = (T *)cv.take(F * 4);
+        w.grad = (T *)cv.take((size_t)G_COUNT * adj_groups() * Npad * sizeof(T));
+        w.rb = (T *)cv.take((size_t)Npad * sizeof(T));
         w.rho = (T *)cv.take((size_t)Npad * sizeof(T));
         w.lam = (T *)cv.take((size_t)Npad * sizeof(T));
         w.mu = (T *)cv.take((size_t)Npad * sizeof(T));
         w.resid = (T *)cv.take((size_t)std::max<int64_t>(trace_elems, 1) * sizeof(T));
         w.loss = (double *)cv.take(64);
+        w.part = (double *)cv.take(MISFIT_BLOCKS * sizeof(double));
     }
     size_t ws_size(int64_t K) const
     {
@@ -539,20 +1070,58 @@ template <typename T> struct ElPlan : ElPlanBase {
         return block(w.seg + (size_t)(n % w.NS) * NF * nshots * Npad);
     }
 
+    static constexpr int VECP = sizeof(T) == 4 ? 4 : 2;  // pitched layout: 16-byte vectors
+    template <int RPT, int VEC>
+    void launch_rect_r(int phase, const Rect &R, const ElC<T> &I, const El<T> &O, const T *rho,
+                       const T *lam, const T *mu, const ElCoef<T> &c, int act, int Sg, hipStream_t st)
+    {
+        TileMap m;
+        m.ntj = (int)cdiv(R.jR - R.jL, WAVE * VEC);
+        m.nstrips = (int)cdiv(R.iR - R.iL, NWAVES * RPT);
+        m.ngroups = (int)cdiv(act, Sg);
+        m.nb = m.ntj * m.nstrips * m.ngroups;
+        const unsigned nb = cdiv(m.nb, NXCD) * NXCD;
+        if (phase == 0) el_stress_rect<T, RPT, VEC><<<nb, WAVE * NWAVES, 0, st>>>(I, O, lam, mu, c, R, m, act, Sg);
+        else el_vel_rect<T, RPT, VEC><<<nb, WAVE * NWAVES, 0, st>>>(I, O, rho, c, R, m, act, Sg);
+    }
+    template <int VEC>
+    void launch_rect(int phase, const Rect &R, const ElC<T> &I, const El<T> &O, const T *rho,
+                     const T *lam, const T *mu, const ElCoef<T> &c, int act, int Sg, hipStream_t st)
+    {
+        if (rpt == 2) launch_rect_r<2, VEC>(phase, R, I, O, rho, lam, mu, c, act, Sg, st);
+        else launch_rect_r<1, VEC>(phase, R, I, O, rho, lam, mu, c, act, Sg, st);
+    }
+
     int step(const El<T> &I, const El<T> &O, int n, int act, const T *rho, const T *lam,
              const T *mu, const T *srcv, T *traces, int record, int P, int li, hipStream_t st)
     {
         if (act <= 0) return ADS_OK;
         const ElCoef<T> c = coef(P);
-        const double cells = (double)act * (double)NX * (double)NY;
+        const double cells = (double)act * (double)PX * (double)PY;  // interior (or block) cells
         // algorithmic bytes: stress kernel reads v (2) + sigma (3) + lam, mu (2), writes sigma (3),
         // memory m1..m4 read+write in their strips; velocity kernel reads sigma (3) + v (2) + rho,
         // writes v (2), m5..m8 in their strips
+        const Rect &R = rect[li];
+        const bool rr = R.iR > R.iL && R.jR > R.jL;
+        const int Sg = std::min(sg, act);
+        const dim3 fg(cdiv(nitems[li], NWAVES), 1, act);
         int pr = prof_start(ADS_PROF_EL_FWD_STRESS, st);
-        el_stress_fwd<T><<<grid(act), 256, 0, st>>>(cst(I), O, lam, mu, c);
+        if (rr) {
+            if (li == 0) launch_rect<VECP>(0, R, cst(I), O, rho, lam, mu, c, act, Sg, st);
+            else launch_rect<1>(0, R, cst(I), O, rho, lam, mu, c, act, Sg, st);
+        }
+        if (nitems[li])
+            el_stress_frame<T><<<fg, WAVE * NWAVES, 0, st>>>(cst(I), O, lam, mu, c,
+                                                              d_items[li].template as<int4>(), nitems[li]);
         prof_stop(pr, st, cells * sizeof(T) * (10.0 + 4.0 * f_x + 4.0 * f_y));
         pr = prof_start(ADS_PROF_EL_FWD_VEL, st);
-        el_vel_fwd<T><<<grid(act), 256, 0, st>>>(cst(I), O, rho, c);
+        if (rr) {
+            if (li == 0) launch_rect<VECP>(1, R, cst(I), O, rho, lam, mu, c, act, Sg, st);
+            else launch_rect<1>(1, R, cst(I), O, rho, lam, mu, c, act, Sg, st);
+        }
+        if (nitems[li])
+            el_vel_frame<T><<<fg, WAVE * NWAVES, 0, st>>>(cst(I), O, rho, c,
+                                                           d_items[li].template as<int4>(), nitems[li]);
         prof_stop(pr, st, cells * sizeof(T) * (8.0 + 4.0 * f_x + 4.0 * f_y));
         el_src_rcv<T><<<act, 256, 0, st>>>(O, n, d_us_ptr.as<int64_t>(), d_us_key[li].template as<int64_t>(),
                                            d_us_eptr.as<int64_t>(), d_us_ent.as<int64_t>(), srcv,
@@ -603,6 +1172,9 @@ template <typename T> struct ElPlan : ElPlanBase {
         ADS_TRY(check_ws(w, ws, wsb, K));
         ADS_TRY(prepare_models(w.rho, w.lam, w.mu, rho, lam, mu, st));
         ADS_HIP(hipMemsetAsync(slot(w, 0).p[0], 0, (size_t)NF * nshots * Npad * sizeof(T), st));
+        // rectangle cells never write their (zero) memory variables: clear them in every slot
+        for (int64_t q = 1; q < w.NS; q++)
+            ADS_HIP(hipMemsetAsync(slot(w, q).p[M1], 0, (size_t)8 * nshots * Npad * sizeof(T), st));
         if (traces && trace_elems) ADS_HIP(hipMemsetAsync(traces, 0, trace_elems * sizeof(T), st));
         for (int64_t m = 0; m < w.M; m++) {
             int64_t a = m * w.K, b = std::min<int64_t>(a + w.K, NMAX);
@@ -614,8 +1186,8 @@ template <typename T> struct ElPlan : ElPlanBase {
 
     // one reverse level: cotangents of level n -> level n-1.  Sn: state of level n, Sm: n-1.
     int adj_level(const El<T> &A, const El<T> &Sn, const El<T> &Sm, int n, int act,
-                  const T *resid, const ElC<T> *dense, T *tmp, T *grad, const T *rho,
-                  const T *lam, const T *mu, const T *srcv, int P, int li, hipStream_t st)
+                  const T *resid, const ElC<T> *dense, const WS &w, const T *srcv, int P, int li,
+                  hipStream_t st)
     {
         if (act <= 0) return ADS_OK;
         const ElCoef<T> c = coef(P);
@@ -632,19 +1204,49 @@ template <typename T> struct ElPlan : ElPlanBase {
                                            d_rcv_ptr.as<int64_t>(), d_rcv_key[li].template as<int64_t>(),
                                            d_nstep.as<int64_t>(), nullptr,
                                            d_trace_off.as<int64_t>(), 2);
-        const int64_t F = (int64_t)nshots * Npad;  // temporaries / gradients: [k][shot][Npad]
-        const double cells = (double)act * (double)NX * (double)NY;
-        T *grho = grad, *glam = grad + F, *gmu = grad + 2 * F;
+        ElAdj<T> a;
+        a.A = A;
+        a.Sn = cst(Sn);
+        a.Sm = cst(Sm);
+        a.Tb = w.tmp;
+        a.G = w.grad;
+        a.npad = Npad;
+        a.tstride = (int64_t)nshots * Npad;
+        a.gstride = adj_groups() * Npad;
+        a.rho = w.rho;
+        a.lam = w.lam;
+        a.mu = w.mu;
+        a.rb = w.rb;
+        a.R = rect[li];
+        a.act = act;
+        a.Sg = std::min(sg, act);
+        const int ng = (int)cdiv(act, a.Sg);
+        const dim3 fg(cdiv(nitems[li], NWAVES), ng);
+        const int4 *items = d_items[li].template as<int4>();
+        const bool rr = a.R.iR > a.R.iL && a.R.jR > a.R.jL;
+        TileMap m{};
+        if (rr) {
+            m.ntj = (int)cdiv(a.R.jR - a.R.jL, LT_J);
+            m.nstrips = (int)cdiv(a.R.iR - a.R.iL, LT_I);
+            m.ngroups = ng;
+            m.nb = m.ntj * m.nstrips * m.ngroups;
+        }
+        const unsigned rg = cdiv(m.nb, NXCD) * NXCD;
+        const double cells = (double)act * (double)PX * (double)PY;  // interior (or block) cells
+        // algorithmic bytes per cell: velocity phase reads the velocity cotangents (2) and the
+        // stresses (3), reads and writes the stress cotangents (6), reads rho; stress phase reads
+        // the stress cotangents (3) and velocities (2), reads and writes the velocity cotangents
+        // (4), reads lambda and mu; CPML memory values and cotangents in their strips
         int pr = prof_start(ADS_PROF_EL_ADJ_VEL, st);
-        el_adj_vel_a<T><<<grid(act), 256, 0, st>>>(A, cst(Sn), tmp, F, Npad, grho, Npad, rho, c);
-        el_adj_vel_b<T><<<grid(act), 256, 0, st>>>(A, tmp, F, Npad, grho, Npad, c);
-        // velocity VJP: reads v-bar (2), sigma (3), rho, stress cotangents r+w (6), rho grad r+w (2),
-        // memory values and cotangents in the strips
-        prof_stop(pr, st, cells * sizeof(T) * (14.0 + 6.0 * (f_x + f_y)));
+        if (nitems[li]) el_adjv_frame_a<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
+        if (rr) el_adjv_rect<T><<<rg, WAVE * NWAVES, 0, st>>>(a, c, m);
+        if (nitems[li]) el_adjv_frame_b<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
+        prof_stop(pr, st, cells * sizeof(T) * (12.0 + 12.0 * (f_x + f_y)));
         pr = prof_start(ADS_PROF_EL_ADJ_STRESS, st);
-        el_adj_stress_a<T><<<grid(act), 256, 0, st>>>(A, cst(Sm), cst(Sn), tmp, F, Npad, lam, mu, c);
-        el_adj_stress_b<T><<<grid(act), 256, 0, st>>>(A, tmp, F, Npad, glam, gmu, Npad, c);
-        prof_stop(pr, st, cells * sizeof(T) * (14.0 + 6.0 * (f_x + f_y)));
+        if (nitems[li]) el_adjs_frame_a<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
+        if (rr) el_adjs_rect<T><<<rg, WAVE * NWAVES, 0, st>>>(a, c, m);
+        if (nitems[li]) el_adjs_frame_b<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
+        prof_stop(pr, st, cells * sizeof(T) * (11.0 + 12.0 * (f_x + f_y)));
         ADS_LAUNCH_CHECK();
         return ADS_OK;
     }
@@ -656,8 +1258,9 @@ template <typename T> struct ElPlan : ElPlanBase {
         ADS_TRY(check_ws(w, ws, wsb, K));
         const size_t F = (size_t)nshots * Npad * sizeof(T);
         ADS_HIP(hipMemsetAsync(w.adj, 0, F * NF, st));
-        ADS_HIP(hipMemsetAsync(w.grad, 0, F * 3, st));
-        ADS_HIP(hipMemsetAsync(w.tmp, 0, F * 7, st));
+        ADS_HIP(hipMemsetAsync(w.grad, 0, (size_t)G_COUNT * adj_groups() * Npad * sizeof(T), st));
+        ADS_HIP(hipMemsetAsync(w.tmp, 0, F * 4, st));
+        el_rho_bar<T><<<dim3(cdiv(Pitch, 256), NX + 2), 256, 0, st>>>(w.rho, w.rb, (int)NX, (int)NY, (int)Pitch, sub);
         const T *sv = (const T *)srcv;
         const El<T> A = block(w.adj);
         const int64_t L = NMAX + 1;
@@ -684,25 +1287,20 @@ template <typename T> struct ElPlan : ElPlanBase {
                     for (int q = 5; q < NF; q++) D.p[q] = nullptr, D.st[q] = 0;
                     Dp = &D;
                 }
-                ADS_TRY(adj_level(A, slot(w, n), slot(w, n - 1), (int)n, active[n], resid, Dp,
-                                  w.tmp, w.grad, w.rho, w.lam, w.mu, sv, (int)Pitch, 0, st));
+                ADS_TRY(adj_level(A, slot(w, n), slot(w, n - 1), (int)n, active[n], resid, Dp, w,
+                                  sv, (int)Pitch, 0, st));
             }
         }
         dim3 g(cdiv(NY + 2, 256), NX + 2);
-        const int64_t gs = (int64_t)nshots * Npad;
-        el_grad_out<T><<<g, 256, 0, st>>>(w.grad, Npad, (int)nshots, (T *)grho, (int)NX, (int)NY, (int)Pitch);
-        el_grad_out<T><<<g, 256, 0, st>>>(w.grad + gs, Npad, (int)nshots, (T *)glam, (int)NX, (int)NY, (int)Pitch);
-        el_grad_out<T><<<g, 256, 0, st>>>(w.grad + 2 * gs, Npad, (int)nshots, (T *)gmu, (int)NX, (int)NY, (int)Pitch);
+        el_grad_map<T><<<g, 256, 0, st>>>(w.grad, Npad, adj_groups() * Npad, (int)adj_groups(),
+                                          (T *)grho, (T *)glam, (T *)gmu, (int)NX, (int)NY, (int)Pitch, sub);
         ADS_LAUNCH_CHECK();
         return ADS_OK;
     }
 
     int misfit(const WS &w, const T *traces, const T *obs, hipStream_t st)
     {
-        ADS_HIP(hipMemsetAsync(w.loss, 0, sizeof(double), st));
-        unsigned nb = std::min<unsigned>(cdiv(trace_elems, 256), 2048u);
-        if (nb == 0) nb = 1;
-        el_misfit<T><<<nb, 256, 0, st>>>(traces, obs, w.resid, trace_elems, w.loss);
+        launch_misfit<T>(traces, obs, w.resid, trace_elems, w.part, w.loss, st);
         ADS_LAUNCH_CHECK();
         return ADS_OK;
     }
@@ -742,12 +1340,11 @@ static int validate_params(const ads_elastic_params *p)
     if (p->NX < 3 || p->NY < 3 || p->NSTEP < 1)
         ADS_FAIL(ADS_ERR_SHAPE, "NX, NY must be >= 3 and NSTEP >= 1 (got %lld, %lld, %lld)",
                  (long long)p->NX, (long long)p->NY, (long long)p->NSTEP);
-    if ((p->NX + 2) * (((p->NY + 2 + 63) / 64) * 64) >= (int64_t)1 << 31 || p->NX + 2 > 65535)
+    if ((p->NX + 6) * (((p->NY + 6 + 63) / 64) * 64) >= (int64_t)1 << 31 || p->NX + 6 > 65535)
         ADS_FAIL(ADS_ERR_SHAPE, "grid too large");
     if (!(p->DELTAT > 0) || !(p->DELTAX > 0) || !(p->DELTAY > 0))
         ADS_FAIL(ADS_ERR_ARG, "DELTAT, DELTAX, DELTAY must be > 0");
-    if (p->subdomain)
-        ADS_FAIL(ADS_ERR_ARG, "the MPIElastic subdomain variant is not implemented by the elastic plan");
+
     return ADS_OK;
 }
 
@@ -758,11 +1355,17 @@ static int build_el_plan(const ads_elastic_params *p, const ads_shot_geometry *g
     if (!g || g->nshots < 1 || !g->src_ptr || !g->rcv_ptr)
         ADS_FAIL(ADS_ERR_ARG, "geometry: need nshots >= 1, src_ptr and rcv_ptr");
     P->p = *p;
-    P->NX = p->NX;
-    P->NY = p->NY;
-    P->Pitch = ((p->NY + 2 + 63) / 64) * 64;
-    P->Npad = (p->NX + 2) * P->Pitch;
-    P->Ndense = (p->NX + 2) * (p->NY + 2);
+    // subdomain form: a block of NX x NY cells inside a 2-cell halo, stored as an
+    // (NX+2) x (NY+2) "interior" with the usual 1-cell ring, i.e. (NX+4) x (NY+4) arrays
+    const int sub = p->subdomain ? 1 : 0;
+    P->sub = sub;
+    P->PX = p->NX;
+    P->PY = p->NY;
+    P->NX = p->NX + 2 * sub;
+    P->NY = p->NY + 2 * sub;
+    P->Pitch = ((P->NY + 2 + 63) / 64) * 64;
+    P->Npad = (P->NX + 2) * P->Pitch;
+    P->Ndense = (P->NX + 2) * (P->NY + 2);
     P->nshots = g->nshots;
     P->dt_ = (T)p->DELTAT;
     P->ihx = (T)(1.0 / (24.0 * p->DELTAX));
@@ -800,12 +1403,20 @@ static int build_el_plan(const ads_elastic_params *p, const ads_shot_geometry *g
     }
     P->srcv_elems = so;
     P->trace_elems = to;
+    // 1-based index ranges: sources on the padded grid ([3, NX+2] x [3, NY+2], the block, in the
+    // subdomain form: MPIElastic.jl:85-86 adds the halo offset 2); receivers on the padded grid,
+    // or on the block itself in the subdomain form (MPIElastic.jl:110-111)
+    const int64_t slo = 1 + 2 * sub, shi_x = P->NX + 2 - 2 * sub, shi_y = P->NY + 2 - 2 * sub;
+    const int64_t rlo = 1, rhi_x = sub ? p->NX : P->NX + 2, rhi_y = sub ? p->NY : P->NY + 2;
     auto check = [&](const int64_t *I, const int64_t *J, const int64_t *Ty, int64_t e, const char *what) -> int {
         if (!Ty) ADS_FAIL(ADS_ERR_ARG, "%s types are null", what);
         int64_t i = I[e], j = J[e], t = Ty[e];
-        if (i < 1 || i > p->NX + 2 || j < 1 || j > p->NY + 2)
-            ADS_FAIL(ADS_ERR_INDEX, "%s %lld at (%lld, %lld) is outside the padded grid [1, %lld] x [1, %lld]",
-                     what, (long long)e, (long long)i, (long long)j, (long long)(p->NX + 2), (long long)(p->NY + 2));
+        const bool src = what[0] == 's';
+        const int64_t lo = src ? slo : rlo, hx = src ? shi_x : rhi_x, hy = src ? shi_y : rhi_y;
+        if (i < lo || i > hx || j < lo || j > hy)
+            ADS_FAIL(ADS_ERR_INDEX, "%s %lld at (%lld, %lld) is outside [%lld, %lld] x [%lld, %lld]%s", what,
+                     (long long)e, (long long)i, (long long)j, (long long)lo, (long long)hx, (long long)lo,
+                     (long long)hy, sub ? " (subdomain block)" : " (padded grid)");
         if (t < 0 || t > 4)
             ADS_FAIL(ADS_ERR_ARG, "%s %lld has type %lld (must be 0..4: vx, vy, sxx, syy, sxy)", what,
                      (long long)e, (long long)t);
@@ -824,8 +1435,8 @@ static int build_el_plan(const ads_elastic_params *p, const ads_shot_geometry *g
         for (int64_t e = g->rcv_ptr[s]; e < g->rcv_ptr[s + 1]; e++) {
             if (!g->rcvi || !g->rcvj) ADS_FAIL(ADS_ERR_ARG, "rcvi/rcvj are null");
             ADS_TRY(check(g->rcvi, g->rcvj, g->rcvtype, e, "receiver"));
-            ri.push_back(g->rcvi[e] - 1);
-            rj.push_back(g->rcvj[e] - 1);
+            ri.push_back(g->rcvi[e] - 1 + 2 * sub);
+            rj.push_back(g->rcvj[e] - 1 + 2 * sub);
             rt.push_back(g->rcvtype[e]);
         }
         sp[q + 1] = (int64_t)si.size();
@@ -835,7 +1446,7 @@ static int build_el_plan(const ads_elastic_params *p, const ads_shot_geometry *g
         toff[q] = P->trace_off_caller[s];
         nsv[q] = nstep[s];
     }
-    const int64_t pitches[2] = {P->Pitch, p->NY + 2};
+    const int64_t pitches[2] = {P->Pitch, P->NY + 2};
     for (int L = 0; L < 2; L++) {
         std::vector<int64_t> skey(si.size()), rkey(ri.size());
         for (size_t e = 0; e < si.size(); e++) skey[e] = (si[e] * pitches[L] + sj[e]) * 8 + st_[e];
@@ -869,6 +1480,24 @@ static int build_el_plan(const ads_elastic_params *p, const ads_shot_geometry *g
     for (int64_t k = 0; k < p->NY; k++) cy += (ay[k] != 0.0 || ay[p->NY + k] != 0.0);
     P->f_x = (double)cx / (double)p->NX;
     P->f_y = (double)cy / (double)p->NY;
+    {
+        // plain rows/columns: every update range holds and the CPML is the identity there
+        const int64_t NX = P->NX, NY = P->NY, po = 1 + sub;  // po: profile index offset
+        std::vector<uint8_t> px(NX + 2, 0), py(NY + 2, 0);
+        const T one = T(1), zero = T(0);
+        for (int64_t i = 2; i <= NX - 1; i++)
+            px[i] = (T)ax[i - po] == zero && (T)ax[p->NX + i - po] == zero && (T)bx[i - po] == one &&
+                    (T)bx[p->NX + i - po] == one;
+        for (int64_t j = 2; j <= NY - 1; j++)
+            py[j] = (T)ay[j - po] == zero && (T)ay[p->NY + j - po] == zero && (T)by[j - po] == one &&
+                    (T)by[p->NY + j - po] == one;
+        for (int li = 0; li < 2; li++) {
+            std::vector<int4> items;
+            split_grid(px, py, NX, NY, li == 0 ? ElPlan<T>::VECP : 1, P->rect[li], items);
+            P->nitems[li] = (int)items.size();
+            ADS_TRY(P->d_items[li].upload(items));
+        }
+    }
     ADS_TRY(P->d_nsrc.upload(nsrc));
     ADS_TRY(P->d_srcv_off.upload(soff));
     ADS_TRY(P->d_rcv_ptr.upload(rp));

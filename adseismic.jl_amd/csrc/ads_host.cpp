@@ -49,6 +49,8 @@ hipEvent_t pool_get()
 }
 }  // namespace
 
+bool prof_enabled() { return g_prof_on; }
+
 int prof_start(int cls, hipStream_t st)
 {
     if (!g_prof_on) return -1;

@@ -15,10 +15,12 @@ import torch
 
 from . import _lib
 from .acoustic import _dcode, _dev, _stream, _vp
-from .params import ElasticPropagatorParams, ElasticReceiver, ElasticSource, Geometry
+from .params import (ElasticPropagatorParams, ElasticReceiver, ElasticSource, Geometry,
+                     MPIElasticPropagatorParams, MPIElasticReceiver, MPIElasticSource)
 
 __all__ = ["compute_PML_Params", "ElasticPlan", "ElasticPropagator", "ElasticPropagatorSolver",
-           "SimulatedObservation_elastic", "elastic_traces"]
+           "SimulatedObservation_elastic", "elastic_traces", "MPIElasticPropagator",
+           "MPIElasticPropagatorSolver", "MPISimulatedObservation_"]
 
 
 def compute_PML_Params(param: ElasticPropagatorParams):
@@ -44,6 +46,9 @@ class ElasticPlan:
         self.nstep = (np.full(self.nshots, param.NSTEP, np.int64) if nstep is None
                       else np.asarray(nstep, np.int64))
         self._pc = param.to_c()
+        # padded array size: (NX+2) x (NY+2), or (n+4) x (n+4) for one MPIElastic block
+        pad = 4 if self._pc.subdomain else 2
+        self.gx, self.gy = int(self._pc.NX) + pad, int(self._pc.NY) + pad
         gc = self.geom.to_c()
         h = C.c_void_p()
         _lib.check(self.lib.ads_elastic_plan_create(C.byref(self._pc), C.byref(gc), _dcode(dtype),
@@ -148,8 +153,8 @@ class ElasticPlan:
 
     def forward_history(self, rho, lam, mu, srcv):
         r, l, m = self._models(rho, lam, mu)
-        NX, NY, L = self.param.NX, self.param.NY, self.param.NSTEP + 1
-        hist = torch.empty((self.nshots, 5, L, NX + 2, NY + 2), dtype=self.dtype, device="cuda")
+        L = self.param.NSTEP + 1
+        hist = torch.empty((self.nshots, 5, L, self.gx, self.gy), dtype=self.dtype, device="cuda")
         need = int(self.lib.ads_elastic_plan_history_scratch_size(self.h))
         if self._scratch is None or self._scratch.numel() < need:
             self._scratch = torch.empty(need, dtype=torch.uint8, device="cuda")
@@ -247,6 +252,52 @@ def SimulatedObservation_elastic(ep: ElasticPropagator, rcv: ElasticReceiver):
     t = np.asarray(rcv.rcvtype, np.int64)
     if len(idx) and (idx.min() < 0 or idx.max() >= fields.shape[2] or t.min() < 0 or t.max() > 4):
         raise IndexError("receiver outside the padded grid or bad type")
+    dev = fields.device
+    rcv.rcvv = fields[torch.as_tensor(t, device=dev), :, torch.as_tensor(idx, device=dev)]
+    return rcv.rcvv
+
+
+@dataclass
+class MPIElasticPropagator:
+    """MPIElasticPropagator, MPIElastic.jl:132-142: block histories of shape (NSTEP+1, n, n)."""
+    param: MPIElasticPropagatorParams
+    src: MPIElasticSource
+    vx: Any
+    vy: Any
+    sigmaxx: Any
+    sigmayy: Any
+    sigmaxy: Any
+
+
+def MPIElasticPropagatorSolver(param: MPIElasticPropagatorParams, src: MPIElasticSource, ρ, λ, μ
+                               ) -> MPIElasticPropagator:
+    """MPIElasticPropagatorSolver(param, src, rho, lambda, mu), MPIElastic.jl:374-466, for one
+    block stepped with zero halos: (rho, lambda, mu) are the block's (n+4) x (n+4) arrays; the
+    scheme updates every block cell without parameter averaging (MPIElastic.jl:483-620).
+    Histories are differentiable through the hand-written adjoint."""
+    n = int(param.n)
+    dtype = ρ.dtype if isinstance(ρ, torch.Tensor) else torch.float64
+    models = [m if isinstance(m, torch.Tensor) and m.is_cuda else _dev(m, dtype) for m in (ρ, λ, μ)]
+    models = [m.reshape(n + 4, n + 4) for m in models]
+    nsrc = len(src.srci)
+    srcv = _dev(src.srcv, dtype).reshape(param.NSTEP, nsrc)
+    esrc = ElasticSource(src.srci, src.srcj, src.srctype, srcv)
+    none = np.zeros(0, np.int64)
+    plan = ElasticPlan(param, [esrc], [ElasticReceiver(none, none, none)], dtype=dtype)
+    hist = _ElHistoryFn.apply(*models, plan, srcv.contiguous())[0]
+    blk = hist[:, :, 2:n + 2, 2:n + 2]
+    return MPIElasticPropagator(param, src, blk[0], blk[1], blk[2], blk[3], blk[4])
+
+
+def MPISimulatedObservation_(ep: MPIElasticPropagator, rcv: MPIElasticReceiver):
+    """MPISimulatedObservation!(ep, rcv), MPIElastic.jl:662-688 (GetReceive over the n x n block
+    histories): rcvv of shape (nrcv, NSTEP+1), rcvv[r, n] = field_r[n][(rcvi-1) n + rcvj-1]."""
+    n = int(ep.param.n)
+    fields = torch.stack([ep.vx, ep.vy, ep.sigmaxx, ep.sigmayy, ep.sigmaxy]).reshape(5, ep.vx.shape[0], -1)
+    idx = (np.asarray(rcv.rcvi, np.int64) - 1) * n + (np.asarray(rcv.rcvj, np.int64) - 1)
+    t = np.asarray(rcv.rcvtype, np.int64)
+    if len(idx) and (idx.min() < 0 or idx.max() >= n * n or t.min() < 0 or t.max() > 4):
+        raise IndexError("receiver outside the block or bad type")
     dev = fields.device
     rcv.rcvv = fields[torch.as_tensor(t, device=dev), :, torch.as_tensor(idx, device=dev)]
     return rcv.rcvv

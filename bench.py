@@ -33,6 +33,27 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBPS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_acoustic_1000x1000_32shots.txt")
+
+
+def measured_traffic(kernel, shape):
+    """HBM bytes per launch of a stencil class, from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a plain device copy of known size).
+    Lines: '<class> <bytes> <rows,shots,batch>'.  Returns (bytes, profiled shape); bytes is None
+    when the summary is missing or was taken at another kernel shape."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            lines = [ln.split() for ln in f if ln.strip() and not ln.startswith("#")]
+    except OSError:
+        return None, None
+    kv = {ln[0]: ln[1:] for ln in lines if len(ln) >= 3}
+    if kernel not in kv:
+        return None, None
+    prof_shape = tuple(int(x) for x in kv[kernel][1].split(","))
+    half = None if shape is None else tuple(shape[:3] if kernel == "ac_fwd_step" else shape[3:])
+    if half != prof_shape:
+        return None, prof_shape
+    return float(kv[kernel][0]), prof_shape
 
 
 def velocity_model(NX, NY, seed):
@@ -108,6 +129,90 @@ def cpu_baseline(p, c0, src, nsteps):
             "host_threads_available": len(os.sched_getaffinity(0))}
 
 
+def elastic_workload(ads, nstep=0):
+    """Elastic FP32 workload: 600 x 600, h = 10 m, dt = 0.8 ms, 3000 steps, 30-cell CPML.
+    Layered Vp 2000-4500 m/s (seed 3) with a 5% smooth random perturbation, Vs = Vp/1.73,
+    rho = 310 Vp^0.25 (Gardner), lambda = rho (Vp^2 - 2 Vs^2), mu = rho Vs^2.  16 explosive
+    sources (sxx and syy, the same 12 Hz Ricker) evenly spaced below the top CPML, 600 receivers
+    recording vx and vy.  The gradient is taken at a sigma = 10 cell smoothing of the model."""
+    from scipy.ndimage import gaussian_filter
+
+    NX = NY = 600
+    NSTEP = nstep or 3000
+    npml, nshots = 30, 16
+    p = ads.ElasticPropagatorParams(NX=NX, NY=NY, NSTEP=NSTEP, DELTAX=10.0, DELTAY=10.0,
+                                    DELTAT=0.8e-3, f0=12.0, vp_ref=4500.0, NPOINTS_PML=npml)
+    rng = np.random.default_rng(3)
+    nl = 10
+    depths = np.sort(rng.choice(np.arange(1, NY + 1), size=nl - 1, replace=False))
+    vels = np.sort(rng.uniform(2000.0, 4500.0, size=nl))
+    vp = np.repeat(vels[np.searchsorted(depths, np.arange(NY + 2), side="right")][None, :], NX + 2, 0)
+    noise = gaussian_filter(rng.standard_normal((NX + 2, NY + 2)), 8.0)
+    vp = vp * (1.0 + 0.05 * np.clip(noise / noise.std(), -1.0, 1.0))
+
+    def lame(v):
+        vs = v / 1.73
+        rho = 310.0 * v ** 0.25
+        return rho, rho * (v * v - 2.0 * vs * vs), rho * vs * vs
+
+    true = lame(vp)
+    start = lame(gaussian_filter(vp, 10.0, mode="reflect"))
+    w = ads.ricker_freq(12.0, p.DELTAT, NSTEP, 1e6)
+    xs = np.round(np.linspace(npml + 2, NX + 1 - npml, nshots)).astype(np.int64)
+    srcs, rcvs = [], []
+    for s in range(nshots):
+        srcs.append(ads.ElasticSource([int(xs[s])] * 2, [npml + 3] * 2, [2, 3],
+                                      np.stack([w, w], 1)))
+        ri = np.concatenate([np.arange(2, NX + 2)] * 2)
+        rcvs.append(ads.ElasticReceiver(ri, np.full(2 * NX, npml + 3), np.repeat([0, 1], NX)))
+    return p, true, start, srcs, rcvs
+
+
+def elastic_leg(ads, L, lib, steps, nstep=0):
+    """Batched FP32 elastic FWI gradient (d/d rho, lambda, mu) throughput and kernel roofline."""
+    import ctypes as C
+    import torch
+
+    p, true, start, srcs, rcvs = elastic_workload(ads, nstep)
+    plan = ads.ElasticPlan(p, srcs, rcvs, dtype=torch.float32)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    K = plan.choose_interval(int(torch.cuda.mem_get_info()[0] * 0.80))
+    tt = [torch.as_tensor(x, dtype=torch.float32, device="cuda") for x in true]
+    st = [torch.as_tensor(x, dtype=torch.float32, device="cuda") for x in start]
+    obs = plan.forward(*tt, srcv, K).clone()
+    plan.gradient(*st, srcv, obs, K)
+    torch.cuda.synchronize()
+    L.check(lib.ads_profile_begin())
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss, grads, _ = plan.gradient(*st, srcv, obs, K)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    L.check(lib.ads_profile_end())
+    cells = float(p.NX * p.NY * np.sum(plan.nstep))
+    kern = {}
+    for name, cls in (("el_fwd_stress", 2), ("el_fwd_vel", 3), ("el_adj_vel", 4), ("el_adj_stress", 5)):
+        tm, n, b = C.c_double(), C.c_int64(), C.c_double()
+        L.check(lib.ads_profile_query(cls, C.byref(tm), C.byref(n), C.byref(b)))
+        kern[name] = {"total_ms": round(tm.value, 2), "launches": n.value,
+                      "avg_us": round(tm.value / n.value * 1e3, 2) if n.value else None,
+                      "alg_GBps": round(b.value / (tm.value * 1e-3) / 1e9, 1) if tm.value else None}
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    ach = kern[dom]["alg_GBps"] or 0.0
+    out = {"workload": (f"elastic {p.NX}x{p.NY}, h=10 m, dt=0.8 ms, {p.NSTEP} steps, "
+                        f"{p.NPOINTS_PML}-cell CPML, {plan.nshots} explosive shots x {p.NX} "
+                        "two-component receivers, FP32 forward + hand-written adjoint for "
+                        "(rho, lambda, mu), one batched call per gradient"),
+           "value": round(cells / t / 1e9, 4), "unit": "Gcell-steps/s",
+           "ms_per_gradient": round(t * 1e3, 2), "checkpoint_interval": K,
+           "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": PEAK_HBM_GBPS,
+                        "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBPS, 4)},
+           "kernels": kern, "loss": float(loss.item())}
+    plan._ws = None
+    torch.cuda.empty_cache()
+    return out
+
+
 def plain_single_shot(ads, p, c0, src, rcv, obs_shot):
     """The plain drop-in: history solver + SimulatedObservation_ + torch autograd, one shot."""
     import torch
@@ -151,6 +256,11 @@ def main():
                          "per batch of the forward and adjoint kernels) instead of the autotuner")
     ap.add_argument("--nstep", type=int, default=0,
                     help="override the record length (profiling runs only; not the metric)")
+    ap.add_argument("--no-elastic", action="store_true", help="skip the elastic gradient leg")
+    ap.add_argument("--elastic-only", action="store_true",
+                    help="run only the elastic gradient leg (profiling runs; no bench line)")
+    ap.add_argument("--elastic-nstep", type=int, default=0,
+                    help="override the elastic record length (profiling runs only)")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -167,6 +277,10 @@ def main():
     L = importlib.import_module("adseismic_amd._lib")
     lib = ads.load_library()
     torch.cuda.init()
+    if args.elastic_only:
+        print(json.dumps({"elastic": elastic_leg(ads, L, lib, args.steps, args.elastic_nstep)}),
+              flush=True)
+        return
     p, c_true, c0, srcs, rcvs = workload(ads, small=args.small, nstep=args.nstep)
     if args.calib_copy:
         n = args.calib_copy * (1 << 20) // 4
@@ -211,6 +325,7 @@ def main():
     dom = max(prof, key=lambda k: prof[k][0])
     tm, n, b = prof[dom]
     achieved = (b / n) / (tm / n * 1e-3) / 1e9 if n else 0.0
+    traffic, traffic_shape = measured_traffic(dom, shape)
     out = {
         "metric": "FWI gradient Gcell-steps/s (fwd+adj), batched",
         "value": round(value, 4), "unit": "Gcell-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -227,7 +342,10 @@ def main():
                    "shots": plan.nshots, "cells": p.NX * p.NY, "nstep": p.NSTEP},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
-                     "traffic": None, "avg_launch_us": round(tm / n * 1e3, 2) if n else None,
+                     "traffic": traffic, "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)
+                     if traffic else (f"profiled at shape {traffic_shape}, not this run's"
+                                      if traffic_shape else None),
+                     "avg_launch_us": round(tm / n * 1e3, 2) if n else None,
                      "launches": n, "alg_bytes_per_launch": round(b / n) if n else None},
         "kernels": {k: {"total_ms": round(v[0], 2), "launches": v[1],
                         "avg_us": round(v[0] / v[1] * 1e3, 2) if v[1] else None,
@@ -239,6 +357,8 @@ def main():
     plan._ws = None  # release the batched workspace before the full-history run
     del grad
     torch.cuda.empty_cache()
+    if not args.no_elastic:
+        out["elastic"] = elastic_leg(ads, L, lib, 1, args.elastic_nstep)
     if not args.no_plain:
         out["plain_single_shot"] = plain_single_shot(ads, p, c0, srcs[0], rcvs[0], tr_true)
         out["plain_single_shot"]["batched_over_plain"] = round(value / out["plain_single_shot"]["value"], 2)

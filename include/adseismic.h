@@ -45,7 +45,7 @@ int ads_version(void);
 
 /* ======================================================================================
  * Drop-in per-step operators.  Each replaces one TensorFlow custom op of the reference
- * (deps/CustomOps/*) with the same argument meaning; fields are device arrays of
+ * (under deps/CustomOps) with the same argument meaning; fields are device arrays of
  * N = (NX+2)*(NY+2) elements; scalar arguments are host values.
  * ==================================================================================== */
 
@@ -175,10 +175,19 @@ typedef struct {
     int64_t NPOINTS_PML;
     double NPOWER, K_MAX_PML, ALPHA_MAX_PML, Rcoef;
     /* 0: ElasticPropagatorSolver semantics (parameter averaging, reduced cell ranges,
-     *    src/Core.jl:96-213); 1: one MPIElastic subdomain (no averaging, every local cell,
-     *    MPIElastic.jl:483-620). */
+     *    src/Core.jl:96-213), arrays of (NX+2) x (NY+2).
+     * 1: one MPIElastic block stepped with zero halos (MPIElasticPropagatorSolver,
+     *    MPIElastic.jl:374-620): NX x NY is the block (n x n there), arrays are
+     *    (NX+4) x (NY+4) with the block at 1-based [3, NX+2] x [3, NY+2]; every fw updates the
+     *    whole block, parameters are not averaged, the CPML profiles have NX (NY) entries
+     *    indexed by the block cell.  Sources are 1-based into the (NX+4) x (NY+4) arrays (the
+     *    MPIElasticSource local index, MPIElastic.jl:85-86); receivers are 1-based into the
+     *    block (MPIElasticReceiver, MPIElastic.jl:110-111). */
     int32_t subdomain;
-    double origin_x, origin_y; /* physical x/y of profile index 0 */
+    /* physical x/y of profile index 0 (MPIElastic.jl:231, 306: (II-1) n DELTAX and, as the
+     * reference writes it, (JJ-1) n DELTAX) and the global grid size whose far edge places the
+     * right/top CPML (0: NX, NY) */
+    double origin_x, origin_y;
     int64_t NX_global, NY_global;
 } ads_elastic_params;
 
@@ -258,6 +267,14 @@ int ads_acoustic_plan_autotune(ads_acoustic_plan *plan, const void *model, const
                                void *workspace, size_t ws_bytes, int64_t K, int reps,
                                int *shape_out, ads_stream_t stream);
 int ads_acoustic_plan_set_shape(ads_acoustic_plan *plan, const int *shape);
+
+/* hipGraph replay (no counterpart in the reference): with enable = 1, each checkpoint segment
+ * of the forward sweep, of the recomputation and of the adjoint sweep is captured once into a
+ * graph (on a private stream) and replayed on the caller's stream afterwards.  A graph is keyed
+ * by the segment, the workspace, srcv, traces and residual addresses; calls with other buffers
+ * capture new graphs.  Graphs are not used while ads_profile_begin() timing is active, and are
+ * dropped by set_shape/autotune and at plan destruction. */
+int ads_acoustic_plan_set_graph(ads_acoustic_plan *plan, int enable);
 
 /* Plain drop-in of AcousticPropagatorSolver (Core.jl:562-620) on the plan's shots (every shot
  * must have nstep == NSTEP): writes the full histories u, phi, psi, each laid out

@@ -201,20 +201,22 @@ def elastic_pml(NX, NY, hx, hy, dt, npml, npower, kmax, alpha_max, Rcoef, vp_ref
     return ax, bx, kx, ay, by, ky
 
 
-def elastic_step(NX, NY, hx, hy, dt, rho, lam, mu, ax, bx, ay, by, f, m, srci, srcj, srctype, srcv):
-    """One elastic step in place (Core.jl:215-228 + AddSource.cpp:57-85)."""
+def elastic_step(NX, NY, hx, hy, dt, rho, lam, mu, ax, bx, ay, by, f, m, srci, srcj, srctype, srcv,
+                 sub=False):
+    """One elastic step in place (Core.jl:215-228 + AddSource.cpp:57-85).  sub: the
+    one-subdomain form of MPIElastic.jl (NX, NY = block + 2; profiles of the block length)."""
     f = _f64(f)
     m = _f64(m)
     srci, srcj, srctype = _i64(srci), _i64(srcj), _i64(srctype)
     lib().or_elastic_step(i64(NX), i64(NY), f64(hx), f64(hy), f64(dt), _d(_f64(rho).ravel()),
                           _d(_f64(lam).ravel()), _d(_f64(mu).ravel()), _d(_f64(ax)), _d(_f64(bx)),
                           _d(_f64(ay)), _d(_f64(by)), _d(f), _d(m), i64(len(srci)), _i(srci),
-                          _i(srcj), _i(srctype), _d(_f64(srcv)))
+                          _i(srcj), _i(srctype), _d(_f64(srcv)), C.c_int(int(sub)))
     return f, m
 
 
 def elastic_forward(NX, NY, NSTEP, hx, hy, dt, rho, lam, mu, pml, srci, srcj, srctype, srcv,
-                    keep_mem=True):
+                    keep_mem=True, sub=False):
     """ElasticPropagatorSolver (Core.jl:31-94): histories (5, NSTEP+1, NX+2, NY+2) in the order
     vx, vy, sxx, syy, sxy, and memory variables (8, NSTEP+1, ...)."""
     ax, bx, _, ay, by, _ = pml
@@ -227,7 +229,7 @@ def elastic_forward(NX, NY, NSTEP, hx, hy, dt, rho, lam, mu, pml, srci, srcj, sr
                              _d(_f64(rho).ravel()), _d(_f64(lam).ravel()), _d(_f64(mu).ravel()),
                              _d(_f64(ax)), _d(_f64(bx)), _d(_f64(ay)), _d(_f64(by)),
                              i64(len(srci)), _i(srci), _i(srcj), _i(srctype), _d(srcv), _d(hist),
-                             _d(mem) if mem is not None else None)
+                             _d(mem) if mem is not None else None, C.c_int(int(sub)))
     sh = (NSTEP + 1, NX + 2, NY + 2)
     return hist.reshape((5,) + sh), (mem.reshape((8,) + sh) if mem is not None else None)
 
@@ -243,7 +245,7 @@ def elastic_receive(hist, rcvi, rcvj, rcvtype, NX, NY):
 
 
 def elastic_adjoint(NX, NY, NSTEP, hx, hy, dt, rho, lam, mu, pml, hist, mem_hist, g_hist,
-                    srci=None, srcj=None, srctype=None):
+                    srci=None, srcj=None, srctype=None, sub=False):
     """Reverse sweep of the elastic loop: grads wrt rho, lambda, mu (and srcv if sources given)."""
     ax, bx, _, ay, by, _ = pml
     N = (NX + 2) * (NY + 2)
@@ -258,7 +260,8 @@ def elastic_adjoint(NX, NY, NSTEP, hx, hy, dt, rho, lam, mu, pml, hist, mem_hist
                              _d(_f64(ax)), _d(_f64(bx)), _d(_f64(ay)), _d(_f64(by)),
                              _d(_f64(hist).ravel()), _d(_f64(mem_hist).ravel()),
                              _d(_f64(g_hist).ravel()), _d(gr), _d(gl), _d(gm), i64(nsrc), _i(si),
-                             _i(sj), _i(st), _d(gsrc) if gsrc is not None else None)
+                             _i(sj), _i(st), _d(gsrc) if gsrc is not None else None,
+                             C.c_int(int(sub)))
     sh = (NX + 2, NY + 2)
     out = (gr.reshape(sh), gl.reshape(sh), gm.reshape(sh))
     if gsrc is not None:
@@ -267,10 +270,11 @@ def elastic_adjoint(NX, NY, NSTEP, hx, hy, dt, rho, lam, mu, pml, hist, mem_hist
 
 
 def elastic_misfit_gradient(NX, NY, NSTEP, hx, hy, dt, rho, lam, mu, pml, srci, srcj, srctype, srcv,
-                            rcvi, rcvj, rcvtype, observed):
-    """loss = sum((rcvv - observed)^2) and grads wrt rho, lambda, mu."""
+                            rcvi, rcvj, rcvtype, observed, sub=False):
+    """loss = sum((rcvv - observed)^2) and grads wrt rho, lambda, mu.  Receiver indices are
+    1-based into the padded grid (for sub, the block index + 2)."""
     hist, mem = elastic_forward(NX, NY, NSTEP, hx, hy, dt, rho, lam, mu, pml, srci, srcj, srctype,
-                                srcv)
+                                srcv, sub=sub)
     rc = elastic_receive(hist, rcvi, rcvj, rcvtype, NX, NY)
     res = rc - observed
     loss = float(np.sum(res * res))
@@ -278,7 +282,7 @@ def elastic_misfit_gradient(NX, NY, NSTEP, hx, hy, dt, rho, lam, mu, pml, srci, 
     idx = flat_index(rcvi, rcvj, NY)
     for r in range(len(idx)):
         g[int(rcvtype[r]), :, idx[r]] += 2.0 * res[r]
-    gr, gl, gm = elastic_adjoint(NX, NY, NSTEP, hx, hy, dt, rho, lam, mu, pml, hist, mem, g)
+    gr, gl, gm = elastic_adjoint(NX, NY, NSTEP, hx, hy, dt, rho, lam, mu, pml, hist, mem, g, sub=sub)
     return loss, (gr, gl, gm), rc
 
 

@@ -344,7 +344,28 @@ typedef struct {
     double hx, hy, dt;
     const double *rho, *lam, *mu;
     const double *axf, *axh, *bxf, *bxh, *ayf, *ayh, *byf, *byh;
+    /* sub = 1: the one-subdomain form of MPIElastic.jl (a block of (NX-2) x (NY-2) cells at
+     * [2, NX-1] x [2, NY-1] inside a 2-cell zero halo): every fw updates the whole block
+     * (MPIElastic.jl:483-620, index set ij = kk x kk), parameters are not averaged
+     * (:499-502, 537-538, 605-606), and the profile index is i-2 / j-2 (length NX-2). */
+    int sub;
 } el_ctx;
+
+/* update range of fw1..fw4 (Core.jl:100-107, 132-139, 162-169, 190-197; whole block if sub) */
+static void el_range(const el_ctx *e, int fw, i64 *ilo, i64 *ihi, i64 *jlo, i64 *jhi)
+{
+    if (e->sub) {
+        *ilo = 2, *ihi = e->NX - 1, *jlo = 2, *jhi = e->NY - 1;
+        return;
+    }
+    switch (fw) {
+    case 1: *ilo = 1, *ihi = e->NX - 1, *jlo = 2, *jhi = e->NY; break;
+    case 2: *ilo = 2, *ihi = e->NX, *jlo = 1, *jhi = e->NY - 1; break;
+    case 3: *ilo = 2, *ihi = e->NX, *jlo = 2, *jhi = e->NY; break;
+    default: *ilo = 1, *ihi = e->NX - 1, *jlo = 1, *jhi = e->NY - 1; break;
+    }
+}
+#define EL_PO(e) ((e)->sub ? 2 : 1) /* profile index offset */
 
 /* fw1: sxx, syy, m1, m2.  Core.jl:96-126.  i in [1, NX-1], j in [2, NY] (0-based). */
 static void el_fw1(const el_ctx *e, const double *vx, const double *vy, double *sxx, double *syy,
@@ -353,17 +374,20 @@ static void el_fw1(const el_ctx *e, const double *vx, const double *vy, double *
     const i64 S = e->NY + 2, NX = e->NX, NY = e->NY;
     const i64 N = (NX + 2) * S;
     double *n1 = (double *)calloc(N, sizeof(double)), *n2 = (double *)calloc(N, sizeof(double));
+    i64 ilo, ihi, jlo, jhi;
+    el_range(e, 1, &ilo, &ihi, &jlo, &jhi);
+    const i64 po = EL_PO(e);
 #pragma omp parallel for schedule(static)
-    for (i64 i = 1; i <= NX - 1; i++)
-        for (i64 j = 2; j <= NY; j++) {
+    for (i64 i = ilo; i <= ihi; i++)
+        for (i64 j = jlo; j <= jhi; j++) {
             i64 ij = i * S + j;
-            double lb = 0.5 * (e->lam[ij + S] + e->lam[ij]);
-            double mb = 0.5 * (e->mu[ij + S] + e->mu[ij]);
+            double lb = e->sub ? e->lam[ij] : 0.5 * (e->lam[ij + S] + e->lam[ij]);
+            double mb = e->sub ? e->mu[ij] : 0.5 * (e->mu[ij + S] + e->mu[ij]);
             double lm = lb + 2 * mb;
             double dx = (27 * vx[ij + S] - 27 * vx[ij] - vx[ij + 2 * S] + vx[ij - S]) / (24 * e->hx);
             double dy = (27 * vy[ij] - 27 * vy[ij - 1] - vy[ij + 1] + vy[ij - 2]) / (24 * e->hy);
-            n1[ij] = e->bxh[i - 1] * m1[ij] + e->axh[i - 1] * dx;
-            n2[ij] = e->byf[j - 1] * m2[ij] + e->ayf[j - 1] * dy;
+            n1[ij] = e->bxh[i - po] * m1[ij] + e->axh[i - po] * dx;
+            n2[ij] = e->byf[j - po] * m2[ij] + e->ayf[j - po] * dy;
             dx = dx + n1[ij];
             dy = dy + n2[ij];
             sxx[ij] += (lm * dx + lb * dy) * e->dt;
@@ -382,15 +406,18 @@ static void el_fw2(const el_ctx *e, const double *vx, const double *vy, double *
     const i64 S = e->NY + 2, NX = e->NX, NY = e->NY;
     const i64 N = (NX + 2) * S;
     double *n3 = (double *)calloc(N, sizeof(double)), *n4 = (double *)calloc(N, sizeof(double));
+    i64 ilo, ihi, jlo, jhi;
+    el_range(e, 2, &ilo, &ihi, &jlo, &jhi);
+    const i64 po = EL_PO(e);
 #pragma omp parallel for schedule(static)
-    for (i64 i = 2; i <= NX; i++)
-        for (i64 j = 1; j <= NY - 1; j++) {
+    for (i64 i = ilo; i <= ihi; i++)
+        for (i64 j = jlo; j <= jhi; j++) {
             i64 ij = i * S + j;
-            double mb = 0.5 * (e->mu[ij] + e->mu[ij + 1]);
+            double mb = e->sub ? e->mu[ij] : 0.5 * (e->mu[ij] + e->mu[ij + 1]);
             double dx = (27 * vy[ij] - 27 * vy[ij - S] - vy[ij + S] + vy[ij - 2 * S]) / (24 * e->hx);
             double dy = (27 * vx[ij + 1] - 27 * vx[ij] - vx[ij + 2] + vx[ij - 1]) / (24 * e->hy);
-            n3[ij] = e->bxf[i - 1] * m3[ij] + e->axf[i - 1] * dx;
-            n4[ij] = e->byh[j - 1] * m4[ij] + e->ayh[j - 1] * dy;
+            n3[ij] = e->bxf[i - po] * m3[ij] + e->axf[i - po] * dx;
+            n4[ij] = e->byh[j - po] * m4[ij] + e->ayh[j - po] * dy;
             dx = dx + n3[ij];
             dy = dy + n4[ij];
             sxy[ij] += mb * (dx + dy) * e->dt;
@@ -408,14 +435,17 @@ static void el_fw3(const el_ctx *e, const double *sxx, const double *sxy, double
     const i64 S = e->NY + 2, NX = e->NX, NY = e->NY;
     const i64 N = (NX + 2) * S;
     double *n5 = (double *)calloc(N, sizeof(double)), *n6 = (double *)calloc(N, sizeof(double));
+    i64 ilo, ihi, jlo, jhi;
+    el_range(e, 3, &ilo, &ihi, &jlo, &jhi);
+    const i64 po = EL_PO(e);
 #pragma omp parallel for schedule(static)
-    for (i64 i = 2; i <= NX; i++)
-        for (i64 j = 2; j <= NY; j++) {
+    for (i64 i = ilo; i <= ihi; i++)
+        for (i64 j = jlo; j <= jhi; j++) {
             i64 ij = i * S + j;
             double dx = (27 * sxx[ij] - 27 * sxx[ij - S] - sxx[ij + S] + sxx[ij - 2 * S]) / (24 * e->hx);
             double dy = (27 * sxy[ij] - 27 * sxy[ij - 1] - sxy[ij + 1] + sxy[ij - 2]) / (24 * e->hy);
-            n5[ij] = e->bxf[i - 1] * m5[ij] + e->axf[i - 1] * dx;
-            n6[ij] = e->byf[j - 1] * m6[ij] + e->ayf[j - 1] * dy;
+            n5[ij] = e->bxf[i - po] * m5[ij] + e->axf[i - po] * dx;
+            n6[ij] = e->byf[j - po] * m6[ij] + e->ayf[j - po] * dy;
             dx = dx + n5[ij];
             dy = dy + n6[ij];
             vx[ij] += (dx + dy) * e->dt / e->rho[ij];
@@ -433,15 +463,19 @@ static void el_fw4(const el_ctx *e, const double *sxy, const double *syy, double
     const i64 S = e->NY + 2, NX = e->NX, NY = e->NY;
     const i64 N = (NX + 2) * S;
     double *n7 = (double *)calloc(N, sizeof(double)), *n8 = (double *)calloc(N, sizeof(double));
+    i64 ilo, ihi, jlo, jhi;
+    el_range(e, 4, &ilo, &ihi, &jlo, &jhi);
+    const i64 po = EL_PO(e);
 #pragma omp parallel for schedule(static)
-    for (i64 i = 1; i <= NX - 1; i++)
-        for (i64 j = 1; j <= NY - 1; j++) {
+    for (i64 i = ilo; i <= ihi; i++)
+        for (i64 j = jlo; j <= jhi; j++) {
             i64 ij = i * S + j;
-            double rb = 0.25 * (e->rho[ij] + e->rho[ij + S] + e->rho[ij + S + 1] + e->rho[ij + 1]);
+            double rb = e->sub ? e->rho[ij]
+                               : 0.25 * (e->rho[ij] + e->rho[ij + S] + e->rho[ij + S + 1] + e->rho[ij + 1]);
             double dx = (27 * sxy[ij + S] - 27 * sxy[ij] - sxy[ij + 2 * S] + sxy[ij - S]) / (24 * e->hx);
             double dy = (27 * syy[ij + 1] - 27 * syy[ij] - syy[ij + 2] + syy[ij - 1]) / (24 * e->hy);
-            n7[ij] = e->bxh[i - 1] * m7[ij] + e->axh[i - 1] * dx;
-            n8[ij] = e->byh[j - 1] * m8[ij] + e->ayh[j - 1] * dy;
+            n7[ij] = e->bxh[i - po] * m7[ij] + e->axh[i - po] * dx;
+            n8[ij] = e->byh[j - po] * m8[ij] + e->ayh[j - po] * dy;
             dx = dx + n7[ij];
             dy = dy + n8[ij];
             vy[ij] += (dx + dy) * e->dt / rb;
@@ -454,8 +488,10 @@ static void el_fw4(const el_ctx *e, const double *sxy, const double *syy, double
 
 static void el_ctx_init(el_ctx *e, i64 NX, i64 NY, double hx, double hy, double dt,
                         const double *rho, const double *lam, const double *mu, const double *ax,
-                        const double *bx, const double *ay, const double *by)
+                        const double *bx, const double *ay, const double *by, int sub)
 {
+    const i64 PX = sub ? NX - 2 : NX, PY = sub ? NY - 2 : NY; /* profile lengths */
+    e->sub = sub;
     e->NX = NX;
     e->NY = NY;
     e->hx = hx;
@@ -465,13 +501,13 @@ static void el_ctx_init(el_ctx *e, i64 NX, i64 NY, double hx, double hy, double 
     e->lam = lam;
     e->mu = mu;
     e->axf = ax;
-    e->axh = ax + NX;
+    e->axh = ax + PX;
     e->bxf = bx;
-    e->bxh = bx + NX;
+    e->bxh = bx + PX;
     e->ayf = ay;
-    e->ayh = ay + NY;
+    e->ayh = ay + PY;
     e->byf = by;
-    e->byh = by + NY;
+    e->byh = by + PY;
 }
 
 /* ----------------------------------------------------------------------------------------
@@ -483,10 +519,11 @@ static void el_ctx_init(el_ctx *e, i64 NX, i64 NY, double hx, double hy, double 
 void or_elastic_step(i64 NX, i64 NY, double hx, double hy, double dt, const double *rho,
                      const double *lam, const double *mu, const double *ax, const double *bx,
                      const double *ay, const double *by, double *f, double *m, i64 nsrc,
-                     const i64 *srci, const i64 *srcj, const i64 *srctype, const double *srcv)
+                     const i64 *srci, const i64 *srcj, const i64 *srctype, const double *srcv,
+                     int sub)
 {
     el_ctx e;
-    el_ctx_init(&e, NX, NY, hx, hy, dt, rho, lam, mu, ax, bx, ay, by);
+    el_ctx_init(&e, NX, NY, hx, hy, dt, rho, lam, mu, ax, bx, ay, by, sub);
     const i64 N = (NX + 2) * (NY + 2), S = NY + 2;
     double *vx = f, *vy = f + N, *sxx = f + 2 * N, *syy = f + 3 * N, *sxy = f + 4 * N;
     el_fw1(&e, vx, vy, sxx, syy, m, m + N);
@@ -507,7 +544,7 @@ void or_elastic_forward(i64 NX, i64 NY, i64 NSTEP, double hx, double hy, double 
                         const double *rho, const double *lam, const double *mu, const double *ax,
                         const double *bx, const double *ay, const double *by, i64 nsrc,
                         const i64 *srci, const i64 *srcj, const i64 *srctype, const double *srcv,
-                        double *hist, double *mem_hist)
+                        double *hist, double *mem_hist, int sub)
 {
     const i64 N = (NX + 2) * (NY + 2), L = (NSTEP + 1) * N;
     double *f = (double *)calloc(5 * N, sizeof(double));
@@ -517,7 +554,7 @@ void or_elastic_forward(i64 NX, i64 NY, i64 NSTEP, double hx, double hy, double 
         for (int k = 0; k < 8; k++) memset(mem_hist + k * L, 0, N * sizeof(double));
     for (i64 n = 1; n <= NSTEP; n++) {
         or_elastic_step(NX, NY, hx, hy, dt, rho, lam, mu, ax, bx, ay, by, f, m, nsrc, srci, srcj,
-                        srctype, srcv + (n - 1) * nsrc);
+                        srctype, srcv + (n - 1) * nsrc, sub);
         for (int k = 0; k < 5; k++) memcpy(hist + k * L + n * N, f + k * N, N * sizeof(double));
         if (mem_hist)
             for (int k = 0; k < 8; k++) memcpy(mem_hist + k * L + n * N, m + k * N, N * sizeof(double));
@@ -552,27 +589,35 @@ static void el_fw4_vjp(const el_ctx *e, const double *sxy, const double *syy, co
     double *o7 = (double *)calloc(N, sizeof(double)), *o8 = (double *)calloc(N, sizeof(double));
     const double c1 = 27.0 / (24 * e->hx), c2 = 1.0 / (24 * e->hx);
     const double d1 = 27.0 / (24 * e->hy), d2 = 1.0 / (24 * e->hy);
-    for (i64 i = 1; i <= NX - 1; i++)
-        for (i64 j = 1; j <= NY - 1; j++) {
+    i64 ilo, ihi, jlo, jhi;
+    el_range(e, 4, &ilo, &ihi, &jlo, &jhi);
+    const i64 po = EL_PO(e);
+    for (i64 i = ilo; i <= ihi; i++)
+        for (i64 j = jlo; j <= jhi; j++) {
             i64 ij = i * S + j;
-            double rb = 0.25 * (e->rho[ij] + e->rho[ij + S] + e->rho[ij + S + 1] + e->rho[ij + 1]);
+            double rb = e->sub ? e->rho[ij]
+                               : 0.25 * (e->rho[ij] + e->rho[ij + S] + e->rho[ij + S + 1] + e->rho[ij + 1]);
             double dx = (27 * sxy[ij + S] - 27 * sxy[ij] - sxy[ij + 2 * S] + sxy[ij - S]) / (24 * e->hx);
             double dy = (27 * syy[ij + 1] - 27 * syy[ij] - syy[ij + 2] + syy[ij - 1]) / (24 * e->hy);
-            double n7 = e->bxh[i - 1] * m7[ij] + e->axh[i - 1] * dx;
-            double n8 = e->byh[j - 1] * m8[ij] + e->ayh[j - 1] * dy;
+            double n7 = e->bxh[i - po] * m7[ij] + e->axh[i - po] * dx;
+            double n8 = e->byh[j - po] * m8[ij] + e->ayh[j - po] * dy;
             double g = Gvy[ij];
             double sum = dx + n7 + dy + n8;
             double grb = -g * sum * e->dt / (rb * rb);
-            gr[ij] += 0.25 * grb;
-            gr[ij + S] += 0.25 * grb;
-            gr[ij + S + 1] += 0.25 * grb;
-            gr[ij + 1] += 0.25 * grb;
+            if (e->sub) {
+                gr[ij] += grb;
+            } else {
+                gr[ij] += 0.25 * grb;
+                gr[ij + S] += 0.25 * grb;
+                gr[ij + S + 1] += 0.25 * grb;
+                gr[ij + 1] += 0.25 * grb;
+            }
             double gs = g * e->dt / rb;               /* cotangent of (dx+n7+dy+n8) */
             double g7 = Gm7[ij] + gs, g8 = Gm8[ij] + gs; /* total cotangent of n7, n8 */
-            double gdx = gs + e->axh[i - 1] * g7;
-            double gdy = gs + e->ayh[j - 1] * g8;
-            o7[ij] = e->bxh[i - 1] * g7;
-            o8[ij] = e->byh[j - 1] * g8;
+            double gdx = gs + e->axh[i - po] * g7;
+            double gdy = gs + e->ayh[j - po] * g8;
+            o7[ij] = e->bxh[i - po] * g7;
+            o8[ij] = e->byh[j - po] * g8;
             Gsxy[ij + S] += c1 * gdx;
             Gsxy[ij] -= c1 * gdx;
             Gsxy[ij + 2 * S] -= c2 * gdx;
@@ -596,22 +641,25 @@ static void el_fw3_vjp(const el_ctx *e, const double *sxx, const double *sxy, co
     double *o5 = (double *)calloc(N, sizeof(double)), *o6 = (double *)calloc(N, sizeof(double));
     const double c1 = 27.0 / (24 * e->hx), c2 = 1.0 / (24 * e->hx);
     const double d1 = 27.0 / (24 * e->hy), d2 = 1.0 / (24 * e->hy);
-    for (i64 i = 2; i <= NX; i++)
-        for (i64 j = 2; j <= NY; j++) {
+    i64 ilo, ihi, jlo, jhi;
+    el_range(e, 3, &ilo, &ihi, &jlo, &jhi);
+    const i64 po = EL_PO(e);
+    for (i64 i = ilo; i <= ihi; i++)
+        for (i64 j = jlo; j <= jhi; j++) {
             i64 ij = i * S + j;
             double dx = (27 * sxx[ij] - 27 * sxx[ij - S] - sxx[ij + S] + sxx[ij - 2 * S]) / (24 * e->hx);
             double dy = (27 * sxy[ij] - 27 * sxy[ij - 1] - sxy[ij + 1] + sxy[ij - 2]) / (24 * e->hy);
-            double n5 = e->bxf[i - 1] * m5[ij] + e->axf[i - 1] * dx;
-            double n6 = e->byf[j - 1] * m6[ij] + e->ayf[j - 1] * dy;
+            double n5 = e->bxf[i - po] * m5[ij] + e->axf[i - po] * dx;
+            double n6 = e->byf[j - po] * m6[ij] + e->ayf[j - po] * dy;
             double g = Gvx[ij];
             double r = e->rho[ij];
             gr[ij] += -g * (dx + n5 + dy + n6) * e->dt / (r * r);
             double gs = g * e->dt / r;
             double g5 = Gm5[ij] + gs, g6 = Gm6[ij] + gs;
-            double gdx = gs + e->axf[i - 1] * g5;
-            double gdy = gs + e->ayf[j - 1] * g6;
-            o5[ij] = e->bxf[i - 1] * g5;
-            o6[ij] = e->byf[j - 1] * g6;
+            double gdx = gs + e->axf[i - po] * g5;
+            double gdy = gs + e->ayf[j - po] * g6;
+            o5[ij] = e->bxf[i - po] * g5;
+            o6[ij] = e->byf[j - po] * g6;
             Gsxx[ij] += c1 * gdx;
             Gsxx[ij - S] -= c1 * gdx;
             Gsxx[ij + S] -= c2 * gdx;
@@ -635,24 +683,31 @@ static void el_fw2_vjp(const el_ctx *e, const double *vx, const double *vy, cons
     double *o3 = (double *)calloc(N, sizeof(double)), *o4 = (double *)calloc(N, sizeof(double));
     const double c1 = 27.0 / (24 * e->hx), c2 = 1.0 / (24 * e->hx);
     const double d1 = 27.0 / (24 * e->hy), d2 = 1.0 / (24 * e->hy);
-    for (i64 i = 2; i <= NX; i++)
-        for (i64 j = 1; j <= NY - 1; j++) {
+    i64 ilo, ihi, jlo, jhi;
+    el_range(e, 2, &ilo, &ihi, &jlo, &jhi);
+    const i64 po = EL_PO(e);
+    for (i64 i = ilo; i <= ihi; i++)
+        for (i64 j = jlo; j <= jhi; j++) {
             i64 ij = i * S + j;
-            double mb = 0.5 * (e->mu[ij] + e->mu[ij + 1]);
+            double mb = e->sub ? e->mu[ij] : 0.5 * (e->mu[ij] + e->mu[ij + 1]);
             double dx = (27 * vy[ij] - 27 * vy[ij - S] - vy[ij + S] + vy[ij - 2 * S]) / (24 * e->hx);
             double dy = (27 * vx[ij + 1] - 27 * vx[ij] - vx[ij + 2] + vx[ij - 1]) / (24 * e->hy);
-            double n3 = e->bxf[i - 1] * m3[ij] + e->axf[i - 1] * dx;
-            double n4 = e->byh[j - 1] * m4[ij] + e->ayh[j - 1] * dy;
+            double n3 = e->bxf[i - po] * m3[ij] + e->axf[i - po] * dx;
+            double n4 = e->byh[j - po] * m4[ij] + e->ayh[j - po] * dy;
             double g = Gsxy[ij];
             double gmb = g * (dx + n3 + dy + n4) * e->dt;
-            gm[ij] += 0.5 * gmb;
-            gm[ij + 1] += 0.5 * gmb;
+            if (e->sub) {
+                gm[ij] += gmb;
+            } else {
+                gm[ij] += 0.5 * gmb;
+                gm[ij + 1] += 0.5 * gmb;
+            }
             double gs = g * mb * e->dt;
             double g3 = Gm3[ij] + gs, g4 = Gm4[ij] + gs;
-            double gdx = gs + e->axf[i - 1] * g3;
-            double gdy = gs + e->ayh[j - 1] * g4;
-            o3[ij] = e->bxf[i - 1] * g3;
-            o4[ij] = e->byh[j - 1] * g4;
+            double gdx = gs + e->axf[i - po] * g3;
+            double gdy = gs + e->ayh[j - po] * g4;
+            o3[ij] = e->bxf[i - po] * g3;
+            o4[ij] = e->byh[j - po] * g4;
             Gvy[ij] += c1 * gdx;
             Gvy[ij - S] -= c1 * gdx;
             Gvy[ij + S] -= c2 * gdx;
@@ -676,31 +731,39 @@ static void el_fw1_vjp(const el_ctx *e, const double *vx, const double *vy, cons
     double *o1 = (double *)calloc(N, sizeof(double)), *o2 = (double *)calloc(N, sizeof(double));
     const double c1 = 27.0 / (24 * e->hx), c2 = 1.0 / (24 * e->hx);
     const double d1 = 27.0 / (24 * e->hy), d2 = 1.0 / (24 * e->hy);
-    for (i64 i = 1; i <= NX - 1; i++)
-        for (i64 j = 2; j <= NY; j++) {
+    i64 ilo, ihi, jlo, jhi;
+    el_range(e, 1, &ilo, &ihi, &jlo, &jhi);
+    const i64 po = EL_PO(e);
+    for (i64 i = ilo; i <= ihi; i++)
+        for (i64 j = jlo; j <= jhi; j++) {
             i64 ij = i * S + j;
-            double lb = 0.5 * (e->lam[ij + S] + e->lam[ij]);
-            double mb = 0.5 * (e->mu[ij + S] + e->mu[ij]);
+            double lb = e->sub ? e->lam[ij] : 0.5 * (e->lam[ij + S] + e->lam[ij]);
+            double mb = e->sub ? e->mu[ij] : 0.5 * (e->mu[ij + S] + e->mu[ij]);
             double lm = lb + 2 * mb;
             double dx = (27 * vx[ij + S] - 27 * vx[ij] - vx[ij + 2 * S] + vx[ij - S]) / (24 * e->hx);
             double dy = (27 * vy[ij] - 27 * vy[ij - 1] - vy[ij + 1] + vy[ij - 2]) / (24 * e->hy);
-            double n1 = e->bxh[i - 1] * m1[ij] + e->axh[i - 1] * dx;
-            double n2 = e->byf[j - 1] * m2[ij] + e->ayf[j - 1] * dy;
+            double n1 = e->bxh[i - po] * m1[ij] + e->axh[i - po] * dx;
+            double n2 = e->byf[j - po] * m2[ij] + e->ayf[j - po] * dy;
             double X = dx + n1, Y = dy + n2;
             double gx = Gsxx[ij] * e->dt, gy = Gsyy[ij] * e->dt;
             double glm = X * gx + Y * gy;
             double glb = Y * gx + X * gy + glm;
             double gmb = 2 * glm;
-            gl[ij] += 0.5 * glb;
-            gl[ij + S] += 0.5 * glb;
-            gm[ij] += 0.5 * gmb;
-            gm[ij + S] += 0.5 * gmb;
+            if (e->sub) {
+                gl[ij] += glb;
+                gm[ij] += gmb;
+            } else {
+                gl[ij] += 0.5 * glb;
+                gl[ij + S] += 0.5 * glb;
+                gm[ij] += 0.5 * gmb;
+                gm[ij + S] += 0.5 * gmb;
+            }
             double gX = lm * gx + lb * gy, gY = lb * gx + lm * gy;
             double g1 = Gm1[ij] + gX, g2 = Gm2[ij] + gY;
-            double gdx = gX + e->axh[i - 1] * g1;
-            double gdy = gY + e->ayf[j - 1] * g2;
-            o1[ij] = e->bxh[i - 1] * g1;
-            o2[ij] = e->byf[j - 1] * g2;
+            double gdx = gX + e->axh[i - po] * g1;
+            double gdy = gY + e->ayf[j - po] * g2;
+            o1[ij] = e->bxh[i - po] * g1;
+            o2[ij] = e->byf[j - po] * g2;
             Gvx[ij + S] += c1 * gdx;
             Gvx[ij] -= c1 * gdx;
             Gvx[ij + 2 * S] -= c2 * gdx;
@@ -728,10 +791,10 @@ void or_elastic_adjoint(i64 NX, i64 NY, i64 NSTEP, double hx, double hy, double 
                         const double *bx, const double *ay, const double *by, const double *hist,
                         const double *mem_hist, const double *g_hist, double *grho, double *glam,
                         double *gmu, i64 nsrc, const i64 *srci, const i64 *srcj,
-                        const i64 *srctype, double *gsrcv)
+                        const i64 *srctype, double *gsrcv, int sub)
 {
     el_ctx e;
-    el_ctx_init(&e, NX, NY, hx, hy, dt, rho, lam, mu, ax, bx, ay, by);
+    el_ctx_init(&e, NX, NY, hx, hy, dt, rho, lam, mu, ax, bx, ay, by, sub);
     const i64 N = (NX + 2) * (NY + 2), L = (NSTEP + 1) * N, S = NY + 2;
     double *G = (double *)calloc(5 * N, sizeof(double));
     double *Gm = (double *)calloc(8 * N, sizeof(double));

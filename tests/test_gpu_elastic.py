@@ -48,7 +48,8 @@ def make_case(orc, ads, NX=34, NY=28, NSTEP=60, npml=6, nshots=2, seed=0, uneven
 def oracle(orc, p, models, src, rcv, nstep, scale=None, obs=None):
     rho, lam, mu = models
     pml = orc.elastic_pml(p.NX, p.NY, p.DELTAX, p.DELTAY, p.DELTAT, p.NPOINTS_PML, p.NPOWER, p.K_MAX_PML,
-                          p.ALPHA_MAX_PML, p.Rcoef, p.vp_ref)
+                          p.ALPHA_MAX_PML, p.Rcoef, p.vp_ref,
+                          use=(p.USE_PML_XMIN, p.USE_PML_XMAX, p.USE_PML_YMIN, p.USE_PML_YMAX))
     if scale is not None:
         rho, lam, mu = rho * scale[0], lam * scale[1], mu * scale[2]
     if obs is None:
@@ -121,4 +122,123 @@ def test_elastic_history_dropin_and_autograd(orc, ads, cuda):
     L_ref, g_ref, _ = oracle(orc, p, models, src, rcv, p.NSTEP, obs=obs)
     assert loss.item() == pytest.approx(L_ref, rel=1e-10)
     for t, b, name in zip(mt, g_ref, ("rho", "lambda", "mu")):
+        assert rel(t.grad.cpu(), b) < 1e-10, (name, rel(t.grad.cpu(), b))
+
+
+# interior rectangle / CPML frame splits: partial CPML sides, odd sizes, no CPML (the rectangle
+# is the whole update range), and a CPML that leaves no interior rectangle
+EL_SPLITS = [
+    dict(NX=29, NY=33, npml=5, use=(False, True, True, False)),
+    dict(NX=26, NY=21, npml=4, use=(True, True, True, True)),
+    dict(NX=17, NY=23, npml=4, use=(False, False, False, False)),
+    dict(NX=14, NY=14, npml=7, use=(True, True, True, True)),
+]
+
+
+@pytest.mark.parametrize("split", range(len(EL_SPLITS)))
+def test_elastic_grid_splits_f64(orc, ads, cuda, split):
+    cfg = EL_SPLITS[split]
+    p, models, srcs, rcvs, nstep = make_case(orc, ads, NX=cfg["NX"], NY=cfg["NY"], NSTEP=40,
+                                             npml=cfg["npml"], nshots=2, seed=30 + split,
+                                             uneven=True)
+    p.USE_PML_XMIN, p.USE_PML_XMAX, p.USE_PML_YMIN, p.USE_PML_YMAX = cfg["use"]
+    obs, L_ref, g_ref = [], 0.0, [0.0, 0.0, 0.0]
+    for s in range(2):
+        o, _ = oracle(orc, p, models, srcs[s], rcvs[s], nstep[s], scale=(1.02, 0.97, 1.04))
+        L, g, _ = oracle(orc, p, models, srcs[s], rcvs[s], nstep[s], obs=o)
+        obs.append(o)
+        L_ref += L
+        g_ref = [a + b for a, b in zip(g_ref, g)]
+    plan = ads.ElasticPlan(p, srcs, rcvs, dtype=torch.float64, nstep=nstep)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    mt = [T(m) for m in models]
+    loss, g, tr = plan.gradient(*mt, srcv, plan.pack_traces(obs), 7, want_traces=True)
+    per = plan.unpack_traces(tr)
+    for s in range(2):
+        ref, _ = oracle(orc, p, models, srcs[s], rcvs[s], nstep[s])
+        assert rel(per[s].cpu(), ref) < 1e-12, (s, rel(per[s].cpu(), ref))
+    assert loss.item() == pytest.approx(L_ref, rel=1e-10)
+    for a, b, name in zip(g, g_ref, ("rho", "lambda", "mu")):
+        assert rel(a.cpu(), b) < 1e-10, (name, rel(a.cpu(), b))
+    # the full-history drop-in (dense layout) on the same split
+    ep = ads.ElasticPropagatorSolver(p, srcs[0], *mt)
+    _, hist_ref = oracle(orc, p, models, srcs[0], rcvs[0], p.NSTEP)
+    for k, f in enumerate((ep.vx, ep.vy, ep.sigmaxx, ep.sigmayy, ep.sigmaxy)):
+        assert rel(f.detach().cpu(), hist_ref[k]) < 1e-12, k
+
+
+# ------------------------------------------------------------------------------------------
+# one MPIElastic block (MPIElastic.jl), stepped with zero halos
+# ------------------------------------------------------------------------------------------
+
+def _mpi_case(ads, orc, n=16, NSTEP=40, seed=41):
+    """Global 2n x 2n grid split into 2 x 2 blocks; rank 1 steps block (II, JJ) = (1, 2).
+    DELTAX != DELTAY exercises the reference's y origin (JJ-1) n DELTAX."""
+    p = ads.MPIElasticPropagatorParams(NX=2 * n, NY=2 * n, n=n, NSTEP=NSTEP, DELTAX=10.0,
+                                       DELTAY=8.0, DELTAT=8e-4, NPOINTS_PML=5, vp_ref=2500.0,
+                                       f0=8.0)
+    ads.compute_PML_Params_mpi(p, rank=1, size=4)
+    assert (p.II, p.JJ) == (1, 2)
+    rng = np.random.default_rng(seed)
+    vp = 2300.0 + 400.0 * rng.random((n + 4, n + 4))
+    vs = vp / 1.73
+    rho = 310 * vp ** 0.25
+    models = (rho, rho * (vp ** 2 - 2 * vs ** 2), rho * vs ** 2)
+    w = orc.ricker(NSTEP, 5.0, 10.0, 1e8)
+    # global indices; the first two lie in block (1, 2), the third in another block
+    src = ads.MPIElasticSource(p, [6, 6, 30], [n + 7, n + 7, 3], [2, 3, 0],
+                               np.stack([w, w, w], 1))
+    assert list(src.srci) == [8, 8] and list(src.srcj) == [9, 9]
+    gi = np.arange(2, n, 2)
+    rcv = ads.MPIElasticReceiver(p, np.concatenate([gi, gi]), np.full(2 * len(gi), n + 3),
+                                 np.concatenate([np.zeros(len(gi)), np.ones(len(gi))]).astype(np.int64))
+    assert np.all(rcv.rcvj == 3)
+    # the oracle's block profiles: this block's origin on the global axes
+    ax, bx, kx = orc.elastic_pml_axis(n, p.DELTAX, 0.0, p.NX, p.NPOINTS_PML, p.NPOWER,
+                                      p.K_MAX_PML, p.ALPHA_MAX_PML, p.Rcoef, p.vp_ref, p.DELTAT)
+    ay, by, ky = orc.elastic_pml_axis(n, p.DELTAY, (p.JJ - 1) * n * p.DELTAX, p.NY,
+                                      p.NPOINTS_PML, p.NPOWER, p.K_MAX_PML, p.ALPHA_MAX_PML,
+                                      p.Rcoef, p.vp_ref, p.DELTAT)
+    pml = (ax, bx, kx, ay, by, ky)
+    return p, models, src, rcv, pml
+
+
+def test_mpi_elastic_block_plan_and_dropin_f64(orc, ads, cuda):
+    n = 16
+    p, models, src, rcv, pml = _mpi_case(ads, orc, n=n)
+    NX = n + 2   # the oracle's padded-interior size of a block
+    rr_i, rr_j = rcv.rcvi + 2, rcv.rcvj + 2  # padded 1-based for the oracle
+    scaled = (models[0] * 1.02, models[1] * 0.97, models[2] * 1.04)
+    hist_o, _ = orc.elastic_forward(NX, NX, p.NSTEP, p.DELTAX, p.DELTAY, p.DELTAT, *scaled, pml,
+                                    src.srci, src.srcj, src.srctype, src.srcv, keep_mem=False,
+                                    sub=True)
+    obs = orc.elastic_receive(hist_o, rr_i, rr_j, rcv.rcvtype, NX, NX)
+    L_ref, g_ref, tr_ref = orc.elastic_misfit_gradient(NX, NX, p.NSTEP, p.DELTAX, p.DELTAY,
+                                                       p.DELTAT, *models, pml, src.srci, src.srcj,
+                                                       src.srctype, src.srcv, rr_i, rr_j,
+                                                       rcv.rcvtype, obs, sub=True)
+    # batched plan
+    plan = ads.ElasticPlan(p, [ads.ElasticSource(src.srci, src.srcj, src.srctype, src.srcv)],
+                           [ads.ElasticReceiver(rcv.rcvi, rcv.rcvj, rcv.rcvtype)],
+                           dtype=torch.float64)
+    srcv = plan.pack_srcv([src.srcv])
+    mt = [T(m) for m in models]
+    loss, g, tr = plan.gradient(*mt, srcv, plan.pack_traces([obs]), 9, want_traces=True)
+    assert rel(plan.unpack_traces(tr)[0].cpu(), tr_ref) < 1e-12
+    assert loss.item() == pytest.approx(L_ref, rel=1e-10)
+    for a, b, name in zip(g, g_ref, ("rho", "lambda", "mu")):
+        assert rel(a.cpu(), b) < 1e-10, (name, rel(a.cpu(), b))
+    # the drop-in: block histories (NSTEP+1, n, n), MPISimulatedObservation_, autograd
+    mg = [T(m).requires_grad_(True) for m in models]
+    ep = ads.MPIElasticPropagatorSolver(p, src, *mg)
+    assert tuple(ep.vx.shape) == (p.NSTEP + 1, n, n)
+    h_ref, _ = orc.elastic_forward(NX, NX, p.NSTEP, p.DELTAX, p.DELTAY, p.DELTAT, *models, pml,
+                                   src.srci, src.srcj, src.srctype, src.srcv, keep_mem=False,
+                                   sub=True)
+    for k, f in enumerate((ep.vx, ep.vy, ep.sigmaxx, ep.sigmayy, ep.sigmaxy)):
+        assert rel(f.detach().cpu(), h_ref[k][:, 2:n + 2, 2:n + 2]) < 1e-12, k
+    out = ads.MPISimulatedObservation_(ep, rcv)
+    assert tuple(out.shape) == (len(rcv.rcvi), p.NSTEP + 1)
+    torch.sum((out - T(obs)) ** 2).backward()
+    for t, b, name in zip(mg, g_ref, ("rho", "lambda", "mu")):
         assert rel(t.grad.cpu(), b) < 1e-10, (name, rel(t.grad.cpu(), b))

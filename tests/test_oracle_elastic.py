@@ -114,3 +114,108 @@ def test_elastic_p_wave_arrival(orc):
     lag = np.argmax(np.abs(tr)) - (shift - 1)
     expected = dist_cells * hx / vp / dt
     assert abs(lag - expected) <= 12, (lag, expected)
+
+
+# ------------------------------------------------------------------------------------------
+# one-subdomain form of MPIElastic.jl (a block of n x n cells inside a 2-cell zero halo)
+# ------------------------------------------------------------------------------------------
+
+def _identity_profiles(n):
+    z, o = np.zeros(2 * n), np.ones(2 * n)
+    return z, o
+
+
+def test_subdomain_form_equals_full_form_away_from_the_edges(orc):
+    """With homogeneous parameters the averages of the full form are exact (lambda-bar = lambda,
+    rho-bar = rho) and without CPML the two forms differ only in their update ranges, so a
+    wavefield that has not reached the edges is bitwise the same (pins the subdomain update
+    formulas and their index offsets against the already pinned full form)."""
+    n = 30
+    NX = NY = n + 2                     # padded (n+4)^2 either way
+    shape = (NX + 2, NY + 2)
+    rho = np.full(shape, 2048.0)
+    lam = np.full(shape, 4.0e9)
+    mu = np.full(shape, 2.0e9)
+    a0, b0 = _identity_profiles(n)
+    A0, B0 = _identity_profiles(NX)
+    w = orc.ricker(3, 2.0, 2.0, 1e6)
+    src = dict(srci=[17, 17, 16], srcj=[17, 17, 18], srctype=[2, 3, 0],
+               srcv=np.stack([w, w, 0.5 * w], 1))
+    h_sub, _ = orc.elastic_forward(NX, NY, 3, 10.0, 10.0, 1e-3, rho, lam, mu,
+                                   (a0, b0, None, a0, b0, None), **src, sub=True)
+    h_full, _ = orc.elastic_forward(NX, NY, 3, 10.0, 10.0, 1e-3, rho, lam, mu,
+                                    (A0, B0, None, A0, B0, None), **src)
+    assert np.abs(h_full[:, 3]).max() > 0
+    np.testing.assert_array_equal(h_sub, h_full)
+
+
+def test_subdomain_keeps_the_halo_zero_and_uses_block_profile_index(orc):
+    """One step with a source in the block: the 2-cell halo stays 0, and a memory variable of
+    a block cell is a[i-2] * D (profile index i-2, MPIElastic.jl:483-620) -- probed with a
+    profile that is non-zero at one index only."""
+    n = 12
+    NX = NY = n + 2
+    shape = (NX + 2, NY + 2)
+    rng = np.random.default_rng(3)
+    rho = 2000.0 + rng.random(shape)
+    lam = 4e9 * (1 + 0.1 * rng.random(shape))
+    mu = 2e9 * (1 + 0.1 * rng.random(shape))
+    N = (NX + 2) * (NY + 2)
+    ax = np.zeros(2 * n)
+    bx = np.ones(2 * n)
+    k = 5
+    ax[k] = 0.25          # a_x (full grid) at block index k only
+    a0, b0 = _identity_profiles(n)
+    f = np.zeros(5 * N)
+    f.reshape(5, NX + 2, NY + 2)[2, 2:NX, 2:NY] = rng.standard_normal((n, n))  # sxx in the block
+    m = np.zeros(8 * N)
+    f1, m1 = orc.elastic_step(NX, NY, 10.0, 10.0, 1e-3, rho, lam, mu, ax, bx, a0, b0, f.copy(), m,
+                              [7], [7], [0], [0.0], sub=True)
+    F = f1.reshape(5, NX + 2, NY + 2)
+    Mv = m1.reshape(8, NX + 2, NY + 2)
+    halo = np.ones((NX + 2, NY + 2), bool)
+    halo[2:NX, 2:NY] = False
+    assert np.all(F[:, halo] == 0) and np.all(Mv[:, halo] == 0)
+    sxx = f.reshape(5, NX + 2, NY + 2)[2]
+    i = k + 2
+    for j in range(2, NY):
+        dx = (27 * sxx[i, j] - 27 * sxx[i - 1, j] - sxx[i + 1, j] + sxx[i - 2, j]) / 240.0
+        assert Mv[4, i, j] == pytest.approx(0.25 * dx, rel=1e-12)      # m5 of fw3
+    assert np.all(Mv[4, 2:NX, 2:NY][np.arange(n) != k] == 0)
+
+
+@pytest.mark.parametrize("param", ["rho", "lam", "mu"])
+def test_subdomain_gradient_finite_differences(orc, param):
+    n, NSTEP = 18, 40
+    NX = NY = n + 2
+    hx = hy = 10.0
+    dt = 1e-3
+    rho, lam, mu = _model(NX, NY, seed=7)
+    pml = orc.elastic_pml(n, n, hx, hy, dt, 4, 2.0, 1.0, 2 * np.pi * 5, 1e-3, 2500.0)
+    w = orc.ricker(NSTEP, 6.0, 15.0, 1e9)
+    srci, srcj, st = [11, 11, 6], [5, 5, 12], [2, 3, 1]
+    srcv = np.stack([w, w, 0.3 * w], 1)
+    rcvi = np.concatenate([np.arange(3, n + 3), np.arange(3, n + 3)])   # block row 2, padded
+    rcvj = np.full(len(rcvi), 4)
+    rt = np.concatenate([np.zeros(n), np.ones(n)]).astype(np.int64)
+    hist, _ = orc.elastic_forward(NX, NY, NSTEP, hx, hy, dt, rho * 1.02, lam * 0.98, mu * 1.03,
+                                  pml, srci, srcj, st, srcv, sub=True)
+    obs = orc.elastic_receive(hist, rcvi, rcvj, rt, NX, NY)
+    _, grads, _ = orc.elastic_misfit_gradient(NX, NY, NSTEP, hx, hy, dt, rho, lam, mu, pml, srci,
+                                              srcj, st, srcv, rcvi, rcvj, rt, obs, sub=True)
+    base = {"rho": rho, "lam": lam, "mu": mu}
+    g = dict(zip(("rho", "lam", "mu"), grads))[param]
+    halo = np.ones(rho.shape, bool)
+    halo[2:NX, 2:NY] = False
+    assert np.all(g[halo] == 0)      # no averaging: only block parameters are read
+    v = np.random.default_rng(9).standard_normal(rho.shape) * base[param] * 1e-6
+
+    def L(x):
+        a = dict(base)
+        a[param] = x
+        return orc.elastic_misfit_gradient(NX, NY, NSTEP, hx, hy, dt, a["rho"], a["lam"], a["mu"],
+                                           pml, srci, srcj, st, srcv, rcvi, rcvj, rt, obs,
+                                           sub=True)[0]
+
+    fd = (L(base[param] + v) - L(base[param] - v)) / 2
+    assert np.sum(g * v) == pytest.approx(fd, rel=1e-6)
