@@ -89,10 +89,10 @@ template <typename T> struct FwdArgs {
 // depth cells); a workgroup loops over a group of Sg shots, SB at a time, issuing every load of
 // those SB shots before the first store, so each wavefront keeps SB shots' rows in flight.
 template <typename T, int RPT, int VEC, int SB>
-__global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_rect(const FwdArgs<T> a, const Rect R)
+__device__ inline void fwd_rect_body(const FwdArgs<T> &a, const Rect &R, int bid)
 {
     int g, strip, tj;
-    if (!decode_tile(a.map, g, strip, tj)) return;
+    if (!decode_tile(a.map, g, strip, tj, bid)) return;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     const int j0 = R.jL + (tj * WAVE + lane) * VEC;
     const int ib = R.iL + (strip * NWAVES + wv) * RPT;
@@ -159,12 +159,11 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_rect(const FwdArgs<T> a, 
 // Only lanes with a cell near the PML read phi/psi; cells of the padding ring get u = 0
 // (sources are added afterwards).
 template <typename T, int V>
-__global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_frame(const FwdArgs<T> a,
-                                                              const int4 *__restrict__ items,
-                                                              int nitems)
+__device__ inline void fwd_frame_body(const FwdArgs<T> &a, const TileMap &map,
+                                      const int4 *__restrict__ items, int nitems, int bid)
 {
     int g, blk, tj;
-    if (!decode_tile(a.map, g, blk, tj)) return;
+    if (!decode_tile(map, g, blk, tj, bid)) return;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     const int it = blk;  // one item per workgroup; its 4 wavefronts take every 4th shot
     if (it >= nitems) return;  // workgroup-uniform
@@ -360,11 +359,11 @@ template <typename T> struct AdjArgs {
 // Rectangle adjoint: P^n = 2 P^{n+1} + lap(C P^{n+1}) - P^{n+2} (+ the dense cotangent of u^n
 // in the full-history mode), dL/dC += P^n lap(u^{n-1}); no PML term reaches these cells.
 template <typename T, int RPT, int VEC, int SB>
-__global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_rect(const AdjArgs<T> a, const Rect R)
+__device__ inline void adj_rect_body(const AdjArgs<T> &a, const Rect &R, int bid)
 {
     constexpr bool DENSE = VEC == 1;  // the dense cotangent only exists in the history layout
     int g, strip, tj;
-    if (!decode_tile(a.map, g, strip, tj)) return;
+    if (!decode_tile(a.map, g, strip, tj, bid)) return;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     const int j0 = R.jL + (tj * WAVE + lane) * VEC;
     const int ib = R.iL + (strip * NWAVES + wv) * RPT;
@@ -479,12 +478,11 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_rect(const AdjArgs<T> a, 
 // F^n/G^n written for kernel 1 on PML lanes), zero on the padding ring.  Work split as in the
 // frame forward; the wavefronts' dL/dC partial sums are combined through LDS.
 template <typename T, int V>
-__global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_frame(const AdjArgs<T> a,
-                                                              const int4 *__restrict__ items,
-                                                              int nitems)
+__device__ inline void adj_frame_body(const AdjArgs<T> &a, const TileMap &map,
+                                      const int4 *__restrict__ items, int nitems, int bid)
 {
     int g, blk, tj;
-    if (!decode_tile(a.map, g, blk, tj)) return;
+    if (!decode_tile(map, g, blk, tj, bid)) return;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     const int it = blk;  // one item per workgroup; its 4 wavefronts take every 4th shot
     if (it >= nitems) return;  // workgroup-uniform
@@ -620,6 +618,55 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_frame(const AdjArgs<T> a,
         for (int q = 1; q < NWAVES; q++) sum += red[q][r][lane];
         gC[(int64_t)(ib + r) * P + j] += sum;
     }
+}
+
+// Kernels.  ac_*_rect and ac_*_frame run one part of the grid each (the dense history layout
+// uses them).  ac_*_level runs both in one launch: workgroups [0, nfb) take the frame items
+// (tile map fmap), the rest the rectangle tiles (a.map), so the latency-bound frame work
+// overlaps the bandwidth-bound rectangle work instead of following it.
+template <typename T, int RPT, int VEC, int SB>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_rect(const FwdArgs<T> a, const Rect R)
+{
+    fwd_rect_body<T, RPT, VEC, SB>(a, R, (int)blockIdx.x);
+}
+template <typename T, int V>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_frame(const FwdArgs<T> a,
+                                                              const int4 *__restrict__ items,
+                                                              int nitems)
+{
+    fwd_frame_body<T, V>(a, a.map, items, nitems, (int)blockIdx.x);
+}
+template <typename T, int RPT, int VEC, int SB, int V>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_level(const FwdArgs<T> a, const Rect R,
+                                                              const TileMap fmap,
+                                                              const int4 *__restrict__ items,
+                                                              int nitems, int nfb)
+{
+    const int b = (int)blockIdx.x;
+    if (b < nfb) fwd_frame_body<T, V>(a, fmap, items, nitems, b);
+    else fwd_rect_body<T, RPT, VEC, SB>(a, R, b - nfb);
+}
+template <typename T, int RPT, int VEC, int SB>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_rect(const AdjArgs<T> a, const Rect R)
+{
+    adj_rect_body<T, RPT, VEC, SB>(a, R, (int)blockIdx.x);
+}
+template <typename T, int V>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_frame(const AdjArgs<T> a,
+                                                              const int4 *__restrict__ items,
+                                                              int nitems)
+{
+    adj_frame_body<T, V>(a, a.map, items, nitems, (int)blockIdx.x);
+}
+template <typename T, int RPT, int VEC, int SB, int V>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_level(const AdjArgs<T> a, const Rect R,
+                                                              const TileMap fmap,
+                                                              const int4 *__restrict__ items,
+                                                              int nitems, int nfb)
+{
+    const int b = (int)blockIdx.x;
+    if (b < nfb) adj_frame_body<T, V>(a, fmap, items, nitems, b);
+    else adj_rect_body<T, RPT, VEC, SB>(a, R, b - nfb);
 }
 
 // Receiver residual injection at level n after the stencil pass.  One wavefront owns each
@@ -790,15 +837,6 @@ template <typename T> struct AcPlan : AcPlanBase {
         m.nb = m.ntj * m.nstrips * m.ngroups;
         return m;
     }
-    static TileMap frame_tiles_n(int n, int act, int sg)
-    {
-        TileMap m;
-        m.ntj = 1;
-        m.nstrips = n;
-        m.ngroups = (int)cdiv(act, sg);
-        m.nb = m.ntj * m.nstrips * m.ngroups;
-        return m;
-    }
     TileMap frame_tiles(int li, int act, int sg) const
     {
         TileMap m;
@@ -809,41 +847,74 @@ template <typename T> struct AcPlan : AcPlanBase {
         return m;
     }
     static bool empty(const Rect &R) { return R.iR <= R.iL || R.jR <= R.jL; }
-    // rectangle kernel instances: rows per wavefront 1/2/4, shots per batch 1/2 (2 with 1-2 rows)
-    template <int RPT, int VEC, int SB> void launch_fwd_rect(FwdArgs<T> a, const Rect &R, hipStream_t st)
+    // rectangle kernel instances: rows per wavefront 1/2/4, shots per batch 1/2 (2 with 1-2 rows).
+    // With `fuse` (pitched layout only) the frame items of layout li ride in the same launch.
+    template <int RPT, int VEC, int SB>
+    void launch_fwd_rect(FwdArgs<T> a, const Rect &R, int li, bool fuse, hipStream_t st)
     {
         a.map = rect_tiles(R, VEC, RPT, a.act, a.Sg);
+        if constexpr (VEC > 1) {
+            if (fuse) {
+                const TileMap fm = frame_tiles(li, a.act, a.Sg);
+                const int nfb = (int)grid_of(fm);
+                const unsigned nb = (unsigned)nfb + grid_of(a.map);
+                const int4 *it = d_items[li].template as<int4>();
+                if (variant == 1)
+                    ac_fwd_level<T, RPT, VEC, SB, 1><<<nb, WAVE * NWAVES, 0, st>>>(a, R, fm, it, nitems[li], nfb);
+                else
+                    ac_fwd_level<T, RPT, VEC, SB, 0><<<nb, WAVE * NWAVES, 0, st>>>(a, R, fm, it, nitems[li], nfb);
+                return;
+            }
+        }
         ac_fwd_rect<T, RPT, VEC, SB><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, R);
     }
-    template <int VEC> void launch_fwd_shape(const FwdArgs<T> &a, const Rect &R, const Shape &sh, hipStream_t st)
+    template <int VEC>
+    void launch_fwd_shape(const FwdArgs<T> &a, const Rect &R, const Shape &sh, int li, bool fuse, hipStream_t st)
     {
         if (empty(R)) return;
-        if (sh.sb == 2 && sh.rpt == 1) launch_fwd_rect<1, VEC, 2>(a, R, st);
-        else if (sh.sb == 2 && sh.rpt == 2) launch_fwd_rect<2, VEC, 2>(a, R, st);
-        else if (sh.rpt == 1) launch_fwd_rect<1, VEC, 1>(a, R, st);
-        else if (sh.rpt == 4) launch_fwd_rect<4, VEC, 1>(a, R, st);
-        else launch_fwd_rect<2, VEC, 1>(a, R, st);
+        if (sh.sb == 2 && sh.rpt == 1) launch_fwd_rect<1, VEC, 2>(a, R, li, fuse, st);
+        else if (sh.sb == 2 && sh.rpt == 2) launch_fwd_rect<2, VEC, 2>(a, R, li, fuse, st);
+        else if (sh.rpt == 1) launch_fwd_rect<1, VEC, 1>(a, R, li, fuse, st);
+        else if (sh.rpt == 4) launch_fwd_rect<4, VEC, 1>(a, R, li, fuse, st);
+        else launch_fwd_rect<2, VEC, 1>(a, R, li, fuse, st);
     }
-    template <int RPT, int VEC, int SB> void launch_adj_rect(AdjArgs<T> a, const Rect &R, hipStream_t st)
+    template <int RPT, int VEC, int SB>
+    void launch_adj_rect(AdjArgs<T> a, const Rect &R, int li, bool fuse, hipStream_t st)
     {
         a.map = rect_tiles(R, VEC, RPT, a.act, a.Sg);
+        if constexpr (VEC > 1) {
+            if (fuse) {
+                const TileMap fm = frame_tiles(li, a.act, a.Sg);
+                const int nfb = (int)grid_of(fm);
+                const unsigned nb = (unsigned)nfb + grid_of(a.map);
+                const int4 *it = d_items[li].template as<int4>();
+                if (variant == 1)
+                    ac_adj_level<T, RPT, VEC, SB, 1><<<nb, WAVE * NWAVES, 0, st>>>(a, R, fm, it, nitems[li], nfb);
+                else
+                    ac_adj_level<T, RPT, VEC, SB, 0><<<nb, WAVE * NWAVES, 0, st>>>(a, R, fm, it, nitems[li], nfb);
+                return;
+            }
+        }
         ac_adj_rect<T, RPT, VEC, SB><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, R);
     }
-    template <int VEC> void launch_adj_shape(const AdjArgs<T> &a, const Rect &R, const Shape &sh, hipStream_t st)
+    template <int VEC>
+    void launch_adj_shape(const AdjArgs<T> &a, const Rect &R, const Shape &sh, int li, bool fuse, hipStream_t st)
     {
         if (empty(R)) return;
-        if (sh.sb == 2 && sh.rpt == 1) launch_adj_rect<1, VEC, 2>(a, R, st);
-        else if (sh.sb == 2 && sh.rpt == 2) launch_adj_rect<2, VEC, 2>(a, R, st);
-        else if (sh.rpt == 1) launch_adj_rect<1, VEC, 1>(a, R, st);
-        else if (sh.rpt == 4) launch_adj_rect<4, VEC, 1>(a, R, st);
-        else launch_adj_rect<2, VEC, 1>(a, R, st);
+        if (sh.sb == 2 && sh.rpt == 1) launch_adj_rect<1, VEC, 2>(a, R, li, fuse, st);
+        else if (sh.sb == 2 && sh.rpt == 2) launch_adj_rect<2, VEC, 2>(a, R, li, fuse, st);
+        else if (sh.rpt == 1) launch_adj_rect<1, VEC, 1>(a, R, li, fuse, st);
+        else if (sh.rpt == 4) launch_adj_rect<4, VEC, 1>(a, R, li, fuse, st);
+        else launch_adj_rect<2, VEC, 1>(a, R, li, fuse, st);
     }
-    // the rectangle, then the frame (disjoint cells, same inputs)
+    // The rectangle and the frame (disjoint cells, same inputs): one fused launch in the
+    // pitched layout, two launches in the dense history layout.
     void launch_fwd_all(FwdArgs<T> a, const Layout &L, const Shape &sh, hipStream_t st)
     {
-        if (L.li == 0) launch_fwd_shape<VECP>(a, rect[0], sh, st);
-        else launch_fwd_shape<1>(a, rect[1], sh, st);
-        if (nitems[L.li] == 0) return;
+        const bool fuse = L.li == 0 && !empty(rect[0]) && nitems[0] > 0;
+        if (L.li == 0) launch_fwd_shape<VECP>(a, rect[0], sh, 0, fuse, st);
+        else launch_fwd_shape<1>(a, rect[1], sh, 1, false, st);
+        if (fuse || nitems[L.li] == 0) return;
         a.map = frame_tiles(L.li, a.act, a.Sg);
         const int4 *it = d_items[L.li].template as<int4>();
         if (variant == 1) ac_fwd_frame<T, 1><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, it, nitems[L.li]);
@@ -851,9 +922,10 @@ template <typename T> struct AcPlan : AcPlanBase {
     }
     void launch_adj_all(AdjArgs<T> a, const Layout &L, const Shape &sh, hipStream_t st)
     {
-        if (L.li == 0) launch_adj_shape<VECP>(a, rect[0], sh, st);
-        else launch_adj_shape<1>(a, rect[1], sh, st);
-        if (nitems[L.li] == 0) return;
+        const bool fuse = L.li == 0 && !empty(rect[0]) && nitems[0] > 0;
+        if (L.li == 0) launch_adj_shape<VECP>(a, rect[0], sh, 0, fuse, st);
+        else launch_adj_shape<1>(a, rect[1], sh, 1, false, st);
+        if (fuse || nitems[L.li] == 0) return;
         a.map = frame_tiles(L.li, a.act, a.Sg);
         const int4 *it = d_items[L.li].template as<int4>();
         if (variant == 1) ac_adj_frame<T, 1><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, it, nitems[L.li]);
@@ -1571,13 +1643,16 @@ size_t ads_acoustic_plan_workspace_size(const ads_acoustic_plan *plan, int64_t K
 int64_t ads_acoustic_plan_choose_interval(const ads_acoustic_plan *plan, size_t budget)
 {
     if (!plan || !plan->impl) return 0;
-    int64_t hi = std::max<int64_t>(plan->impl->NMAX - 1, 1), best = 0;
-    for (int64_t K = hi; K >= 1; K = (K > 64 ? K - std::max<int64_t>(K / 64, 1) : K - 1))
-        if (ads_acoustic_plan_workspace_size(plan, K) <= budget) {
-            best = K;
-            break;
-        }
-    return best;
+    // Fewest segments M whose interval fits, then the shortest interval with that M: every
+    // segment but the last is recomputed, so K = ceil(levels / M) minimises the recomputation
+    // ((M - 1) K levels) among the intervals that need M segments.
+    const int64_t L = std::max<int64_t>(plan->impl->NMAX - 1, 1);
+    for (int64_t M = 1; M <= L; M++) {
+        const int64_t K = (L + M - 1) / M;
+        if (ads_acoustic_plan_workspace_size(plan, K) <= budget) return K;
+        if (K == 1) break;
+    }
+    return 0;
 }
 
 int ads_acoustic_plan_forward(ads_acoustic_plan *plan, const void *model, const void *srcv,

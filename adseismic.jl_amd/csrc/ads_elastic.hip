@@ -1579,13 +1579,14 @@ size_t ads_elastic_plan_workspace_size(const ads_elastic_plan *plan, int64_t K)
 int64_t ads_elastic_plan_choose_interval(const ads_elastic_plan *plan, size_t budget)
 {
     if (!plan || !plan->impl) return 0;
-    int64_t hi = std::max<int64_t>(plan->impl->NMAX, 1), best = 0;
-    for (int64_t K = hi; K >= 1; K = (K > 64 ? K - std::max<int64_t>(K / 64, 1) : K - 1))
-        if (ads_elastic_plan_workspace_size(plan, K) <= budget) {
-            best = K;
-            break;
-        }
-    return best;
+    // as for the acoustic plan: fewest segments, then the shortest interval with that count
+    const int64_t L = std::max<int64_t>(plan->impl->NMAX, 1);
+    for (int64_t M = 1; M <= L; M++) {
+        const int64_t K = (L + M - 1) / M;
+        if (ads_elastic_plan_workspace_size(plan, K) <= budget) return K;
+        if (K == 1) break;
+    }
+    return 0;
 }
 
 int ads_elastic_plan_forward(ads_elastic_plan *plan, const void *rho, const void *lam,

@@ -20,10 +20,9 @@ struct TileMap {
     int ntj, nstrips, ngroups, nb;
 };
 
-__device__ inline bool decode_tile(const TileMap &m, int &g, int &strip, int &tj)
+__device__ inline bool decode_tile(const TileMap &m, int &g, int &strip, int &tj, int b)
 {
     const int nb8 = (m.nb + NXCD - 1) / NXCD;
-    const int b = blockIdx.x;
     const int L = (b % NXCD) * nb8 + b / NXCD;
     if (L >= m.nb) return false;
     tj = L % m.ntj;
@@ -31,6 +30,10 @@ __device__ inline bool decode_tile(const TileMap &m, int &g, int &strip, int &tj
     strip = r % m.nstrips;
     g = r / m.nstrips;
     return true;
+}
+__device__ inline bool decode_tile(const TileMap &m, int &g, int &strip, int &tj)
+{
+    return decode_tile(m, g, strip, tj, (int)blockIdx.x);
 }
 
 // Vector access of VEC consecutive depth cells per lane (128-bit loads in the pitched layout).

@@ -235,7 +235,9 @@ int ads_acoustic_plan_sizes(const ads_acoustic_plan *plan, int64_t *srcv_elems,
 /* Workspace bytes needed for a checkpoint interval K (levels between checkpoints; K >= 1).
  * K >= NSTEP-1 stores every level (no recomputation). */
 size_t ads_acoustic_plan_workspace_size(const ads_acoustic_plan *plan, int64_t K);
-/* Largest K whose workspace fits in budget_bytes (0 if none). */
+/* Checkpoint interval for a memory budget (0 if none fits): the fewest segments M whose
+ * interval fits in budget_bytes, and among the intervals with M segments the shortest,
+ * K = ceil((NSTEP-1)/M), which minimises the recomputed levels ((M-1) K). */
 int64_t ads_acoustic_plan_choose_interval(const ads_acoustic_plan *plan, size_t budget_bytes);
 
 /* Forward: traces of every shot (model = velocity or C as square_velocity says).  Keeps the
@@ -301,6 +303,7 @@ int ads_elastic_plan_destroy(ads_elastic_plan *plan);
 int ads_elastic_plan_sizes(const ads_elastic_plan *plan, int64_t *srcv_elems,
                            int64_t *trace_elems, int64_t *srcv_off, int64_t *trace_off);
 size_t ads_elastic_plan_workspace_size(const ads_elastic_plan *plan, int64_t K);
+/* As ads_acoustic_plan_choose_interval, over the NSTEP elastic levels. */
 int64_t ads_elastic_plan_choose_interval(const ads_elastic_plan *plan, size_t budget_bytes);
 /* Forward: traces of every shot from rho, lambda, mu (device, N each). */
 int ads_elastic_plan_forward(ads_elastic_plan *plan, const void *rho, const void *lam,

@@ -292,6 +292,24 @@ def test_workspace_too_small_is_an_error(orc, ads, cuda):
     assert rc == 6
 
 
+def test_choose_interval_minimises_recomputation(orc, ads, cuda):
+    """For a budget, the chosen K needs the fewest segments that fit and, among the K with that
+    segment count, recomputes the fewest levels; checked by brute force over every K."""
+    p, c, srcs, rcvs, nstep = make_case(orc, ads, NSTEP=80)
+    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float32)
+    L = p.NSTEP - 1
+    sizes = {K: plan.workspace_size(K) for K in range(1, L + 1)}
+    for budget in (sizes[L], sizes[L] - 1, sizes[40], sizes[27] + 7, sizes[9], sizes[1]):
+        K = plan.choose_interval(budget)
+        assert sizes[K] <= budget
+        segs = lambda k: -(-L // k)
+        fit = [k for k in sizes if sizes[k] <= budget]
+        best_m = min(segs(k) for k in fit)
+        assert segs(K) == best_m, (budget, K)
+        assert K == min(k for k in fit if segs(k) == best_m), (budget, K)
+    assert plan.choose_interval(sizes[L]) == L  # everything fits: no recomputation
+
+
 @pytest.mark.parametrize("kernel", [0, 1])
 def test_kernel_shapes_and_autotune_agree(orc, ads, cuda, kernel):
     """Every (rows per wavefront, shots per workgroup, shots per batch) shape, and the autotuned
