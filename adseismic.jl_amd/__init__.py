@@ -20,5 +20,9 @@ from .elastic import (ElasticPlan, ElasticPropagator, ElasticPropagatorSolver,  
 from .elastic import compute_PML_Params as compute_PML_Params_elastic  # noqa: F401
 from .utils import (Gauss, Ricker, compute_lame_parameters, compute_properties,  # noqa: F401
                     ricker_freq)
+from .io_mat import (load_acoustic_model, load_acoustic_receiver, load_acoustic_source,  # noqa: F401
+                     load_elastic_model, load_elastic_receiver, load_elastic_source, load_params,
+                     matread, safe_vec)
+from .fwi import AcousticFWI, Adam_, LBFGS_, compute_loss_and_grads  # noqa: F401
 
 SimulatedObservation = SimulatedObservation_

@@ -7,6 +7,11 @@
 2. wavelet_fixtures.npz: the source wavelets, sampling and geometry the reference ships in
    examples/nn_fwi/models/{marmousi2,BP}-model-true.mat (read with scipy.io.loadmat; no
    pickle), used to pin the Ricker restatement and the index conventions.
+3. marmousi2_models.npz: the velocity models (true, smooth), the update mask and dx, dy of
+   examples/nn_fwi/models/marmousi2-model-{true,smooth}.mat (models as float32), for the MAT loader and
+   FWI-driver tests (which rebuild a .mat of the same layout from them).
+
+``python make_golden.py models`` writes only item 3.
 """
 import ctypes as C
 import os
@@ -26,7 +31,18 @@ def P(a):
     return a.ctypes.data_as(dp)
 
 
+def models():
+    import scipy.io as sio
+    t = sio.loadmat(os.path.join(REF, "examples/nn_fwi/models/marmousi2-model-true.mat"))
+    m = sio.loadmat(os.path.join(REF, "examples/nn_fwi/models/marmousi2-model-smooth.mat"))
+    np.savez_compressed(os.path.join(HERE, "marmousi2_models.npz"),
+                        vp_true=np.asarray(t["vp"], np.float32), vp_smooth=np.asarray(m["vp"], np.float32),
+                        mask=np.asarray(m["mask"], np.float32), dx_dy=np.array([float(t["dx"][0, 0]),
+                                                                                float(t["dy"][0, 0])]))
+
+
 def main():
+    models()
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     R = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libadseismic_ref.so"))
     rng = np.random.default_rng(233)
@@ -101,4 +117,4 @@ def main():
 
 
 if __name__ == "__main__":
-    sys.exit(main())
+    sys.exit(models() if sys.argv[1:] == ["models"] else main())
