@@ -239,6 +239,16 @@ class AcousticPlan:
         _lib.check(self.lib.ads_acoustic_plan_set_shape(self.h, arr))
         self.shape = tuple(shape)
 
+    def set_blocking(self, levels: int = 2):
+        """Levels per interior-rectangle pass of the forward sweeps (2 = temporal blocking,
+        the default; bit-identical to 1)."""
+        _lib.check(self.lib.ads_acoustic_plan_set_blocking(self.h, int(levels)))
+
+    @property
+    def blocking(self) -> int:
+        """Levels per pass in effect: 2 only when no source lies in the interior rectangle."""
+        return int(self.lib.ads_acoustic_plan_blocking(self.h))
+
     def set_graph(self, enable: bool = True):
         """Replay checkpoint segments as captured hipGraphs (ads_acoustic_plan_set_graph)."""
         _lib.check(self.lib.ads_acoustic_plan_set_graph(self.h, int(bool(enable))))

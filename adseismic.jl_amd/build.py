@@ -2,7 +2,8 @@
 
 Usage: ``python adseismic.jl_amd/build.py`` (or ``__graft_entry__.build()``).  Every HIP
 translation unit is compiled with packed-FP32 instruction selection disabled; ads_ops.hip
-(the bit-exact drop-in operators) is additionally compiled without FMA contraction.
+(the bit-exact drop-in operators) and ads_acoustic.hip (whose single- and two-level stencil
+passes must agree bit for bit) are additionally compiled without FMA contraction.
 """
 from __future__ import annotations
 
@@ -26,7 +27,9 @@ COMMON = [
 ]
 SOURCES = {
     "ads_host.cpp": ["-x", "hip"],
-    "ads_acoustic.hip": [],
+    # no FMA contraction: the rectangle, two-level rectangle and frame kernels evaluate the
+    # same stencil expression in different kernels and must round identically
+    "ads_acoustic.hip": ["-ffp-contract=off"],
     "ads_elastic.hip": [],
     "ads_ops.hip": ["-ffp-contract=off"],
 }

@@ -154,6 +154,128 @@ __device__ inline void fwd_rect_body(const FwdArgs<T> &a, const Rect &R, int bid
     }
 }
 
+// Two-level rectangle forward (temporal blocking).  From w = u^{n-1} and wold = u^{n-2} one pass
+// writes u^n on the rectangle R and u^{n+1} on R shrunk by one cell; the ring of R at level n+1
+// is left to that level's frame launch.  A wavefront owns RPT rows of u^{n+1}: it recomputes u^n
+// on one halo row above and below and, on its segment's edge lanes, one halo column on each
+// side, so no value crosses wavefronts.  Every value is the single-level formula in the same
+// order, so results equal two single-level steps bit for bit.  HBM per cell and two levels:
+// read u^{n-1} and u^{n-2}, write u^n and u^{n+1} (two thirds of the single-level traffic).
+// Valid only when no source lies in R (sources are added between the levels by ac_src_rcv).
+template <typename T, int RPT, int VEC>
+__device__ inline void fwd_rect2_body(const FwdArgs<T> &a, const Lv<T> &u2, const Rect &R, int bid)
+{
+    int g, strip, tj;
+    if (!decode_tile(a.map, g, strip, tj, bid)) return;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int ja = R.jL + tj * WAVE * VEC;  // first column of the segment
+    const int j0 = ja + lane * VEC;
+    const int ib = R.iL + (strip * NWAVES + wv) * RPT;
+    if (ib >= R.iR) return;  // wave-uniform
+    const int P = a.P, nr = min(RPT, R.iR - ib);
+    const bool act = j0 < R.jR;
+    const int jb = min(ja + WAVE * VEC, R.jR);  // one past the segment's last column
+    const bool le = lane == 0;                    // ja < R.jR, so lane 0 is active
+    const bool re = act && (lane == WAVE - 1 || j0 + VEC == R.jR);
+    const bool hl = le && ja - 1 >= R.jL;  // the halo column left of the segment lies in R
+    const bool hr = re && jb < R.jR;       // the halo column right of it
+    const AcCoef<T> k = a.k;
+    constexpr int NW = RPT + 4, NU = RPT + 2;  // w rows ib-2+q; u^n, wold, C rows ib-1+q
+    bool urow[NU];
+    T Cr[NU][VEC], CL[NU], CR[NU];
+#pragma unroll
+    for (int q = 0; q < NU; q++) {
+        const int i = ib - 1 + q;
+        urow[q] = q <= nr + 1 && i >= R.iL && i < R.iR;
+        const int64_t o = (int64_t)i * P;
+        if (act && urow[q]) vld(Cr[q], a.C + o + j0);
+        else vzero(Cr[q]);
+        CL[q] = hl && urow[q] ? a.C[o + ja - 1] : T(0);
+        CR[q] = hr && urow[q] ? a.C[o + jb] : T(0);
+    }
+    const int s1 = min(a.act, (g + 1) * a.Sg);
+    for (int s = g * a.Sg; s < s1; s++) {
+        const T *__restrict__ w = a.w.at(s);
+        const T *__restrict__ wo = a.wold.at(s);
+        T W[NW][VEC], WL1[NW], WR1[NW];  // w rows; w at columns ja-1 and jb
+        T WO[NU][VEC], WL2[NU], WR2[NU], WOL[NU], WOR[NU];
+#pragma unroll
+        for (int q = 0; q < NW; q++) {  // rows R.iL-1 .. R.iR lie inside [0, NX+1]
+            const int x = ib - 2 + q;
+            const bool ok = q <= nr + 3 && x >= R.iL - 1 && x <= R.iR;
+            const int64_t o = (int64_t)x * P;
+            if (act && ok) vld(W[q], w + o + j0);
+            else vzero(W[q]);
+            WL1[q] = le && ok ? w[o + ja - 1] : T(0);
+            WR1[q] = re && ok ? w[o + jb] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < NU; q++) {
+            const int64_t o = (int64_t)(ib - 1 + q) * P;
+            if (act && urow[q]) vld(WO[q], wo + o + j0);
+            else vzero(WO[q]);
+            WL2[q] = hl && urow[q] ? w[o + ja - 2] : T(0);
+            WR2[q] = hr && urow[q] ? w[o + jb + 1] : T(0);
+            WOL[q] = hl && urow[q] ? wo[o + ja - 1] : T(0);
+            WOR[q] = hr && urow[q] ? wo[o + jb] : T(0);
+        }
+        // u^n on rows ib-1 .. ib+nr (and the halo columns on the edge lanes)
+        T U[NU][VEC], UL[NU], UR[NU];
+#pragma unroll
+        for (int q = 0; q < NU; q++) {
+            T wl, wr;
+            lr_of(W[q + 1], WL1[q + 1], WR1[q + 1], lane, re, wl, wr);
+#pragma unroll
+            for (int e = 0; e < VEC; e++) {
+                const T wc = W[q + 1][e];
+                const T ww = e == 0 ? wl : W[q + 1][e - 1];
+                const T we = e == VEC - 1 ? wr : W[q + 1][e + 1];
+                const T lap = k.cx * (W[q + 2][e] + W[q][e] - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
+                U[q][e] = T(2) * wc - WO[q][e] + Cr[q][e] * lap;
+            }
+            {
+                const T wc = WL1[q + 1];
+                const T lap = k.cx * (WL1[q + 2] + WL1[q] - T(2) * wc) + k.cy * (W[q + 1][0] + WL2[q] - T(2) * wc);
+                UL[q] = T(2) * wc - WOL[q] + CL[q] * lap;
+            }
+            {
+                const T wc = WR1[q + 1];
+                const T lap = k.cx * (WR1[q + 2] + WR1[q] - T(2) * wc) + k.cy * (WR2[q] + W[q + 1][VEC - 1] - T(2) * wc);
+                UR[q] = T(2) * wc - WOR[q] + CR[q] * lap;
+            }
+        }
+        T *__restrict__ u1 = a.u.at(s);
+        T *__restrict__ up = u2.at(s);
+#pragma unroll
+        for (int q = 1; q <= RPT; q++) {
+            if (q > nr) break;  // uniform
+            const int i = ib - 1 + q;
+            const int64_t o = (int64_t)i * P + j0;
+            if (act) vst(u1 + o, U[q]);
+            if (i <= R.iL || i >= R.iR - 1) continue;  // uniform: u^{n+1} only inside R shrunk by 1
+            T ul, ur;
+            lr_of(U[q], UL[q], UR[q], lane, re, ul, ur);
+            T out[VEC];
+#pragma unroll
+            for (int e = 0; e < VEC; e++) {
+                const T uc = U[q][e];
+                const T uw = e == 0 ? ul : U[q][e - 1];
+                const T ue = e == VEC - 1 ? ur : U[q][e + 1];
+                const T lap = k.cx * (U[q + 1][e] + U[q - 1][e] - T(2) * uc) + k.cy * (ue + uw - T(2) * uc);
+                out[e] = T(2) * uc - W[q + 1][e] + Cr[q][e] * lap;
+            }
+            if (!act) continue;
+            if (j0 > R.jL && j0 + VEC < R.jR) {
+                vst(up + o, out);
+            } else {
+#pragma unroll
+                for (int e = 0; e < VEC; e++)
+                    if (j0 + e > R.jL && j0 + e < R.jR - 1) up[o + e] = out[e];
+            }
+        }
+    }
+}
+
 // Frame forward, one depth cell per lane.  A workgroup takes one frame item and its four
 // wavefronts take every fourth shot of the group, so the thin PML strips still fill the GPU.
 // Only lanes with a cell near the PML read phi/psi; cells of the padding ring get u = 0
@@ -646,6 +768,17 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_level(const FwdArgs<T> a,
     if (b < nfb) fwd_frame_body<T, V>(a, fmap, items, nitems, b);
     else fwd_rect_body<T, RPT, VEC, SB>(a, R, b - nfb);
 }
+// Levels n and n+1 of the rectangle (two-level body) with level n of the frame, in one launch.
+template <typename T, int RPT, int VEC, int V>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_level2(const FwdArgs<T> a, const Lv<T> u2,
+                                                               const Rect R, const TileMap fmap,
+                                                               const int4 *__restrict__ items,
+                                                               int nitems, int nfb)
+{
+    const int b = (int)blockIdx.x;
+    if (b < nfb) fwd_frame_body<T, V>(a, fmap, items, nitems, b);
+    else fwd_rect2_body<T, RPT, VEC>(a, u2, R, b - nfb);
+}
 template <typename T, int RPT, int VEC, int SB>
 __global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_rect(const AdjArgs<T> a, const Rect R)
 {
@@ -773,6 +906,15 @@ struct AcPlanBase {
     Rect rect[2]{};
     DevBuf d_items[2];
     int nitems[2] = {0, 0};
+    // Two-level forward (temporal blocking) in the pitched layout: rect1 is rect[0] shrunk by
+    // one cell and items1 the frame items of its complement (level n+1's frame launch).
+    // blk_ok: rect1 is not empty and no source lies in rect[0]; blocking: levels per
+    // rectangle pass asked for (1 or 2).
+    Rect rect1{};
+    DevBuf d_items1;
+    int nitems1 = 0;
+    bool blk_ok = false;
+    int blocking = 2;
     // hipGraph replay of whole checkpoint segments (forward sweep, recomputation, adjoint
     // sweep), keyed by everything the captured launches depend on
     struct GraphKey {
@@ -932,30 +1074,46 @@ template <typename T> struct AcPlan : AcPlanBase {
         else ac_adj_frame<T, 0><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, it, nitems[L.li]);
     }
 
+    double fwd_level_bytes(int act) const
+    {
+        return (double)act * (double)NX * (double)NY * sizeof(T) * (4.0 + (variant == 1 ? 4.0 : 2.0) * f_pml);
+    }
+    FwdArgs<T> fwd_args(const Layout &L, int act, Lv<T> uo, Lv<const T> w, Lv<const T> wo,
+                        Lv<const T> phi, Lv<const T> psi, Lv<T> phio, Lv<T> psio, const T *C,
+                        const Shape &shape) const
+    {
+        return FwdArgs<T>{uo, w, wo, phi, psi, phio, psio, C, d_sx.as<T>(), d_ty.as<T>(),
+                          d_nearx.as<uint8_t>(), d_neary.as<uint8_t>(), (int)NX, (int)NY, L.P, act,
+                          std::min(shape.sg, act), TileMap{}, k};
+    }
+    // after a level's stencil launch: phi/psi from the new u (kernel 0), sources, receivers
+    void fwd_post(const Layout &L, const FwdArgs<T> &a, int n, const T *srcv, T *traces, int record,
+                  hipStream_t st)
+    {
+        if (variant == 0) {
+            dim3 g(cdiv(NY + 2, 64), cdiv(NX + 2, 4), a.act);
+            ac_fwd_phipsi<T><<<g, 256, 0, st>>>(cst(a.u), a.phi, a.psi, a.phio, a.psio, a.C, d_sx.as<T>(),
+                                                d_ty.as<T>(), d_pmlx.as<uint8_t>(),
+                                                d_pmly.as<uint8_t>(), (int)NX, (int)NY, L.P, k);
+        }
+        ac_src_rcv<T><<<a.act, 256, 0, st>>>(a.u, n, d_us_ptr.as<int64_t>(), d_us_cell[L.li].template as<int64_t>(),
+                                             d_us_eptr.as<int64_t>(), d_us_ent.as<int64_t>(), srcv,
+                                             d_srcv_off.as<int64_t>(), d_nsrc.as<int64_t>(), k.dt2,
+                                             d_rcv_ptr.as<int64_t>(), d_rcv_cell[L.li].template as<int64_t>(),
+                                             traces, d_trace_off.as<int64_t>(), record);
+    }
+
     int fwd_step(const Layout &L, int n, int act, Lv<T> uo, Lv<const T> w, Lv<const T> wo,
                  Lv<const T> phi, Lv<const T> psi, Lv<T> phio, Lv<T> psio, const T *C,
                  const T *srcv, T *traces, int record, hipStream_t st, const Shape *sh = nullptr)
     {
         if (act <= 0) return ADS_OK;
         const Shape &shape = sh ? *sh : fwd_shape;
-        const double cells = (double)act * (double)NX * (double)NY;
-        FwdArgs<T> a{uo, w, wo, phi, psi, phio, psio, C, d_sx.as<T>(), d_ty.as<T>(),
-                     d_nearx.as<uint8_t>(), d_neary.as<uint8_t>(), (int)NX, (int)NY, L.P, act,
-                     std::min(shape.sg, act), TileMap{}, k};
+        FwdArgs<T> a = fwd_args(L, act, uo, w, wo, phi, psi, phio, psio, C, shape);
         int pr = prof_start(ADS_PROF_AC_FWD, st);
         launch_fwd_all(a, L, shape, st);
-        prof_stop(pr, st, cells * sizeof(T) * (4.0 + (variant == 1 ? 4.0 : 2.0) * f_pml));
-        if (variant == 0) {
-            dim3 g(cdiv(NY + 2, 64), cdiv(NX + 2, 4), act);
-            ac_fwd_phipsi<T><<<g, 256, 0, st>>>(cst(uo), phi, psi, phio, psio, C, d_sx.as<T>(),
-                                                d_ty.as<T>(), d_pmlx.as<uint8_t>(),
-                                                d_pmly.as<uint8_t>(), (int)NX, (int)NY, L.P, k);
-        }
-        ac_src_rcv<T><<<act, 256, 0, st>>>(uo, n, d_us_ptr.as<int64_t>(), d_us_cell[L.li].template as<int64_t>(),
-                                           d_us_eptr.as<int64_t>(), d_us_ent.as<int64_t>(), srcv,
-                                           d_srcv_off.as<int64_t>(), d_nsrc.as<int64_t>(), k.dt2,
-                                           d_rcv_ptr.as<int64_t>(), d_rcv_cell[L.li].template as<int64_t>(),
-                                           traces, d_trace_off.as<int64_t>(), record);
+        prof_stop(pr, st, fwd_level_bytes(act));
+        fwd_post(L, a, n, srcv, traces, record, st);
         ADS_LAUNCH_CHECK();
         return ADS_OK;
     }
@@ -1133,6 +1291,55 @@ template <typename T> struct AcPlan : AcPlanBase {
         return ADS_OK;
     }
 
+    // Levels n and n+1 of the plan layout with the two-level rectangle pass: [rectangle
+    // levels n, n+1 + frame level n], phi/psi and sources of level n, [frame of the shrunk
+    // rectangle, level n+1], phi/psi and sources of level n+1.
+    template <int RPT>
+    void launch_level2(FwdArgs<T> a, Lv<T> u2, hipStream_t st)
+    {
+        const Rect &R = rect[0];
+        a.map = rect_tiles(R, VECP, RPT, a.act, a.Sg);
+        const TileMap fm = frame_tiles(0, a.act, a.Sg);
+        const int nfb = (int)grid_of(fm);
+        const unsigned nb = (unsigned)nfb + grid_of(a.map);
+        const int4 *it = d_items[0].template as<int4>();
+        if (variant == 1)
+            ac_fwd_level2<T, RPT, VECP, 1><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, R, fm, it, nitems[0], nfb);
+        else
+            ac_fwd_level2<T, RPT, VECP, 0><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, R, fm, it, nitems[0], nfb);
+    }
+    int fwd_pair(const WS &w, int n, int act, const T *srcv, T *traces, int record, hipStream_t st,
+                 const Shape *sh)
+    {
+        const Layout L = plan_layout();
+        const Shape &shape = sh ? *sh : fwd_shape;
+        Lv<T> ph[2] = {{w.phi[0], Npad}, {w.phi[1], Npad}}, ps[2] = {{w.psi[0], Npad}, {w.psi[1], Npad}};
+        FwdArgs<T> a = fwd_args(L, act, slot(w, n), cst(slot(w, n - 1)), cst(slot(w, n - 2)),
+                                cst(ph[(n - 1) & 1]), cst(ps[(n - 1) & 1]), ph[n & 1], ps[n & 1], w.C, shape);
+        int pr = prof_start(ADS_PROF_AC_FWD, st);
+        if (shape.rpt == 1) launch_level2<1>(a, slot(w, n + 1), st);
+        else if (shape.rpt == 4) launch_level2<4>(a, slot(w, n + 1), st);
+        else launch_level2<2>(a, slot(w, n + 1), st);
+        prof_stop(pr, st, fwd_level_bytes(act));
+        fwd_post(L, a, n, srcv, traces, record, st);
+        FwdArgs<T> b = fwd_args(L, act, slot(w, n + 1), cst(slot(w, n)), cst(slot(w, n - 1)),
+                                cst(ph[n & 1]), cst(ps[n & 1]), ph[(n + 1) & 1], ps[(n + 1) & 1], w.C, shape);
+        pr = prof_start(ADS_PROF_AC_FWD, st);
+        if (nitems1 > 0) {
+            b.map.ntj = 1;
+            b.map.nstrips = nitems1;
+            b.map.ngroups = (int)cdiv(act, b.Sg);
+            b.map.nb = b.map.ntj * b.map.nstrips * b.map.ngroups;
+            const int4 *it = d_items1.template as<int4>();
+            if (variant == 1) ac_fwd_frame<T, 1><<<grid_of(b.map), WAVE * NWAVES, 0, st>>>(b, it, nitems1);
+            else ac_fwd_frame<T, 0><<<grid_of(b.map), WAVE * NWAVES, 0, st>>>(b, it, nitems1);
+        }
+        prof_stop(pr, st, fwd_level_bytes(act));
+        fwd_post(L, b, n + 1, srcv, traces, record, st);
+        ADS_LAUNCH_CHECK();
+        return ADS_OK;
+    }
+
     int run_forward(const WS &w, int64_t lo, int64_t hi, const T *srcv, T *traces, int record,
                     hipStream_t st, const Shape *sh = nullptr)
     {
@@ -1148,6 +1355,11 @@ template <typename T> struct AcPlan : AcPlanBase {
         const Layout L = plan_layout();
         for (int64_t n = lo + 1; n <= hi; n++) {
             int act = active[n];
+            if (blocking == 2 && blk_ok && act > 0 && n + 1 <= hi && active[n + 1] == act) {
+                ADS_TRY(fwd_pair(w, (int)n, act, srcv, traces, record, st, sh));
+                n++;
+                continue;
+            }
             Lv<T> pi{w.phi[(n - 1) & 1], Npad}, si{w.psi[(n - 1) & 1], Npad};
             Lv<T> po{w.phi[n & 1], Npad}, so{w.psi[n & 1], Npad};
             ADS_TRY(fwd_step(L, (int)n, act, slot(w, n), cst(slot(w, n - 1)), cst(slot(w, n - 2)),
@@ -1555,6 +1767,18 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
         P->nitems[li] = (int)items.size();
         ADS_TRY(P->d_items[li].upload(items));
     }
+    {
+        const Rect &R = P->rect[0];
+        P->rect1 = Rect{R.iL + 1, R.iR - 1, R.jL + 1, R.jR - 1};
+        bool src_in = false;
+        for (size_t e = 0; e < si.size(); e++)
+            src_in |= si[e] >= R.iL && si[e] < R.iR && sj[e] >= R.jL && sj[e] < R.jR;
+        P->blk_ok = !src_in && P->rect1.iR > P->rect1.iL && P->rect1.jR > P->rect1.jL;
+        std::vector<int4> items;
+        frame_items(P->blk_ok ? P->rect1 : Rect{1, 1, 1, 1}, p->NX, p->NY, items);
+        P->nitems1 = P->blk_ok ? (int)items.size() : 0;
+        if (P->blk_ok) ADS_TRY(P->d_items1.upload(items));
+    }
     ADS_TRY(P->d_pmlx.upload(P->pmlx));
     ADS_TRY(P->d_pmly.upload(P->pmly));
     ADS_TRY(P->d_nearx.upload(P->nearx));
@@ -1741,6 +1965,23 @@ int ads_acoustic_plan_set_graph(ads_acoustic_plan *plan, int enable)
         if (!enable) P.drop_graphs();
         return ADS_OK;
     });
+}
+
+int ads_acoustic_plan_set_blocking(ads_acoustic_plan *plan, int levels)
+{
+    clear_error();
+    if (levels != 1 && levels != 2) ADS_FAIL(ADS_ERR_ARG, "levels must be 1 or 2, got %d", levels);
+    return dispatch(plan, [&](auto &P) -> int {
+        if (P.blocking != levels) P.drop_graphs();
+        P.blocking = levels;
+        return ADS_OK;
+    });
+}
+
+int ads_acoustic_plan_blocking(const ads_acoustic_plan *plan)
+{
+    if (!plan || !plan->impl) return 0;
+    return plan->impl->blocking == 2 && plan->impl->blk_ok ? 2 : 1;
 }
 
 int ads_acoustic_plan_forward_history(ads_acoustic_plan *plan, const void *model,

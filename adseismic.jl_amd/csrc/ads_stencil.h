@@ -89,6 +89,28 @@ __device__ inline void lr_of(const T (&x)[VEC], T el, T er, int lane, bool re, T
     if (re) r = er;
 }
 
+// Frame items covering every cell of [0, NX+1] x [0, NY+1] outside the rectangle R (all of it
+// when R is empty): frows rows by at most fwidth columns each, in row order within each band.
+static inline void frame_items(const Rect &R, int64_t NX, int64_t NY, std::vector<int4> &items,
+                               int frows = FRPT, int fwidth = WAVE)
+{
+    items.clear();
+    auto add = [&](int64_t r0, int64_t r1, int64_t c0, int64_t c1) {
+        for (int64_t i = r0; i < r1; i += frows)
+            for (int64_t j = c0; j < c1; j += fwidth)
+                items.push_back(make_int4((int)i, (int)std::min<int64_t>(frows, r1 - i), (int)j,
+                                          (int)std::min<int64_t>(j + fwidth, c1)));
+    };
+    if (R.iR <= R.iL || R.jR <= R.jL) {
+        add(0, NX + 2, 0, NY + 2);
+        return;
+    }
+    add(0, R.iL, 0, NY + 2);
+    add(R.iL, R.iR, 0, R.jL);
+    add(R.iL, R.iR, R.jR, NY + 2);
+    add(R.iR, NX + 2, 0, NY + 2);
+}
+
 // Interior rectangle: the longest run of rows and of columns whose "plain" flag is set (the
 // plain scheme is exact there), columns rounded inwards to multiples of vec.  Frame items cover
 // every other cell of [0, NX+1] x [0, NY+1] (the padding columns of a pitched row are never
@@ -118,21 +140,7 @@ static inline void split_grid(const std::vector<uint8_t> &plainx, const std::vec
     jR = jR / vec * vec;
     if (iR <= iL || jR <= jL) iL = iR = jL = jR = 1;
     R = Rect{(int)iL, (int)iR, (int)jL, (int)jR};
-    items.clear();
-    auto add = [&](int64_t r0, int64_t r1, int64_t c0, int64_t c1) {
-        for (int64_t i = r0; i < r1; i += frows)
-            for (int64_t j = c0; j < c1; j += fwidth)
-                items.push_back(make_int4((int)i, (int)std::min<int64_t>(frows, r1 - i), (int)j,
-                                          (int)std::min<int64_t>(j + fwidth, c1)));
-    };
-    if (iR <= iL) {
-        add(0, NX + 2, 0, NY + 2);
-        return;
-    }
-    add(0, iL, 0, NY + 2);
-    add(iL, iR, 0, jL);
-    add(iL, iR, jR, NY + 2);
-    add(iR, NX + 2, 0, NY + 2);
+    frame_items(R, NX, NY, items, frows, fwidth);
 }
 
 // Trace misfit, deterministic: resid = 2 (traces - observed) and one double partial sum of

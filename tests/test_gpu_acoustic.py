@@ -215,6 +215,49 @@ def test_plan_grid_splits_f64(orc, ads, cuda, kernel, split):
     assert rel(ct.grad.cpu(), g0) < 1e-10
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_two_level_blocking_is_bit_identical(orc, ads, cuda, kernel, dtype):
+    """The forward's temporal blocking (two levels per interior-rectangle pass, the default)
+    gives the single-level traces, loss and gradient bit for bit: over the grid splits plus a
+    grid wide and tall enough for several column segments and row strips, three kernel shapes,
+    two checkpoint intervals, an odd record length and uneven records.  A plan with a source
+    inside the rectangle (the last case) steps one level at a time."""
+    cases = SPLITS + [dict(NX=70, NY=300, npml=8, use=(True, True, True, True)),
+                      dict(NX=40, NY=44, npml=5, use=(True, True, True, True), centre=True)]
+    seen = set()
+    for split, cfg in enumerate(cases):
+        p, c, srcs, rcvs, nstep = make_case(orc, ads, NX=cfg["NX"], NY=cfg["NY"], NSTEP=41,
+                                            npml=cfg["npml"], nshots=5, kernel=kernel,
+                                            seed=40 + split, uneven=True)
+        p.USE_PML_XMIN, p.USE_PML_XMAX, p.USE_PML_YMIN, p.USE_PML_YMAX = cfg["use"]
+        if cfg.get("centre"):
+            srcs[1] = ads.AcousticSource([cfg["NX"] // 2], [cfg["NY"] // 2], srcs[1].srcv)
+        plan = ads.AcousticPlan(p, srcs, rcvs, dtype=dtype, nstep=nstep)
+        seen.add(plan.blocking)
+        if cfg.get("centre"):
+            assert plan.blocking == 1
+        if split == len(SPLITS):
+            assert plan.blocking == 2
+        srcv = plan.pack_srcv([s.srcv for s in srcs])
+        ct = T(c, dtype)
+        obs = plan.forward(ct * 1.02, srcv, 9).clone()
+        for shape in [(1, 2, 1, 1, 2, 1), (2, 3, 1, 2, 3, 1), (4, 5, 1, 1, 5, 1)]:
+            plan.set_shape(shape)
+            for K in (5, 1000):
+                res = []
+                for levels in (1, 2):
+                    plan.set_blocking(levels)
+                    loss, g, tr = plan.gradient(ct, srcv, obs, K, want_traces=True)
+                    res.append((loss.clone(), g.clone(), tr.clone()))
+                (l1, g1, t1), (l2, g2, t2) = res
+                assert torch.equal(t1, t2), (split, shape, K)
+                assert torch.equal(g1, g2), (split, shape, K)
+                assert torch.equal(l1, l2), (split, shape, K)
+        plan.set_blocking(2)
+    assert seen == {1, 2}
+
+
 @pytest.mark.parametrize("kernel", [0, 1])
 def test_plan_f32_against_f64_oracle(orc, ads, cuda, kernel):
     """FP32 fields vs the FP64 oracle: traces <= 1e-4 rel L2, gradient <= 1e-3 rel L2."""
