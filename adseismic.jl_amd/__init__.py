@@ -24,5 +24,6 @@ from .io_mat import (load_acoustic_model, load_acoustic_receiver, load_acoustic_
                      load_elastic_model, load_elastic_receiver, load_elastic_source, load_params,
                      matread, safe_vec)
 from .fwi import AcousticFWI, Adam_, LBFGS_, compute_loss_and_grads  # noqa: F401
+from .nn import Generator, add_initial_model  # noqa: F401
 
 SimulatedObservation = SimulatedObservation_
