@@ -666,17 +666,23 @@ __global__ __launch_bounds__(WAVE *NWAVES) void el_adjv_frame_a(ElAdj<T> a, ElCo
     }
 }
 template <typename T>
-__global__ __launch_bounds__(WAVE *NWAVES) void el_adjv_frame_b(ElAdj<T> a, ElCoef<T> c,
-                                                                const int4 *__restrict__ items, int nitems)
+__device__ inline void adjv_frame_b_body(const ElAdj<T> &a, const ElCoef<T> &c,
+                                         const int4 *__restrict__ items, int nitems, int bx, int g)
 {
-    const int it = blockIdx.x * NWAVES + threadIdx.x / WAVE;
+    const int it = bx * NWAVES + threadIdx.x / WAVE;
     if (it >= nitems) return;
     const int4 m = items[it];
     const int j = m.z + (threadIdx.x & (WAVE - 1));
     if (j >= m.w) return;
-    const int g = blockIdx.y, s1 = min(a.act, (g + 1) * a.Sg);
+    const int s1 = min(a.act, (g + 1) * a.Sg);
     for (int r = 0; r < m.y; r++)
         for (int s = g * a.Sg; s < s1; s++) adjv_b_cell(a, c, s, m.x + r, j);
+}
+template <typename T>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_adjv_frame_b(ElAdj<T> a, ElCoef<T> c,
+                                                                const int4 *__restrict__ items, int nitems)
+{
+    adjv_frame_b_body(a, c, items, nitems, (int)blockIdx.x, (int)blockIdx.y);
 }
 template <typename T>
 __global__ __launch_bounds__(WAVE *NWAVES) void el_adjs_frame_a(ElAdj<T> a, ElCoef<T> c,
@@ -699,17 +705,23 @@ __global__ __launch_bounds__(WAVE *NWAVES) void el_adjs_frame_a(ElAdj<T> a, ElCo
     }
 }
 template <typename T>
-__global__ __launch_bounds__(WAVE *NWAVES) void el_adjs_frame_b(ElAdj<T> a, ElCoef<T> c,
-                                                                const int4 *__restrict__ items, int nitems)
+__device__ inline void adjs_frame_b_body(const ElAdj<T> &a, const ElCoef<T> &c,
+                                         const int4 *__restrict__ items, int nitems, int bx, int g)
 {
-    const int it = blockIdx.x * NWAVES + threadIdx.x / WAVE;
+    const int it = bx * NWAVES + threadIdx.x / WAVE;
     if (it >= nitems) return;
     const int4 m = items[it];
     const int j = m.z + (threadIdx.x & (WAVE - 1));
     if (j >= m.w) return;
-    const int g = blockIdx.y, s1 = min(a.act, (g + 1) * a.Sg);
+    const int s1 = min(a.act, (g + 1) * a.Sg);
     for (int r = 0; r < m.y; r++)
         for (int s = g * a.Sg; s < s1; s++) adjs_b_cell(a, c, s, m.x + r, j);
+}
+template <typename T>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_adjs_frame_b(ElAdj<T> a, ElCoef<T> c,
+                                                                const int4 *__restrict__ items, int nitems)
+{
+    adjs_frame_b_body(a, c, items, nitems, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 // Rectangle kernels: a workgroup owns an LT_I x LT_J tile and loops over its shot group.  Per
@@ -727,11 +739,11 @@ template <typename T> __device__ inline T d4(T a0, T a1, T am, T ap)  // 27 a0 -
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void el_adjv_rect(ElAdj<T> a, ElCoef<T> c, TileMap map)
+__device__ inline void adjv_rect_body(const ElAdj<T> &a, const ElCoef<T> &c, const TileMap &map, int bid)
 {
     __shared__ T S[7][LS_I][LS_J];  // t0..t3, then sxx, sxy, syy of level n
     int g, strip, tj;
-    if (!decode_tile(map, g, strip, tj)) return;
+    if (!decode_tile(map, g, strip, tj, bid)) return;
     const int i0 = a.R.iL + strip * LT_I, j0 = a.R.jL + tj * LT_J;
     const int tc = threadIdx.x % LT_J, tr = threadIdx.x / LT_J;
     const int j = j0 + tc;
@@ -798,11 +810,11 @@ __global__ __launch_bounds__(256) void el_adjv_rect(ElAdj<T> a, ElCoef<T> c, Til
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void el_adjs_rect(ElAdj<T> a, ElCoef<T> c, TileMap map)
+__device__ inline void adjs_rect_body(const ElAdj<T> &a, const ElCoef<T> &c, const TileMap &map, int bid)
 {
     __shared__ T S[4][LS_I][LS_J];  // t0..t3
     int g, strip, tj;
-    if (!decode_tile(map, g, strip, tj)) return;
+    if (!decode_tile(map, g, strip, tj, bid)) return;
     const int i0 = a.R.iL + strip * LT_I, j0 = a.R.jL + tj * LT_J;
     const int tc = threadIdx.x % LT_J, tr = threadIdx.x / LT_J;
     const int j = j0 + tc;
@@ -865,6 +877,37 @@ __global__ __launch_bounds__(256) void el_adjs_rect(ElAdj<T> a, ElCoef<T> c, Til
         g_at(a, G_MB1, g, ij) += gM1[r];
         g_at(a, G_MB2, g, ij) += gM2[r];
     }
+}
+
+// Rectangle kernels alone, and fused with frame_b (independent cells, same phase): workgroups
+// [0, nfb) take the frame items (nib item blocks per shot group), the rest the rectangle tiles.
+template <typename T>
+__global__ __launch_bounds__(256) void el_adjv_rect(ElAdj<T> a, ElCoef<T> c, TileMap map)
+{
+    adjv_rect_body(a, c, map, (int)blockIdx.x);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void el_adjs_rect(ElAdj<T> a, ElCoef<T> c, TileMap map)
+{
+    adjs_rect_body(a, c, map, (int)blockIdx.x);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void el_adjv_rb(ElAdj<T> a, ElCoef<T> c, TileMap map,
+                                                  const int4 *__restrict__ items, int nitems, int nib,
+                                                  int nfb)
+{
+    const int b = (int)blockIdx.x;
+    if (b < nfb) adjv_frame_b_body(a, c, items, nitems, b % nib, b / nib);
+    else adjv_rect_body(a, c, map, b - nfb);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void el_adjs_rb(ElAdj<T> a, ElCoef<T> c, TileMap map,
+                                                  const int4 *__restrict__ items, int nitems, int nib,
+                                                  int nfb)
+{
+    const int b = (int)blockIdx.x;
+    if (b < nfb) adjs_frame_b_body(a, c, items, nitems, b % nib, b / nib);
+    else adjs_rect_body(a, c, map, b - nfb);
 }
 
 // rho-bar of fw4 at every cell of [1, NX-1] x [1, NY-1] (Core.jl:201), 0 elsewhere
@@ -1229,14 +1272,19 @@ template <typename T> struct ElPlan : ElPlanBase {
         // the stress cotangents (3) and velocities (2), reads and writes the velocity cotangents
         // (4), reads lambda and mu; CPML memory values and cotangents in their strips
         int pr = prof_start(ADS_PROF_EL_ADJ_VEL, st);
+        const int nib = (int)cdiv(nitems[li], NWAVES), nfb = nib * ng;
         if (nitems[li]) el_adjv_frame_a<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
-        if (rr) el_adjv_rect<T><<<rg, WAVE * NWAVES, 0, st>>>(a, c, m);
-        if (nitems[li]) el_adjv_frame_b<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
+        if (rr && nitems[li])
+            el_adjv_rb<T><<<(unsigned)nfb + rg, WAVE * NWAVES, 0, st>>>(a, c, m, items, nitems[li], nib, nfb);
+        else if (rr) el_adjv_rect<T><<<rg, WAVE * NWAVES, 0, st>>>(a, c, m);
+        else if (nitems[li]) el_adjv_frame_b<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
         prof_stop(pr, st, cells * sizeof(T) * (12.0 + 12.0 * (f_x + f_y)));
         pr = prof_start(ADS_PROF_EL_ADJ_STRESS, st);
         if (nitems[li]) el_adjs_frame_a<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
-        if (rr) el_adjs_rect<T><<<rg, WAVE * NWAVES, 0, st>>>(a, c, m);
-        if (nitems[li]) el_adjs_frame_b<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
+        if (rr && nitems[li])
+            el_adjs_rb<T><<<(unsigned)nfb + rg, WAVE * NWAVES, 0, st>>>(a, c, m, items, nitems[li], nib, nfb);
+        else if (rr) el_adjs_rect<T><<<rg, WAVE * NWAVES, 0, st>>>(a, c, m);
+        else if (nitems[li]) el_adjs_frame_b<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
         prof_stop(pr, st, cells * sizeof(T) * (11.0 + 12.0 * (f_x + f_y)));
         ADS_LAUNCH_CHECK();
         return ADS_OK;
