@@ -564,10 +564,14 @@ __device__ inline void adjv_a_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s,
 // velocity phase gathers at (i, j): sxx <- (D-x)^T t0 ; sxy <- (D-y)^T t1 + (D+x)^T t2 ;
 // syy <- (D+y)^T t3
 template <typename T>
-__device__ inline void adjv_b_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s, int i, int j)
+__device__ inline void adjv_b_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s, int i, int j,
+                                   bool from_tb = false)
 {
     const int64_t ij = (int64_t)i * c.P + j;
-    auto t = [&](int k, int ii, int jj) { return tv_vel(a, c, k, s, ii, jj); };
+    auto t = [&](int k, int ii, int jj) {
+        if (!from_tb) return tv_vel(a, c, k, s, ii, jj);
+        return ii >= 1 && ii <= c.NX && jj >= 1 && jj <= c.NY ? tb_at(a, k, s, (int64_t)ii * c.P + jj) : T(0);
+    };
     const T gxx = (T(27) * t(0, i, j) - T(27) * t(0, i + 1, j) - t(0, i - 1, j) + t(0, i + 2, j)) * c.ihx;
     const T gxy = (T(27) * t(1, i, j) - T(27) * t(1, i, j + 1) - t(1, i, j - 1) + t(1, i, j + 2)) * c.ihy +
                   (T(27) * t(2, i - 1, j) - T(27) * t(2, i, j) - t(2, i - 2, j) + t(2, i + 1, j)) * c.ihx;
@@ -633,10 +637,14 @@ __device__ inline void adjs_a_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s,
 
 // stress phase gathers: vx <- (D+x)^T t0 + (D+y)^T t3 ; vy <- (D-y)^T t1 + (D-x)^T t2
 template <typename T>
-__device__ inline void adjs_b_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s, int i, int j)
+__device__ inline void adjs_b_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s, int i, int j,
+                                   bool from_tb = false)
 {
     const int64_t ij = (int64_t)i * c.P + j;
-    auto t = [&](int k, int ii, int jj) { return tv_str(a, c, k, s, ii, jj); };
+    auto t = [&](int k, int ii, int jj) {
+        if (!from_tb) return tv_str(a, c, k, s, ii, jj);
+        return ii >= 1 && ii <= c.NX && jj >= 1 && jj <= c.NY ? tb_at(a, k, s, (int64_t)ii * c.P + jj) : T(0);
+    };
     const T gvx = (T(27) * t(0, i - 1, j) - T(27) * t(0, i, j) - t(0, i - 2, j) + t(0, i + 1, j)) * c.ihx +
                   (T(27) * t(3, i, j - 1) - T(27) * t(3, i, j) - t(3, i, j - 2) + t(3, i, j + 1)) * c.ihy;
     const T gvy = (T(27) * t(1, i, j) - T(27) * t(1, i, j + 1) - t(1, i, j - 1) + t(1, i, j + 2)) * c.ihy +
@@ -647,15 +655,15 @@ __device__ inline void adjs_b_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s,
 
 // frame kernels: one column per lane, FRPT rows per wavefront item, grid y = shot group
 template <typename T>
-__global__ __launch_bounds__(WAVE *NWAVES) void el_adjv_frame_a(ElAdj<T> a, ElCoef<T> c,
-                                                                const int4 *__restrict__ items, int nitems)
+__device__ inline void adjv_frame_a_body(const ElAdj<T> &a, const ElCoef<T> &c,
+                                         const int4 *__restrict__ items, int nitems, int bx, int g)
 {
-    const int it = blockIdx.x * NWAVES + threadIdx.x / WAVE;
+    const int it = bx * NWAVES + threadIdx.x / WAVE;
     if (it >= nitems) return;
     const int4 m = items[it];
     const int j = m.z + (threadIdx.x & (WAVE - 1));
     if (j >= m.w) return;
-    const int g = blockIdx.y, s1 = min(a.act, (g + 1) * a.Sg);
+    const int s1 = min(a.act, (g + 1) * a.Sg);
     for (int r = 0; r < m.y; r++) {
         const int i = m.x + r;
         T gR = 0, gB = 0;
@@ -665,6 +673,53 @@ __global__ __launch_bounds__(WAVE *NWAVES) void el_adjv_frame_a(ElAdj<T> a, ElCo
         g_at(a, G_RB, g, ij) += gB;
     }
 }
+// Stress phase, ring of the rectangle (its cells within two of its edge, which the frame
+// cells' gathers reach): t0..t3 recomputed as on rectangle cells and stored in the
+// temporaries, so that frame_b reads every neighbour's t-value from the temporaries alone.
+// One column per lane.
+template <typename T>
+__device__ inline void adjs_ring_body(const ElAdj<T> &a, const ElCoef<T> &c,
+                                      const int4 *__restrict__ items, int nitems, int bx, int g)
+{
+    const int it = bx * NWAVES + threadIdx.x / WAVE;
+    if (it >= nitems) return;
+    const int4 m = items[it];
+    const int j = m.z + (threadIdx.x & (WAVE - 1));
+    if (j >= m.w) return;
+    const int s1 = min(a.act, (g + 1) * a.Sg);
+    for (int r = 0; r < m.y; r++) {
+        const int i = m.x + r;
+        const int64_t ij = (int64_t)i * c.P + j;
+        for (int s = g * a.Sg; s < s1; s++) {
+            T *tb = a.Tb + (int64_t)s * a.npad + ij;
+            tb[0] = tv_str(a, c, 0, s, i, j);
+            tb[a.tstride] = tv_str(a, c, 1, s, i, j);
+            const T t2 = tv_str(a, c, 2, s, i, j);
+            tb[2 * a.tstride] = t2;
+            tb[3 * a.tstride] = t2;
+        }
+    }
+}
+// frame_a (and, in the stress phase, the ring) in one launch: blocks [0, nfa) frame items,
+// then ring items (nib and nrb item blocks per shot group)
+template <typename T>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_adjv_a(ElAdj<T> a, ElCoef<T> c, const int4 *__restrict__ items,
+                                                          int nitems, int nib, const int4 *__restrict__,
+                                                          int, int, int nfa)
+{
+    const int b = (int)blockIdx.x;
+    if (b < nfa) adjv_frame_a_body(a, c, items, nitems, b % nib, b / nib);
+}
+template <typename T>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_adjs_a(ElAdj<T> a, ElCoef<T> c, const int4 *__restrict__ items,
+                                                          int nitems, int nib, const int4 *__restrict__ ring,
+                                                          int nring, int nrb, int nfa)
+{
+    const int b = (int)blockIdx.x;
+    if (b < nfa) adjs_frame_a_body(a, c, items, nitems, b % nib, b / nib);
+    else adjs_ring_body(a, c, ring, nring, (b - nfa) % nrb, (b - nfa) / nrb);
+}
+
 template <typename T>
 __device__ inline void adjv_frame_b_body(const ElAdj<T> &a, const ElCoef<T> &c,
                                          const int4 *__restrict__ items, int nitems, int bx, int g)
@@ -685,15 +740,15 @@ __global__ __launch_bounds__(WAVE *NWAVES) void el_adjv_frame_b(ElAdj<T> a, ElCo
     adjv_frame_b_body(a, c, items, nitems, (int)blockIdx.x, (int)blockIdx.y);
 }
 template <typename T>
-__global__ __launch_bounds__(WAVE *NWAVES) void el_adjs_frame_a(ElAdj<T> a, ElCoef<T> c,
-                                                                const int4 *__restrict__ items, int nitems)
+__device__ inline void adjs_frame_a_body(const ElAdj<T> &a, const ElCoef<T> &c,
+                                         const int4 *__restrict__ items, int nitems, int bx, int g)
 {
-    const int it = blockIdx.x * NWAVES + threadIdx.x / WAVE;
+    const int it = bx * NWAVES + threadIdx.x / WAVE;
     if (it >= nitems) return;
     const int4 m = items[it];
     const int j = m.z + (threadIdx.x & (WAVE - 1));
     if (j >= m.w) return;
-    const int g = blockIdx.y, s1 = min(a.act, (g + 1) * a.Sg);
+    const int s1 = min(a.act, (g + 1) * a.Sg);
     for (int r = 0; r < m.y; r++) {
         const int i = m.x + r;
         T gL = 0, gM1 = 0, gM2 = 0;
@@ -715,7 +770,7 @@ __device__ inline void adjs_frame_b_body(const ElAdj<T> &a, const ElCoef<T> &c,
     if (j >= m.w) return;
     const int s1 = min(a.act, (g + 1) * a.Sg);
     for (int r = 0; r < m.y; r++)
-        for (int s = g * a.Sg; s < s1; s++) adjs_b_cell(a, c, s, m.x + r, j);
+        for (int s = g * a.Sg; s < s1; s++) adjs_b_cell(a, c, s, m.x + r, j, true);
 }
 template <typename T>
 __global__ __launch_bounds__(WAVE *NWAVES) void el_adjs_frame_b(ElAdj<T> a, ElCoef<T> c,
@@ -1015,6 +1070,8 @@ struct ElPlanBase {
     Rect rect[2]{};
     DevBuf d_items[2];
     int nitems[2] = {0, 0};
+    DevBuf d_ring[2];  // the rectangle's cells within two of its edge (adjoint t-values)
+    int nring[2] = {0, 0};
     int rpt = 1, sg = 4;  // rectangle kernels: rows per wavefront, shots per workgroup
     int sub = 0;          // one MPIElastic subdomain (see ElCoef)
     int64_t PX = 0, PY = 0;  // CPML profile lengths (NX, NY; the block size in the subdomain form)
@@ -1273,14 +1330,22 @@ template <typename T> struct ElPlan : ElPlanBase {
         // (4), reads lambda and mu; CPML memory values and cotangents in their strips
         int pr = prof_start(ADS_PROF_EL_ADJ_VEL, st);
         const int nib = (int)cdiv(nitems[li], NWAVES), nfb = nib * ng;
-        if (nitems[li]) el_adjv_frame_a<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
+        // velocity phase: frame_a alone (its gathers recompute rectangle t-values, two loads
+        // each); stress phase: frame_a with the rectangle's ring, whose t-values cost five
+        // parameter loads each to recompute, so frame_b reads them from the temporaries
+        const int nrb = (int)cdiv(nring[li], NWAVES), nfa = nib * ng;
+        const unsigned ga = (unsigned)(nfa + nrb * ng);
+        const int4 *ring = d_ring[li].template as<int4>();
+        if (nfa) el_adjv_a<T><<<nfa, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li], std::max(nib, 1), ring,
+                                                             0, 1, nfa);
         if (rr && nitems[li])
             el_adjv_rb<T><<<(unsigned)nfb + rg, WAVE * NWAVES, 0, st>>>(a, c, m, items, nitems[li], nib, nfb);
         else if (rr) el_adjv_rect<T><<<rg, WAVE * NWAVES, 0, st>>>(a, c, m);
         else if (nitems[li]) el_adjv_frame_b<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
         prof_stop(pr, st, cells * sizeof(T) * (12.0 + 12.0 * (f_x + f_y)));
         pr = prof_start(ADS_PROF_EL_ADJ_STRESS, st);
-        if (nitems[li]) el_adjs_frame_a<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
+        if (ga) el_adjs_a<T><<<ga, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li], std::max(nib, 1), ring,
+                                                           nring[li], std::max(nrb, 1), nfa);
         if (rr && nitems[li])
             el_adjs_rb<T><<<(unsigned)nfb + rg, WAVE * NWAVES, 0, st>>>(a, c, m, items, nitems[li], nib, nfb);
         else if (rr) el_adjs_rect<T><<<rg, WAVE * NWAVES, 0, st>>>(a, c, m);
@@ -1535,6 +1600,9 @@ static int build_el_plan(const ads_elastic_params *p, const ads_shot_geometry *g
             split_grid(px, py, NX, NY, li == 0 ? ElPlan<T>::VECP : 1, P->rect[li], items);
             P->nitems[li] = (int)items.size();
             ADS_TRY(P->d_items[li].upload(items));
+            ring_items(P->rect[li], 2, items);
+            P->nring[li] = (int)items.size();
+            if (!items.empty()) ADS_TRY(P->d_ring[li].upload(items));
         }
     }
     ADS_TRY(P->d_nsrc.upload(nsrc));

@@ -111,6 +111,26 @@ static inline void frame_items(const Rect &R, int64_t NX, int64_t NY, std::vecto
     add(R.iR, NX + 2, 0, NY + 2);
 }
 
+// Items covering the cells of R within w of its edge (each cell once), frows rows by at most
+// fwidth columns each.
+static inline void ring_items(const Rect &R, int w, std::vector<int4> &items, int frows = FRPT,
+                              int fwidth = WAVE)
+{
+    items.clear();
+    if (R.iR <= R.iL || R.jR <= R.jL) return;
+    auto add = [&](int r0, int r1, int c0, int c1) {
+        for (int i = r0; i < r1; i += frows)
+            for (int j = c0; j < c1; j += fwidth)
+                items.push_back(make_int4(i, std::min(frows, r1 - i), j, std::min(j + fwidth, c1)));
+    };
+    const int top = std::min(R.iL + w, R.iR), bot = std::max(R.iR - w, top);
+    add(R.iL, top, R.jL, R.jR);
+    add(bot, R.iR, R.jL, R.jR);
+    const int left = std::min(R.jL + w, R.jR), right = std::max(R.jR - w, left);
+    add(top, bot, R.jL, left);
+    add(top, bot, right, R.jR);
+}
+
 // Interior rectangle: the longest run of rows and of columns whose "plain" flag is set (the
 // plain scheme is exact there), columns rounded inwards to multiples of vec.  Frame items cover
 // every other cell of [0, NX+1] x [0, NY+1] (the padding columns of a pitched row are never
