@@ -803,8 +803,27 @@ __device__ inline void adjv_rect_body(const ElAdj<T> &a, const ElCoef<T> &c, con
     const int tc = threadIdx.x % LT_J, tr = threadIdx.x / LT_J;
     const int j = j0 + tc;
     const int s1 = min(a.act, (g + 1) * a.Sg);
-    T gR[4] = {}, gB[4] = {};
+    T gR[4] = {}, gB[4] = {}, irho[4], irb[4];
+    bool own[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {  // shot-invariant: 1/rho, 1/rho-bar of this thread's cells
+        const int i = i0 + tr + 4 * r;
+        own[r] = i < a.R.iR && j < a.R.jR;
+        const int64_t ij = (int64_t)i * c.P + j;
+        irho[r] = own[r] ? T(1) / a.rho[ij] : T(0);
+        irb[r] = own[r] ? T(1) / a.rb[ij] : T(0);
+    }
     for (int s = g * a.Sg; s < s1; s++) {
+        // this shot's stress cotangents at the thread's cells, loaded before the staging so
+        // that their latency overlaps it
+        T axx[4], axy[4], ayy[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int64_t ij = (int64_t)(i0 + tr + 4 * r) * c.P + j;
+            axx[r] = own[r] ? a.A.f(SXX, s)[ij] : T(0);
+            axy[r] = own[r] ? a.A.f(SXY, s)[ij] : T(0);
+            ayy[r] = own[r] ? a.A.f(SYY, s)[ij] : T(0);
+        }
         for (int q = threadIdx.x; q < LS_I * LS_J; q += 256) {
             const int li = q / LS_J, lj = q % LS_J;
             const int i = i0 - LH + li, jj = j0 - LH + lj;
@@ -833,7 +852,7 @@ __device__ inline void adjv_rect_body(const ElAdj<T> &a, const ElCoef<T> &c, con
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int li = tr + 4 * r, i = i0 + li;
-            if (i >= a.R.iR || j >= a.R.jR) continue;
+            if (!own[r]) continue;
             const int I = li + LH, J = tc + LH;
             const int64_t ij = (int64_t)i * c.P + j;
             // parameter terms (fw3: rho; fw4: rho-bar); t0 = g_vx dt / rho, t2 = g_vy dt / rho-bar
@@ -841,16 +860,16 @@ __device__ inline void adjv_rect_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                          d4(S[5][I][J], S[5][I][J - 1], S[5][I][J + 1], S[5][I][J - 2]) * c.ihy;
             const T S4 = d4(S[5][I + 1][J], S[5][I][J], S[5][I + 2][J], S[5][I - 1][J]) * c.ihx +
                          d4(S[6][I][J + 1], S[6][I][J], S[6][I][J + 2], S[6][I][J - 1]) * c.ihy;
-            gR[r] -= S[0][I][J] * S3 / a.rho[ij];
-            gB[r] -= S[2][I][J] * S4 / a.rb[ij];
+            gR[r] -= S[0][I][J] * S3 * irho[r];
+            gB[r] -= S[2][I][J] * S4 * irb[r];
             // sxx <- (D-x)^T t0 ; sxy <- (D-y)^T t1 + (D+x)^T t2 ; syy <- (D+y)^T t3
             const T gxx = d4(S[0][I][J], S[0][I + 1][J], S[0][I - 1][J], S[0][I + 2][J]) * c.ihx;
             const T gxy = d4(S[1][I][J], S[1][I][J + 1], S[1][I][J - 1], S[1][I][J + 2]) * c.ihy +
                           d4(S[2][I - 1][J], S[2][I][J], S[2][I - 2][J], S[2][I + 1][J]) * c.ihx;
             const T gyy = d4(S[3][I][J - 1], S[3][I][J], S[3][I][J - 2], S[3][I][J + 1]) * c.ihy;
-            a.A.f(SXX, s)[ij] += gxx;
-            a.A.f(SXY, s)[ij] += gxy;
-            a.A.f(SYY, s)[ij] += gyy;
+            a.A.f(SXX, s)[ij] = axx[r] + gxx;
+            a.A.f(SXY, s)[ij] = axy[r] + gxy;
+            a.A.f(SYY, s)[ij] = ayy[r] + gyy;
         }
         __syncthreads();
     }
@@ -875,7 +894,22 @@ __device__ inline void adjs_rect_body(const ElAdj<T> &a, const ElCoef<T> &c, con
     const int j = j0 + tc;
     const int s1 = min(a.act, (g + 1) * a.Sg);
     T gL[4] = {}, gM1[4] = {}, gM2[4] = {};
+    bool own[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) own[r] = i0 + tr + 4 * r < a.R.iR && j < a.R.jR;
     for (int s = g * a.Sg; s < s1; s++) {
+        // this shot's cotangents at the thread's cells, loaded before the staging so that
+        // their latency overlaps it
+        T avx[4], avy[4], axx[4], ayy[4], axy[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int64_t ij = (int64_t)(i0 + tr + 4 * r) * c.P + j;
+            avx[r] = own[r] ? a.A.f(VX, s)[ij] : T(0);
+            avy[r] = own[r] ? a.A.f(VY, s)[ij] : T(0);
+            axx[r] = own[r] ? a.A.f(SXX, s)[ij] : T(0);
+            ayy[r] = own[r] ? a.A.f(SYY, s)[ij] : T(0);
+            axy[r] = own[r] ? a.A.f(SXY, s)[ij] : T(0);
+        }
         for (int q = threadIdx.x; q < LS_I * LS_J; q += 256) {
             const int li = q / LS_J, lj = q % LS_J;
             const int i = i0 - LH + li, jj = j0 - LH + lj;
@@ -898,7 +932,7 @@ __device__ inline void adjs_rect_body(const ElAdj<T> &a, const ElCoef<T> &c, con
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int li = tr + 4 * r, i = i0 + li;
-            if (i >= a.R.iR || j >= a.R.jR) continue;
+            if (!own[r]) continue;
             const int I = li + LH, J = tc + LH, P = c.P;
             const int64_t ij = (int64_t)i * P + j;
             // parameter terms: fw1 (lambda-bar, mu-bar) and fw2 (mu-bar), from level n-1
@@ -906,20 +940,20 @@ __device__ inline void adjs_rect_body(const ElAdj<T> &a, const ElCoef<T> &c, con
             const T *__restrict__ vy = a.Sm.f(VY, s);
             const T X = d4(vx[ij + P], vx[ij], vx[ij + 2 * P], vx[ij - P]) * c.ihx;
             const T Y = d4(vy[ij], vy[ij - 1], vy[ij + 1], vy[ij - 2]) * c.ihy;
-            const T gx = a.A.f(SXX, s)[ij] * c.dt, gy = a.A.f(SYY, s)[ij] * c.dt;
+            const T gx = axx[r] * c.dt, gy = ayy[r] * c.dt;
             const T glm = X * gx + Y * gy;
             gL[r] += Y * gx + X * gy + glm;
             gM1[r] += T(2) * glm;
             const T dx = d4(vy[ij], vy[ij - P], vy[ij + P], vy[ij - 2 * P]) * c.ihx;
             const T dy = d4(vx[ij + 1], vx[ij], vx[ij + 2], vx[ij - 1]) * c.ihy;
-            gM2[r] += a.A.f(SXY, s)[ij] * (dx + dy) * c.dt;
+            gM2[r] += axy[r] * (dx + dy) * c.dt;
             // vx <- (D+x)^T t0 + (D+y)^T t3 ; vy <- (D-y)^T t1 + (D-x)^T t2
             const T gvx = d4(S[0][I - 1][J], S[0][I][J], S[0][I - 2][J], S[0][I + 1][J]) * c.ihx +
                           d4(S[3][I][J - 1], S[3][I][J], S[3][I][J - 2], S[3][I][J + 1]) * c.ihy;
             const T gvy = d4(S[1][I][J], S[1][I][J + 1], S[1][I][J - 1], S[1][I][J + 2]) * c.ihy +
                           d4(S[2][I][J], S[2][I + 1][J], S[2][I - 1][J], S[2][I + 2][J]) * c.ihx;
-            a.A.f(VX, s)[ij] += gvx;
-            a.A.f(VY, s)[ij] += gvy;
+            a.A.f(VX, s)[ij] = avx[r] + gvx;
+            a.A.f(VY, s)[ij] = avy[r] + gvy;
         }
         __syncthreads();
     }
