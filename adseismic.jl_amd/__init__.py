@@ -23,7 +23,8 @@ from .utils import (Gauss, Ricker, compute_lame_parameters, compute_properties, 
 from .io_mat import (load_acoustic_model, load_acoustic_receiver, load_acoustic_source,  # noqa: F401
                      load_elastic_model, load_elastic_receiver, load_elastic_source, load_params,
                      matread, safe_vec)
-from .fwi import AcousticFWI, Adam_, LBFGS_, compute_loss_and_grads  # noqa: F401
+from .fwi import (AcousticFWI, Adam_, LBFGS_, acoustic_gradient_in_batches,  # noqa: F401
+                  acoustic_traces_in_batches, compute_loss_and_grads)
 from .nn import Generator, add_initial_model  # noqa: F401
 
 SimulatedObservation = SimulatedObservation_

@@ -142,3 +142,23 @@ def check(rc: int) -> None:
     if rc != ADS_OK:
         msg = load().ads_last_error()
         raise ADSError(rc, msg.decode() if msg else "")
+
+
+def available_device_bytes() -> int:
+    """Device memory a new workspace can use: the free HBM plus what torch's caching allocator
+    holds but does not use (released workspaces of earlier plans stay cached)."""
+    import torch
+    free, _ = torch.cuda.mem_get_info()
+    return int(free + torch.cuda.memory_reserved() - torch.cuda.memory_allocated())
+
+
+def device_bytes(n: int):
+    """A uint8 device buffer of n bytes for a workspace.  A large request that torch's cache
+    cannot place (cached blocks of earlier, smaller workspaces) is retried once after the
+    cache is returned to the device."""
+    import torch
+    try:
+        return torch.empty(int(n), dtype=torch.uint8, device="cuda")
+    except torch.OutOfMemoryError:
+        torch.cuda.empty_cache()
+        return torch.empty(int(n), dtype=torch.uint8, device="cuda")

@@ -170,8 +170,7 @@ class AcousticPlan:
 
     def choose_interval(self, budget_bytes: Optional[int] = None) -> int:
         if budget_bytes is None:
-            free, _ = torch.cuda.mem_get_info()
-            budget_bytes = int(free * 0.6)
+            budget_bytes = int(_lib.available_device_bytes() * 0.6)
         K = int(self.lib.ads_acoustic_plan_choose_interval(self.h, int(budget_bytes)))
         if K < 1:
             raise MemoryError("no checkpoint interval fits the memory budget")
@@ -181,7 +180,7 @@ class AcousticPlan:
         need = self.workspace_size(K)
         if self._ws is None or self._ws.numel() < need:
             self._ws = None
-            self._ws = torch.empty(need, dtype=torch.uint8, device="cuda")
+            self._ws = _lib.device_bytes(need)
         self._ws_K = K
         return self._ws
 
@@ -256,7 +255,7 @@ class AcousticPlan:
     def _scratch_buf(self):
         need = int(self.lib.ads_acoustic_plan_history_scratch_size(self.h))
         if self._scratch is None or self._scratch.numel() < need:
-            self._scratch = torch.empty(need, dtype=torch.uint8, device="cuda")
+            self._scratch = _lib.device_bytes(need)
         return self._scratch
 
     def forward_history(self, model, srcv, with_aux: bool = True):

@@ -101,7 +101,7 @@ class ElasticPlan:
 
     def choose_interval(self, budget_bytes: Optional[int] = None) -> int:
         if budget_bytes is None:
-            budget_bytes = int(torch.cuda.mem_get_info()[0] * 0.6)
+            budget_bytes = int(_lib.available_device_bytes() * 0.6)
         K = int(self.lib.ads_elastic_plan_choose_interval(self.h, int(budget_bytes)))
         if K < 1:
             raise MemoryError("no checkpoint interval fits the memory budget")
@@ -111,7 +111,7 @@ class ElasticPlan:
         need = self.workspace_size(K)
         if self._ws is None or self._ws.numel() < need:
             self._ws = None
-            self._ws = torch.empty(need, dtype=torch.uint8, device="cuda")
+            self._ws = _lib.device_bytes(need)
         self._ws_K = K
         return self._ws
 
@@ -157,7 +157,7 @@ class ElasticPlan:
         hist = torch.empty((self.nshots, 5, L, self.gx, self.gy), dtype=self.dtype, device="cuda")
         need = int(self.lib.ads_elastic_plan_history_scratch_size(self.h))
         if self._scratch is None or self._scratch.numel() < need:
-            self._scratch = torch.empty(need, dtype=torch.uint8, device="cuda")
+            self._scratch = _lib.device_bytes(need)
         _lib.check(self.lib.ads_elastic_forward_history(self.h, _vp(r), _vp(l), _vp(m), _vp(srcv),
                                                         _vp(hist), _vp(self._scratch), _stream()))
         return hist

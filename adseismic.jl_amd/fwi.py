@@ -24,6 +24,7 @@ from typing import Callable, List, Optional, Sequence, Tuple
 import numpy as np
 import torch
 
+from . import _lib
 from .acoustic import AcousticPlan, SimulatedObservation_
 from .params import AcousticPropagatorParams, AcousticReceiver, AcousticSource
 
@@ -73,6 +74,61 @@ class AcousticFWI:
                              device="cuda")
         loss, g = self.loss_and_grad_device(xd)
         return float(loss.item()), g.double().cpu().numpy().reshape(-1)
+
+
+def _plans_in_batches(param, srcs, rcvs, dtype, max_shots: int, nstep, budget_fraction: float):
+    """Plans of at most ``max_shots`` consecutive shots, one after another, sharing one device
+    workspace (grown when a batch needs more), so that released workspaces do not pin torch's
+    cache.  Yields (shot indices, plan, K)."""
+    n = len(srcs)
+    ws = None
+    for b0 in range(0, n, int(max_shots)):
+        idx = list(range(b0, min(n, b0 + int(max_shots))))
+        plan = AcousticPlan(param, [srcs[i] for i in idx], [rcvs[i] for i in idx], dtype=dtype,
+                            nstep=None if nstep is None else [int(nstep[i]) for i in idx])
+        held = 0 if ws is None else ws.numel()
+        K = plan.choose_interval(int((_lib.available_device_bytes() + held) * budget_fraction))
+        need = plan.workspace_size(K)
+        if ws is None or ws.numel() < need:
+            ws = None
+            torch.cuda.empty_cache()
+            ws = _lib.device_bytes(need)
+        plan._ws = ws
+        yield idx, plan, K
+        plan._ws = None
+        del plan
+
+
+def acoustic_traces_in_batches(param: AcousticPropagatorParams, srcs: Sequence[AcousticSource],
+                               rcvs: Sequence[AcousticReceiver], c: torch.Tensor, max_shots: int,
+                               nstep=None, budget_fraction: float = 0.6) -> List[torch.Tensor]:
+    """Traces of every shot, (NSTEP_s+1, nrcv_s) each, through plans of at most ``max_shots``
+    shots (see :func:`acoustic_gradient_in_batches`)."""
+    out: List[torch.Tensor] = []
+    for idx, plan, K in _plans_in_batches(param, srcs, rcvs, c.dtype, max_shots, nstep, budget_fraction):
+        tr = plan.forward(c, plan.pack_srcv([srcs[i].srcv for i in idx]), K)
+        out += [t.clone() for t in plan.unpack_traces(tr)]
+    return out
+
+
+def acoustic_gradient_in_batches(param: AcousticPropagatorParams, srcs: Sequence[AcousticSource],
+                                 rcvs: Sequence[AcousticReceiver], observed, c: torch.Tensor,
+                                 max_shots: int, nstep=None, budget_fraction: float = 0.6):
+    """Misfit and gradient over a shot set too large for one plan's workspace: the shots go
+    through plans of at most ``max_shots`` consecutive shots each (one checkpointed batched
+    gradient per plan, interval chosen from ``budget_fraction`` of the available HBM, one
+    shared workspace), and the losses and gradients are summed on the device in shot-batch
+    order (the compute_loss_and_grads_GPU reduction, Utils.jl:325-331, on one GPU).  ``c`` is
+    the velocity (device tensor, the plans' dtype); ``nstep`` the per-shot record lengths
+    (default NSTEP)."""
+    loss, grad = None, None
+    for idx, plan, K in _plans_in_batches(param, srcs, rcvs, c.dtype, max_shots, nstep, budget_fraction):
+        srcv = plan.pack_srcv([srcs[i].srcv for i in idx])
+        obs = plan.pack_traces([observed[i] for i in idx])
+        l, g, _ = plan.gradient(c, srcv, obs, K)
+        loss = l if loss is None else loss + l
+        grad = g if grad is None else grad + g
+    return loss, grad
 
 
 def compute_loss_and_grads(model: Callable, src: Sequence[AcousticSource],

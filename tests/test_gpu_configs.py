@@ -64,3 +64,68 @@ def test_plain_fp64_configuration(orc, ads, cuda):
     rr = ads.SimulatedObservation_(ap, rcv)
     torch.sum((rr - torch.as_tensor(obs, device="cuda")) ** 2).backward()
     assert rel(cg.grad.cpu(), g_ref) < 1e-10
+
+
+@pytest.mark.timeout(900)
+def test_large_uneven_batch_configuration(ads, cuda):
+    """The 2000 x 2000, 64-shot configuration with uneven records (NSTEP_s ~ U{3000..6000},
+    spreads of 400..2000 receivers centred on each source), FP32, dt = 0.5 ms (chosen: CFL
+    0.5 at 5000 m/s), 40-cell PML, smooth random model 1500..5000 m/s (seed 4).  Its
+    checkpointed workspace does not fit one plan (about 300 GB for 64 shots at the best
+    interval), so the shots go through plans of 16 and of 8 (acoustic_gradient_in_batches).
+    The oracle is out of reach at this size; checked instead through size-independent
+    properties: (1) the two batchings (different batch sizes, shot groupings and checkpoint
+    intervals) give the same misfit and gradient to FP32 reassociation; (2) a central finite
+    difference of the misfit along a perturbation of 0.5% of the model in the direction of the
+    (smoothed) gradient matches <gradient, perturbation> within 2%."""
+    import time
+    from scipy.ndimage import gaussian_filter
+
+    rng = np.random.default_rng(4)
+    NX = NY = 2000
+    nshots = 64
+    dt, h = 5e-4, 5.0
+    nst = rng.integers(3000, 6001, nshots)
+    p = ads.AcousticPropagatorParams(NX=NX, NY=NY, NSTEP=int(nst.max()), DELTAX=h, DELTAY=h,
+                                     DELTAT=dt, NPOINTS_PML=40, vp_ref=3000.0, PropagatorKernel=1)
+    noise = gaussian_filter(rng.standard_normal((NX + 2, NY + 2)), 40.0)
+    noise = (noise - noise.min()) / (noise.max() - noise.min())
+    c_true = 1500.0 + 3500.0 * noise
+    ads.check_cfl(p, float(c_true.max()))
+    c0 = gaussian_filter(c_true, 30.0)
+    xs = np.round(np.linspace(41, NX - 40, nshots)).astype(int)
+    srcs, rcvs = [], []
+    for s in range(nshots):
+        w = ads.ricker_freq(8.0 + 4.0 * rng.random(), dt, int(nst[s]), 1e4)
+        srcs.append(ads.AcousticSource([int(xs[s])], [3], w.reshape(-1, 1)))
+        nr = int(rng.integers(400, 2001))
+        lo = int(np.clip(xs[s] - nr // 2, 1, NX + 2 - nr))
+        rcvs.append(ads.AcousticReceiver(np.arange(lo, lo + nr), np.full(nr, 3)))
+    T = lambda x: torch.as_tensor(x, dtype=torch.float32, device="cuda")
+    # observed data from the true model, 16 shots per plan
+    obs = ads.acoustic_traces_in_batches(p, srcs, rcvs, T(c_true), 16, nstep=nst)
+    cells = NX * NY * int(nst.sum())
+    res = {}
+    for ms in (16, 8):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res[ms] = ads.acoustic_gradient_in_batches(p, srcs, rcvs, obs, T(c0), ms, nstep=nst)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        print(f"{ms} shots per plan: {el:.2f} s, {cells / el / 1e9:.1f} Gcell-steps/s", flush=True)
+    (L16, g16), (L8, g8) = res[16], res[8]
+    assert L16.item() == pytest.approx(L8.item(), rel=1e-6)
+    assert rel(g8.cpu(), g16.cpu()) < 1e-5
+    # directional derivative along the (smoothed) gradient itself, scaled to a perturbation of
+    # at most 0.5% of the model: the largest first-order signal against FP32 rounding
+    gs = gaussian_filter(g16.double().cpu().numpy(), 5.0)
+    d = gs * (0.005 * c0.min() / np.abs(gs).max())
+    def misfit(cm):
+        tr = ads.acoustic_traces_in_batches(p, srcs, rcvs, T(cm), 16, nstep=nst)
+        return sum(float(torch.sum((a.double() - b.double()) ** 2)) for a, b in zip(tr, obs))
+
+    fd = (misfit(c0 + d) - misfit(c0 - d)) / 2.0
+    an = float(torch.sum(g16.double() * torch.as_tensor(d, device="cuda")).item())
+    print(f"directional derivative: finite difference {fd:.6e}, gradient {an:.6e}, "
+          f"misfit {L16.item():.6e}", flush=True)
+    assert abs(fd - an) <= 0.02 * abs(an)
