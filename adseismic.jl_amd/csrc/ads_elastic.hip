@@ -89,7 +89,7 @@ __device__ inline void el_stress_cell(const ElC<T> &I, const El<T> &O, const T *
                                       const T *__restrict__ mu, const ElCoef<T> &c, int s, int i,
                                       int j)
 {
-    const int NX = c.NX, NY = c.NY, P = c.P;
+    const int P = c.P;
     const int64_t ij = (int64_t)i * P + j;
     const T *__restrict__ vx = I.f(VX, s);
     const T *__restrict__ vy = I.f(VY, s);
@@ -131,7 +131,7 @@ template <typename T>
 __device__ inline void el_vel_cell(const ElC<T> &I, const El<T> &O, const T *__restrict__ rho,
                                    const ElCoef<T> &c, int s, int i, int j)
 {
-    const int NX = c.NX, NY = c.NY, P = c.P;
+    const int P = c.P;
     const int64_t ij = (int64_t)i * P + j;
     const T *__restrict__ sxx = O.f(SXX, s);
     const T *__restrict__ syy = O.f(SYY, s);
@@ -514,7 +514,7 @@ template <typename T>
 __device__ inline void adjv_a_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s, int i, int j,
                                    T &gR, T &gB)
 {
-    const int NX = c.NX, NY = c.NY, P = c.P;
+    const int P = c.P;
     const int64_t ij = (int64_t)i * P + j;
     T t0 = 0, t1 = 0, t2 = 0, t3 = 0;
     T *a5 = a.A.f(M5, s), *a6 = a.A.f(M6, s), *a7 = a.A.f(M7, s), *a8 = a.A.f(M8, s);
@@ -586,7 +586,7 @@ template <typename T>
 __device__ inline void adjs_a_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s, int i, int j,
                                    T &gL, T &gM1, T &gM2)
 {
-    const int NX = c.NX, NY = c.NY, P = c.P;
+    const int P = c.P;
     const int64_t ij = (int64_t)i * P + j;
     const T *__restrict__ vx = a.Sm.f(VX, s);
     const T *__restrict__ vy = a.Sm.f(VY, s);
@@ -1023,7 +1023,6 @@ __global__ void el_grad_map(const T *__restrict__ G, int64_t npad, int64_t gstri
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
     if (j >= NY + 2) return;
-    const int64_t ij = (int64_t)i * P + j;
     auto at = [&](int k, int g, int ii, int jj) -> T {
         return ii >= 0 && jj >= 0 ? G[(int64_t)k * gstride + (int64_t)g * npad + (int64_t)ii * P + jj] : T(0);
     };

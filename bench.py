@@ -276,8 +276,9 @@ def main():
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the "
-              "multi-GPU shard design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); shots shard across "
+              "GPUs with one gradient all-reduce per iteration, which is not implemented here",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
