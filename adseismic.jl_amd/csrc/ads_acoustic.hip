@@ -74,6 +74,16 @@ template <typename T> __device__ inline T ldg(const T *p, int64_t i, bool ok) { 
 // ---------------------------------------------------------------------------------------------
 // forward step: u^n (pre-source) and, for kernel 1, phi^n/psi^n
 // ---------------------------------------------------------------------------------------------
+// Sources added by the frame work of a level (PropagatorKernel 1, no source in the interior
+// rectangle): per cell, cell_idx is -1 or a source cell whose entries [ptr[c], ptr[c+1]) are
+// (shot, srcv offset of the entry's column, sources of that shot) in (shot, source) order.
+template <typename T> struct SrcIn {
+    const int32_t *cell_idx, *ptr, *shot;
+    const int64_t *base, *ns;
+    const T *srcv;
+    int n;  // level
+};
+
 template <typename T> struct FwdArgs {
     Lv<T> u;
     Lv<const T> w, wold, phi, psi;
@@ -83,7 +93,17 @@ template <typename T> struct FwdArgs {
     int NX, NY, P, act, Sg;
     TileMap map;
     AcCoef<T> k;
+    SrcIn<T> src;  // cell_idx null: sources are added by ac_src_rcv
 };
+
+// u^n + dt^2 srcv[n-1, e] over the cell's entries of shot s, in source order (Core.jl:593-601)
+template <typename T>
+__device__ inline T add_sources(const SrcIn<T> &q, int c, int s, T v, T dt2)
+{
+    for (int e = q.ptr[c]; e < q.ptr[c + 1]; e++)
+        if (q.shot[e] == s) v += q.srcv[q.base[e] + (int64_t)(q.n - 1) * q.ns[e]] * dt2;
+    return v;
+}
 
 // Rectangle forward: u^n = 2 w - wold + C lap(w).  A wavefront owns RPT rows x (64 lanes x VEC
 // depth cells); a workgroup loops over a group of Sg shots, SB at a time, issuing every load of
@@ -314,6 +334,10 @@ __device__ inline void fwd_frame_body(const FwdArgs<T> &a, const TileMap &map,
         lp |= rin[r] && jin && (nx[r] || ny);
     }
     lp = lp && act;
+    int cidx[FRPT];
+#pragma unroll
+    for (int r = 0; r < FRPT; r++)
+        cidx[r] = V == 1 && a.src.cell_idx && act && r < nr ? a.src.cell_idx[(int64_t)(ib + r) * P + j] : -1;
     const int s1 = min(a.act, (g + 1) * a.Sg);
     for (int s = g * a.Sg + wv; s < s1; s += NWAVES) {
         const T *__restrict__ w = a.w.at(s);
@@ -355,7 +379,8 @@ __device__ inline void fwd_frame_body(const FwdArgs<T> &a, const TileMap &map,
             const T plain = T(2) * wc - WO[r] + Cij * lap;
             const T num = (T(2) - s_ * tu * k.dt2) * wc + Cij * lap + k.ex * (PH[r + 2] - PH[r]) +
                           k.ey * (pe - pw) - (T(1) - (s_ + tu) * k.hdt) * WO[r];
-            const T out = in ? (near ? num / (T(1) + (s_ + tu) * k.hdt) : plain) : T(0);
+            T out = in ? (near ? num / (T(1) + (s_ + tu) * k.hdt) : plain) : T(0);
+            if (V == 1 && cidx[r] >= 0) out = add_sources(a.src, cidx[r], s, out, k.dt2);
             if (act) u[row + j] = out;
             if (V == 1 && lp) {
                 const bool pml = in && (s_ != T(0) || tu != T(0));
@@ -412,6 +437,25 @@ __global__ __launch_bounds__(256) void ac_src_rcv(
     __syncthreads();
     const int64_t r0 = rcv_ptr[s], nr = rcv_ptr[s + 1] - r0;
     T *__restrict__ tr = traces + trace_off[s] + (int64_t)n * nr;
+    for (int64_t r = threadIdx.x; r < nr; r += blockDim.x) tr[r] = u[rcv_cell[r0 + r]];
+}
+
+// Receiver sampling of a whole forward segment at once, trace[n, r] = u^n[rcv_r] for levels
+// lo < n <= hi, from the checkpoint slot ring (level n in slot n mod NS; sources already added).
+// Used when the frame work adds the sources.  Grid: (levels, shots).
+template <typename T>
+__global__ __launch_bounds__(256) void ac_record(const T *__restrict__ seg, int64_t NS, int64_t slot_stride,
+                                                 int64_t npad, int64_t lo, const int64_t *__restrict__ nstep,
+                                                 const int64_t *__restrict__ rcv_ptr,
+                                                 const int64_t *__restrict__ rcv_cell, T *__restrict__ traces,
+                                                 const int64_t *__restrict__ trace_off)
+{
+    const int64_t n = lo + 1 + blockIdx.x;
+    const int s = blockIdx.y;
+    if (nstep[s] < n) return;
+    const T *__restrict__ u = seg + (n % NS) * slot_stride + (int64_t)s * npad;
+    const int64_t r0 = rcv_ptr[s], nr = rcv_ptr[s + 1] - r0;
+    T *__restrict__ tr = traces + trace_off[s] + n * nr;
     for (int64_t r = threadIdx.x; r < nr; r += blockDim.x) tr[r] = u[rcv_cell[r0 + r]];
 }
 
@@ -915,6 +959,10 @@ struct AcPlanBase {
     int nitems1 = 0;
     bool blk_ok = false;
     int blocking = 2;
+    // sources added inside the frame work (kernel 1, no source in rect[0]); receivers then
+    // recorded per forward segment (ac_record)
+    bool src_fused = false;
+    DevBuf d_sc_idx, d_sc_ptr, d_sc_shot, d_sc_base, d_sc_ns, d_nstep_sorted;
     // hipGraph replay of whole checkpoint segments (forward sweep, recomputation, adjoint
     // sweep), keyed by everything the captured launches depend on
     struct GraphKey {
@@ -1080,16 +1128,21 @@ template <typename T> struct AcPlan : AcPlanBase {
     }
     FwdArgs<T> fwd_args(const Layout &L, int act, Lv<T> uo, Lv<const T> w, Lv<const T> wo,
                         Lv<const T> phi, Lv<const T> psi, Lv<T> phio, Lv<T> psio, const T *C,
-                        const Shape &shape) const
+                        const Shape &shape, int n = 0, const T *srcv = nullptr) const
     {
+        SrcIn<T> src{};
+        if (src_fused && L.li == 0 && srcv)
+            src = SrcIn<T>{d_sc_idx.as<int32_t>(), d_sc_ptr.as<int32_t>(), d_sc_shot.as<int32_t>(),
+                           d_sc_base.as<int64_t>(), d_sc_ns.as<int64_t>(), srcv, n};
         return FwdArgs<T>{uo, w, wo, phi, psi, phio, psio, C, d_sx.as<T>(), d_ty.as<T>(),
                           d_nearx.as<uint8_t>(), d_neary.as<uint8_t>(), (int)NX, (int)NY, L.P, act,
-                          std::min(shape.sg, act), TileMap{}, k};
+                          std::min(shape.sg, act), TileMap{}, k, src};
     }
     // after a level's stencil launch: phi/psi from the new u (kernel 0), sources, receivers
     void fwd_post(const Layout &L, const FwdArgs<T> &a, int n, const T *srcv, T *traces, int record,
                   hipStream_t st)
     {
+        if (a.src.cell_idx) return;  // sources added by the frame work, receivers per segment
         if (variant == 0) {
             dim3 g(cdiv(NY + 2, 64), cdiv(NX + 2, 4), a.act);
             ac_fwd_phipsi<T><<<g, 256, 0, st>>>(cst(a.u), a.phi, a.psi, a.phio, a.psio, a.C, d_sx.as<T>(),
@@ -1109,7 +1162,7 @@ template <typename T> struct AcPlan : AcPlanBase {
     {
         if (act <= 0) return ADS_OK;
         const Shape &shape = sh ? *sh : fwd_shape;
-        FwdArgs<T> a = fwd_args(L, act, uo, w, wo, phi, psi, phio, psio, C, shape);
+        FwdArgs<T> a = fwd_args(L, act, uo, w, wo, phi, psi, phio, psio, C, shape, n, srcv);
         int pr = prof_start(ADS_PROF_AC_FWD, st);
         launch_fwd_all(a, L, shape, st);
         prof_stop(pr, st, fwd_level_bytes(act));
@@ -1315,7 +1368,8 @@ template <typename T> struct AcPlan : AcPlanBase {
         const Shape &shape = sh ? *sh : fwd_shape;
         Lv<T> ph[2] = {{w.phi[0], Npad}, {w.phi[1], Npad}}, ps[2] = {{w.psi[0], Npad}, {w.psi[1], Npad}};
         FwdArgs<T> a = fwd_args(L, act, slot(w, n), cst(slot(w, n - 1)), cst(slot(w, n - 2)),
-                                cst(ph[(n - 1) & 1]), cst(ps[(n - 1) & 1]), ph[n & 1], ps[n & 1], w.C, shape);
+                                cst(ph[(n - 1) & 1]), cst(ps[(n - 1) & 1]), ph[n & 1], ps[n & 1], w.C, shape,
+                                n, srcv);
         int pr = prof_start(ADS_PROF_AC_FWD, st);
         if (shape.rpt == 1) launch_level2<1>(a, slot(w, n + 1), st);
         else if (shape.rpt == 4) launch_level2<4>(a, slot(w, n + 1), st);
@@ -1323,7 +1377,8 @@ template <typename T> struct AcPlan : AcPlanBase {
         prof_stop(pr, st, fwd_level_bytes(act));
         fwd_post(L, a, n, srcv, traces, record, st);
         FwdArgs<T> b = fwd_args(L, act, slot(w, n + 1), cst(slot(w, n)), cst(slot(w, n - 1)),
-                                cst(ph[n & 1]), cst(ps[n & 1]), ph[(n + 1) & 1], ps[(n + 1) & 1], w.C, shape);
+                                cst(ph[n & 1]), cst(ps[n & 1]), ph[(n + 1) & 1], ps[(n + 1) & 1], w.C, shape,
+                                n + 1, srcv);
         pr = prof_start(ADS_PROF_AC_FWD, st);
         if (nitems1 > 0) {
             b.map.ntj = 1;
@@ -1364,6 +1419,12 @@ template <typename T> struct AcPlan : AcPlanBase {
             Lv<T> po{w.phi[n & 1], Npad}, so{w.psi[n & 1], Npad};
             ADS_TRY(fwd_step(L, (int)n, act, slot(w, n), cst(slot(w, n - 1)), cst(slot(w, n - 2)),
                              cst(pi), cst(si), po, so, w.C, srcv, traces, record, st, sh));
+        }
+        if (src_fused && srcv && record && traces && hi > lo && trace_elems > 0) {
+            ac_record<T><<<dim3((unsigned)(hi - lo), (unsigned)nshots), 256, 0, st>>>(
+                w.seg, w.NS, nshots * Npad, Npad, lo, d_nstep_sorted.as<int64_t>(), d_rcv_ptr.as<int64_t>(),
+                d_rcv_cell[0].template as<int64_t>(), traces, d_trace_off.as<int64_t>());
+            ADS_LAUNCH_CHECK();
         }
         return ADS_OK;
     }
@@ -1774,6 +1835,42 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
         for (size_t e = 0; e < si.size(); e++)
             src_in |= si[e] >= R.iL && si[e] < R.iR && sj[e] >= R.jL && sj[e] < R.jR;
         P->blk_ok = !src_in && P->rect1.iR > P->rect1.iL && P->rect1.jR > P->rect1.jL;
+        // sources inside the frame work: per source cell, its entries in (shot, source) order
+        P->src_fused = P->variant == 1 && !src_in;
+        if (P->src_fused) {
+            struct Ent {
+                int64_t cell;
+                int32_t shot;
+                int64_t base, ns;
+            };
+            std::vector<Ent> ents;
+            for (int64_t q = 0; q < ns; q++)
+                for (int64_t e = sp[q]; e < sp[q + 1]; e++)
+                    ents.push_back(Ent{si[e] * P->Pitch + sj[e], (int32_t)q, soff[q] + (e - sp[q]), nsrc[q]});
+            std::stable_sort(ents.begin(), ents.end(), [](const Ent &x, const Ent &y) {
+                return x.cell < y.cell || (x.cell == y.cell && x.shot < y.shot);
+            });
+            std::vector<int32_t> idx((size_t)P->Npad, -1), ptr, shot;
+            std::vector<int64_t> base, nsv;
+            for (size_t e = 0; e < ents.size(); e++) {
+                if (e == 0 || ents[e].cell != ents[e - 1].cell) {
+                    idx[(size_t)ents[e].cell] = (int32_t)ptr.size();
+                    ptr.push_back((int32_t)e);
+                }
+                shot.push_back(ents[e].shot);
+                base.push_back(ents[e].base);
+                nsv.push_back(ents[e].ns);
+            }
+            ptr.push_back((int32_t)ents.size());
+            ADS_TRY(P->d_sc_idx.upload(idx));
+            ADS_TRY(P->d_sc_ptr.upload(ptr));
+            if (!ents.empty()) {
+                ADS_TRY(P->d_sc_shot.upload(shot));
+                ADS_TRY(P->d_sc_base.upload(base));
+                ADS_TRY(P->d_sc_ns.upload(nsv));
+            }
+            ADS_TRY(P->d_nstep_sorted.upload(P->nstep_sorted));
+        }
         std::vector<int4> items;
         frame_items(P->blk_ok ? P->rect1 : Rect{1, 1, 1, 1}, p->NX, p->NY, items);
         P->nitems1 = P->blk_ok ? (int)items.size() : 0;
