@@ -11,8 +11,9 @@ recomputation and the adjoint sweep, all in one batched call.
 Prints ONE JSON line.  Besides value/ms_per_step it carries:
   roofline      : the dominant stencil kernel's algorithmic HBM bytes per launch / its average
                   launch duration (HIP events on the launch stream, over the timed region);
-  cpu_baseline  : the FP64 CPU restatement (oracle/, one thread, like the reference's serial
-                  CPU op) on a bounded sample of the same workload;
+  cpu_baseline  : the FP64 CPU restatement (oracle/) on the process's threads, with a one-thread
+                  run beside it (like the reference's serial CPU op), on a bounded sample of
+                  the same workload;
   plain_single_shot : the plain drop-in path (AcousticPropagatorSolver + SimulatedObservation_
                   + autograd, full histories, one shot) on the same grid.
 """
@@ -111,11 +112,10 @@ def workload(ads, small=False, nstep=0):
     return p, c_true, c0, srcs, rcvs
 
 
-def cpu_baseline(p, c0, src, nsteps):
-    """Oracle (FP64, one thread) forward + adjoint of one shot over the first nsteps steps."""
-    import oracle as O
-
-    O.set_num_threads(1)
+def _cpu_gradient(O, p, c0, src, nsteps, threads):
+    """Oracle forward + adjoint of one shot over the first nsteps steps on `threads` threads
+    (1: the serial checker itself, scatter-form step VJPs; more: the gather-form sweep)."""
+    O.set_num_threads(threads)
     NX, NY = p.NX, p.NY
     Sx, Sy = O.acoustic_pml(NX, NY, p.DELTAX, p.DELTAY, p.NPOINTS_PML, p.Rcoef, p.vp_ref)
     srcv = np.asarray(src.srcv)[:nsteps]
@@ -126,15 +126,38 @@ def cpu_baseline(p, c0, src, nsteps):
     tr = O.acoustic_receive(u, rcvi, rcvj, NY)
     gu = np.zeros((nsteps + 1, (NX + 2) * (NY + 2)))
     gu[:, O.flat_index(rcvi, rcvj, NY)] = 2.0 * tr
-    O.acoustic_adjoint(1, NX, NY, nsteps, p.DELTAT, p.DELTAX, p.DELTAY, Sx, Sy, c0 ** 2, u, ph,
-                       ps, gu)
+    if threads == 1:
+        O.acoustic_adjoint(1, NX, NY, nsteps, p.DELTAT, p.DELTAX, p.DELTAY, Sx, Sy, c0 ** 2, u,
+                           ph, ps, gu)
+    else:
+        O.acoustic_adjoint_par(1, NX, NY, nsteps, p.DELTAT, p.DELTAX, p.DELTAY, Sx, Sy, c0 ** 2, u, gu)
     t = time.perf_counter() - t0
-    cells = NX * NY * (nsteps - 1)
-    return {"value": cells / t / 1e9, "unit": "Gcell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"1 shot, {NX}x{NY} grid, first {nsteps} of {p.NSTEP} steps, FP64 "
-                      f"forward+adjoint (oracle/ads_oracle.c, PropagatorKernel 1), {t:.1f} s",
-            "host_cpu": platform.processor() or platform.machine(),
-            "host_threads_available": len(os.sched_getaffinity(0))}
+    return NX * NY * (nsteps - 1) / t / 1e9, t
+
+
+def cpu_baseline(p, c0, src, nsteps):
+    """Oracle (FP64) forward + adjoint of one shot over the first nsteps steps, on all the
+    threads this process may use (OMP_NUM_THREADS caps the affinity set) and on one thread,
+    like the reference's serial custom op."""
+    import oracle as O
+
+    threads = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = max(1, min(threads, int(os.environ["OMP_NUM_THREADS"])))
+    sample = (f"1 shot, {p.NX}x{p.NY} grid, first {nsteps} of {p.NSTEP} steps, FP64 "
+              "forward+adjoint (oracle/ads_oracle.c, PropagatorKernel 1)")
+    v1, t1 = _cpu_gradient(O, p, c0, src, nsteps, 1)
+    out = {"value": v1, "unit": "Gcell-steps/s", "cores": 1, "kind": "port",
+           "sample": f"{sample}, one thread, {t1:.1f} s"}
+    if threads > 1:
+        vm, tm = _cpu_gradient(O, p, c0, src, nsteps, threads)
+        out = {"value": vm, "unit": "Gcell-steps/s", "cores": threads, "kind": "port",
+               "sample": f"{sample}, {threads} threads (OpenMP rows, gather-form adjoint), {tm:.1f} s",
+               "single_thread": out}
+    O.set_num_threads(1)
+    out["host_cpu"] = platform.processor() or platform.machine()
+    out["host_threads_available"] = len(os.sched_getaffinity(0))
+    return out
 
 
 def elastic_workload(ads, nstep=0):

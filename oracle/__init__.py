@@ -161,6 +161,30 @@ def acoustic_adjoint(kernel, NX, NY, NSTEP, dt, hx, hy, sigma, tau, C2, u_hist, 
     return gC.reshape(NX + 2, NY + 2)
 
 
+def acoustic_step_vjp_gather(gu, gphi, gpsi, w, sigma, tau, c, dt, hx, hy, NX, NY):
+    """Kernel-1 step VJP in gather form (parallel over rows): grad_w, grad_wold, grad_phi,
+    grad_psi, grad_c."""
+    N = (NX + 2) * (NY + 2)
+    outs = [np.empty(N) for _ in range(5)]
+    args = [_f64(x).ravel() for x in (gu, gphi, gpsi, w, sigma, tau, c)]
+    lib().or_acoustic_step_vjp_gather(*[_d(a) for a in args], f64(dt), f64(hx), f64(hy), i64(NX),
+                                      i64(NY), *[_d(o) for o in outs])
+    return tuple(outs)
+
+
+def acoustic_adjoint_par(kernel, NX, NY, NSTEP, dt, hx, hy, sigma, tau, C2, u_hist, gu_hist):
+    """acoustic_adjoint for kernel 1 with the gather-form VJP, multi-threaded."""
+    N = (NX + 2) * (NY + 2)
+    gC = np.empty(N)
+    rc = lib().or_acoustic_adjoint_par(C.c_int(kernel), i64(NX), i64(NY), i64(NSTEP), f64(dt),
+                                       f64(hx), f64(hy), _d(_f64(sigma).ravel()),
+                                       _d(_f64(tau).ravel()), _d(_f64(C2).ravel()),
+                                       _d(_f64(u_hist).ravel()), _d(_f64(gu_hist).ravel()), _d(gC))
+    if rc != 0:
+        raise ValueError(f"or_acoustic_adjoint_par failed ({rc}): kernel 1 only")
+    return gC.reshape(NX + 2, NY + 2)
+
+
 def acoustic_misfit_gradient(kernel, NX, NY, NSTEP, dt, hx, hy, sigma, tau, c, srci, srcj, srcv,
                              rcvi, rcvj, observed):
     """loss = sum((rcvv - observed)^2) (Utils.jl:308) and d loss / d c (c = velocity)."""

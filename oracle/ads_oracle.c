@@ -271,6 +271,114 @@ void or_acoustic_adjoint(int kernel, i64 NX, i64 NY, i64 NSTEP, double dt, doubl
     free(buf);
 }
 
+/* ----------------------------------------------------------------------------------------
+ * Gather form of or_acoustic_step_vjp for kernel 1, for the multi-threaded CPU baseline: the
+ * same terms as the scatter form above (AcousticOneStepCpu.h:76-123), each output cell
+ * collecting its own, so rows are independent and run in parallel.  Equal to the scatter form
+ * up to summation order (pinned against it in tests/test_oracle_acoustic.py).
+ * -------------------------------------------------------------------------------------- */
+void or_acoustic_step_vjp_gather(const double *grad_u, const double *grad_phiout,
+                                 const double *grad_psiout, const double *w, const double *sigma,
+                                 const double *tau, const double *c, double dt, double hx,
+                                 double hy, i64 NX, i64 NY, double *grad_w, double *grad_wold,
+                                 double *grad_phi, double *grad_psi, double *grad_c)
+{
+    const i64 S = NY + 2;
+#define IN(a, b) ((a) >= 1 && (a) <= NX && (b) >= 1 && (b) <= NY)
+#define DEN(K) (1 + (sigma[K] + tau[K]) / 2 * dt)
+#pragma omp parallel for schedule(static)
+    for (i64 i = 0; i < NX + 2; i++) {
+        for (i64 j = 0; j < NY + 2; j++) {
+            const i64 IJ = i * S + j;
+            double gw = 0, gwo = 0, gp = 0, gs = 0, gc = 0;
+            if (IN(i, j)) {
+                const i64 IpJ = IJ + S, InJ = IJ - S, IJp = IJ + 1, IJn = IJ - 1;
+                const double den = DEN(IJ);
+                gc += ((-2 * dt * dt / hx / hx - 2 * dt * dt / hy / hy) * w[IJ]
+                       + (dt / hx) * (dt / hx) * (w[IpJ] + w[InJ])
+                       + (dt / hy) * (dt / hy) * (w[IJp] + w[IJn])) * grad_u[IJ] / den;
+                gw += (2 - sigma[IJ] * tau[IJ] * dt * dt - 2 * dt * dt / hx / hx * c[IJ]
+                       - 2 * dt * dt / hy / hy * c[IJ]) * grad_u[IJ] / den;
+                gwo += -(1 - (sigma[IJ] + tau[IJ]) * dt / 2) * grad_u[IJ] / den;
+                gp += (1. - dt * sigma[IJ]) * grad_phiout[IJ];
+                gc += dt * (tau[IJ] - sigma[IJ]) / 2.0 / hx * (w[IpJ] - w[InJ]) * grad_phiout[IJ];
+                gs += (1. - dt * tau[IJ]) * grad_psiout[IJ];
+                gc += dt * (sigma[IJ] - tau[IJ]) / 2.0 / hy * (w[IJp] - w[IJn]) * grad_psiout[IJ];
+            }
+            /* terms of the neighbours K whose stencil reaches IJ */
+            if (IN(i - 1, j)) { /* K = (i-1, j): IJ is its IpJ */
+                const i64 K = IJ - S;
+                gw += c[K] * (dt / hx) * (dt / hx) * grad_u[K] / DEN(K);
+                gp += (dt * dt / (2.0 * hx)) * grad_u[K] / DEN(K);
+                gw += dt * c[K] * (tau[K] - sigma[K]) / 2.0 / hx * grad_phiout[K];
+            }
+            if (IN(i + 1, j)) { /* K = (i+1, j): IJ is its InJ */
+                const i64 K = IJ + S;
+                gw += c[K] * (dt / hx) * (dt / hx) * grad_u[K] / DEN(K);
+                gp += -(dt * dt / (2.0 * hx)) * grad_u[K] / DEN(K);
+                gw += -dt * c[K] * (tau[K] - sigma[K]) / 2.0 / hx * grad_phiout[K];
+            }
+            if (IN(i, j - 1)) { /* K = (i, j-1): IJ is its IJp */
+                const i64 K = IJ - 1;
+                gw += c[K] * (dt / hy) * (dt / hy) * grad_u[K] / DEN(K);
+                gs += (dt * dt / (2.0 * hy)) * grad_u[K] / DEN(K);
+                gw += dt * c[K] * (sigma[K] - tau[K]) / 2.0 / hy * grad_psiout[K];
+            }
+            if (IN(i, j + 1)) { /* K = (i, j+1): IJ is its IJn */
+                const i64 K = IJ + 1;
+                gw += c[K] * (dt / hy) * (dt / hy) * grad_u[K] / DEN(K);
+                gs += -(dt * dt / (2.0 * hy)) * grad_u[K] / DEN(K);
+                gw += -dt * c[K] * (sigma[K] - tau[K]) / 2.0 / hy * grad_psiout[K];
+            }
+            grad_w[IJ] = gw;
+            grad_wold[IJ] = gwo;
+            grad_phi[IJ] = gp;
+            grad_psi[IJ] = gs;
+            grad_c[IJ] = gc;
+        }
+    }
+#undef IN
+#undef DEN
+}
+
+/* The reverse sweep of or_acoustic_adjoint for kernel 1 with the gather-form step VJP and
+ * parallel element-wise updates: the multi-threaded CPU baseline.  Returns -1 for kernel 0. */
+int or_acoustic_adjoint_par(int kernel, i64 NX, i64 NY, i64 NSTEP, double dt, double hx, double hy,
+                            const double *sigma, const double *tau, const double *C,
+                            const double *u_hist, const double *gu_hist, double *gC)
+{
+    if (kernel != 1) return -1;
+    const i64 N = (NX + 2) * (NY + 2);
+    double *buf = (double *)calloc((size_t)N * 10, sizeof(double));
+    if (!buf) return -2;
+    double *gu[3] = {buf, buf + N, buf + 2 * N};
+    double *gphi = buf + 3 * N, *gpsi = buf + 4 * N;
+    double *gw = buf + 5 * N, *gwold = buf + 6 * N, *gp = buf + 7 * N, *gs = buf + 8 * N;
+    double *gc_step = buf + 9 * N;
+    memset(gC, 0, N * sizeof(double));
+    memcpy(gu[0], gu_hist + NSTEP * N, N * sizeof(double));
+    if (NSTEP >= 1) memcpy(gu[1], gu_hist + (NSTEP - 1) * N, N * sizeof(double));
+    for (i64 n = NSTEP; n >= 2; n--) {
+        or_acoustic_step_vjp_gather(gu[0], gphi, gpsi, u_hist + (n - 1) * N, sigma, tau, C, dt, hx,
+                                    hy, NX, NY, gw, gwold, gp, gs, gc_step);
+        const double *d2 = gu_hist + (n - 2) * N;
+#pragma omp parallel for schedule(static)
+        for (i64 k = 0; k < N; k++) {
+            gu[1][k] += gw[k];
+            gu[2][k] = d2[k] + gwold[k];
+            gC[k] += gc_step[k];
+            gphi[k] = gp[k];
+            gpsi[k] = gs[k];
+        }
+        double *t = gu[0];
+        gu[0] = gu[1];
+        gu[1] = gu[2];
+        gu[2] = t;
+    }
+    free(buf);
+    return 0;
+}
+
 /* ========================================================================================
  * ELASTIC
  * ======================================================================================== */

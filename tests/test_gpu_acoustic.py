@@ -114,9 +114,9 @@ def make_case(orc, ads, NX=30, NY=26, NSTEP=80, npml=6, nshots=1, seed=0, uneven
             srcs.append(ads.AcousticSource([i0], [3], w[:, None]))
         ri = np.arange(1, NX + 3, 2 + s % 2)
         rj = np.full(len(ri), 2 + s % 2)
-        if s % 2 == 1:  # duplicated receiver and one in the PML corner
-            ri = np.concatenate([ri, [ri[0], 2]])
-            rj = np.concatenate([rj, [rj[0], NY]])
+        if s % 2 == 1:  # duplicated receivers (on the ring, inside) and one in the PML corner
+            ri = np.concatenate([ri, [ri[0], ri[1], 2]])
+            rj = np.concatenate([rj, [rj[0], rj[1], NY]])
         rcvs.append(ads.AcousticReceiver(ri, rj))
     return p, c, srcs, rcvs, nstep
 

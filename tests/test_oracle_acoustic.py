@@ -32,6 +32,47 @@ def test_step_vjp_kernel1_equals_reference_header(orc, golden, name):
         np.testing.assert_array_equal(o, g[f"{name}/{k}"], err_msg=k)
 
 
+@pytest.mark.parametrize("name", ["10x10", "7x12"])
+def test_step_vjp_gather_form_matches_reference_header(orc, golden, name):
+    """The gather-form VJP (multi-threaded CPU baseline) gives the reference header's
+    backward up to summation order."""
+    g = golden("acoustic_one_step.npz")
+    ins, sc = _golden_inputs(g, name)
+    w, _, _, _, sigma, tau, c = ins
+    prev = orc.num_threads()
+    orc.set_num_threads(3)
+    try:
+        outs = orc.acoustic_step_vjp_gather(g[f"{name}/grad_u"], g[f"{name}/grad_phiout"],
+                                            g[f"{name}/grad_psiout"], w, sigma, tau, c, *sc)
+    finally:
+        orc.set_num_threads(prev)
+    for o, k in zip(outs, ("grad_w", "grad_wold", "grad_phi", "grad_psi", "grad_c")):
+        ref = g[f"{name}/{k}"]
+        np.testing.assert_allclose(o, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max(), err_msg=k)
+
+
+def test_parallel_adjoint_sweep_equals_serial(orc):
+    """The multi-threaded kernel-1 reverse sweep equals the serial one (scatter-form VJPs) on
+    a PML grid with a source, up to summation order; kernel 0 is refused."""
+    NX, NY, NSTEP, h, dt = 23, 19, 60, 10.0, 1e-3
+    rng = np.random.default_rng(11)
+    Sx, Sy = orc.acoustic_pml(NX, NY, h, h, 5, 1e-3, 2000.0)
+    C2 = (1800.0 + 500.0 * rng.random((NX + 2, NY + 2))) ** 2
+    srcv = rng.standard_normal((NSTEP, 1))
+    u, ph, ps = orc.acoustic_forward(1, NX, NY, NSTEP, dt, h, h, Sx, Sy, C2, [7], [9], srcv)
+    gu = rng.standard_normal((NSTEP + 1, (NX + 2) * (NY + 2)))
+    ref = orc.acoustic_adjoint(1, NX, NY, NSTEP, dt, h, h, Sx, Sy, C2, u, ph, ps, gu)
+    prev = orc.num_threads()
+    orc.set_num_threads(4)
+    try:
+        par = orc.acoustic_adjoint_par(1, NX, NY, NSTEP, dt, h, h, Sx, Sy, C2, u, gu)
+    finally:
+        orc.set_num_threads(prev)
+    assert np.abs(par - ref).max() <= 1e-12 * np.abs(ref).max()
+    with pytest.raises(ValueError):
+        orc.acoustic_adjoint_par(0, NX, NY, NSTEP, dt, h, h, Sx, Sy, C2, u, gu)
+
+
 def test_kernel0_equals_kernel1_outside_pml(orc):
     """With sigma = tau = 0 and phi = psi = 0 both variants advance u identically and keep
     phi = psi = 0 (Core.jl:534-548 vs AcousticOneStepCpu.h:34-44)."""
