@@ -391,6 +391,147 @@ __device__ inline void fwd_frame_body(const FwdArgs<T> &a, const TileMap &map,
     }
 }
 
+// Two-level frame work (kernel 1, sources from the per-cell table).  An item is nr <= FRPT rows
+// of the complement of the shrunk rectangle by at most WAVE - 2 columns [m.z, m.w).  Lane L
+// holds column m.z - 1 + L: every lane computes level n at its column on the item's rows and
+// one row above and below (from levels n-1, n-2 and phi/psi^{n-1}); lanes 1 .. m.w - m.z then
+// compute level n+1 on the item's rows from those values, the first and last lane supplying
+// the halo columns.  Level n is stored only outside R (the rectangle pass stores it inside).
+// Every value is fwd_frame_body's expression in the same order, so the results equal
+// single-level stepping bit for bit.
+template <typename T>
+__device__ inline void fwd_frame2_body(const FwdArgs<T> &a, const Lv<T> &u2, const Lv<T> &phi2,
+                                       const Lv<T> &psi2, const Rect &R, const TileMap &map,
+                                       const int4 *__restrict__ items, int nitems, int bid)
+{
+    constexpr int NB = FRPT + 2;  // level-n rows: the item's and one on each side
+    int g, blk, tj;
+    if (!decode_tile(map, g, blk, tj, bid)) return;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int it = blk;
+    if (it >= nitems) return;  // workgroup-uniform
+    const int4 m = items[it];
+    const int ib = m.x, nr = m.y, j = m.z - 1 + lane;
+    const int NX = a.NX, NY = a.NY, P = a.P;
+    const int jlast = min(m.w, NY + 1);  // the block's last column
+    const bool act = j >= 0 && j <= jlast;
+    const bool own = act && lane >= 1 && j < m.w;  // a column of the item
+    const bool le = lane == 0 && act && j >= 1;
+    const bool re = act && j == jlast;
+    const bool reok = re && j + 1 <= NY + 1;
+    const AcCoef<T> k = a.k;
+    const bool jin = j >= 1 && j <= NY;
+    const T tu = act ? a.ty[j] : T(0);
+    const bool ny = act && a.neary[j] != 0;
+    const bool jR = j >= R.jL && j < R.jR;
+    bool rin[NB], nx[NB], lvl[NB];
+    T sg[NB], Cr[NB];
+    int cidx[NB];
+    bool lp = false, lpi = false;  // near cells on the block's rows / on the item's rows
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+        const int i = ib - 1 + q;
+        lvl[q] = q <= nr + 1 && i >= 0 && i <= NX + 1;
+        rin[q] = lvl[q] && i >= 1 && i <= NX;
+        nx[q] = lvl[q] && a.nearx[i] != 0;
+        sg[q] = lvl[q] ? a.sx[i] : T(0);
+        Cr[q] = act && rin[q] ? a.C[(int64_t)i * P + j] : T(0);
+        cidx[q] = act && lvl[q] && a.src.cell_idx ? a.src.cell_idx[(int64_t)i * P + j] : -1;
+        const bool nr_ = rin[q] && jin && (nx[q] || ny);
+        lp |= nr_;
+        if (q >= 1 && q <= nr) lpi |= nr_;
+    }
+    lp = lp && act;
+    lpi = lpi && act;
+    SrcIn<T> src2 = a.src;
+    src2.n = a.src.n + 1;
+    const int s1 = min(a.act, (g + 1) * a.Sg);
+    for (int s = g * a.Sg + wv; s < s1; s += NWAVES) {
+        const T *__restrict__ w = a.w.at(s);
+        const T *__restrict__ wo = a.wold.at(s);
+        const T *__restrict__ phi = a.phi.at(s);
+        const T *__restrict__ psi = a.psi.at(s);
+        // level n-1 on rows ib-2 .. ib+nr+1, level n-2 and psi^{n-1} on the block's rows
+        T W[NB + 2], PH[NB + 2], WO[NB], PS[NB], EL[NB], ER[NB], SL[NB], SR[NB];
+#pragma unroll
+        for (int q = 0; q < NB + 2; q++) {
+            const int i = ib - 2 + q;
+            const bool ok = act && q <= nr + 3 && i >= 0 && i <= NX + 1;
+            W[q] = ok ? w[(int64_t)i * P + j] : T(0);
+            PH[q] = ok && lp ? phi[(int64_t)i * P + j] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int64_t row = (int64_t)(ib - 1 + q) * P;
+            WO[q] = act && rin[q] ? wo[row + j] : T(0);
+            PS[q] = lp && rin[q] ? psi[row + j] : T(0);
+            EL[q] = le && rin[q] ? w[row + j - 1] : T(0);
+            ER[q] = reok && rin[q] ? w[row + j + 1] : T(0);
+            SL[q] = le && lp && rin[q] ? psi[row + j - 1] : T(0);
+            SR[q] = reok && lp && rin[q] ? psi[row + j + 1] : T(0);
+        }
+        // level n on the block
+        T Un[NB], PHn[NB], PSn[NB];
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            T wv1[1] = {W[q + 1]}, pv1[1] = {PS[q]};
+            T ww, we, pw, pe;
+            lr_of(wv1, EL[q], ER[q], lane, re, ww, we);
+            lr_of(pv1, SL[q], SR[q], lane, re, pw, pe);
+            const T wc = W[q + 1], wn = W[q], ws = W[q + 2];
+            const T lap = k.cx * (ws + wn - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
+            const T s_ = sg[q], Cij = Cr[q];
+            const bool in = rin[q] && jin;
+            const bool near = nx[q] || ny;
+            const T plain = T(2) * wc - WO[q] + Cij * lap;
+            const T num = (T(2) - s_ * tu * k.dt2) * wc + Cij * lap + k.ex * (PH[q + 2] - PH[q]) +
+                          k.ey * (pe - pw) - (T(1) - (s_ + tu) * k.hdt) * WO[q];
+            T out = in ? (near ? num / (T(1) + (s_ + tu) * k.hdt) : plain) : T(0);
+            if (cidx[q] >= 0) out = add_sources(a.src, cidx[q], s, out, k.dt2);
+            Un[q] = act && lvl[q] ? out : T(0);
+            const bool pml = in && (s_ != T(0) || tu != T(0));
+            PHn[q] = lp && pml ? (T(1) - k.dt * s_) * PH[q + 1] + Cij * (tu - s_) * k.fx * (ws - wn) : T(0);
+            PSn[q] = lp && pml ? (T(1) - k.dt * tu) * PS[q] + Cij * (s_ - tu) * k.fy * (we - ww) : T(0);
+        }
+        T *__restrict__ u = a.u.at(s);
+        T *__restrict__ uo2 = u2.at(s);
+        // level n stores (item cells outside R) and level n+1 on the item
+#pragma unroll
+        for (int r = 0; r < FRPT; r++) {
+            if (r >= nr) break;  // uniform
+            const int q = r + 1, i = ib + r;
+            const int64_t row = (int64_t)i * P;
+            if (own && !(jR && i >= R.iL && i < R.iR)) {
+                u[row + j] = Un[q];
+                if (lpi) {
+                    a.phio.at(s)[row + j] = PHn[q];
+                    a.psio.at(s)[row + j] = PSn[q];
+                }
+            }
+            T uv1[1] = {Un[q]}, pv1[1] = {PSn[q]};
+            T ww, we, pw, pe;
+            lr_of(uv1, T(0), T(0), lane, false, ww, we);
+            lr_of(pv1, T(0), T(0), lane, false, pw, pe);
+            const T wc = Un[q], wn = Un[q - 1], ws = Un[q + 1];
+            const T lap = k.cx * (ws + wn - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
+            const T s_ = sg[q], Cij = Cr[q];
+            const bool in = rin[q] && jin;
+            const bool near = nx[q] || ny;
+            const T plain = T(2) * wc - W[q + 1] + Cij * lap;
+            const T num = (T(2) - s_ * tu * k.dt2) * wc + Cij * lap + k.ex * (PHn[q + 1] - PHn[q - 1]) +
+                          k.ey * (pe - pw) - (T(1) - (s_ + tu) * k.hdt) * W[q + 1];
+            T out = in ? (near ? num / (T(1) + (s_ + tu) * k.hdt) : plain) : T(0);
+            if (cidx[q] >= 0) out = add_sources(src2, cidx[q], s, out, k.dt2);
+            if (own) uo2[row + j] = out;
+            if (own && lpi) {
+                const bool pml = in && (s_ != T(0) || tu != T(0));
+                phi2.at(s)[row + j] = pml ? (T(1) - k.dt * s_) * PHn[q] + Cij * (tu - s_) * k.fx * (ws - wn) : T(0);
+                psi2.at(s)[row + j] = pml ? (T(1) - k.dt * tu) * PSn[q] + Cij * (s_ - tu) * k.fy * (we - ww) : T(0);
+            }
+        }
+    }
+}
+
 // kernel 0: phi^n/psi^n from the new u (before sources), PML cells only.  One thread per cell.
 template <typename T>
 __global__ __launch_bounds__(256) void ac_fwd_phipsi(
@@ -823,6 +964,19 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_level2(const FwdArgs<T> a
     if (b < nfb) fwd_frame_body<T, V>(a, fmap, items, nitems, b);
     else fwd_rect2_body<T, RPT, VEC>(a, u2, R, b - nfb);
 }
+// Levels n and n+1 of the whole grid in one launch: the two-level rectangle body and the
+// two-level frame body over the complement of the shrunk rectangle (kernel 1).
+template <typename T, int RPT, int VEC>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_pair(const FwdArgs<T> a, const Lv<T> u2,
+                                                             const Lv<T> phi2, const Lv<T> psi2,
+                                                             const Rect R, const TileMap fmap,
+                                                             const int4 *__restrict__ items,
+                                                             int nitems, int nfb)
+{
+    const int b = (int)blockIdx.x;
+    if (b < nfb) fwd_frame2_body<T>(a, u2, phi2, psi2, R, fmap, items, nitems, b);
+    else fwd_rect2_body<T, RPT, VEC>(a, u2, R, b - nfb);
+}
 template <typename T, int RPT, int VEC, int SB>
 __global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_rect(const AdjArgs<T> a, const Rect R)
 {
@@ -962,6 +1116,9 @@ struct AcPlanBase {
     // sources added inside the frame work (kernel 1, no source in rect[0]); receivers then
     // recorded per forward segment (ac_record)
     bool src_fused = false;
+    // items of the two-level frame pass (complement of rect1, WAVE - 2 columns each)
+    DevBuf d_items2;
+    int nitems2 = 0;
     DevBuf d_sc_idx, d_sc_ptr, d_sc_shot, d_sc_base, d_sc_ns, d_nstep_sorted;
     // hipGraph replay of whole checkpoint segments (forward sweep, recomputation, adjoint
     // sweep), keyed by everything the captured launches depend on
@@ -1268,7 +1425,9 @@ template <typename T> struct AcPlan : AcPlanBase {
     // ---- workspace layout for checkpoint interval K ----
     struct WS {
         int64_t K, NS, M;  // interval, slots, segments
-        T *seg, *phi[2], *psi[2], *P[2], *F[2], *G[2], *gC, *ckpt, *C, *resid;
+        // phi/psi of level n in buffer n mod 3: a two-level pass writes level n+1 while other
+        // workgroups of the same launch still read level n-1
+        T *seg, *phi[3], *psi[3], *P[2], *F[2], *G[2], *gC, *ckpt, *C, *resid;
         double *loss, *part;
     };
     int64_t seg_count(int64_t K) const { return NMAX >= 2 ? (NMAX - 1 + K - 1) / K : 0; }
@@ -1281,9 +1440,11 @@ template <typename T> struct AcPlan : AcPlanBase {
         w.NS = std::min<int64_t>(K + 2, NMAX + 1);
         const size_t F = (size_t)nshots * Npad * sizeof(T);
         w.seg = (T *)cv.take(F * w.NS);
-        for (int q = 0; q < 2; q++) {
+        for (int q = 0; q < 3; q++) {
             w.phi[q] = (T *)cv.take(F);
             w.psi[q] = (T *)cv.take(F);
+        }
+        for (int q = 0; q < 2; q++) {
             w.P[q] = (T *)cv.take(F);
             w.F[q] = (T *)cv.take(F);
             w.G[q] = (T *)cv.take(F);
@@ -1295,6 +1456,8 @@ template <typename T> struct AcPlan : AcPlanBase {
         w.loss = (double *)cv.take(64);
         w.part = (double *)cv.take(MISFIT_BLOCKS * sizeof(double));
     }
+    Lv<T> phl(const WS &w, int64_t n) const { return Lv<T>{w.phi[n % 3], Npad}; }
+    Lv<T> psl(const WS &w, int64_t n) const { return Lv<T>{w.psi[n % 3], Npad}; }
     size_t ws_size(int64_t K) const
     {
         WS w;
@@ -1344,9 +1507,10 @@ template <typename T> struct AcPlan : AcPlanBase {
         return ADS_OK;
     }
 
-    // Levels n and n+1 of the plan layout with the two-level rectangle pass: [rectangle
-    // levels n, n+1 + frame level n], phi/psi and sources of level n, [frame of the shrunk
-    // rectangle, level n+1], phi/psi and sources of level n+1.
+    // Levels n and n+1 of the plan layout with the two-level rectangle pass.  Kernel 1 with the
+    // sources added by the frame work: one launch (two-level rectangle + two-level frame).
+    // Otherwise: [rectangle levels n, n+1 + frame level n], phi/psi and sources of level n,
+    // [frame of the shrunk rectangle, level n+1], phi/psi and sources of level n+1.
     template <int RPT>
     void launch_level2(FwdArgs<T> a, Lv<T> u2, hipStream_t st)
     {
@@ -1361,15 +1525,39 @@ template <typename T> struct AcPlan : AcPlanBase {
         else
             ac_fwd_level2<T, RPT, VECP, 0><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, R, fm, it, nitems[0], nfb);
     }
+    // Levels n and n+1 of the whole grid in one launch (kernel 1 with the frame adding sources)
+    template <int RPT>
+    void launch_pair(FwdArgs<T> a, Lv<T> u2, Lv<T> phi2, Lv<T> psi2, hipStream_t st)
+    {
+        const Rect &R = rect[0];
+        a.map = rect_tiles(R, VECP, RPT, a.act, a.Sg);
+        TileMap fm;
+        fm.ntj = 1;
+        fm.nstrips = nitems2;
+        fm.ngroups = (int)cdiv(a.act, a.Sg);
+        fm.nb = fm.ntj * fm.nstrips * fm.ngroups;
+        const int nfb = (int)grid_of(fm);
+        const unsigned nb = (unsigned)nfb + grid_of(a.map);
+        ac_fwd_pair<T, RPT, VECP><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, phi2, psi2, R, fm,
+                                                                d_items2.template as<int4>(), nitems2, nfb);
+    }
     int fwd_pair(const WS &w, int n, int act, const T *srcv, T *traces, int record, hipStream_t st,
                  const Shape *sh)
     {
         const Layout L = plan_layout();
         const Shape &shape = sh ? *sh : fwd_shape;
-        Lv<T> ph[2] = {{w.phi[0], Npad}, {w.phi[1], Npad}}, ps[2] = {{w.psi[0], Npad}, {w.psi[1], Npad}};
         FwdArgs<T> a = fwd_args(L, act, slot(w, n), cst(slot(w, n - 1)), cst(slot(w, n - 2)),
-                                cst(ph[(n - 1) & 1]), cst(ps[(n - 1) & 1]), ph[n & 1], ps[n & 1], w.C, shape,
+                                cst(phl(w, n - 1)), cst(psl(w, n - 1)), phl(w, n), psl(w, n), w.C, shape,
                                 n, srcv);
+        if (variant == 1 && nitems2 > 0 && a.src.cell_idx) {
+            int pr = prof_start(ADS_PROF_AC_FWD, st);
+            if (shape.rpt == 1) launch_pair<1>(a, slot(w, n + 1), phl(w, n + 1), psl(w, n + 1), st);
+            else if (shape.rpt == 4) launch_pair<4>(a, slot(w, n + 1), phl(w, n + 1), psl(w, n + 1), st);
+            else launch_pair<2>(a, slot(w, n + 1), phl(w, n + 1), psl(w, n + 1), st);
+            prof_stop(pr, st, 2.0 * fwd_level_bytes(act), 2);
+            ADS_LAUNCH_CHECK();
+            return ADS_OK;
+        }
         int pr = prof_start(ADS_PROF_AC_FWD, st);
         if (shape.rpt == 1) launch_level2<1>(a, slot(w, n + 1), st);
         else if (shape.rpt == 4) launch_level2<4>(a, slot(w, n + 1), st);
@@ -1377,7 +1565,7 @@ template <typename T> struct AcPlan : AcPlanBase {
         prof_stop(pr, st, fwd_level_bytes(act));
         fwd_post(L, a, n, srcv, traces, record, st);
         FwdArgs<T> b = fwd_args(L, act, slot(w, n + 1), cst(slot(w, n)), cst(slot(w, n - 1)),
-                                cst(ph[n & 1]), cst(ps[n & 1]), ph[(n + 1) & 1], ps[(n + 1) & 1], w.C, shape,
+                                cst(phl(w, n)), cst(psl(w, n)), phl(w, n + 1), psl(w, n + 1), w.C, shape,
                                 n + 1, srcv);
         pr = prof_start(ADS_PROF_AC_FWD, st);
         if (nitems1 > 0) {
@@ -1415,8 +1603,7 @@ template <typename T> struct AcPlan : AcPlanBase {
                 n++;
                 continue;
             }
-            Lv<T> pi{w.phi[(n - 1) & 1], Npad}, si{w.psi[(n - 1) & 1], Npad};
-            Lv<T> po{w.phi[n & 1], Npad}, so{w.psi[n & 1], Npad};
+            Lv<T> pi = phl(w, n - 1), si = psl(w, n - 1), po = phl(w, n), so = psl(w, n);
             ADS_TRY(fwd_step(L, (int)n, act, slot(w, n), cst(slot(w, n - 1)), cst(slot(w, n - 2)),
                              cst(pi), cst(si), po, so, w.C, srcv, traces, record, st, sh));
         }
@@ -1438,8 +1625,8 @@ template <typename T> struct AcPlan : AcPlanBase {
         size_t F = (size_t)nshots * Npad;
         ADS_HIP(hipMemcpyAsync(dst, slot(w, a).p, bytes, hipMemcpyDeviceToDevice, st));
         ADS_HIP(hipMemcpyAsync(dst + F, slot(w, a - 1).p, bytes, hipMemcpyDeviceToDevice, st));
-        ADS_HIP(hipMemcpyAsync(dst + 2 * F, w.phi[a & 1], bytes, hipMemcpyDeviceToDevice, st));
-        ADS_HIP(hipMemcpyAsync(dst + 3 * F, w.psi[a & 1], bytes, hipMemcpyDeviceToDevice, st));
+        ADS_HIP(hipMemcpyAsync(dst + 2 * F, phl(w, a).p, bytes, hipMemcpyDeviceToDevice, st));
+        ADS_HIP(hipMemcpyAsync(dst + 3 * F, psl(w, a).p, bytes, hipMemcpyDeviceToDevice, st));
         return ADS_OK;
     }
 
@@ -1451,15 +1638,15 @@ template <typename T> struct AcPlan : AcPlanBase {
         if (m == 0) {  // level 1: everything zero
             ADS_HIP(hipMemsetAsync(slot(w, 0).p, 0, bytes, st));
             ADS_HIP(hipMemsetAsync(slot(w, 1).p, 0, bytes, st));
-            ADS_HIP(hipMemsetAsync(w.phi[1], 0, bytes, st));
-            ADS_HIP(hipMemsetAsync(w.psi[1], 0, bytes, st));
+            ADS_HIP(hipMemsetAsync(phl(w, 1).p, 0, bytes, st));
+            ADS_HIP(hipMemsetAsync(psl(w, 1).p, 0, bytes, st));
             return ADS_OK;
         }
         const T *src = w.ckpt + (size_t)(m - 1) * 4 * nshots * Npad;
         ADS_HIP(hipMemcpyAsync(slot(w, a).p, src, bytes, hipMemcpyDeviceToDevice, st));
         ADS_HIP(hipMemcpyAsync(slot(w, a - 1).p, src + F, bytes, hipMemcpyDeviceToDevice, st));
-        ADS_HIP(hipMemcpyAsync(w.phi[a & 1], src + 2 * F, bytes, hipMemcpyDeviceToDevice, st));
-        ADS_HIP(hipMemcpyAsync(w.psi[a & 1], src + 3 * F, bytes, hipMemcpyDeviceToDevice, st));
+        ADS_HIP(hipMemcpyAsync(phl(w, a).p, src + 2 * F, bytes, hipMemcpyDeviceToDevice, st));
+        ADS_HIP(hipMemcpyAsync(psl(w, a).p, src + 3 * F, bytes, hipMemcpyDeviceToDevice, st));
         return ADS_OK;
     }
 
@@ -1481,7 +1668,7 @@ template <typename T> struct AcPlan : AcPlanBase {
         ADS_TRY(prepare_C(w.C, model, (int)Pitch, st));
         ADS_HIP(hipMemsetAsync(slot(w, 0).p, 0, F, st));
         ADS_HIP(hipMemsetAsync(slot(w, 1).p, 0, F, st));
-        for (int q = 0; q < 2; q++) {
+        for (int q = 0; q < 3; q++) {
             ADS_HIP(hipMemsetAsync(w.phi[q], 0, F, st));
             ADS_HIP(hipMemsetAsync(w.psi[q], 0, F, st));
         }
@@ -1567,7 +1754,7 @@ template <typename T> struct AcPlan : AcPlanBase {
         ADS_TRY(prepare_C(w.C, model, (int)Pitch, st));
         const size_t F = (size_t)nshots * Npad * sizeof(T);
         for (int64_t q = 0; q < std::min<int64_t>(w.NS, 4); q++) ADS_HIP(hipMemsetAsync(slot(w, q).p, 0, F, st));
-        for (int q = 0; q < 2; q++) {
+        for (int q = 0; q < 3; q++) {
             ADS_HIP(hipMemsetAsync(w.phi[q], 0, F, st));
             ADS_HIP(hipMemsetAsync(w.psi[q], 0, F, st));
         }
@@ -1875,6 +2062,11 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
         frame_items(P->blk_ok ? P->rect1 : Rect{1, 1, 1, 1}, p->NX, p->NY, items);
         P->nitems1 = P->blk_ok ? (int)items.size() : 0;
         if (P->blk_ok) ADS_TRY(P->d_items1.upload(items));
+        if (P->blk_ok && P->src_fused) {
+            frame_items(P->rect1, p->NX, p->NY, items, FRPT, WAVE - 2);
+            P->nitems2 = (int)items.size();
+            ADS_TRY(P->d_items2.upload(items));
+        }
     }
     ADS_TRY(P->d_pmlx.upload(P->pmlx));
     ADS_TRY(P->d_pmly.upload(P->pmly));

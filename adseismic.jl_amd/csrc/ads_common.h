@@ -118,7 +118,9 @@ struct CellLists {
 // Benchmark timing (see ads_profile_* in the header).  prof_start returns -1 when disabled.
 int prof_start(int cls, hipStream_t st);
 bool prof_enabled();
-void prof_stop(int rec, hipStream_t st, double alg_bytes);
+// alg_bytes: algorithmic bytes of the timed launches; levels: time levels they advance (each
+// counts as one launch of the class)
+void prof_stop(int rec, hipStream_t st, double alg_bytes, int levels = 1);
 
 CellLists build_cell_lists(int64_t nshots, const int64_t *ptr, const std::vector<int64_t> &flat);
 

@@ -28,6 +28,7 @@ struct ProfRec {
     int cls;
     hipEvent_t a, b;
     double bytes;
+    int levels;  // time levels the timed launches advance
 };
 std::mutex g_prof_mu;
 bool g_prof_on = false;
@@ -55,19 +56,20 @@ int prof_start(int cls, hipStream_t st)
 {
     if (!g_prof_on) return -1;
     std::lock_guard<std::mutex> lk(g_prof_mu);
-    ProfRec r{cls, pool_get(), pool_get(), 0.0};
+    ProfRec r{cls, pool_get(), pool_get(), 0.0, 1};
     if (!r.a || !r.b) return -1;
     if (hipEventRecord(r.a, st) != hipSuccess) return -1;
     g_recs.push_back(r);
     return (int)g_recs.size() - 1;
 }
 
-void prof_stop(int rec, hipStream_t st, double bytes)
+void prof_stop(int rec, hipStream_t st, double bytes, int levels)
 {
     if (rec < 0) return;
     std::lock_guard<std::mutex> lk(g_prof_mu);
     if (rec >= (int)g_recs.size()) return;
     g_recs[rec].bytes = bytes;
+    g_recs[rec].levels = levels;
     (void)hipEventRecord(g_recs[rec].b, st);
 }
 
@@ -132,7 +134,7 @@ int ads_profile_end(void)
             set_error("event timing failed");
         } else if (r.cls >= 0 && r.cls < ADS_PROF_COUNT) {
             g_ms[r.cls] += ms;
-            g_n[r.cls] += 1;
+            g_n[r.cls] += r.levels;
             g_bytes[r.cls] += r.bytes;
         }
         g_pool.push_back(r.a);

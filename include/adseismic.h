@@ -280,9 +280,10 @@ int ads_acoustic_plan_set_graph(ads_acoustic_plan *plan, int enable);
 
 /* Temporal blocking of the forward sweeps (no counterpart in the reference): with levels = 2
  * (the default) the interior rectangle advances two time levels per pass (u^n and u^{n+1}
- * from u^{n-1}, u^{n-2}), which cuts its HBM traffic by a third; results are bit-identical to
- * levels = 1.  It applies only when no source lies in the interior rectangle and to pairs of
- * levels with the same number of active shots; otherwise levels are stepped one at a time.
+ * from u^{n-1}, u^{n-2}), which cuts its HBM traffic by a third; with PropagatorKernel 1 the
+ * PML frame does too, in the same launch.  Results are bit-identical to levels = 1.  It
+ * applies only when no source lies in the interior rectangle and to pairs of levels with the
+ * same number of active shots; otherwise levels are stepped one at a time.
  * ads_acoustic_plan_blocking returns the levels per pass in effect (1 or 2). */
 int ads_acoustic_plan_set_blocking(ads_acoustic_plan *plan, int levels);
 int ads_acoustic_plan_blocking(const ads_acoustic_plan *plan);
@@ -349,7 +350,8 @@ int ads_elastic_forward_history(ads_elastic_plan *plan, const void *rho, const v
  * the summed duration, the launch count and the algorithmic HBM bytes of those launches
  * (compulsory per-cell traffic: acoustic forward w(4 + 4 f_pml), adjoint w(7 + 4 f_pml) for
  * PropagatorKernel 1; w(4 + 2 f_pml) and w(7 + 2 f_pml) for kernel 0 whose phi/psi updates
- * run in a separate pass; elastic as documented per class).  Not thread-safe across
+ * run in a separate pass; elastic as documented per class).  A launch that advances two time
+ * levels (the acoustic forward's two-level pass) counts as two launches.  Not thread-safe across
  * concurrent callers; meant for one benchmarking thread.
  * ==================================================================================== */
 enum {
