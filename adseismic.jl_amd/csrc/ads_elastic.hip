@@ -226,12 +226,12 @@ __device__ inline void halo_fill(T (&ext)[VEC + 4], const T (&x)[VEC], const T (
 // (64 lanes x VEC depth cells); a workgroup loops over a group of Sg shots with lambda and mu
 // in registers.
 template <typename T, int RPT, int VEC>
-__global__ __launch_bounds__(WAVE *NWAVES) void el_stress_rect(ElC<T> I, El<T> O, const T *__restrict__ lam,
-                                                               const T *__restrict__ mu, ElCoef<T> c,
-                                                               Rect R, TileMap map, int act, int Sg)
+__device__ inline void el_stress_rect_body(const ElC<T> &I, const El<T> &O, const T *__restrict__ lam,
+                                           const T *__restrict__ mu, const ElCoef<T> &c, const Rect &R,
+                                           const TileMap &map, int act, int Sg, int bid)
 {
     int g, strip, tj;
-    if (!decode_tile(map, g, strip, tj)) return;
+    if (!decode_tile(map, g, strip, tj, bid)) return;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     const int jb = R.jL + tj * WAVE * VEC, j0 = jb + lane * VEC;
     const int ib = R.iL + (strip * NWAVES + wv) * RPT;
@@ -322,12 +322,12 @@ __global__ __launch_bounds__(WAVE *NWAVES) void el_stress_rect(ElC<T> I, El<T> O
 
 // Rectangle velocity update (fw3 + fw4 without memory variables), reading the new stresses.
 template <typename T, int RPT, int VEC>
-__global__ __launch_bounds__(WAVE *NWAVES) void el_vel_rect(ElC<T> I, El<T> O, const T *__restrict__ rho,
-                                                            ElCoef<T> c, Rect R, TileMap map, int act,
-                                                            int Sg)
+__device__ inline void el_vel_rect_body(const ElC<T> &I, const El<T> &O, const T *__restrict__ rho,
+                                        const ElCoef<T> &c, const Rect &R, const TileMap &map, int act,
+                                        int Sg, int bid)
 {
     int g, strip, tj;
-    if (!decode_tile(map, g, strip, tj)) return;
+    if (!decode_tile(map, g, strip, tj, bid)) return;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     const int jb = R.jL + tj * WAVE * VEC, j0 = jb + lane * VEC;
     const int ib = R.iL + (strip * NWAVES + wv) * RPT;
@@ -403,6 +403,61 @@ __global__ __launch_bounds__(WAVE *NWAVES) void el_vel_rect(ElC<T> I, El<T> O, c
             }
         }
     }
+}
+
+// Kernels: the rectangle alone, and the rectangle with the frame in one launch (pitched
+// layout): workgroups [0, nfb) take the frame items (nib item blocks of NWAVES items per shot),
+// the rest the rectangle tiles, so the latency-bound frame work overlaps the rectangle's.
+template <typename T, int RPT, int VEC>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_stress_rect(ElC<T> I, El<T> O, const T *__restrict__ lam,
+                                                               const T *__restrict__ mu, ElCoef<T> c,
+                                                               Rect R, TileMap map, int act, int Sg)
+{
+    el_stress_rect_body<T, RPT, VEC>(I, O, lam, mu, c, R, map, act, Sg, (int)blockIdx.x);
+}
+template <typename T, int RPT, int VEC>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_vel_rect(ElC<T> I, El<T> O, const T *__restrict__ rho,
+                                                            ElCoef<T> c, Rect R, TileMap map, int act,
+                                                            int Sg)
+{
+    el_vel_rect_body<T, RPT, VEC>(I, O, rho, c, R, map, act, Sg, (int)blockIdx.x);
+}
+template <typename T, int RPT, int VEC>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_stress_level(ElC<T> I, El<T> O, const T *__restrict__ lam,
+                                                                const T *__restrict__ mu, ElCoef<T> c,
+                                                                Rect R, TileMap map, int act, int Sg,
+                                                                const int4 *__restrict__ items, int nitems,
+                                                                int nib, int nfb)
+{
+    const int b = (int)blockIdx.x;
+    if (b >= nfb) {
+        el_stress_rect_body<T, RPT, VEC>(I, O, lam, mu, c, R, map, act, Sg, b - nfb);
+        return;
+    }
+    const int it = (b % nib) * NWAVES + threadIdx.x / WAVE, s = b / nib;
+    if (it >= nitems || s >= act) return;
+    const int4 m = items[it];
+    const int j = m.z + (threadIdx.x & (WAVE - 1));
+    if (j >= m.w) return;
+    for (int r = 0; r < m.y; r++) el_stress_cell(I, O, lam, mu, c, s, m.x + r, j);
+}
+template <typename T, int RPT, int VEC>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_vel_level(ElC<T> I, El<T> O, const T *__restrict__ rho,
+                                                             ElCoef<T> c, Rect R, TileMap map, int act,
+                                                             int Sg, const int4 *__restrict__ items,
+                                                             int nitems, int nib, int nfb)
+{
+    const int b = (int)blockIdx.x;
+    if (b >= nfb) {
+        el_vel_rect_body<T, RPT, VEC>(I, O, rho, c, R, map, act, Sg, b - nfb);
+        return;
+    }
+    const int it = (b % nib) * NWAVES + threadIdx.x / WAVE, s = b / nib;
+    if (it >= nitems || s >= act) return;
+    const int4 m = items[it];
+    const int j = m.z + (threadIdx.x & (WAVE - 1));
+    if (j >= m.w) return;
+    for (int r = 0; r < m.y; r++) el_vel_cell(I, O, rho, c, s, m.x + r, j);
 }
 
 // Sources at level n (unique (cell, type) keys, entries in source order), then receivers:
@@ -1195,9 +1250,12 @@ template <typename T> struct ElPlan : ElPlanBase {
     }
 
     static constexpr int VECP = sizeof(T) == 4 ? 4 : 2;  // pitched layout: 16-byte vectors
+    // The rectangle of layout li, with the frame items in the same launch when `fuse` (the
+    // pitched layout); otherwise the caller launches the frame.
     template <int RPT, int VEC>
     void launch_rect_r(int phase, const Rect &R, const ElC<T> &I, const El<T> &O, const T *rho,
-                       const T *lam, const T *mu, const ElCoef<T> &c, int act, int Sg, hipStream_t st)
+                       const T *lam, const T *mu, const ElCoef<T> &c, int act, int Sg, int li,
+                       bool fuse, hipStream_t st)
     {
         TileMap m;
         m.ntj = (int)cdiv(R.jR - R.jL, WAVE * VEC);
@@ -1205,15 +1263,27 @@ template <typename T> struct ElPlan : ElPlanBase {
         m.ngroups = (int)cdiv(act, Sg);
         m.nb = m.ntj * m.nstrips * m.ngroups;
         const unsigned nb = cdiv(m.nb, NXCD) * NXCD;
+        if (fuse) {
+            const int nib = (int)cdiv(nitems[li], NWAVES), nfb = nib * act;
+            const int4 *it = d_items[li].template as<int4>();
+            if (phase == 0)
+                el_stress_level<T, RPT, VEC><<<nb + nfb, WAVE * NWAVES, 0, st>>>(I, O, lam, mu, c, R, m, act, Sg,
+                                                                                it, nitems[li], nib, nfb);
+            else
+                el_vel_level<T, RPT, VEC><<<nb + nfb, WAVE * NWAVES, 0, st>>>(I, O, rho, c, R, m, act, Sg,
+                                                                             it, nitems[li], nib, nfb);
+            return;
+        }
         if (phase == 0) el_stress_rect<T, RPT, VEC><<<nb, WAVE * NWAVES, 0, st>>>(I, O, lam, mu, c, R, m, act, Sg);
         else el_vel_rect<T, RPT, VEC><<<nb, WAVE * NWAVES, 0, st>>>(I, O, rho, c, R, m, act, Sg);
     }
     template <int VEC>
     void launch_rect(int phase, const Rect &R, const ElC<T> &I, const El<T> &O, const T *rho,
-                     const T *lam, const T *mu, const ElCoef<T> &c, int act, int Sg, hipStream_t st)
+                     const T *lam, const T *mu, const ElCoef<T> &c, int act, int Sg, int li, bool fuse,
+                     hipStream_t st)
     {
-        if (rpt == 2) launch_rect_r<2, VEC>(phase, R, I, O, rho, lam, mu, c, act, Sg, st);
-        else launch_rect_r<1, VEC>(phase, R, I, O, rho, lam, mu, c, act, Sg, st);
+        if (rpt == 2) launch_rect_r<2, VEC>(phase, R, I, O, rho, lam, mu, c, act, Sg, li, fuse, st);
+        else launch_rect_r<1, VEC>(phase, R, I, O, rho, lam, mu, c, act, Sg, li, fuse, st);
     }
 
     int step(const El<T> &I, const El<T> &O, int n, int act, const T *rho, const T *lam,
@@ -1229,21 +1299,22 @@ template <typename T> struct ElPlan : ElPlanBase {
         const bool rr = R.iR > R.iL && R.jR > R.jL;
         const int Sg = std::min(sg, act);
         const dim3 fg(cdiv(nitems[li], NWAVES), 1, act);
+        const bool fuse = rr && li == 0 && nitems[li] > 0;  // rectangle + frame in one launch
         int pr = prof_start(ADS_PROF_EL_FWD_STRESS, st);
         if (rr) {
-            if (li == 0) launch_rect<VECP>(0, R, cst(I), O, rho, lam, mu, c, act, Sg, st);
-            else launch_rect<1>(0, R, cst(I), O, rho, lam, mu, c, act, Sg, st);
+            if (li == 0) launch_rect<VECP>(0, R, cst(I), O, rho, lam, mu, c, act, Sg, li, fuse, st);
+            else launch_rect<1>(0, R, cst(I), O, rho, lam, mu, c, act, Sg, li, false, st);
         }
-        if (nitems[li])
+        if (nitems[li] && !fuse)
             el_stress_frame<T><<<fg, WAVE * NWAVES, 0, st>>>(cst(I), O, lam, mu, c,
                                                               d_items[li].template as<int4>(), nitems[li]);
         prof_stop(pr, st, cells * sizeof(T) * (10.0 + 4.0 * f_x + 4.0 * f_y));
         pr = prof_start(ADS_PROF_EL_FWD_VEL, st);
         if (rr) {
-            if (li == 0) launch_rect<VECP>(1, R, cst(I), O, rho, lam, mu, c, act, Sg, st);
-            else launch_rect<1>(1, R, cst(I), O, rho, lam, mu, c, act, Sg, st);
+            if (li == 0) launch_rect<VECP>(1, R, cst(I), O, rho, lam, mu, c, act, Sg, li, fuse, st);
+            else launch_rect<1>(1, R, cst(I), O, rho, lam, mu, c, act, Sg, li, false, st);
         }
-        if (nitems[li])
+        if (nitems[li] && !fuse)
             el_vel_frame<T><<<fg, WAVE * NWAVES, 0, st>>>(cst(I), O, rho, c,
                                                            d_items[li].template as<int4>(), nitems[li]);
         prof_stop(pr, st, cells * sizeof(T) * (8.0 + 4.0 * f_x + 4.0 * f_y));
