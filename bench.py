@@ -83,7 +83,7 @@ def velocity_model(NX, NY, seed):
     return c
 
 
-def workload(ads, small=False, nstep=0):
+def workload(ads, small=False, nstep=0, kernel=1):
     import torch
     from scipy.ndimage import gaussian_filter
 
@@ -97,7 +97,7 @@ def workload(ads, small=False, nstep=0):
         NSTEP = nstep
     p = ads.AcousticPropagatorParams(NX=NX, NY=NY, NSTEP=NSTEP, DELTAX=5.0, DELTAY=5.0,
                                      DELTAT=0.5e-3, NPOINTS_PML=npml, vp_ref=3000.0,
-                                     PropagatorKernel=1)
+                                     PropagatorKernel=kernel)
     c_true = velocity_model(NX, NY, seed=2)
     c0 = gaussian_filter(c_true, 10.0, mode="reflect")
     ads.check_cfl(p, float(c_true.max()))
@@ -291,6 +291,9 @@ def main():
     ap.add_argument("--nstep", type=int, default=0,
                     help="override the record length (profiling runs only; not the metric)")
     ap.add_argument("--no-elastic", action="store_true", help="skip the elastic gradient leg")
+    ap.add_argument("--propagator-kernel", type=int, default=1, choices=(0, 1),
+                    help="PropagatorKernel of the acoustic workload (0: the reference's default "
+                         "Core.jl form, phi/psi from the new u; 1: the custom-op form)")
     ap.add_argument("--elastic-only", action="store_true",
                     help="run only the elastic gradient leg (profiling runs; no bench line)")
     ap.add_argument("--elastic-nstep", type=int, default=0,
@@ -316,7 +319,8 @@ def main():
         print(json.dumps({"elastic": elastic_leg(ads, L, lib, args.steps, args.elastic_nstep)}),
               flush=True)
         return
-    p, c_true, c0, srcs, rcvs = workload(ads, small=args.small, nstep=args.nstep)
+    p, c_true, c0, srcs, rcvs = workload(ads, small=args.small, nstep=args.nstep,
+                                         kernel=args.propagator_kernel)
     if args.calib_copy:
         n = args.calib_copy * (1 << 20) // 4
         a = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -360,7 +364,8 @@ def main():
     dom = max(prof, key=lambda k: prof[k][0])
     tm, n, b = prof[dom]
     achieved = (b / n) / (tm / n * 1e-3) / 1e9 if n else 0.0
-    traffic, traffic_shape = measured_traffic(dom, shape)
+    # the committed PMC summary was taken with PropagatorKernel 1
+    traffic, traffic_shape = measured_traffic(dom, shape) if p.PropagatorKernel == 1 else (None, None)
     out = {
         "metric": "FWI gradient Gcell-steps/s (fwd+adj), batched",
         "value": round(value, 4), "unit": "Gcell-steps/s", "n_gpus": 1, "steps": args.steps,
