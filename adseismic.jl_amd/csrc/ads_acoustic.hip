@@ -532,24 +532,29 @@ __device__ inline void fwd_frame2_body(const FwdArgs<T> &a, const Lv<T> &u2, con
     }
 }
 
-// kernel 0: phi^n/psi^n from the new u (before sources), PML cells only.  One thread per cell.
+// kernel 0: phi^n/psi^n from the new u (before sources), PML cells only.  The PML lies in the
+// frame: one workgroup per (frame item, 4 shots), a wavefront per shot, a lane per column.
 template <typename T>
 __global__ __launch_bounds__(256) void ac_fwd_phipsi(
     Lv<const T> u_, Lv<const T> phi_, Lv<const T> psi_, Lv<T> phio, Lv<T> psio,
     const T *__restrict__ C, const T *__restrict__ sx, const T *__restrict__ ty,
     const uint8_t *__restrict__ pmlx, const uint8_t *__restrict__ pmly, int NX, int NY, int P,
-    AcCoef<T> k)
+    AcCoef<T> k, const int4 *__restrict__ items, int act)
 {
-    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int s = blockIdx.z;
-    if (j < 1 || j > NY || i < 1 || i > NX) return;
-    if (!(pmlx[i] | pmly[j])) return;
-    const int64_t ij = (int64_t)i * P + j;
+    const int4 m = items[blockIdx.x];
+    const int j = m.z + (threadIdx.x & 63);
+    const int s = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (s >= act || j >= m.w || j < 1 || j > NY) return;
     const T *__restrict__ u = u_.at(s);
-    const T sg = sx[i], tu = ty[j], Cij = C[ij];
-    phio.at(s)[ij] = (T(1) - k.dt * sg) * phi_.at(s)[ij] + Cij * (tu - sg) * k.fx * (u[ij + P] - u[ij - P]);
-    psio.at(s)[ij] = (T(1) - k.dt * tu) * psi_.at(s)[ij] + Cij * (sg - tu) * k.fy * (u[ij + 1] - u[ij - 1]);
+    const T tu = ty[j];
+    for (int r = 0; r < m.y; r++) {
+        const int i = m.x + r;
+        if (i < 1 || i > NX || !(pmlx[i] | pmly[j])) continue;
+        const int64_t ij = (int64_t)i * P + j;
+        const T sg = sx[i], Cij = C[ij];
+        phio.at(s)[ij] = (T(1) - k.dt * sg) * phi_.at(s)[ij] + Cij * (tu - sg) * k.fx * (u[ij + P] - u[ij - P]);
+        psio.at(s)[ij] = (T(1) - k.dt * tu) * psi_.at(s)[ij] + Cij * (sg - tu) * k.fy * (u[ij + 1] - u[ij - 1]);
+    }
 }
 
 // Per-shot source injection at level n (u[cell] += dt^2 * srcv[n-1, q] in source order,
@@ -603,8 +608,10 @@ __global__ __launch_bounds__(256) void ac_record(const T *__restrict__ seg, int6
 // ---------------------------------------------------------------------------------------------
 // adjoint
 // ---------------------------------------------------------------------------------------------
-// kernel 0 only: F^n, G^n on PML cells and their dL/dC term with v = u^n before sources.
-// One thread per cell and shot group; the group's shots are summed in order into one write.
+// kernel 0 only: F^n, G^n on PML cells and their dL/dC term with v = u^n before sources.  The
+// PML lies in the frame: one workgroup per (frame item, shot group); its 4 wavefronts take every
+// 4th shot of the group, a lane per column, and their dL/dC partial sums are added in wavefront
+// order through LDS into one write per cell.
 template <typename T>
 __global__ __launch_bounds__(256) void ac_adj_phipsi(
     Lv<const T> P1_, Lv<const T> F1_, Lv<const T> G1_, Lv<T> F0_, Lv<T> G0_, Lv<const T> un_,
@@ -613,41 +620,66 @@ __global__ __launch_bounds__(256) void ac_adj_phipsi(
     const int64_t *__restrict__ us_ptr, const int64_t *__restrict__ us_cell,
     const int64_t *__restrict__ us_eptr, const int64_t *__restrict__ us_ent,
     const T *__restrict__ srcv, const int64_t *__restrict__ srcv_off,
-    const int64_t *__restrict__ nsrc, int NX, int NY, int P, AcCoef<T> k)
+    const int64_t *__restrict__ nsrc, int NX, int NY, int P, AcCoef<T> k,
+    const int4 *__restrict__ items)
 {
-    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int g = blockIdx.z;
-    if (j < 1 || j > NY || i < 1 || i > NX) return;
-    if (!(pmlx[i] | pmly[j])) return;
-    const int64_t ij = (int64_t)i * P + j;
-    const T sg = sx[i], tu = ty[j];
-    T acc = T(0);
+    const int4 m = items[blockIdx.x];
+    const int g = blockIdx.y;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int j = m.z + lane;
+    const bool jok = j < m.w && j >= 1 && j <= NY;
+    const T tu = jok ? ty[j] : T(0);
+    T acc[FRPT] = {};
+    bool cell[FRPT];
+#pragma unroll
+    for (int r = 0; r < FRPT; r++) {
+        const int i = m.x + r;
+        cell[r] = jok && r < m.y && i >= 1 && i <= NX && (pmlx[i] | pmly[j]);
+    }
     const int s1 = min(act, (g + 1) * Sg);
-    for (int s = g * Sg; s < s1; s++) {
+    for (int s = g * Sg + wv; s < s1; s += NWAVES) {
         const T *__restrict__ P1 = P1_.at(s);
-        const T F0 = (T(1) - k.dt * sg) * F1_.at(s)[ij] + k.ex * (P1[ij - P] - P1[ij + P]);
-        const T G0 = (T(1) - k.dt * tu) * G1_.at(s)[ij] + k.ey * (P1[ij - 1] - P1[ij + 1]);
-        F0_.at(s)[ij] = F0;
-        G0_.at(s)[ij] = G0;
         const T *__restrict__ u = un_.at(s);
-        T vn = u[ij - P], vs = u[ij + P], vw = u[ij - 1], ve = u[ij + 1];
-        // remove this level's sources from the neighbours (v is u^n before injection)
         const int64_t ns = nsrc[s];
         const T *__restrict__ row = srcv + srcv_off[s] + (int64_t)(n - 1) * ns;
-        for (int64_t c = us_ptr[s]; c < us_ptr[s + 1]; c++) {
-            const int64_t cell = us_cell[c];
-            if (cell != ij - P && cell != ij + P && cell != ij - 1 && cell != ij + 1) continue;
-            T add = T(0);
-            for (int64_t e = us_eptr[c]; e < us_eptr[c + 1]; e++) add += row[us_ent[e]] * k.dt2;
-            if (cell == ij - P) vn -= add;
-            if (cell == ij + P) vs -= add;
-            if (cell == ij - 1) vw -= add;
-            if (cell == ij + 1) ve -= add;
+#pragma unroll
+        for (int r = 0; r < FRPT; r++) {
+            if (!cell[r]) continue;
+            const int i = m.x + r;
+            const int64_t ij = (int64_t)i * P + j;
+            const T sg = sx[i];
+            const T F0 = (T(1) - k.dt * sg) * F1_.at(s)[ij] + k.ex * (P1[ij - P] - P1[ij + P]);
+            const T G0 = (T(1) - k.dt * tu) * G1_.at(s)[ij] + k.ey * (P1[ij - 1] - P1[ij + 1]);
+            F0_.at(s)[ij] = F0;
+            G0_.at(s)[ij] = G0;
+            T vn = u[ij - P], vs = u[ij + P], vw = u[ij - 1], ve = u[ij + 1];
+            // remove this level's sources from the neighbours (v is u^n before injection)
+            for (int64_t c = us_ptr[s]; c < us_ptr[s + 1]; c++) {
+                const int64_t cl = us_cell[c];
+                if (cl != ij - P && cl != ij + P && cl != ij - 1 && cl != ij + 1) continue;
+                T add = T(0);
+                for (int64_t e = us_eptr[c]; e < us_eptr[c + 1]; e++) add += row[us_ent[e]] * k.dt2;
+                if (cl == ij - P) vn -= add;
+                if (cl == ij + P) vs -= add;
+                if (cl == ij - 1) vw -= add;
+                if (cl == ij + 1) ve -= add;
+            }
+            acc[r] += F0 * (tu - sg) * k.fx * (vs - vn) + G0 * (sg - tu) * k.fy * (ve - vw);
         }
-        acc += F0 * (tu - sg) * k.fx * (vs - vn) + G0 * (sg - tu) * k.fy * (ve - vw);
     }
-    gC_.at(g)[ij] += acc;
+    __shared__ T red[NWAVES][FRPT][WAVE];
+#pragma unroll
+    for (int r = 0; r < FRPT; r++) red[wv][r][lane] = acc[r];
+    __syncthreads();
+    if (wv != 0) return;
+#pragma unroll
+    for (int r = 0; r < FRPT; r++) {
+        if (!cell[r]) continue;
+        T sum = red[0][r][lane];
+#pragma unroll
+        for (int q = 1; q < NWAVES; q++) sum += red[q][r][lane];
+        gC_.at(g)[(int64_t)(m.x + r) * P + j] += sum;
+    }
 }
 
 template <typename T> struct AdjArgs {
@@ -1301,10 +1333,11 @@ template <typename T> struct AcPlan : AcPlanBase {
     {
         if (a.src.cell_idx) return;  // sources added by the frame work, receivers per segment
         if (variant == 0) {
-            dim3 g(cdiv(NY + 2, 64), cdiv(NX + 2, 4), a.act);
+            dim3 g((unsigned)nitems[L.li], cdiv(a.act, NWAVES));
             ac_fwd_phipsi<T><<<g, 256, 0, st>>>(cst(a.u), a.phi, a.psi, a.phio, a.psio, a.C, d_sx.as<T>(),
                                                 d_ty.as<T>(), d_pmlx.as<uint8_t>(),
-                                                d_pmly.as<uint8_t>(), (int)NX, (int)NY, L.P, k);
+                                                d_pmly.as<uint8_t>(), (int)NX, (int)NY, L.P, k,
+                                                d_items[L.li].template as<int4>(), a.act);
         }
         ac_src_rcv<T><<<a.act, 256, 0, st>>>(a.u, n, d_us_ptr.as<int64_t>(), d_us_cell[L.li].template as<int64_t>(),
                                              d_us_eptr.as<int64_t>(), d_us_ent.as<int64_t>(), srcv,
@@ -1338,12 +1371,12 @@ template <typename T> struct AcPlan : AcPlanBase {
         const int Sg = adj_shape.sg;  // gC grouping is fixed per plan (see group_arrays())
         const double cells = (double)act * (double)NX * (double)NY;
         if (variant == 0) {
-            dim3 g(cdiv(NY + 2, 64), cdiv(NX + 2, 4), cdiv(act, std::min(Sg, act)));
+            dim3 g((unsigned)nitems[L.li], cdiv(act, std::min(Sg, act)));
             ac_adj_phipsi<T><<<g, 256, 0, st>>>(
                 P1, F1, G1, F0, G0, un, gC, std::min(Sg, act), act, n, d_sx.as<T>(), d_ty.as<T>(), d_pmlx.as<uint8_t>(),
                 d_pmly.as<uint8_t>(), d_us_ptr.as<int64_t>(), d_us_cell[L.li].template as<int64_t>(),
                 d_us_eptr.as<int64_t>(), d_us_ent.as<int64_t>(), srcv, d_srcv_off.as<int64_t>(),
-                d_nsrc.as<int64_t>(), (int)NX, (int)NY, L.P, k);
+                d_nsrc.as<int64_t>(), (int)NX, (int)NY, L.P, k, d_items[L.li].template as<int4>());
         }
         AdjArgs<T> a{P, P1, variant == 1 ? F1 : cst(F0), variant == 1 ? G1 : cst(G0), F0, G0,
                      um1, dense, gC, C, d_sx.as<T>(), d_ty.as<T>(), d_nearx.as<uint8_t>(),
