@@ -296,6 +296,156 @@ __device__ inline void fwd_rect2_body(const FwdArgs<T> &a, const Lv<T> &u2, cons
     }
 }
 
+// Two-level rectangle forward, streaming form: a wavefront owns ROWS rows of one column
+// segment for one shot (a workgroup takes NWAVES shots) and walks down them, holding three rows
+// of u^{n-1}, u^n and C in registers and loading the next rows one step ahead, so each row of
+// u^{n-1} and u^{n-2} is read about once instead of RPT+4 and RPT+2 times per RPT rows.  The
+// values and writes are those of fwd_rect2_body (bit-identical).
+template <typename T, int VEC>
+struct Rect2Row {  // one row of the window: the vector and the segment's halo columns
+    T v[VEC];
+    T l, r;
+};
+template <typename T, int VEC, int ROWS>
+__device__ inline void fwd_rect2s_body(const FwdArgs<T> &a, const Lv<T> &u2, const Rect &R, int bid)
+{
+    int g, strip, tj;
+    if (!decode_tile(a.map, g, strip, tj, bid)) return;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int s = g * NWAVES + wv;
+    if (s >= a.act) return;  // wave-uniform; no barrier below
+    const int ja = R.jL + tj * WAVE * VEC;  // first column of the segment
+    const int j0 = ja + lane * VEC;
+    const int ib = R.iL + strip * ROWS;  // first owned row
+    const int ie = min(ib + ROWS, R.iR);  // one past the last owned row
+    const int P = a.P;
+    const bool act = j0 < R.jR;
+    const int jb = min(ja + WAVE * VEC, R.jR);
+    const bool le = lane == 0;
+    const bool re = act && (lane == WAVE - 1 || j0 + VEC == R.jR);
+    const bool hl = le && ja - 1 >= R.jL;
+    const bool hr = re && jb < R.jR;
+    const AcCoef<T> k = a.k;
+    const T *__restrict__ w = a.w.at(s);
+    const T *__restrict__ wo = a.wold.at(s);
+    T *__restrict__ u1 = a.u.at(s);
+    T *__restrict__ up = u2.at(s);
+    // u^{n-1} row x (rows R.iL-1 .. R.iR lie inside [0, NX+1]) with columns ja-1 and jb
+    auto loadW = [&](int x, Rect2Row<T, VEC> &o) {
+        const bool ok = x >= R.iL - 1 && x <= R.iR;
+        const int64_t q = (int64_t)x * P;
+        if (act && ok) vld(o.v, w + q + j0);
+        else vzero(o.v);
+        o.l = le && ok ? w[q + ja - 1] : T(0);
+        o.r = re && ok ? w[q + jb] : T(0);
+    };
+    // inputs of level n on row x besides u^{n-1}: u^{n-2}, C, and for the halo columns
+    // u^{n-1} at ja-2 / jb+1, u^{n-2} and C at ja-1 / jb
+    struct In {
+        T wo[VEC], c[VEC];
+        T cl, cr, wl2, wr2, wol, wor;
+    };
+    auto loadIn = [&](int x, In &o) {
+        const bool ok = x >= R.iL && x < R.iR;
+        const int64_t q = (int64_t)x * P;
+        if (act && ok) {
+            vld(o.wo, wo + q + j0);
+            vld(o.c, a.C + q + j0);
+        } else {
+            vzero(o.wo);
+            vzero(o.c);
+        }
+        o.cl = hl && ok ? a.C[q + ja - 1] : T(0);
+        o.cr = hr && ok ? a.C[q + jb] : T(0);
+        o.wl2 = hl && ok ? w[q + ja - 2] : T(0);
+        o.wr2 = hr && ok ? w[q + jb + 1] : T(0);
+        o.wol = hl && ok ? wo[q + ja - 1] : T(0);
+        o.wor = hr && ok ? wo[q + jb] : T(0);
+    };
+    // level n on row x from u^{n-1} rows x-1, x, x+1
+    auto level_n = [&](const Rect2Row<T, VEC> &Wn, const Rect2Row<T, VEC> &Wc,
+                       const Rect2Row<T, VEC> &Ws, const In &I, Rect2Row<T, VEC> &U) {
+        T wl, wr;
+        lr_of(Wc.v, Wc.l, Wc.r, lane, re, wl, wr);
+#pragma unroll
+        for (int e = 0; e < VEC; e++) {
+            const T wc = Wc.v[e];
+            const T ww = e == 0 ? wl : Wc.v[e - 1];
+            const T we = e == VEC - 1 ? wr : Wc.v[e + 1];
+            const T lap = k.cx * (Ws.v[e] + Wn.v[e] - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
+            U.v[e] = T(2) * wc - I.wo[e] + I.c[e] * lap;
+        }
+        {
+            const T wc = Wc.l;
+            const T lap = k.cx * (Ws.l + Wn.l - T(2) * wc) + k.cy * (Wc.v[0] + I.wl2 - T(2) * wc);
+            U.l = T(2) * wc - I.wol + I.cl * lap;
+        }
+        {
+            const T wc = Wc.r;
+            const T lap = k.cx * (Ws.r + Wn.r - T(2) * wc) + k.cy * (I.wr2 + Wc.v[VEC - 1] - T(2) * wc);
+            U.r = T(2) * wc - I.wor + I.cr * lap;
+        }
+    };
+    // window: W0 = row x-1, W1 = row x, W2 = row x+1 (u^{n-1}); Wn = row x+2 (next load);
+    // U0 = level n on row x-2, U1 on row x-1; C1 = C on row x-1; I1 = inputs of row x, In_ next
+    Rect2Row<T, VEC> W0, W1, W2, U0, U1;
+    T C1[VEC];
+    In I1;
+    const int x0 = ib - 1;  // first level-n row
+    loadW(x0 - 1, W0);
+    loadW(x0, W1);
+    loadW(x0 + 1, W2);
+    loadIn(x0, I1);
+    vzero(U0.v);
+    vzero(U1.v);
+    U0.l = U0.r = U1.l = U1.r = T(0);
+    vzero(C1);
+#pragma unroll 1
+    for (int x = x0; x <= ie; x++) {
+        // next rows first, so their latency overlaps this step
+        Rect2Row<T, VEC> W3;
+        In I2;
+        loadW(x + 2, W3);
+        loadIn(x + 1, I2);
+        Rect2Row<T, VEC> U2;  // level n on row x
+        level_n(W0, W1, W2, I1, U2);
+        if (x >= ib && x < ie && act) vst(u1 + (int64_t)x * P + j0, U2.v);
+        const int y = x - 1;  // level n+1 on row y from level n rows y-1, y, y+1
+        if (y >= ib && y > R.iL && y < R.iR - 1) {
+            T ul, ur;
+            lr_of(U1.v, U1.l, U1.r, lane, re, ul, ur);
+            T out[VEC];
+#pragma unroll
+            for (int e = 0; e < VEC; e++) {
+                const T uc = U1.v[e];
+                const T uw = e == 0 ? ul : U1.v[e - 1];
+                const T ue = e == VEC - 1 ? ur : U1.v[e + 1];
+                const T lap = k.cx * (U2.v[e] + U0.v[e] - T(2) * uc) + k.cy * (ue + uw - T(2) * uc);
+                out[e] = T(2) * uc - W0.v[e] + C1[e] * lap;
+            }
+            if (act) {
+                const int64_t o = (int64_t)y * P + j0;
+                if (j0 > R.jL && j0 + VEC < R.jR) {
+                    vst(up + o, out);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VEC; e++)
+                        if (j0 + e > R.jL && j0 + e < R.jR - 1) up[o + e] = out[e];
+                }
+            }
+        }
+        // rotate
+        W0 = W1;
+        W1 = W2;
+        W2 = W3;
+        U0 = U1;
+        U1 = U2;
+#pragma unroll
+        for (int e = 0; e < VEC; e++) C1[e] = I1.c[e];
+        I1 = I2;
+    }
+}
+
 // Frame forward, one depth cell per lane.  A workgroup takes one frame item and its four
 // wavefronts take every fourth shot of the group, so the thin PML strips still fill the GPU.
 // Only lanes with a cell near the PML read phi/psi; cells of the padding ring get u = 0
@@ -1009,6 +1159,18 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_pair(const FwdArgs<T> a, 
     if (b < nfb) fwd_frame2_body<T>(a, u2, phi2, psi2, R, fmap, items, nitems, b);
     else fwd_rect2_body<T, RPT, VEC>(a, u2, R, b - nfb);
 }
+// The same with the streaming rectangle body (ROWS rows per wavefront, one shot each).
+template <typename T, int VEC, int ROWS>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_pair_s(const FwdArgs<T> a, const Lv<T> u2,
+                                                               const Lv<T> phi2, const Lv<T> psi2,
+                                                               const Rect R, const TileMap fmap,
+                                                               const int4 *__restrict__ items,
+                                                               int nitems, int nfb)
+{
+    const int b = (int)blockIdx.x;
+    if (b < nfb) fwd_frame2_body<T>(a, u2, phi2, psi2, R, fmap, items, nitems, b);
+    else fwd_rect2s_body<T, VEC, ROWS>(a, u2, R, b - nfb);
+}
 template <typename T, int RPT, int VEC, int SB>
 __global__ __launch_bounds__(WAVE *NWAVES) void ac_adj_rect(const AdjArgs<T> a, const Rect R)
 {
@@ -1118,7 +1280,7 @@ struct AcPlanBase {
     std::vector<double> sx, ty;
     std::vector<uint8_t> pmlx, pmly, nearx, neary;
     double f_pml = 0.0;  // fraction of interior cells inside the PML strips
-    Shape fwd_shape, adj_shape;
+    Shape fwd_shape{8, 8, 1}, adj_shape;  // forward: the streaming two-level rectangle
     int tuned = 0;
     // device tables; cell tables exist for the pitched (plan) and dense (history) layouts
     DevBuf d_sx, d_ty, d_pmlx, d_pmly, d_nearx, d_neary;
@@ -1574,6 +1736,24 @@ template <typename T> struct AcPlan : AcPlanBase {
         ac_fwd_pair<T, RPT, VECP><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, phi2, psi2, R, fm,
                                                                 d_items2.template as<int4>(), nitems2, nfb);
     }
+    template <int ROWS>
+    void launch_pair_s(FwdArgs<T> a, Lv<T> u2, Lv<T> phi2, Lv<T> psi2, hipStream_t st)
+    {
+        const Rect &R = rect[0];
+        a.map.ntj = (int)cdiv(R.jR - R.jL, WAVE * VECP);
+        a.map.nstrips = (int)cdiv(R.iR - R.iL, ROWS);
+        a.map.ngroups = (int)cdiv(a.act, NWAVES);
+        a.map.nb = a.map.ntj * a.map.nstrips * a.map.ngroups;
+        TileMap fm;
+        fm.ntj = 1;
+        fm.nstrips = nitems2;
+        fm.ngroups = (int)cdiv(a.act, a.Sg);
+        fm.nb = fm.ntj * fm.nstrips * fm.ngroups;
+        const int nfb = (int)grid_of(fm);
+        const unsigned nb = (unsigned)nfb + grid_of(a.map);
+        ac_fwd_pair_s<T, VECP, ROWS><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, phi2, psi2, R, fm,
+                                                                   d_items2.template as<int4>(), nitems2, nfb);
+    }
     int fwd_pair(const WS &w, int n, int act, const T *srcv, T *traces, int record, hipStream_t st,
                  const Shape *sh)
     {
@@ -1586,6 +1766,8 @@ template <typename T> struct AcPlan : AcPlanBase {
             int pr = prof_start(ADS_PROF_AC_FWD, st);
             if (shape.rpt == 1) launch_pair<1>(a, slot(w, n + 1), phl(w, n + 1), psl(w, n + 1), st);
             else if (shape.rpt == 4) launch_pair<4>(a, slot(w, n + 1), phl(w, n + 1), psl(w, n + 1), st);
+            else if (shape.rpt == 8) launch_pair_s<8>(a, slot(w, n + 1), phl(w, n + 1), psl(w, n + 1), st);
+            else if (shape.rpt == 16) launch_pair_s<16>(a, slot(w, n + 1), phl(w, n + 1), psl(w, n + 1), st);
             else launch_pair<2>(a, slot(w, n + 1), phl(w, n + 1), psl(w, n + 1), st);
             prof_stop(pr, st, 2.0 * fwd_level_bytes(act), 2);
             ADS_LAUNCH_CHECK();
@@ -1792,6 +1974,8 @@ template <typename T> struct AcPlan : AcPlanBase {
             ADS_HIP(hipMemsetAsync(w.psi[q], 0, F, st));
         }
         const int rpts[3] = {1, 2, 4};
+        // the forward also tries the streaming two-level rectangle (8 rows per wavefront)
+        const int frpts[4] = {1, 2, 4, 8};
         std::vector<int> sgs;
         for (int sg = 1; sg <= std::max<int64_t>(nshots, 1) && sg <= 32; sg *= 2) sgs.push_back(sg);
         if (reps < 1) reps = 3;
@@ -1811,10 +1995,10 @@ template <typename T> struct AcPlan : AcPlanBase {
         int64_t n0 = std::min<int64_t>(2, NMAX);
         float best = 1e30f;
         Shape bf = fwd_shape;
-        for (int rp : rpts)
+        for (int rp : frpts)
             for (int sg : sgs)
                 for (int sb = 1; sb <= std::min(2, sg); sb++) {
-                    if (sb == 2 && rp == 4) continue;
+                    if (sb == 2 && rp >= 4) continue;
                     Shape sh{rp, sg, sb};
                     float ms = timeit([&] { (void)run_forward(w, n0 - 1, n0 + 1 <= NMAX ? n0 + 1 : n0, sv, nullptr, 0, st, &sh); });
                     if (ms < best) best = ms, bf = sh;
@@ -2266,10 +2450,11 @@ int ads_acoustic_plan_set_shape(ads_acoustic_plan *plan, const int *shape)
     if (!shape) ADS_FAIL(ADS_ERR_ARG, "null shape");
     for (int q = 0; q < 2; q++) {
         const int rpt = shape[3 * q], sg = shape[3 * q + 1], sb = shape[3 * q + 2];
-        if (rpt != 1 && rpt != 2 && rpt != 4) ADS_FAIL(ADS_ERR_ARG, "rows per wavefront must be 1, 2 or 4");
+        if (rpt != 1 && rpt != 2 && rpt != 4 && !(q == 0 && (rpt == 8 || rpt == 16)))
+            ADS_FAIL(ADS_ERR_ARG, "rows per wavefront must be 1, 2 or 4 (8 or 16 for the forward)");
         if (sg < 1) ADS_FAIL(ADS_ERR_ARG, "shots per workgroup must be >= 1");
         if (sb != 1 && sb != 2) ADS_FAIL(ADS_ERR_ARG, "shots per batch must be 1 or 2");
-        if (sb == 2 && rpt == 4) ADS_FAIL(ADS_ERR_ARG, "shots per batch 2 needs 1 or 2 rows per wavefront");
+        if (sb == 2 && rpt >= 4) ADS_FAIL(ADS_ERR_ARG, "shots per batch 2 needs 1 or 2 rows per wavefront");
     }
     return dispatch(plan, [&](auto &P) -> int {
         P.fwd_shape = Shape{shape[0], shape[1], shape[2]};

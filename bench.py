@@ -37,12 +37,13 @@ PEAK_HBM_GBPS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_acoustic_1000x1000_32shots.txt")
 
 
-# Kernel shape of the acoustic plan on the benchmark workload: forward 2 rows per wavefront,
-# 16 shots per workgroup, 1 shot per batch; adjoint the same.  The fastest of the shapes timed
-# back to back on one MI355X with the forward's two-level rectangle pass (forward 111 us and
-# adjoint 152 us per level at 1000 x 1000 x 32 shots); ads_acoustic_plan_autotune picks shapes
-# within a few percent of it.
-DEFAULT_SHAPE = "2,16,1,2,16,1"
+# Kernel shape of the acoustic plan on the benchmark workload: forward 8 rows per wavefront (the
+# streaming two-level rectangle, one shot per wavefront; 16 shots per workgroup for the frame
+# items), adjoint 2 rows per wavefront, 16 shots per workgroup, 1 shot per batch.  The fastest
+# of the shapes timed back to back on one MI355X (forward 96-98 us and adjoint 165 us per level
+# at 1000 x 1000 x 32 shots; 4, 12 or 16 streaming rows within 1-4%); ads_acoustic_plan_autotune
+# picks shapes within a few percent of it.
+DEFAULT_SHAPE = "8,16,1,2,16,1"
 
 
 def measured_traffic(kernel, shape):

@@ -260,11 +260,14 @@ int ads_acoustic_plan_gradient(ads_acoustic_plan *plan, const void *model, const
                                ads_stream_t stream);
 
 /* Kernel-shape autotuner: times every candidate shape (rows per wavefront 1/2/4 x shots per
- * workgroup 1..32 x shots per batch 1/2, batch 2 only with 1-2 rows) of the forward and adjoint
- * stencil kernels on this plan's own grid and shots for `reps` steps each, keeps the fastest in
- * the plan, and writes it to shape_out[6] = {fwd rows, fwd shots, fwd batch, adj rows, adj
- * shots, adj batch} (may be NULL).  Synchronises the stream; leaves the workspace contents
- * undefined.  ads_acoustic_plan_set_shape() sets a shape directly (same 6-entry layout). */
+ * workgroup 1..32 x shots per batch 1/2, batch 2 only with 1-2 rows; the forward also 8 rows,
+ * the streaming form of its two-level rectangle pass, one shot per wavefront) of the forward
+ * and adjoint stencil kernels on this plan's own grid and shots for `reps` steps each, keeps
+ * the fastest in the plan, and writes it to shape_out[6] = {fwd rows, fwd shots, fwd batch,
+ * adj rows, adj shots, adj batch} (may be NULL).  Synchronises the stream; leaves the
+ * workspace contents undefined.  ads_acoustic_plan_set_shape() sets a shape directly (same
+ * 6-entry layout; forward rows 8 or 16 select the streaming rectangle where the two-level pass
+ * applies and 2 rows elsewhere). */
 int ads_acoustic_plan_autotune(ads_acoustic_plan *plan, const void *model, const void *srcv,
                                void *workspace, size_t ws_bytes, int64_t K, int reps,
                                int *shape_out, ads_stream_t stream);

@@ -75,17 +75,20 @@ def test_plan_rejects_bad_shapes_and_params(ads):
 
 
 def test_set_shape_validates_before_touching_the_plan(ads):
-    """Shapes are checked on the host: rows 1/2/4, shots >= 1, batch 1/2 (2 only with 1-2 rows)."""
+    """Shapes are checked on the host: rows 1/2/4 (the forward also 8/16, its streaming
+    rectangle), shots >= 1, batch 1/2 (2 only with 1-2 rows)."""
     import ctypes as C
     import importlib
     L = importlib.import_module("adseismic_amd._lib")
     lib = L.load()
-    for bad in [(3, 1, 1, 1, 1, 1), (1, 0, 1, 1, 1, 1), (1, 1, 3, 1, 1, 1), (1, 1, 1, 4, 2, 2)]:
+    for bad in [(3, 1, 1, 1, 1, 1), (1, 0, 1, 1, 1, 1), (1, 1, 3, 1, 1, 1), (1, 1, 1, 4, 2, 2),
+                (1, 1, 1, 8, 1, 1), (8, 2, 2, 1, 1, 1), (12, 1, 1, 1, 1, 1)]:
         rc = lib.ads_acoustic_plan_set_shape(None, (C.c_int * 6)(*bad))
         assert rc == 1, bad
         assert b"must" in lib.ads_last_error() or b"needs" in lib.ads_last_error()
-    rc = lib.ads_acoustic_plan_set_shape(None, (C.c_int * 6)(1, 8, 2, 2, 8, 2))
-    assert rc == 1 and b"null plan" in lib.ads_last_error()  # a valid shape still needs a plan
+    for good in [(1, 8, 2, 2, 8, 2), (8, 16, 1, 2, 16, 1), (16, 4, 1, 1, 4, 1)]:
+        rc = lib.ads_acoustic_plan_set_shape(None, (C.c_int * 6)(*good))
+        assert rc == 1 and b"null plan" in lib.ads_last_error()  # a valid shape still needs a plan
 
 
 def test_cfl_check(ads):

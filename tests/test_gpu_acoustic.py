@@ -220,7 +220,8 @@ def test_plan_grid_splits_f64(orc, ads, cuda, kernel, split):
 def test_two_level_blocking_is_bit_identical(orc, ads, cuda, kernel, dtype):
     """The forward's temporal blocking (two levels per interior-rectangle pass, the default)
     gives the single-level traces, loss and gradient bit for bit: over the grid splits plus a
-    grid wide and tall enough for several column segments and row strips, three kernel shapes,
+    grid wide and tall enough for several column segments and row strips, five kernel shapes
+    (the last two with the streaming rectangle, 8 and 16 rows per wavefront),
     two checkpoint intervals, an odd record length and uneven records.  A plan with a source
     inside the rectangle (the last case) steps one level at a time."""
     cases = SPLITS + [dict(NX=70, NY=300, npml=8, use=(True, True, True, True)),
@@ -242,7 +243,8 @@ def test_two_level_blocking_is_bit_identical(orc, ads, cuda, kernel, dtype):
         srcv = plan.pack_srcv([s.srcv for s in srcs])
         ct = T(c, dtype)
         obs = plan.forward(ct * 1.02, srcv, 9).clone()
-        for shape in [(1, 2, 1, 1, 2, 1), (2, 3, 1, 2, 3, 1), (4, 5, 1, 1, 5, 1)]:
+        for shape in [(1, 2, 1, 1, 2, 1), (2, 3, 1, 2, 3, 1), (4, 5, 1, 1, 5, 1),
+                      (8, 3, 1, 2, 3, 1), (16, 5, 1, 1, 5, 1)]:
             plan.set_shape(shape)
             for K in (5, 1000):
                 res = []
@@ -355,8 +357,9 @@ def test_choose_interval_minimises_recomputation(orc, ads, cuda):
 
 @pytest.mark.parametrize("kernel", [0, 1])
 def test_kernel_shapes_and_autotune_agree(orc, ads, cuda, kernel):
-    """Every (rows per wavefront, shots per workgroup, shots per batch) shape, and the autotuned
-    one, gives the oracle's gradient (shot groups change only the order of the dL/dC sum)."""
+    """Every (rows per wavefront, shots per workgroup, shots per batch) shape, the streaming
+    forward rectangle (8 and 16 rows) included, and the autotuned one, gives the oracle's
+    gradient (shot groups change only the order of the dL/dC sum)."""
     p, c, srcs, rcvs, nstep = make_case(orc, ads, NX=37, NY=70, NSTEP=50, nshots=5, kernel=kernel,
                                         seed=11, uneven=True)
     obs, g_ref = [], np.zeros_like(c)
@@ -368,12 +371,12 @@ def test_kernel_shapes_and_autotune_agree(orc, ads, cuda, kernel):
     srcv = plan.pack_srcv([s.srcv for s in srcs])
     ct, ob = T(c), plan.pack_traces(obs)
     for shape in [(2, 1, 1, 2, 1, 1), (4, 2, 1, 1, 3, 2), (1, 5, 2, 4, 2, 1), (2, 32, 2, 2, 32, 2),
-                  (1, 3, 2, 1, 5, 2)]:
+                  (1, 3, 2, 1, 5, 2), (8, 2, 1, 2, 2, 1), (16, 4, 1, 1, 4, 1)]:
         plan.set_shape(shape)
         _, g, _ = plan.gradient(ct, srcv, ob, 6)
         assert rel(g.cpu(), g_ref) < 1e-10, (shape, rel(g.cpu(), g_ref))
     shape = plan.autotune(ct, srcv, 6, reps=1)
-    assert len(shape) == 6 and shape[0] in (1, 2, 4) and shape[3] in (1, 2, 4)
+    assert len(shape) == 6 and shape[0] in (1, 2, 4, 8) and shape[3] in (1, 2, 4)
     assert shape[2] in (1, 2) and shape[5] in (1, 2)
     _, g, _ = plan.gradient(ct, srcv, ob, 6)
     assert rel(g.cpu(), g_ref) < 1e-10
