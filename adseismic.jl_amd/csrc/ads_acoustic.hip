@@ -1159,6 +1159,17 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_pair(const FwdArgs<T> a, 
     if (b < nfb) fwd_frame2_body<T>(a, u2, phi2, psi2, R, fmap, items, nitems, b);
     else fwd_rect2_body<T, RPT, VEC>(a, u2, R, b - nfb);
 }
+// Levels n and n+1 of the rectangle (streaming body) with level n of the frame (kernel 0).
+template <typename T, int VEC, int ROWS, int V>
+__global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_level2_s(const FwdArgs<T> a, const Lv<T> u2,
+                                                                 const Rect R, const TileMap fmap,
+                                                                 const int4 *__restrict__ items,
+                                                                 int nitems, int nfb)
+{
+    const int b = (int)blockIdx.x;
+    if (b < nfb) fwd_frame_body<T, V>(a, fmap, items, nitems, b);
+    else fwd_rect2s_body<T, VEC, ROWS>(a, u2, R, b - nfb);
+}
 // The same with the streaming rectangle body (ROWS rows per wavefront, one shot each).
 template <typename T, int VEC, int ROWS>
 __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_pair_s(const FwdArgs<T> a, const Lv<T> u2,
@@ -1706,6 +1717,23 @@ template <typename T> struct AcPlan : AcPlanBase {
     // sources added by the frame work: one launch (two-level rectangle + two-level frame).
     // Otherwise: [rectangle levels n, n+1 + frame level n], phi/psi and sources of level n,
     // [frame of the shrunk rectangle, level n+1], phi/psi and sources of level n+1.
+    template <int ROWS>
+    void launch_level2_s(FwdArgs<T> a, Lv<T> u2, hipStream_t st)
+    {
+        const Rect &R = rect[0];
+        a.map.ntj = (int)cdiv(R.jR - R.jL, WAVE * VECP);
+        a.map.nstrips = (int)cdiv(R.iR - R.iL, ROWS);
+        a.map.ngroups = (int)cdiv(a.act, NWAVES);
+        a.map.nb = a.map.ntj * a.map.nstrips * a.map.ngroups;
+        const TileMap fm = frame_tiles(0, a.act, a.Sg);
+        const int nfb = (int)grid_of(fm);
+        const unsigned nb = (unsigned)nfb + grid_of(a.map);
+        const int4 *it = d_items[0].template as<int4>();
+        if (variant == 1)
+            ac_fwd_level2_s<T, VECP, ROWS, 1><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, R, fm, it, nitems[0], nfb);
+        else
+            ac_fwd_level2_s<T, VECP, ROWS, 0><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, R, fm, it, nitems[0], nfb);
+    }
     template <int RPT>
     void launch_level2(FwdArgs<T> a, Lv<T> u2, hipStream_t st)
     {
@@ -1776,6 +1804,8 @@ template <typename T> struct AcPlan : AcPlanBase {
         int pr = prof_start(ADS_PROF_AC_FWD, st);
         if (shape.rpt == 1) launch_level2<1>(a, slot(w, n + 1), st);
         else if (shape.rpt == 4) launch_level2<4>(a, slot(w, n + 1), st);
+        else if (shape.rpt == 8) launch_level2_s<8>(a, slot(w, n + 1), st);
+        else if (shape.rpt == 16) launch_level2_s<16>(a, slot(w, n + 1), st);
         else launch_level2<2>(a, slot(w, n + 1), st);
         prof_stop(pr, st, fwd_level_bytes(act));
         fwd_post(L, a, n, srcv, traces, record, st);
