@@ -176,12 +176,14 @@ __device__ inline void fwd_rect_body(const FwdArgs<T> &a, const Rect &R, int bid
 
 // Two-level rectangle forward (temporal blocking).  From w = u^{n-1} and wold = u^{n-2} one pass
 // writes u^n on the rectangle R and u^{n+1} on R shrunk by one cell; the ring of R at level n+1
-// is left to that level's frame launch.  A wavefront owns RPT rows of u^{n+1}: it recomputes u^n
+// is left to the frame work (the two-level frame body in the same launch, or a frame launch for
+// level n+1 with kernel 0).  A wavefront owns RPT rows of u^{n+1}: it recomputes u^n
 // on one halo row above and below and, on its segment's edge lanes, one halo column on each
 // side, so no value crosses wavefronts.  Every value is the single-level formula in the same
 // order, so results equal two single-level steps bit for bit.  HBM per cell and two levels:
 // read u^{n-1} and u^{n-2}, write u^n and u^{n+1} (two thirds of the single-level traffic).
-// Valid only when no source lies in R (sources are added between the levels by ac_src_rcv).
+// Valid only when no source lies in R: level n must carry its sources before level n+1 reads it,
+// and the sources are added by the frame work or between the launches of a pair.
 template <typename T, int RPT, int VEC>
 __device__ inline void fwd_rect2_body(const FwdArgs<T> &a, const Lv<T> &u2, const Rect &R, int bid)
 {
@@ -1310,7 +1312,8 @@ struct AcPlanBase {
     DevBuf d_items[2];
     int nitems[2] = {0, 0};
     // Two-level forward (temporal blocking) in the pitched layout: rect1 is rect[0] shrunk by
-    // one cell and items1 the frame items of its complement (level n+1's frame launch).
+    // one cell and items1 the frame items of its complement (level n+1's frame launch of a
+    // kernel-0 pair).
     // blk_ok: rect1 is not empty and no source lies in rect[0]; blocking: levels per
     // rectangle pass asked for (1 or 2).
     Rect rect1{};
