@@ -684,6 +684,128 @@ __device__ inline void fwd_frame2_body(const FwdArgs<T> &a, const Lv<T> &u2, con
     }
 }
 
+// Frame work of kernel 0 with its phi/psi update and its sources (one level; used when the
+// sources come from the per-cell table).  Items of at most WAVE - 2 columns [m.z, m.w); lane L
+// holds column m.z - 1 + L.  Every lane computes u^n before the sources at its column on the
+// item's rows and one row above and below; the item's lanes then update phi/psi from those
+// values (the expressions of ac_fwd_phipsi) and store u^n with the sources added (those of
+// ac_src_rcv), so neither pass is launched.
+template <typename T>
+__device__ inline void fwd_frame0h_body(const FwdArgs<T> &a, const TileMap &map,
+                                        const int4 *__restrict__ items, int nitems, int bid)
+{
+    constexpr int NB = FRPT + 2;
+    int g, blk, tj;
+    if (!decode_tile(map, g, blk, tj, bid)) return;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int it = blk;
+    if (it >= nitems) return;  // workgroup-uniform
+    const int4 m = items[it];
+    const int ib = m.x, nr = m.y, j = m.z - 1 + lane;
+    const int NX = a.NX, NY = a.NY, P = a.P;
+    const int jlast = min(m.w, NY + 1);
+    const bool act = j >= 0 && j <= jlast;
+    const bool own = act && lane >= 1 && j < m.w;
+    const bool le = lane == 0 && act && j >= 1;
+    const bool re = act && j == jlast;
+    const bool reok = re && j + 1 <= NY + 1;
+    const AcCoef<T> k = a.k;
+    const bool jin = j >= 1 && j <= NY;
+    const T tu = act ? a.ty[j] : T(0);
+    const bool ny = act && a.neary[j] != 0;
+    bool rin[NB], nx[NB], lvl[NB];
+    T sg[NB], Cr[NB];
+    int cidx[NB];
+    bool lp = false, lpi = false;
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+        const int i = ib - 1 + q;
+        lvl[q] = q <= nr + 1 && i >= 0 && i <= NX + 1;
+        rin[q] = lvl[q] && i >= 1 && i <= NX;
+        nx[q] = lvl[q] && a.nearx[i] != 0;
+        sg[q] = lvl[q] ? a.sx[i] : T(0);
+        Cr[q] = act && rin[q] ? a.C[(int64_t)i * P + j] : T(0);
+        cidx[q] = own && q >= 1 && q <= nr && a.src.cell_idx ? a.src.cell_idx[(int64_t)i * P + j] : -1;
+        const bool nr_ = rin[q] && jin && (nx[q] || ny);
+        lp |= nr_;
+        if (q >= 1 && q <= nr) lpi |= nr_;
+    }
+    lp = lp && act;
+    lpi = lpi && act;
+    const int s1 = min(a.act, (g + 1) * a.Sg);
+    for (int s = g * a.Sg + wv; s < s1; s += NWAVES) {
+        const T *__restrict__ w = a.w.at(s);
+        const T *__restrict__ wo = a.wold.at(s);
+        const T *__restrict__ phi = a.phi.at(s);
+        const T *__restrict__ psi = a.psi.at(s);
+        T W[NB + 2], PH[NB + 2], WO[NB], PS[NB], EL[NB], ER[NB], SL[NB], SR[NB];
+#pragma unroll
+        for (int q = 0; q < NB + 2; q++) {
+            const int i = ib - 2 + q;
+            const bool ok = act && q <= nr + 3 && i >= 0 && i <= NX + 1;
+            W[q] = ok ? w[(int64_t)i * P + j] : T(0);
+            PH[q] = ok && lp ? phi[(int64_t)i * P + j] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int64_t row = (int64_t)(ib - 1 + q) * P;
+            WO[q] = act && rin[q] ? wo[row + j] : T(0);
+            PS[q] = lp && rin[q] ? psi[row + j] : T(0);
+            EL[q] = le && rin[q] ? w[row + j - 1] : T(0);
+            ER[q] = reok && rin[q] ? w[row + j + 1] : T(0);
+            SL[q] = le && lp && rin[q] ? psi[row + j - 1] : T(0);
+            SR[q] = reok && lp && rin[q] ? psi[row + j + 1] : T(0);
+        }
+        T Un[NB];  // u^n before the sources
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            T wv1[1] = {W[q + 1]}, pv1[1] = {PS[q]};
+            T ww, we, pw, pe;
+            lr_of(wv1, EL[q], ER[q], lane, re, ww, we);
+            lr_of(pv1, SL[q], SR[q], lane, re, pw, pe);
+            const T wc = W[q + 1], wn = W[q], ws = W[q + 2];
+            const T lap = k.cx * (ws + wn - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
+            const T s_ = sg[q], Cij = Cr[q];
+            const bool in = rin[q] && jin;
+            const bool near = nx[q] || ny;
+            const T plain = T(2) * wc - WO[q] + Cij * lap;
+            const T num = (T(2) - s_ * tu * k.dt2) * wc + Cij * lap + k.ex * (PH[q + 2] - PH[q]) +
+                          k.ey * (pe - pw) - (T(1) - (s_ + tu) * k.hdt) * WO[q];
+            const T out = in ? (near ? num / (T(1) + (s_ + tu) * k.hdt) : plain) : T(0);
+            Un[q] = act && lvl[q] ? out : T(0);
+        }
+        T *__restrict__ u = a.u.at(s);
+#pragma unroll
+        for (int r = 0; r < FRPT; r++) {
+            if (r >= nr) break;  // uniform
+            const int q = r + 1, i = ib + r;
+            const int64_t ij = (int64_t)i * P + j;
+            T uv1[1] = {Un[q]};
+            T uw, ue;
+            lr_of(uv1, T(0), T(0), lane, false, uw, ue);
+            if (own && lpi) {
+                const T s_ = sg[q], Cij = Cr[q];
+                const bool pml = rin[q] && jin && (s_ != T(0) || tu != T(0));
+                a.phio.at(s)[ij] = pml ? (T(1) - k.dt * s_) * PH[q + 1] + Cij * (tu - s_) * k.fx * (Un[q + 1] - Un[q - 1]) : T(0);
+                a.psio.at(s)[ij] = pml ? (T(1) - k.dt * tu) * PS[q] + Cij * (s_ - tu) * k.fy * (ue - uw) : T(0);
+            }
+            T out = Un[q];
+            if (cidx[q] >= 0) out = add_sources(a.src, cidx[q], s, out, k.dt2);
+            if (own) u[ij] = out;
+        }
+    }
+}
+
+// The frame work of a level: V = 0, 1 the PropagatorKernel; V = 2 kernel 0 with its phi/psi
+// update and sources in the frame work (fwd_frame0h_body, items of WAVE - 2 columns).
+template <typename T, int V>
+__device__ inline void frame_work(const FwdArgs<T> &a, const TileMap &map, const int4 *__restrict__ items,
+                                  int nitems, int bid)
+{
+    if constexpr (V == 2) fwd_frame0h_body<T>(a, map, items, nitems, bid);
+    else fwd_frame_body<T, V>(a, map, items, nitems, bid);
+}
+
 // kernel 0: phi^n/psi^n from the new u (before sources), PML cells only.  The PML lies in the
 // frame: one workgroup per (frame item, 4 shots), a wavefront per shot, a lane per column.
 template <typename T>
@@ -1125,7 +1247,7 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_frame(const FwdArgs<T> a,
                                                               const int4 *__restrict__ items,
                                                               int nitems)
 {
-    fwd_frame_body<T, V>(a, a.map, items, nitems, (int)blockIdx.x);
+    frame_work<T, V>(a, a.map, items, nitems, (int)blockIdx.x);
 }
 template <typename T, int RPT, int VEC, int SB, int V>
 __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_level(const FwdArgs<T> a, const Rect R,
@@ -1134,7 +1256,7 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_level(const FwdArgs<T> a,
                                                               int nitems, int nfb)
 {
     const int b = (int)blockIdx.x;
-    if (b < nfb) fwd_frame_body<T, V>(a, fmap, items, nitems, b);
+    if (b < nfb) frame_work<T, V>(a, fmap, items, nitems, b);
     else fwd_rect_body<T, RPT, VEC, SB>(a, R, b - nfb);
 }
 // Levels n and n+1 of the rectangle (two-level body) with level n of the frame, in one launch.
@@ -1145,7 +1267,7 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_level2(const FwdArgs<T> a
                                                                int nitems, int nfb)
 {
     const int b = (int)blockIdx.x;
-    if (b < nfb) fwd_frame_body<T, V>(a, fmap, items, nitems, b);
+    if (b < nfb) frame_work<T, V>(a, fmap, items, nitems, b);
     else fwd_rect2_body<T, RPT, VEC>(a, u2, R, b - nfb);
 }
 // Levels n and n+1 of the whole grid in one launch: the two-level rectangle body and the
@@ -1169,7 +1291,7 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_level2_s(const FwdArgs<T>
                                                                  int nitems, int nfb)
 {
     const int b = (int)blockIdx.x;
-    if (b < nfb) fwd_frame_body<T, V>(a, fmap, items, nitems, b);
+    if (b < nfb) frame_work<T, V>(a, fmap, items, nitems, b);
     else fwd_rect2s_body<T, VEC, ROWS>(a, u2, R, b - nfb);
 }
 // The same with the streaming rectangle body (ROWS rows per wavefront, one shot each).
@@ -1327,6 +1449,10 @@ struct AcPlanBase {
     // items of the two-level frame pass (complement of rect1, WAVE - 2 columns each)
     DevBuf d_items2;
     int nitems2 = 0;
+    // kernel 0 with the sources in the frame work: items of WAVE - 2 columns covering the
+    // complement of rect[0] and of rect1, for the frame body that also updates phi/psi
+    DevBuf d_items_h, d_items1h;
+    int nitems_h = 0, nitems1h = 0;
     DevBuf d_sc_idx, d_sc_ptr, d_sc_shot, d_sc_base, d_sc_ns, d_nstep_sorted;
     // hipGraph replay of whole checkpoint segments (forward sweep, recomputation, adjoint
     // sweep), keyed by everything the captured launches depend on
@@ -1392,6 +1518,21 @@ template <typename T> struct AcPlan : AcPlanBase {
         m.nb = m.ntj * m.nstrips * m.ngroups;
         return m;
     }
+    static TileMap item_tiles(int n, int act, int sg)
+    {
+        TileMap m;
+        m.ntj = 1;
+        m.nstrips = n;
+        m.ngroups = (int)cdiv(act, sg);
+        m.nb = m.ntj * m.nstrips * m.ngroups;
+        return m;
+    }
+    // kernel 0 in the pitched layout with the sources from the per-cell table: its frame work
+    // also updates phi/psi and adds the sources (fwd_frame0h_body)
+    bool frame_h(const FwdArgs<T> &a, int li) const
+    {
+        return variant == 0 && li == 0 && a.src.cell_idx != nullptr && nitems_h > 0;
+    }
     TileMap frame_tiles(int li, int act, int sg) const
     {
         TileMap m;
@@ -1410,14 +1551,18 @@ template <typename T> struct AcPlan : AcPlanBase {
         a.map = rect_tiles(R, VEC, RPT, a.act, a.Sg);
         if constexpr (VEC > 1) {
             if (fuse) {
-                const TileMap fm = frame_tiles(li, a.act, a.Sg);
+                const bool h = frame_h(a, li);
+                const int ni = h ? nitems_h : nitems[li];
+                const TileMap fm = item_tiles(ni, a.act, a.Sg);
                 const int nfb = (int)grid_of(fm);
                 const unsigned nb = (unsigned)nfb + grid_of(a.map);
-                const int4 *it = d_items[li].template as<int4>();
+                const int4 *it = h ? d_items_h.template as<int4>() : d_items[li].template as<int4>();
                 if (variant == 1)
-                    ac_fwd_level<T, RPT, VEC, SB, 1><<<nb, WAVE * NWAVES, 0, st>>>(a, R, fm, it, nitems[li], nfb);
+                    ac_fwd_level<T, RPT, VEC, SB, 1><<<nb, WAVE * NWAVES, 0, st>>>(a, R, fm, it, ni, nfb);
+                else if (h)
+                    ac_fwd_level<T, RPT, VEC, SB, 2><<<nb, WAVE * NWAVES, 0, st>>>(a, R, fm, it, ni, nfb);
                 else
-                    ac_fwd_level<T, RPT, VEC, SB, 0><<<nb, WAVE * NWAVES, 0, st>>>(a, R, fm, it, nitems[li], nfb);
+                    ac_fwd_level<T, RPT, VEC, SB, 0><<<nb, WAVE * NWAVES, 0, st>>>(a, R, fm, it, ni, nfb);
                 return;
             }
         }
@@ -1470,10 +1615,13 @@ template <typename T> struct AcPlan : AcPlanBase {
         if (L.li == 0) launch_fwd_shape<VECP>(a, rect[0], sh, 0, fuse, st);
         else launch_fwd_shape<1>(a, rect[1], sh, 1, false, st);
         if (fuse || nitems[L.li] == 0) return;
-        a.map = frame_tiles(L.li, a.act, a.Sg);
-        const int4 *it = d_items[L.li].template as<int4>();
-        if (variant == 1) ac_fwd_frame<T, 1><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, it, nitems[L.li]);
-        else ac_fwd_frame<T, 0><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, it, nitems[L.li]);
+        const bool h = frame_h(a, L.li);
+        const int ni = h ? nitems_h : nitems[L.li];
+        a.map = item_tiles(ni, a.act, a.Sg);
+        const int4 *it = h ? d_items_h.template as<int4>() : d_items[L.li].template as<int4>();
+        if (variant == 1) ac_fwd_frame<T, 1><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, it, ni);
+        else if (h) ac_fwd_frame<T, 2><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, it, ni);
+        else ac_fwd_frame<T, 0><<<grid_of(a.map), WAVE * NWAVES, 0, st>>>(a, it, ni);
     }
     void launch_adj_all(AdjArgs<T> a, const Layout &L, const Shape &sh, hipStream_t st)
     {
@@ -1728,28 +1876,36 @@ template <typename T> struct AcPlan : AcPlanBase {
         a.map.nstrips = (int)cdiv(R.iR - R.iL, ROWS);
         a.map.ngroups = (int)cdiv(a.act, NWAVES);
         a.map.nb = a.map.ntj * a.map.nstrips * a.map.ngroups;
-        const TileMap fm = frame_tiles(0, a.act, a.Sg);
+        const bool h = frame_h(a, 0);
+        const int ni = h ? nitems_h : nitems[0];
+        const TileMap fm = item_tiles(ni, a.act, a.Sg);
         const int nfb = (int)grid_of(fm);
         const unsigned nb = (unsigned)nfb + grid_of(a.map);
-        const int4 *it = d_items[0].template as<int4>();
+        const int4 *it = h ? d_items_h.template as<int4>() : d_items[0].template as<int4>();
         if (variant == 1)
-            ac_fwd_level2_s<T, VECP, ROWS, 1><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, R, fm, it, nitems[0], nfb);
+            ac_fwd_level2_s<T, VECP, ROWS, 1><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, R, fm, it, ni, nfb);
+        else if (h)
+            ac_fwd_level2_s<T, VECP, ROWS, 2><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, R, fm, it, ni, nfb);
         else
-            ac_fwd_level2_s<T, VECP, ROWS, 0><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, R, fm, it, nitems[0], nfb);
+            ac_fwd_level2_s<T, VECP, ROWS, 0><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, R, fm, it, ni, nfb);
     }
     template <int RPT>
     void launch_level2(FwdArgs<T> a, Lv<T> u2, hipStream_t st)
     {
         const Rect &R = rect[0];
         a.map = rect_tiles(R, VECP, RPT, a.act, a.Sg);
-        const TileMap fm = frame_tiles(0, a.act, a.Sg);
+        const bool h = frame_h(a, 0);
+        const int ni = h ? nitems_h : nitems[0];
+        const TileMap fm = item_tiles(ni, a.act, a.Sg);
         const int nfb = (int)grid_of(fm);
         const unsigned nb = (unsigned)nfb + grid_of(a.map);
-        const int4 *it = d_items[0].template as<int4>();
+        const int4 *it = h ? d_items_h.template as<int4>() : d_items[0].template as<int4>();
         if (variant == 1)
-            ac_fwd_level2<T, RPT, VECP, 1><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, R, fm, it, nitems[0], nfb);
+            ac_fwd_level2<T, RPT, VECP, 1><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, R, fm, it, ni, nfb);
+        else if (h)
+            ac_fwd_level2<T, RPT, VECP, 2><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, R, fm, it, ni, nfb);
         else
-            ac_fwd_level2<T, RPT, VECP, 0><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, R, fm, it, nitems[0], nfb);
+            ac_fwd_level2<T, RPT, VECP, 0><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, R, fm, it, ni, nfb);
     }
     // Levels n and n+1 of the whole grid in one launch (kernel 1 with the frame adding sources)
     template <int RPT>
@@ -1817,13 +1973,13 @@ template <typename T> struct AcPlan : AcPlanBase {
                                 n + 1, srcv);
         pr = prof_start(ADS_PROF_AC_FWD, st);
         if (nitems1 > 0) {
-            b.map.ntj = 1;
-            b.map.nstrips = nitems1;
-            b.map.ngroups = (int)cdiv(act, b.Sg);
-            b.map.nb = b.map.ntj * b.map.nstrips * b.map.ngroups;
-            const int4 *it = d_items1.template as<int4>();
-            if (variant == 1) ac_fwd_frame<T, 1><<<grid_of(b.map), WAVE * NWAVES, 0, st>>>(b, it, nitems1);
-            else ac_fwd_frame<T, 0><<<grid_of(b.map), WAVE * NWAVES, 0, st>>>(b, it, nitems1);
+            const bool h = frame_h(b, 0);
+            const int ni = h ? nitems1h : nitems1;
+            b.map = item_tiles(ni, act, b.Sg);
+            const int4 *it = h ? d_items1h.template as<int4>() : d_items1.template as<int4>();
+            if (variant == 1) ac_fwd_frame<T, 1><<<grid_of(b.map), WAVE * NWAVES, 0, st>>>(b, it, ni);
+            else if (h) ac_fwd_frame<T, 2><<<grid_of(b.map), WAVE * NWAVES, 0, st>>>(b, it, ni);
+            else ac_fwd_frame<T, 0><<<grid_of(b.map), WAVE * NWAVES, 0, st>>>(b, it, ni);
         }
         prof_stop(pr, st, fwd_level_bytes(act));
         fwd_post(L, b, n + 1, srcv, traces, record, st);
@@ -2273,7 +2429,7 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
             src_in |= si[e] >= R.iL && si[e] < R.iR && sj[e] >= R.jL && sj[e] < R.jR;
         P->blk_ok = !src_in && P->rect1.iR > P->rect1.iL && P->rect1.jR > P->rect1.jL;
         // sources inside the frame work: per source cell, its entries in (shot, source) order
-        P->src_fused = P->variant == 1 && !src_in;
+        P->src_fused = !src_in;
         if (P->src_fused) {
             struct Ent {
                 int64_t cell;
@@ -2312,10 +2468,20 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
         frame_items(P->blk_ok ? P->rect1 : Rect{1, 1, 1, 1}, p->NX, p->NY, items);
         P->nitems1 = P->blk_ok ? (int)items.size() : 0;
         if (P->blk_ok) ADS_TRY(P->d_items1.upload(items));
-        if (P->blk_ok && P->src_fused) {
+        if (P->blk_ok && P->src_fused && P->variant == 1) {
             frame_items(P->rect1, p->NX, p->NY, items, FRPT, WAVE - 2);
             P->nitems2 = (int)items.size();
             ADS_TRY(P->d_items2.upload(items));
+        }
+        if (P->src_fused && P->variant == 0) {
+            frame_items(P->rect[0], p->NX, p->NY, items, FRPT, WAVE - 2);
+            P->nitems_h = (int)items.size();
+            ADS_TRY(P->d_items_h.upload(items));
+            if (P->blk_ok) {
+                frame_items(P->rect1, p->NX, p->NY, items, FRPT, WAVE - 2);
+                P->nitems1h = (int)items.size();
+                ADS_TRY(P->d_items1h.upload(items));
+            }
         }
     }
     ADS_TRY(P->d_pmlx.upload(P->pmlx));
