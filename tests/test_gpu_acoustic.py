@@ -168,6 +168,28 @@ def test_plan_forward_and_gradient_f64(orc, ads, cuda, kernel, K):
     assert rel(gb.cpu(), g_ref) < 1e-10
 
 
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_shots_without_sources(orc, ads, cuda, kernel):
+    """A shot with no source records zeros and adds nothing to the gradient, beside shots with
+    sources in the same plan (the frame work's source table has no entry for it)."""
+    p, c, srcs, rcvs, nstep = make_case(orc, ads, nshots=3, kernel=kernel, seed=17)
+    srcs[1] = ads.AcousticSource(np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros((p.NSTEP, 0)))
+    obs, g_ref = [], np.zeros_like(c)
+    for s in (0, 2):
+        o = oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s], cmod=c * 1.03)
+        g_ref += oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s], obs=o)[2]
+        obs.append(o)
+    obs.insert(1, np.zeros((p.NSTEP + 1, len(rcvs[1].rcvi))))
+    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float64)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    per = plan.unpack_traces(plan.forward(T(c), srcv, 9))
+    assert not per[1].abs().max().item()
+    for s in (0, 2):
+        assert rel(per[s].cpu(), oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s])) < 1e-12, s
+    _, g, _ = plan.gradient(T(c), srcv, plan.pack_traces(obs), 9)
+    assert rel(g.cpu(), g_ref) < 1e-10
+
+
 # interior rectangle / PML frame splits: partial PML sides, odd sizes, an interior narrower than
 # one vector, no PML at all, and a PML that covers the whole grid (empty rectangle)
 SPLITS = [
