@@ -1415,7 +1415,8 @@ struct AcPlanBase {
     std::vector<double> sx, ty;
     std::vector<uint8_t> pmlx, pmly, nearx, neary;
     double f_pml = 0.0;  // fraction of interior cells inside the PML strips
-    Shape fwd_shape{8, 8, 1}, adj_shape;  // forward: the streaming two-level rectangle
+    // defaults: the benchmark shape (forward: the streaming two-level rectangle)
+    Shape fwd_shape{8, 16, 1}, adj_shape{2, 16, 1};
     int tuned = 0;
     // device tables; cell tables exist for the pitched (plan) and dense (history) layouts
     DevBuf d_sx, d_ty, d_pmlx, d_pmly, d_nearx, d_neary;

@@ -41,8 +41,9 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_acoustic_1000x1000_32shot
 # streaming two-level rectangle, one shot per wavefront; 16 shots per workgroup for the frame
 # items), adjoint 2 rows per wavefront, 16 shots per workgroup, 1 shot per batch.  The fastest
 # of the shapes timed back to back on one MI355X (forward 96-98 us and adjoint 165 us per level
-# at 1000 x 1000 x 32 shots; 4, 12 or 16 streaming rows within 1-4%); ads_acoustic_plan_autotune
-# picks shapes within a few percent of it.
+# at 1000 x 1000 x 32 shots; 4, 12 or 16 streaming rows within 1-4%; 32 shots per frame group
+# the same within noise, 8 and 4 slower); it is also the plan's default shape, and
+# ads_acoustic_plan_autotune picks shapes within a few percent of it.
 DEFAULT_SHAPE = "8,16,1,2,16,1"
 
 
