@@ -41,7 +41,12 @@ def compute_lame_parameters(*args):
     """compute_lame_parameters, src/Utils.jl:223-247.
 
     (NX, NY, vp, vs, rho) -> constant (NX+2, NY+2) lambda, mu, rho;
+    (param, vp, vs, rho) with an MPIElasticPropagatorParams -> the same on its (n+4)^2 block
+                          arrays (Utils.jl:243-245);
     (vp, vs, rho) arrays  -> lambda = rho (vp^2 - 2 vs^2), mu = rho vs^2, rho."""
+    if len(args) == 4:
+        param, vp, vs, rho = args
+        return compute_lame_parameters(param.n + 2, param.n + 2, vp, vs, rho)
     if len(args) == 5:
         NX, NY, vp, vs, rho = args
         shape = (NX + 2, NY + 2)

@@ -219,3 +219,213 @@ def test_subdomain_gradient_finite_differences(orc, param):
 
     fd = (L(base[param] + v) - L(base[param] - v)) / 2
     assert np.sum(g * v) == pytest.approx(fd, rel=1e-6)
+
+
+# ------------------------------------------------------------------------------------------
+# one-step probes of the full form (Core.jl:96-213, 231-407).  Each probe is built so that a
+# plausible slip in the oracle -- one CPML profile row or index swapped, one averaging
+# neighbour moved, one update range off by a cell, the right-edge CPML origin misplaced, the
+# sign of the memory recursion -- changes which cells move, not just a value.
+# Field order f[0..4] = vx, vy, sxx, syy, sxy; memory m[0..7] = mem[1..8] of Core.jl.
+# ------------------------------------------------------------------------------------------
+
+# 0-based update ranges (i_lo, i_hi, j_lo, j_hi) read off the getid() index sets of
+# fw1..fw4: fw1 ij = getid(2:NX, 3:NY+1) (Core.jl:100), fw2 getid(3:NX+1, 2:NY) (:134),
+# fw3 getid(3:NX+1, 3:NY+1) (:164), fw4 getid(2:NX, 2:NY) (:191); getid is 1-based.
+def _ranges(NX, NY):
+    fw1 = (1, NX - 1, 2, NY)
+    fw2 = (2, NX, 1, NY - 1)
+    fw3 = (2, NX, 2, NY)
+    fw4 = (1, NX - 1, 1, NY - 1)
+    field = {0: fw3, 1: fw4, 2: fw1, 3: fw1, 4: fw2}            # vx, vy, sxx, syy, sxy
+    mem = {0: fw1, 1: fw1, 2: fw2, 3: fw2, 4: fw3, 5: fw3, 6: fw4, 7: fw4}
+    return field, mem
+
+
+def _mask(NX, NY, r):
+    m = np.zeros((NX + 2, NY + 2), bool)
+    m[r[0]:r[1] + 1, r[2]:r[3] + 1] = True
+    return m
+
+
+def _probe_setup(NX, NY, seed):
+    rng = np.random.default_rng(seed)
+    rho, lam, mu = _model(NX, NY, seed=seed)
+    N = (NX + 2) * (NY + 2)
+    f = rng.standard_normal(5 * N) * np.repeat([1e-3, 1e-3, 1e5, 1e5, 1e5], N)
+    m = rng.standard_normal(8 * N) * 1e-4
+    return rho, lam, mu, f, m
+
+
+def _ident(n):
+    return np.zeros(2 * n), np.ones(2 * n)         # a = 0, b = 1: the CPML is the identity
+
+
+def _step(orc, NX, NY, rho, lam, mu, ax, bx, ay, by, f, m):
+    f1, m1 = orc.elastic_step(NX, NY, 10.0, 12.0, 1e-3, rho, lam, mu, ax, bx, ay, by, f.copy(),
+                              m.copy(), [2], [2], [0], [0.0])
+    sh = (NX + 2, NY + 2)
+    return f1.reshape((5,) + sh), m1.reshape((8,) + sh)
+
+
+def test_update_ranges_and_memory_ranges_are_the_reference_index_sets(orc):
+    """Identity CPML, random fields and memory variables: after one step every field changes
+    on every cell of its fw range and is bit-unchanged outside it; every memory variable equals
+    its old value on its range (b = 1, a = 0) and is 0 outside (makevector scatters into
+    zeros, Core.jl:697-699)."""
+    NX, NY = 13, 11
+    rho, lam, mu, f, m = _probe_setup(NX, NY, 11)
+    ax, bx = _ident(NX)
+    ay, by = _ident(NY)
+    F, Mv = _step(orc, NX, NY, rho, lam, mu, ax, bx, ay, by, f, m)
+    sh = (NX + 2, NY + 2)
+    f0, m0 = f.reshape((5,) + sh), m.reshape((8,) + sh)
+    fr, mr = _ranges(NX, NY)
+    for k, r in fr.items():
+        ins = _mask(NX, NY, r)
+        assert np.all(F[k][~ins] == f0[k][~ins]), f"field {k} moved outside its range"
+        assert np.all(F[k][ins] != f0[k][ins]), f"field {k} did not move inside its range"
+    for k, r in mr.items():
+        ins = _mask(NX, NY, r)
+        assert np.all(Mv[k][ins] == m0[k][ins]), f"mem[{k + 1}] changed under b = 1, a = 0"
+        assert np.all(Mv[k][~ins] == 0.0), f"mem[{k + 1}] not zero outside its range"
+
+
+# which memory variables each CPML profile row drives (Core.jl:116-117, 146-147, 174-175,
+# 205-206): bx/ax row 1 (full grid) -> mem[3], mem[5]; row 2 (half grid) -> mem[1], mem[7];
+# by/ay row 1 -> mem[2], mem[6]; row 2 -> mem[4], mem[8]  (0-based memory indices below)
+_PROFILE_MEMS = {("x", 0): (2, 4), ("x", 1): (0, 6), ("y", 0): (1, 5), ("y", 1): (3, 7)}
+
+
+@pytest.mark.parametrize("axis,row", sorted(_PROFILE_MEMS))
+@pytest.mark.parametrize("coef", ["a", "b"])
+def test_each_cpml_profile_drives_its_own_memory_variables(orc, axis, row, coef):
+    """One profile entry (row `row`, index k) is made non-identity, all others stay a = 0,
+    b = 1.  Exactly the two memory variables the reference assigns to that row change, and only
+    on the cells whose profile index i-1 (x) or j-1 (y) equals k (adbroadcast, Core.jl:675-682):
+    b = 0.5 halves the old value there, a = 0.25 adds a quarter of the derivative there."""
+    NX, NY = 14, 12
+    k = 5
+    rho, lam, mu, f, m = _probe_setup(NX, NY, 12)
+    if coef == "a":
+        m = np.zeros_like(m)
+    ax, bx = _ident(NX)
+    ay, by = _ident(NY)
+    prof = {("x", "a"): ax, ("x", "b"): bx, ("y", "a"): ay, ("y", "b"): by}[(axis, coef)]
+    n = NX if axis == "x" else NY
+    prof[row * n + k] = 0.5 if coef == "b" else 0.25
+    F, Mv = _step(orc, NX, NY, rho, lam, mu, ax, bx, ay, by, f, m)
+    sh = (NX + 2, NY + 2)
+    m0 = m.reshape((8,) + sh)
+    _, mr = _ranges(NX, NY)
+    line = np.zeros(sh, bool)
+    if axis == "x":
+        line[k + 1, :] = True
+    else:
+        line[:, k + 1] = True
+    for q in range(8):
+        ins = _mask(NX, NY, mr[q])
+        moved = ins & (Mv[q] != m0[q])
+        if q in _PROFILE_MEMS[(axis, row)]:
+            np.testing.assert_array_equal(moved, ins & line, err_msg=f"mem[{q + 1}]")
+            if coef == "b":
+                np.testing.assert_array_equal(Mv[q][ins & line], 0.5 * m0[q][ins & line])
+        else:
+            assert not moved.any(), f"mem[{q + 1}] moved under a profile it does not use"
+
+
+def _moved(orc, NX, NY, rho, lam, mu, f, m, pml):
+    ax, bx, _, ay, by, _ = pml
+    return _step(orc, NX, NY, rho, lam, mu, ax, bx, ay, by, f, m)
+
+
+@pytest.mark.parametrize("param", ["lam", "mu", "rho"])
+def test_parameter_averaging_neighbours(orc, param):
+    """One cell (i0, j0) of lambda, mu or rho is perturbed; after one step the stresses and
+    velocities differ exactly on the cells whose update reads that parameter value:
+    lambda-bar, mu-bar of fw1 average (i, j) and (i+1, j) (Core.jl:109-110); mu-bar of fw2
+    averages (i, j) and (i, j+1) (:141); fw3 divides by rho(i, j) (:181); rho-bar of fw4 is the
+    mean of (i, j), (i+1, j), (i+1, j+1), (i, j+1) (:200).  fw1/fw2 do not read rho; fw3/fw4
+    read lambda and mu only through the new stresses, so v is checked for rho only."""
+    NX, NY = 15, 13
+    i0, j0 = 7, 6
+    rho, lam, mu, f, m = _probe_setup(NX, NY, 13)
+    pml = orc.elastic_pml(NX, NY, 10.0, 12.0, 1e-3, 4, 2.0, 1.0, 2 * np.pi * 5, 1e-3, 2500.0)
+    base = {"rho": rho, "lam": lam, "mu": mu}
+    pert = {k: v.copy() for k, v in base.items()}
+    pert[param][i0, j0] *= 1.001
+    F0, _ = _moved(orc, NX, NY, base["rho"], base["lam"], base["mu"], f, m, pml)
+    F1, _ = _moved(orc, NX, NY, pert["rho"], pert["lam"], pert["mu"], f, m, pml)
+
+    def cells(*ij):
+        s = np.zeros((NX + 2, NY + 2), bool)
+        for a, b in ij:
+            s[a, b] = True
+        return s
+
+    none = cells()
+    ij_x = cells((i0, j0), (i0 - 1, j0))                       # (i, j) and (i+1, j) averages
+    ij_y = cells((i0, j0), (i0, j0 - 1))                       # (i, j) and (i, j+1) averages
+    ij_4 = cells((i0, j0), (i0 - 1, j0), (i0 - 1, j0 - 1), (i0, j0 - 1))
+    expect = {"lam": {2: ij_x, 3: ij_x, 4: none},
+              "mu": {2: ij_x, 3: ij_x, 4: ij_y},
+              "rho": {2: none, 3: none, 4: none, 0: cells((i0, j0)), 1: ij_4}}[param]
+    for q, want in expect.items():
+        np.testing.assert_array_equal(F1[q] != F0[q], want, err_msg=f"{param}: field {q}")
+
+
+def test_right_edge_cpml_origin_mirrors_the_left_edge(orc):
+    """The right/top CPML origin is (NX - 0.5) h - thickness (Core.jl:256, 342) and profile
+    entry k sits at x = k h (full) or k h + h/2 (half).  The staggered grid then mirrors the
+    left edge onto the right with full and half grids exchanged: full[k] = half[NX-1-k] and
+    half[k] = full[NX-1-k] for a, b.  Spot values: the last half-grid entry is at the outer
+    edge (normalised abscissa 1, b = exp(-(d0 + alpha_max) dt)); the right strip's full grid
+    starts at NX - npml, its half grid has abscissa 0 (a = 0, b = 1) at NX - 1 - npml."""
+    for n, h in ((37, 10.0), (50, 7.5)):
+        npml, npower, amax, R, vref, dt = 6, 2.0, 2 * np.pi * 6, 1e-3, 2400.0, 8e-4
+        a, b, _ = orc.elastic_pml_axis(n, h, 0.0, n, npml, npower, 1.0, amax, R, vref, dt)
+        for q in (a, b):
+            np.testing.assert_allclose(q[0], q[1][::-1], rtol=1e-12, atol=1e-300)
+            np.testing.assert_allclose(q[1], q[0][::-1], rtol=1e-12, atol=1e-300)
+        d0 = -(npower + 1) * vref * np.log(R) / (2.0 * npml * h)
+        assert b[1, n - 1] == pytest.approx(np.exp(-(d0 + amax) * dt), rel=1e-13)
+        assert b[0, n - npml] < 1.0 and b[0, n - npml - 1] == 1.0 and a[0, n - npml - 1] == 0.0
+        assert b[1, n - 1 - npml] == 1.0 and a[1, n - 1 - npml] == 0.0
+        assert b[1, n - npml] < 1.0
+        # the right strip alone (no left PML) is the mirror of the left strip alone
+        al, bl, _ = orc.elastic_pml_axis(n, h, 0.0, n, npml, npower, 1.0, amax, R, vref, dt,
+                                         use_max=False)
+        ar, br, _ = orc.elastic_pml_axis(n, h, 0.0, n, npml, npower, 1.0, amax, R, vref, dt,
+                                         use_min=False)
+        np.testing.assert_allclose(bl[0], br[1][::-1], rtol=1e-12)
+        assert np.all(bl[:, n // 2:] == 1.0) and np.all(br[:, :n // 2] == 1.0)
+
+
+def test_energy_decays_in_the_cpml(orc):
+    """Homogeneous medium, explosive source in the middle: with the CPML on all four sides the
+    field energy falls to well below 1% of its peak once the waves have left the box; with the
+    CPML off (zero-field outer ring, a reflecting box) it stays of the order of its peak.  A
+    wrong sign or a swapped a/b in the memory recursion would not absorb (or would blow up)."""
+    NX = NY = 60
+    NSTEP = 700
+    hx = hy = 10.0
+    dt = 1e-3
+    vp, vs = 3000.0, 1700.0
+    rho = np.full((NX + 2, NY + 2), 2000.0)
+    lam, mu = rho * (vp ** 2 - 2 * vs ** 2), rho * vs ** 2
+    w = orc.ricker(NSTEP, 6.0, 25.0, 1e6)
+    src = ([31, 31], [31, 31], [2, 3], np.stack([w, w], 1))
+
+    def energy(use):
+        pml = orc.elastic_pml(NX, NY, hx, hy, dt, 12, 2.0, 1.0, 2 * np.pi * 10, 1e-3, vp, use=use)
+        hist, _ = orc.elastic_forward(NX, NY, NSTEP, hx, hy, dt, rho, lam, mu, pml, *src,
+                                      keep_mem=False)
+        kin = 0.5 * rho[None] * (hist[0] ** 2 + hist[1] ** 2)
+        strain = (hist[2] ** 2 + hist[3] ** 2 + 2 * hist[4] ** 2) / (4 * mu[None])
+        return (kin + strain).sum(axis=(1, 2))
+
+    e_pml = energy((True, True, True, True))
+    e_box = energy((False, False, False, False))
+    assert np.all(np.isfinite(e_pml))
+    assert e_pml[-1] < 1e-2 * e_pml.max(), e_pml[-1] / e_pml.max()
+    assert e_box[-1] > 0.2 * e_box.max(), e_box[-1] / e_box.max()
