@@ -6,6 +6,7 @@ immediately; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import itertools
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -142,6 +143,26 @@ def check(rc: int) -> None:
     if rc != ADS_OK:
         msg = load().ads_last_error()
         raise ADSError(rc, msg.decode() if msg else "")
+
+
+# Which call last filled a plan workspace.  A forward leaves checkpoints, the last segment's
+# levels and the model copy in the workspace for the backward that follows; any other call
+# on the same buffer (another forward, a gradient, an autotune, the backward itself, which
+# recomputes segments over the slot ring, or another plan sharing the buffer) overwrites them.
+_ws_state = {}
+_ws_count = itertools.count(1)
+
+
+def mark_workspace(ws) -> int:
+    """Records that the caller has just (re)filled ``ws``; returns its token."""
+    tok = next(_ws_count)
+    _ws_state[ws.data_ptr()] = tok
+    return tok
+
+
+def workspace_holds(ws, token) -> bool:
+    """True while ``ws`` still holds the state of the call that returned ``token``."""
+    return ws is not None and token is not None and _ws_state.get(ws.data_ptr()) == token
 
 
 def available_device_bytes() -> int:

@@ -128,6 +128,7 @@ class AcousticPlan:
         self.srcv_off, self.trace_off = so, to
         self._ws = None
         self._ws_K = None
+        self._tok = None  # workspace token of this plan's last forward (see _lib.mark_workspace)
         self._scratch = None
 
     def __del__(self):
@@ -191,7 +192,13 @@ class AcousticPlan:
         ws = self.workspace(K)
         _lib.check(self.lib.ads_acoustic_plan_forward(self.h, _vp(model), _vp(srcv), _vp(tr),
                                                       _vp(ws), ws.numel(), int(K), _stream()))
+        self._tok = _lib.mark_workspace(ws)
         return tr
+
+    def holds_forward(self, token, K: int) -> bool:
+        """True while the workspace still holds the state the forward that returned ``token``
+        (``self._tok`` right after it) left for its backward."""
+        return self._ws_K == K and _lib.workspace_holds(self._ws, token)
 
     def backward(self, model: torch.Tensor, srcv: torch.Tensor, grad_traces: torch.Tensor,
                  K: int) -> torch.Tensor:
@@ -203,6 +210,8 @@ class AcousticPlan:
         _lib.check(self.lib.ads_acoustic_plan_backward(self.h, _vp(model), _vp(srcv), _vp(gt),
                                                        _vp(g), _vp(self._ws), self._ws.numel(),
                                                        int(K), _stream()))
+        # the sweep recomputed segments over the slot ring: the forward's state is spent
+        _lib.mark_workspace(self._ws)
         return g
 
     def gradient(self, model: torch.Tensor, srcv: torch.Tensor, observed: torch.Tensor, K: int,
@@ -217,6 +226,7 @@ class AcousticPlan:
         _lib.check(self.lib.ads_acoustic_plan_gradient(self.h, _vp(model), _vp(srcv), _vp(obs),
                                                        _vp(loss), _vp(g), _vp(tr), _vp(ws),
                                                        ws.numel(), int(K), _stream()))
+        _lib.mark_workspace(ws)
         return loss, g, tr
 
     def autotune(self, model: torch.Tensor, srcv: torch.Tensor, K: int, reps: int = 3):
@@ -228,6 +238,7 @@ class AcousticPlan:
         _lib.check(self.lib.ads_acoustic_plan_autotune(self.h, _vp(model), _vp(srcv), _vp(ws),
                                                        ws.numel(), int(K), int(reps), out,
                                                        _stream()))
+        _lib.mark_workspace(ws)
         self.shape = tuple(out)
         return self.shape
 
@@ -304,14 +315,20 @@ class _TracesFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, model, plan, srcv, K):
         tr = plan.forward(model, srcv, K)
-        ctx.plan, ctx.K = plan, K
+        ctx.plan, ctx.K, ctx.tok = plan, K, plan._tok
         ctx.save_for_backward(model, srcv)
         return tr
 
     @staticmethod
     def backward(ctx, g):
         model, srcv = ctx.saved_tensors
-        return ctx.plan.backward(model, srcv, g.contiguous(), ctx.K), None, None, None
+        plan = ctx.plan
+        if not plan.holds_forward(ctx.tok, ctx.K):
+            # another call used the workspace since this forward (a second acoustic_traces on
+            # the plan, a gradient, an autotune, a plan sharing the buffer, or an earlier
+            # backward): rebuild this forward's checkpoints before the sweep
+            plan.forward(model, srcv, ctx.K)
+        return plan.backward(model, srcv, g.contiguous(), ctx.K), None, None, None
 
 
 def acoustic_traces(plan: AcousticPlan, model: torch.Tensor, srcv: torch.Tensor, K: int):

@@ -1467,13 +1467,34 @@ struct AcPlanBase {
                    std::tie(o.kind, o.record, o.lo, o.hi, o.ns, o.ws, o.srcv, o.traces, o.resid);
         }
     };
-    std::map<GraphKey, hipGraphExec_t> graphs;
+    // A cached graph also records its last use: the key holds the caller's trace / residual
+    // addresses, so fresh buffers on every FWI iteration would otherwise add graphs without
+    // bound.  The cache keeps at most graph_cap() graphs and evicts the least recently used.
+    struct CachedGraph {
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    std::map<GraphKey, CachedGraph> graphs;
+    uint64_t graph_clock = 0;
     hipStream_t cap_stream = nullptr;  // capture needs a stream other than the null stream
     int use_graph = 0;
     void drop_graphs()
     {
-        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        if (!graphs.empty()) (void)hipDeviceSynchronize();  // none may still be executing
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second.exec);
         graphs.clear();
+    }
+    // one gradient's worth of segment graphs (forward, recomputation and adjoint sweeps of M
+    // segments) for two buffer sets, so alternating calls keep their graphs
+    static size_t graph_cap(int64_t M) { return (size_t)std::max<int64_t>(16, 6 * M + 8); }
+    void evict_lru()
+    {
+        auto victim = graphs.begin();
+        for (auto it = graphs.begin(); it != graphs.end(); ++it)
+            if (it->second.used < victim->second.used) victim = it;
+        (void)hipDeviceSynchronize();  // the evicted graph may still be executing
+        (void)hipGraphExecDestroy(victim->second.exec);
+        graphs.erase(victim);
     }
     virtual ~AcPlanBase()
     {
@@ -1840,11 +1861,12 @@ template <typename T> struct AcPlan : AcPlanBase {
     // forward over levels lo+1..hi from the state held in slots/phi (record traces if asked)
     // Runs body(stream) directly, or -- with graphs on and no event profiling -- captures it
     // once on a private stream into a graph and replays that graph on st.
-    template <class F> int replay(const GraphKey &key, hipStream_t st, F &&body)
+    template <class F> int replay(const GraphKey &key, int64_t M, hipStream_t st, F &&body)
     {
         if (!use_graph || prof_enabled()) return body(st);
         auto it = graphs.find(key);
         if (it == graphs.end()) {
+            while (!graphs.empty() && graphs.size() >= graph_cap(M)) evict_lru();
             if (!cap_stream) ADS_HIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
             ADS_HIP(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));
             const int rc = body(cap_stream);
@@ -1859,9 +1881,10 @@ template <typename T> struct AcPlan : AcPlanBase {
             const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             ADS_HIP(ei);
-            it = graphs.emplace(key, ex).first;
+            it = graphs.emplace(key, CachedGraph{ex, 0}).first;
         }
-        ADS_HIP(hipGraphLaunch(it->second, st));
+        it->second.used = ++graph_clock;
+        ADS_HIP(hipGraphLaunch(it->second.exec, st));
         return ADS_OK;
     }
 
@@ -1993,7 +2016,7 @@ template <typename T> struct AcPlan : AcPlanBase {
     {
         if (!sh) {
             GraphKey key{0, record, lo, hi, w.NS, w.seg, srcv, traces, nullptr};
-            return replay(key, st, [&](hipStream_t s) { return run_forward_levels(w, lo, hi, srcv, traces, record, s); });
+            return replay(key, w.M, st, [&](hipStream_t s) { return run_forward_levels(w, lo, hi, srcv, traces, record, s); });
         }
         return run_forward_levels(w, lo, hi, srcv, traces, record, st, sh);
     }
@@ -2125,7 +2148,7 @@ template <typename T> struct AcPlan : AcPlanBase {
             if (resid && max_gi_cells > 0 && inj_sg != adj_shape.sg)
                 ADS_TRY(build_inject_tables(adj_shape.sg));  // host uploads: not inside a capture
             GraphKey key{1, 0, a, b, w.NS, w.seg, sv, nullptr, resid};
-            ADS_TRY(replay(key, st, [&](hipStream_t s) -> int {
+            ADS_TRY(replay(key, w.M, st, [&](hipStream_t s) -> int {
                 for (int64_t n = b; n > a; n--) ADS_TRY(adj_at(w, n, sv, resid, s));
                 return ADS_OK;
             }));

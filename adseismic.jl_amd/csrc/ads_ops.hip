@@ -314,11 +314,30 @@ static int scatter_nd(const int64_t *ii, const T *upd, int64_t n, int64_t m, T *
 }
 
 // Host-side validation of 1-based (i, j, type) triples into a small device table owned by a
-// per-thread cache (grown, never shrunk).
+// per-thread cache (grown, never shrunk).  The kernel that reads the table may still run when
+// the same thread calls again on another stream, so release_entries() records an event after
+// that kernel and the next upload waits for it before overwriting or reallocating the table.
+struct EntryCache {
+    DevBuf buf;
+    hipEvent_t ev = nullptr;
+    bool pending = false;  // the event (kept until the process ends) guards the table
+};
+static thread_local EntryCache entry_cache;
+
+static int release_entries(hipStream_t st)
+{
+    EntryCache &c = entry_cache;
+    if (!c.ev) ADS_HIP(hipEventCreateWithFlags(&c.ev, hipEventDisableTiming));
+    ADS_HIP(hipEventRecord(c.ev, st));
+    c.pending = true;
+    return ADS_OK;
+}
+
 static int upload_entries(const int64_t *I, const int64_t *J, const int64_t *type, int64_t n,
                           int64_t nx, int64_t ny, const char *what, SrcEnt **dev, hipStream_t st)
 {
-    static thread_local DevBuf buf;
+    EntryCache &c = entry_cache;
+    DevBuf &buf = c.buf;
     std::vector<SrcEnt> h(n);
     for (int64_t s = 0; s < n; s++) {
         if (I[s] < 1 || I[s] > nx + 2 || J[s] < 1 || J[s] > ny + 2)
@@ -326,6 +345,10 @@ static int upload_entries(const int64_t *I, const int64_t *J, const int64_t *typ
                      (long long)s, (long long)I[s], (long long)J[s]);
         h[s].cell = (I[s] - 1) * (ny + 2) + (J[s] - 1);
         h[s].type = (int)type[s];
+    }
+    if (c.pending) {  // the previous call's kernel may still read the table
+        ADS_HIP(hipEventSynchronize(c.ev));
+        c.pending = false;
     }
     if (buf.bytes < n * sizeof(SrcEnt)) ADS_TRY(buf.alloc(std::max<size_t>(n, 64) * sizeof(SrcEnt)));
     ADS_HIP(hipMemcpyAsync(buf.p, h.data(), n * sizeof(SrcEnt), hipMemcpyHostToDevice, st));
@@ -446,6 +469,7 @@ int ads_add_source_f64(const double *sxx, const double *syy, const double *sxy, 
     ADS_TRY(upload_entries(srci, srcj, srctype, nsrc, nx, ny, "source", &e, st));
     op_add_source<double><<<1, 1, 0, st>>>(vx_out, vy_out, sxx_out, syy_out, sxy_out, e, srcv, nsrc);
     ADS_LAUNCH_CHECK();
+    ADS_TRY(release_entries(st));
     return ADS_OK;
 }
 
@@ -464,6 +488,7 @@ int ads_add_source_grad_f64(const double *g_sxx, const double *g_syy, const doub
     op_add_source_grad<double><<<cdiv(nsrc, 256), 256, 0, st>>>(g_vx, g_vy, g_sxx, g_syy, g_sxy, e,
                                                                 grad_srcv, nsrc);
     ADS_LAUNCH_CHECK();
+    ADS_TRY(release_entries(st));
     return ADS_OK;
 }
 
@@ -482,6 +507,7 @@ int ads_get_receive_f64(const double *vx, const double *vy, const double *sxx, c
     op_get_receive<double><<<cdiv(nrcv * nt, 256), 256, 0, st>>>(vx, vy, sxx, syy, sxy, nt,
                                                                  (nx + 2) * (ny + 2), e, nrcv, out);
     ADS_LAUNCH_CHECK();
+    ADS_TRY(release_entries(st));
     return ADS_OK;
 }
 

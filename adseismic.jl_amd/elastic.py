@@ -64,6 +64,7 @@ class ElasticPlan:
         self.srcv_off, self.trace_off = so, to
         self._ws = None
         self._ws_K = None
+        self._tok = None  # workspace token of this plan's last forward (see _lib.mark_workspace)
         self._scratch = None
 
     def __del__(self):
@@ -124,7 +125,13 @@ class ElasticPlan:
         ws = self.workspace(K)
         _lib.check(self.lib.ads_elastic_plan_forward(self.h, _vp(r), _vp(l), _vp(m), _vp(srcv),
                                                      _vp(tr), _vp(ws), ws.numel(), int(K), _stream()))
+        self._tok = _lib.mark_workspace(ws)
         return tr
+
+    def holds_forward(self, token, K: int) -> bool:
+        """True while the workspace still holds the state of the forward that returned
+        ``token`` (``self._tok`` right after it)."""
+        return self._ws_K == K and _lib.workspace_holds(self._ws, token)
 
     def backward(self, rho, lam, mu, srcv, grad_traces, K: int):
         if self._ws is None or self._ws_K != K:
@@ -135,6 +142,7 @@ class ElasticPlan:
         _lib.check(self.lib.ads_elastic_plan_backward(self.h, _vp(r), _vp(l), _vp(m), _vp(srcv),
                                                       _vp(gt), *[_vp(x) for x in g], _vp(self._ws),
                                                       self._ws.numel(), int(K), _stream()))
+        _lib.mark_workspace(self._ws)  # the sweep recomputed segments: the forward state is spent
         return tuple(g)
 
     def gradient(self, rho, lam, mu, srcv, observed, K: int, want_traces: bool = False):
@@ -149,6 +157,7 @@ class ElasticPlan:
                                                       _vp(obs), _vp(loss), *[_vp(x) for x in g],
                                                       _vp(tr), _vp(ws), ws.numel(), int(K),
                                                       _stream()))
+        _lib.mark_workspace(ws)
         return loss, tuple(g), tr
 
     def forward_history(self, rho, lam, mu, srcv):
@@ -173,6 +182,7 @@ class ElasticPlan:
                                                             _vp(srcv), _vp(gh),
                                                             *[_vp(x) for x in g], _vp(ws),
                                                             ws.numel(), int(K), _stream()))
+        _lib.mark_workspace(ws)
         return tuple(g)
 
 
@@ -180,13 +190,15 @@ class _ElTracesFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, rho, lam, mu, plan, srcv, K):
         tr = plan.forward(rho, lam, mu, srcv, K)
-        ctx.plan, ctx.K = plan, K
+        ctx.plan, ctx.K, ctx.tok = plan, K, plan._tok
         ctx.save_for_backward(rho, lam, mu, srcv)
         return tr
 
     @staticmethod
     def backward(ctx, g):
         rho, lam, mu, srcv = ctx.saved_tensors
+        if not ctx.plan.holds_forward(ctx.tok, ctx.K):
+            ctx.plan.forward(rho, lam, mu, srcv, ctx.K)  # the workspace was used since
         gr, gl, gm = ctx.plan.backward(rho, lam, mu, srcv, g.contiguous(), ctx.K)
         return gr, gl, gm, None, None, None
 

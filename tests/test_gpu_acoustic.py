@@ -346,6 +346,32 @@ def test_traces_autograd_function(orc, ads, cuda):
     assert np.sum(ct.grad.cpu().numpy() * v) == pytest.approx(fd, rel=1e-6)
 
 
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_traces_autograd_interleaved_calls(orc, ads, cuda, kernel):
+    """Two acoustic_traces calls with different models on one plan in one autograd graph, plus
+    a plan.gradient call between the forwards and the backward: each pullback must still see
+    its own forward's checkpoints (the workspace token makes it re-run the forward when another
+    call used the workspace).  Both gradients equal the oracle's."""
+    p, c, srcs, rcvs, nstep = make_case(orc, ads, nshots=2, kernel=kernel, seed=8)
+    c2 = c * (1.0 + 0.05 * np.random.default_rng(9).random(c.shape))
+    obs = [oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s], cmod=c * 1.03) for s in range(2)]
+    g1_ref = sum(oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s], obs=obs[s])[2] for s in range(2))
+    g2_ref = sum(oracle_shot(orc, p, c2, srcs[s], rcvs[s], nstep[s], obs=obs[s])[2] for s in range(2))
+    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float64)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    ob = plan.pack_traces(obs)
+    K = 9  # several segments: the pullback needs the checkpoints
+    a = T(c).requires_grad_(True)
+    b = T(c2).requires_grad_(True)
+    t1 = ads.acoustic_traces(plan, a, srcv, K)
+    t2 = ads.acoustic_traces(plan, b, srcv, K)
+    plan.gradient(T(c * 0.97), srcv, ob, K)          # overwrites the workspace once more
+    loss = torch.sum((t1 - ob) ** 2) + torch.sum((t2 - ob) ** 2)
+    loss.backward()
+    assert rel(a.grad.cpu(), g1_ref) < 1e-10, rel(a.grad.cpu(), g1_ref)
+    assert rel(b.grad.cpu(), g2_ref) < 1e-10, rel(b.grad.cpu(), g2_ref)
+
+
 def test_workspace_too_small_is_an_error(orc, ads, cuda):
     import ctypes as C
     p, c, srcs, rcvs, nstep = make_case(orc, ads)

@@ -1,0 +1,276 @@
+"""The 1000 x 1000 acoustic (32 shots, 4000 steps) and 600 x 600 elastic (16 shots, 3000
+steps) workloads run at full size on the GPU.  The workloads are bench.py's own (same grids,
+models, wavelets and geometry).
+
+1. Full records, FP32 against FP64 (two shots): FP32 accumulation over thousands of levels must
+   keep traces within 1e-4 and gradients within 1e-3 (relative L2) of the FP64 plan, which is
+   oracle-pinned to 1e-10 on every smaller case and, here, on the full grid over 300 steps.
+2. The full grid over the first 300 steps (one shot) against the FP64 CPU oracle: FP64 traces
+   1e-12 and gradients 1e-10, FP32 traces 1e-4 and gradients 1e-3.  Observed data come from a
+   3% faster model, so the residual is the direct arrival's shift.
+3. Every shot once, through size-independent properties: two shot groupings (one plan and two
+   half plans, different checkpoint intervals) give the same misfit and gradient within FP32
+   reassociation, and a central finite difference of the misfit along a smoothed gradient
+   direction matches <gradient, direction> within 2%.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def T(x, dt=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=dt, device="cuda")
+
+
+def _bench():
+    import bench
+    return bench
+
+
+def _release(*plans):
+    for p in plans:
+        p._ws = None
+    torch.cuda.empty_cache()
+
+
+# ------------------------------------------------------------------------------------------
+# acoustic, 1000 x 1000, 40-cell PML, h = 5 m, dt = 0.5 ms
+# ------------------------------------------------------------------------------------------
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_acoustic_full_record_f32_against_f64(ads, cuda, kernel):
+    p, c_true, c0, srcs, rcvs = _bench().workload(ads, kernel=kernel)
+    pick = [0, 21]
+    S, R = [srcs[i] for i in pick], [rcvs[i] for i in pick]
+    print(f"acoustic 1000x1000, {p.NSTEP} steps, kernel {kernel}, shots {pick}", flush=True)
+    p64 = ads.AcousticPlan(p, S, R, dtype=torch.float64)
+    sv64 = p64.pack_srcv([s.srcv for s in S])
+    K64 = p64.choose_interval()
+    obs = p64.forward(T(c_true, torch.float64), sv64, K64).clone()
+    L64, g64, tr64 = p64.gradient(T(c0, torch.float64), sv64, obs, K64, want_traces=True)
+    L64, g64, tr64 = L64.item(), g64.cpu().numpy(), tr64.cpu().numpy()
+    _release(p64)
+    p32 = ads.AcousticPlan(p, S, R, dtype=torch.float32)
+    sv32 = p32.pack_srcv([s.srcv for s in S])
+    K32 = p32.choose_interval()
+    L32, g32, tr32 = p32.gradient(T(c0), sv32, obs.float(), K32, want_traces=True)
+    torch.cuda.synchronize()
+    per64 = [t.numpy() for t in p32.unpack_traces(torch.as_tensor(tr64))]
+    per32 = [t.cpu().numpy() for t in p32.unpack_traces(tr32)]
+    errs = [rel(a, b) for a, b in zip(per32, per64)]
+    gerr = rel(g32.cpu().numpy(), g64)
+    print(f"FP32 vs FP64: traces {errs}, gradient {gerr:.2e}, misfit {L32.item():.6e} vs "
+          f"{L64:.6e} (K32 {K32}, K64 {K64})", flush=True)
+    _release(p32)
+    assert max(errs) <= 1e-4
+    assert gerr <= 1e-3
+    assert L32.item() == pytest.approx(L64, rel=1e-3)
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_acoustic_full_grid_300_steps_against_oracle(orc, ads, cuda, kernel):
+    nst = 300
+    p, c_true, c0, srcs, rcvs = _bench().workload(ads, nstep=nst, kernel=kernel)
+    src, rcv = srcs[5], rcvs[5]
+    print(f"acoustic 1000x1000, {nst} steps, kernel {kernel}, oracle", flush=True)
+    orc.set_num_threads(min(16, len(os.sched_getaffinity(0))))
+    NX, NY = p.NX, p.NY
+    Sx, Sy = orc.acoustic_pml(NX, NY, p.DELTAX, p.DELTAY, p.NPOINTS_PML, p.Rcoef, p.vp_ref)
+    u, _, _ = orc.acoustic_forward(kernel, NX, NY, nst, p.DELTAT, p.DELTAX, p.DELTAY, Sx, Sy,
+                                   (1.03 * c0) ** 2, src.srci, src.srcj, src.srcv)
+    obs = orc.acoustic_receive(u, rcv.rcvi, rcv.rcvj, NY)
+    del u
+    L_ref, g_ref, tr_ref = orc.acoustic_misfit_gradient(kernel, NX, NY, nst, p.DELTAT, p.DELTAX,
+                                                        p.DELTAY, Sx, Sy, c0, src.srci, src.srcj,
+                                                        src.srcv, rcv.rcvi, rcv.rcvj, obs)
+    orc.set_num_threads(1)
+    for dt, tr_bar, g_bar, K in ((torch.float64, 1e-12, 1e-10, 77), (torch.float32, 1e-4, 1e-3, 77)):
+        plan = ads.AcousticPlan(p, [src], [rcv], dtype=dt)
+        sv = plan.pack_srcv([src.srcv])
+        L, g, tr = plan.gradient(T(c0, dt), sv, plan.pack_traces([obs]), K, want_traces=True)
+        e_tr = rel(plan.unpack_traces(tr)[0].cpu(), tr_ref)
+        e_g = rel(g.cpu(), g_ref)
+        print(f"{dt}: traces {e_tr:.2e}, gradient {e_g:.2e}, misfit {L.item():.6e} vs "
+              f"{L_ref:.6e}", flush=True)
+        _release(plan)
+        assert e_tr <= tr_bar
+        assert e_g <= g_bar
+
+
+def _directional_fd(misfit, g, c0, frac=0.005, sigma=5.0):
+    """Central difference of the misfit along the smoothed gradient, scaled to at most `frac`
+    of the model, against <g, d>."""
+    from scipy.ndimage import gaussian_filter
+    gs = gaussian_filter(np.asarray(g, np.float64), sigma)
+    d = gs * (frac * np.abs(c0).min() / np.abs(gs).max())
+    fd = (misfit(c0 + d) - misfit(c0 - d)) / 2.0
+    return fd, float(np.sum(np.asarray(g, np.float64) * d))
+
+
+@pytest.mark.timeout(900)
+def test_acoustic_all_32_shots_groupings_and_directional_derivative(ads, cuda):
+    p, c_true, c0, srcs, rcvs = _bench().workload(ads)
+    print(f"acoustic 1000x1000, {p.NSTEP} steps, all {len(srcs)} shots", flush=True)
+    full = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float32)
+    sv = full.pack_srcv([s.srcv for s in srcs])
+    K = full.choose_interval()
+    obs = full.forward(T(c_true), sv, K).clone()
+    L_all, g_all, _ = full.gradient(T(c0), sv, obs, K)
+    L_all, g_all = L_all.item(), g_all.cpu().numpy()
+    obs_per = [t.clone() for t in full.unpack_traces(obs)]
+
+    def misfit(cm):
+        tr = full.forward(T(cm), sv, K)
+        return float(torch.sum((tr.double() - obs.double()) ** 2).item())
+
+    fd, an = _directional_fd(misfit, g_all, c0)
+    _release(full)
+    L_h, g_h = 0.0, 0.0
+    for half in (range(0, 32, 2), range(1, 32, 2)):       # interleaved halves
+        idx = list(half)
+        hp = ads.AcousticPlan(p, [srcs[i] for i in idx], [rcvs[i] for i in idx], dtype=torch.float32)
+        hs = hp.pack_srcv([srcs[i].srcv for i in idx])
+        Kh = hp.choose_interval() // 2 + 1                 # another checkpoint interval
+        l, g, _ = hp.gradient(T(c0), hs, hp.pack_traces([obs_per[i] for i in idx]), Kh)
+        L_h += l.item()
+        g_h = g_h + g.cpu().numpy()
+        _release(hp)
+    print(f"misfit {L_all:.6e} (halves {L_h:.6e}), gradient groupings {rel(g_h, g_all):.2e}, "
+          f"directional derivative: fd {fd:.6e}, gradient {an:.6e}", flush=True)
+    assert L_h == pytest.approx(L_all, rel=1e-5)
+    assert rel(g_h, g_all) < 1e-5
+    assert abs(fd - an) <= 0.02 * abs(an)
+
+
+# ------------------------------------------------------------------------------------------
+# elastic, 600 x 600, 30-cell CPML, h = 10 m, dt = 0.8 ms
+# ------------------------------------------------------------------------------------------
+
+def _el_models(models, dt):
+    return [T(m, dt) for m in models]
+
+
+@pytest.mark.timeout(600)
+def test_elastic_full_record_f32_against_f64(ads, cuda):
+    p, true, start, srcs, rcvs = _bench().elastic_workload(ads)
+    pick = [0, 11]
+    S, R = [srcs[i] for i in pick], [rcvs[i] for i in pick]
+    print(f"elastic 600x600, {p.NSTEP} steps, shots {pick}", flush=True)
+    p64 = ads.ElasticPlan(p, S, R, dtype=torch.float64)
+    sv64 = p64.pack_srcv([s.srcv for s in S])
+    K64 = p64.choose_interval()
+    obs = p64.forward(*_el_models(true, torch.float64), sv64, K64).clone()
+    L64, g64, tr64 = p64.gradient(*_el_models(start, torch.float64), sv64, obs, K64, want_traces=True)
+    L64, g64, tr64 = L64.item(), [g.cpu().numpy() for g in g64], tr64.cpu().numpy()
+    _release(p64)
+    p32 = ads.ElasticPlan(p, S, R, dtype=torch.float32)
+    sv32 = p32.pack_srcv([s.srcv for s in S])
+    K32 = p32.choose_interval()
+    L32, g32, tr32 = p32.gradient(*_el_models(start, torch.float32), sv32, obs.float(), K32,
+                                  want_traces=True)
+    torch.cuda.synchronize()
+    per64 = [t.numpy() for t in p32.unpack_traces(torch.as_tensor(tr64))]
+    per32 = [t.cpu().numpy() for t in p32.unpack_traces(tr32)]
+    errs = [rel(a, b) for a, b in zip(per32, per64)]
+    gerrs = [rel(a.cpu().numpy(), b) for a, b in zip(g32, g64)]
+    print(f"FP32 vs FP64: traces {errs}, gradients (rho, lambda, mu) {gerrs}, misfit "
+          f"{L32.item():.6e} vs {L64:.6e} (K32 {K32}, K64 {K64})", flush=True)
+    _release(p32)
+    assert max(errs) <= 1e-4
+    assert max(gerrs) <= 1e-3
+
+
+@pytest.mark.timeout(600)
+def test_elastic_full_grid_300_steps_against_oracle(orc, ads, cuda):
+    nst = 300
+    p, true, start, srcs, rcvs = _bench().elastic_workload(ads, nstep=nst)
+    src, rcv = srcs[3], rcvs[3]
+    print(f"elastic 600x600, {nst} steps, oracle", flush=True)
+    orc.set_num_threads(min(16, len(os.sched_getaffinity(0))))
+    pml = orc.elastic_pml(p.NX, p.NY, p.DELTAX, p.DELTAY, p.DELTAT, p.NPOINTS_PML, p.NPOWER,
+                          p.K_MAX_PML, p.ALPHA_MAX_PML, p.Rcoef, p.vp_ref)
+    fast = (start[0], start[1] * 1.06, start[2] * 1.06)
+    h, _ = orc.elastic_forward(p.NX, p.NY, nst, p.DELTAX, p.DELTAY, p.DELTAT, *fast, pml, src.srci,
+                               src.srcj, src.srctype, src.srcv, keep_mem=False)
+    obs = orc.elastic_receive(h, rcv.rcvi, rcv.rcvj, rcv.rcvtype, p.NX, p.NY)
+    del h
+    L_ref, g_ref, tr_ref = orc.elastic_misfit_gradient(p.NX, p.NY, nst, p.DELTAX, p.DELTAY,
+                                                       p.DELTAT, *start, pml, src.srci, src.srcj,
+                                                       src.srctype, src.srcv, rcv.rcvi, rcv.rcvj,
+                                                       rcv.rcvtype, obs)
+    orc.set_num_threads(1)
+    for dt, tr_bar, g_bar in ((torch.float64, 1e-12, 1e-10), (torch.float32, 1e-4, 1e-3)):
+        plan = ads.ElasticPlan(p, [src], [rcv], dtype=dt)
+        sv = plan.pack_srcv([src.srcv])
+        L, g, tr = plan.gradient(*_el_models(start, dt), sv, plan.pack_traces([obs]), 61,
+                                 want_traces=True)
+        e_tr = rel(plan.unpack_traces(tr)[0].cpu(), tr_ref)
+        e_g = [rel(a.cpu(), b) for a, b in zip(g, g_ref)]
+        print(f"{dt}: traces {e_tr:.2e}, gradients {e_g}, misfit {L.item():.6e} vs {L_ref:.6e}",
+              flush=True)
+        _release(plan)
+        assert e_tr <= tr_bar
+        assert max(e_g) <= g_bar
+
+
+@pytest.mark.timeout(900)
+def test_elastic_all_16_shots_groupings_and_directional_derivative(ads, cuda):
+    p, true, start, srcs, rcvs = _bench().elastic_workload(ads)
+    print(f"elastic 600x600, {p.NSTEP} steps, all {len(srcs)} shots", flush=True)
+    full = ads.ElasticPlan(p, srcs, rcvs, dtype=torch.float32)
+    sv = full.pack_srcv([s.srcv for s in srcs])
+    K = full.choose_interval()
+    obs = full.forward(*_el_models(true, torch.float32), sv, K).clone()
+    L_all, g_all, _ = full.gradient(*_el_models(start, torch.float32), sv, obs, K)
+    L_all, g_all = L_all.item(), [g.cpu().numpy() for g in g_all]
+    obs_per = [t.clone() for t in full.unpack_traces(obs)]
+    # directional derivative along the smoothed (rho, lambda, mu) gradients, each scaled to at
+    # most 0.5% of its parameter
+    from scipy.ndimage import gaussian_filter
+    ds = []
+    for g, m in zip(g_all, start):
+        gs = gaussian_filter(g.astype(np.float64), 5.0)
+        ds.append(gs * (0.005 * np.abs(m).min() / np.abs(gs).max()))
+
+    def misfit(sign):
+        ms = [m + sign * d for m, d in zip(start, ds)]
+        tr = full.forward(*_el_models(ms, torch.float32), sv, K)
+        return float(torch.sum((tr.double() - obs.double()) ** 2).item())
+
+    fd = (misfit(1.0) - misfit(-1.0)) / 2.0
+    an = float(sum(np.sum(g.astype(np.float64) * d) for g, d in zip(g_all, ds)))
+    _release(full)
+    L_h, g_h = 0.0, [0.0, 0.0, 0.0]
+    for half in (range(0, 16, 2), range(1, 16, 2)):
+        idx = list(half)
+        hp = ads.ElasticPlan(p, [srcs[i] for i in idx], [rcvs[i] for i in idx], dtype=torch.float32)
+        hs = hp.pack_srcv([srcs[i].srcv for i in idx])
+        Kh = hp.choose_interval() // 2 + 1
+        l, g, _ = hp.gradient(*_el_models(start, torch.float32), hs,
+                              hp.pack_traces([obs_per[i] for i in idx]), Kh)
+        L_h += l.item()
+        g_h = [a + b.cpu().numpy() for a, b in zip(g_h, g)]
+        _release(hp)
+    gerrs = [rel(a, b) for a, b in zip(g_h, g_all)]
+    print(f"misfit {L_all:.6e} (halves {L_h:.6e}), gradient groupings {gerrs}, directional "
+          f"derivative: fd {fd:.6e}, gradient {an:.6e}", flush=True)
+    assert L_h == pytest.approx(L_all, rel=1e-5)
+    assert max(gerrs) < 1e-5
+    assert abs(fd - an) <= 0.02 * abs(an)

@@ -429,3 +429,44 @@ def test_energy_decays_in_the_cpml(orc):
     assert np.all(np.isfinite(e_pml))
     assert e_pml[-1] < 1e-2 * e_pml.max(), e_pml[-1] / e_pml.max()
     assert e_box[-1] > 0.2 * e_box.max(), e_box[-1] / e_box.max()
+
+
+def test_source_gradient_finite_differences(orc):
+    """The oracle's d misfit / d srcv (the AddSource backward of AddSource.cpp:91-159, composed
+    through the time loop) against a central finite difference along a random srcv
+    direction, with duplicated (cell, type) sources and every source type."""
+    NX, NY, NSTEP = 22, 18, 40
+    hx, hy, dt = 10.0, 10.0, 1e-3
+    rho, lam, mu = _model(NX, NY, seed=4)
+    pml = orc.elastic_pml(NX, NY, hx, hy, dt, 4, 2.0, 1.0, 2 * np.pi * 5, 1e-3, 2500.0)
+    w = orc.ricker(NSTEP, 5.0, 12.0, 1e8)
+    srci = [11, 11, 8, 14, 6, 11]
+    srcj = [6, 6, 9, 5, 12, 6]
+    st = [2, 2, 0, 1, 4, 3]
+    srcv = np.stack([w, 0.5 * w, 1e-7 * w, -1e-7 * w, 0.3 * w, w], 1)
+    rcvi = np.concatenate([np.arange(2, NX), np.arange(2, NX)])
+    rcvj = np.full(len(rcvi), 3)
+    rt = np.concatenate([np.zeros(NX - 2), np.ones(NX - 2)]).astype(np.int64)
+    hist, mem = orc.elastic_forward(NX, NY, NSTEP, hx, hy, dt, rho, lam, mu, pml, srci, srcj, st,
+                                    srcv)
+    rc = orc.elastic_receive(hist, rcvi, rcvj, rt, NX, NY)
+    obs = 0.9 * rc
+    res = rc - obs
+    g = np.zeros_like(hist).reshape(5, NSTEP + 1, -1)
+    idx = orc.flat_index(rcvi, rcvj, NY)
+    for r in range(len(idx)):
+        g[int(rt[r]), :, idx[r]] += 2.0 * res[r]
+    *_, gsrc = orc.elastic_adjoint(NX, NY, NSTEP, hx, hy, dt, rho, lam, mu, pml, hist, mem, g,
+                                   srci, srcj, st)
+    assert gsrc.shape == (NSTEP, len(srci))
+    v = np.random.default_rng(6).standard_normal(srcv.shape) * np.abs(srcv).max(0) * 1e-4
+
+    def L(sv):
+        h2, _ = orc.elastic_forward(NX, NY, NSTEP, hx, hy, dt, rho, lam, mu, pml, srci, srcj, st,
+                                    sv, keep_mem=False)
+        return float(np.sum((orc.elastic_receive(h2, rcvi, rcvj, rt, NX, NY) - obs) ** 2))
+
+    fd = (L(srcv + v) - L(srcv - v)) / 2
+    assert np.sum(gsrc * v) == pytest.approx(fd, rel=1e-6)
+    # the duplicated (cell, type) pair sees the same cotangent
+    np.testing.assert_array_equal(gsrc[:, 0], gsrc[:, 1])
