@@ -72,6 +72,7 @@ _SIGS = {
     "ads_scatter_nd_f64": (C.c_int, [vp, vp, i64, i64, vp, vp]),
     "ads_scatter_nd_f32": (C.c_int, [vp, vp, i64, i64, vp, vp]),
     "ads_add_source_f64": (C.c_int, [vp] * 5 + [i64p] * 3 + [i64] * 3 + [vp] * 6 + [vp]),
+    "ads_add_source_noalloc_f64": (C.c_int, [vp] * 5 + [i64p] * 3 + [i64] * 3 + [vp] * 6 + [vp]),
     "ads_add_source_grad_f64": (C.c_int, [vp] * 5 + [i64p] * 3 + [i64] * 3 + [vp, vp]),
     "ads_get_receive_f64": (C.c_int, [vp] * 5 + [i64] + [i64p] * 3 + [i64] * 3 + [vp, vp]),
     "ads_acoustic_pml_profiles": (C.c_int, [C.POINTER(AcousticParamsC), vp, vp]),

@@ -2,6 +2,8 @@
 // ops one for one (deps/CustomOps/*).  This translation unit is compiled with
 // -ffp-contract=off and evaluates every expression in the reference's order, so the FP64
 // results equal the reference CPU kernels bit for bit.
+#include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "ads_common.h"
@@ -357,6 +359,53 @@ static int upload_entries(const int64_t *I, const int64_t *J, const int64_t *typ
     return ADS_OK;
 }
 
+// ---- fixed source table of the allocation-free AddSource entry point ----
+// One table per device, in the code object's static data: each device holds its own copy
+// until the process ends.  `scratch` is only a host-side stand-in for it; host code uses the
+// device address hipGetSymbolAddress returns, fetched once per device with that device current.
+constexpr int64_t NOALLOC_MAX_SOURCES = ADS_NOALLOC_MAX_SOURCES;  // 1 MiB of 16-byte entries
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static_assert(sizeof(SrcEnt) == 16, "table entries are 16 bytes");
+static __device__ __align__(256) SrcEnt scratch[NOALLOC_MAX_SOURCES];
+static std::mutex scratch_mu;
+static SrcEnt *scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+static int scratch_table(SrcEnt **out)
+{
+    int dev = -1;
+    ADS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        ADS_FAIL(ADS_ERR_ARG, "ads_add_source_noalloc: device %d beyond the %d supported", dev,
+                 NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        ADS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        scratch_dev[dev] = static_cast<SrcEnt *>(p);
+    }
+    *out = scratch_dev[dev];
+    return ADS_OK;
+}
+
+// Validated (cell, type) entries into the device table `dev` through a stack staging block;
+// nothing is allocated on the host or the device.
+static int upload_entries_fixed(const int64_t *I, const int64_t *J, const int64_t *type,
+                                int64_t n, int64_t nx, int64_t ny, SrcEnt *dev, hipStream_t st)
+{
+    constexpr int64_t CH = 256;
+    SrcEnt h[CH];
+    for (int64_t s0 = 0; s0 < n; s0 += CH) {
+        const int64_t m = std::min(CH, n - s0);
+        for (int64_t k = 0; k < m; k++) {
+            h[k].cell = (I[s0 + k] - 1) * (ny + 2) + (J[s0 + k] - 1);
+            h[k].type = (int)type[s0 + k];
+        }
+        ADS_HIP(hipMemcpyAsync(dev + s0, h, m * sizeof(SrcEnt), hipMemcpyHostToDevice, st));
+        ADS_HIP(hipStreamSynchronize(st));  // h is reused by the next block
+    }
+    return ADS_OK;
+}
+
 }  // namespace ads
 
 using namespace ads;
@@ -470,6 +519,41 @@ int ads_add_source_f64(const double *sxx, const double *syy, const double *sxy, 
     op_add_source<double><<<1, 1, 0, st>>>(vx_out, vy_out, sxx_out, syy_out, sxy_out, e, srcv, nsrc);
     ADS_LAUNCH_CHECK();
     ADS_TRY(release_entries(st));
+    return ADS_OK;
+}
+
+int ads_add_source_noalloc_f64(const double *sxx, const double *syy, const double *sxy,
+                               const double *vx, const double *vy, const int64_t *srci,
+                               const int64_t *srcj, const int64_t *srctype, int64_t nsrc,
+                               int64_t nx, int64_t ny, const double *srcv, double *sxx_out,
+                               double *syy_out, double *sxy_out, double *vx_out, double *vy_out,
+                               ads_stream_t stream)
+{
+    clear_error();
+    if (!sxx || !syy || !sxy || !vx || !vy || !sxx_out || !syy_out || !sxy_out || !vx_out || !vy_out)
+        ADS_FAIL(ADS_ERR_ARG, "ads_add_source_noalloc: null field pointer");
+    if (nsrc > 0 && (!srci || !srcj || !srctype || !srcv))
+        ADS_FAIL(ADS_ERR_ARG, "ads_add_source_noalloc: null source arrays");
+    if (nsrc > NOALLOC_MAX_SOURCES)
+        ADS_FAIL(ADS_ERR_SHAPE, "ads_add_source_noalloc: %lld sources exceed the fixed table of %lld",
+                 (long long)nsrc, (long long)NOALLOC_MAX_SOURCES);
+    for (int64_t s = 0; s < nsrc; s++)  // validate everything before touching the outputs
+        if (srci[s] < 1 || srci[s] > nx + 2 || srcj[s] < 1 || srcj[s] > ny + 2)
+            ADS_FAIL(ADS_ERR_INDEX, "source %lld at (%lld, %lld) outside the padded grid",
+                     (long long)s, (long long)srci[s], (long long)srcj[s]);
+    const int64_t N = (nx + 2) * (ny + 2);
+    hipStream_t st = S(stream);
+    const double *in[5] = {sxx, syy, sxy, vx, vy};
+    double *out[5] = {sxx_out, syy_out, sxy_out, vx_out, vy_out};
+    for (int q = 0; q < 5; q++)
+        if (in[q] != out[q])
+            ADS_HIP(hipMemcpyAsync(out[q], in[q], N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (nsrc <= 0) return ADS_OK;
+    SrcEnt *e = nullptr;
+    ADS_TRY(scratch_table(&e));
+    ADS_TRY(upload_entries_fixed(srci, srcj, srctype, nsrc, nx, ny, e, st));
+    op_add_source<double><<<1, 1, 0, st>>>(vx_out, vy_out, sxx_out, syy_out, sxy_out, e, srcv, nsrc);
+    ADS_LAUNCH_CHECK();
     return ADS_OK;
 }
 

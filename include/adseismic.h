@@ -119,6 +119,21 @@ int ads_add_source_f64(const double *sxx, const double *syy, const double *sxy, 
                        const int64_t *srctype, int64_t nsrc, int64_t nx, int64_t ny,
                        const double *srcv, double *sxx_out, double *syy_out, double *sxy_out,
                        double *vx_out, double *vy_out, ads_stream_t stream);
+/* AddSource forward that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, semantics and results as
+ * ads_add_source_f64, but the source table lives in a fixed per-device array of
+ * ADS_NOALLOC_MAX_SOURCES entries held until the process ends; nsrc above that is
+ * ADS_ERR_SHAPE.  Sources are validated before any output is written.  Every call on one
+ * device shares that array: calls that overlap on one device (other streams, other threads)
+ * race. */
+#define ADS_NOALLOC_MAX_SOURCES 65536
+int ads_add_source_noalloc_f64(const double *sxx, const double *syy, const double *sxy,
+                               const double *vx, const double *vy, const int64_t *srci,
+                               const int64_t *srcj, const int64_t *srctype, int64_t nsrc,
+                               int64_t nx, int64_t ny, const double *srcv, double *sxx_out,
+                               double *syy_out, double *sxy_out, double *vx_out, double *vy_out,
+                               ads_stream_t stream);
 /* AddSource backward (AddSource.cpp:91-159): grads pass through; grad_srcv[s] = cotangent of
  * the source's field at its cell. */
 int ads_add_source_grad_f64(const double *g_sxx, const double *g_syy, const double *g_sxy,
