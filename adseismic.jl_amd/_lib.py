@@ -84,6 +84,10 @@ _SIGS = {
     "ads_acoustic_plan_sizes": (C.c_int, [vp, i64p, i64p, i64p, i64p]),
     "ads_acoustic_plan_workspace_size": (C.c_size_t, [vp, i64]),
     "ads_acoustic_plan_choose_interval": (i64, [vp, C.c_size_t]),
+    "ads_acoustic_plan_set_snapshots": (C.c_int, [vp, i64]),
+    "ads_acoustic_plan_snapshots": (i64, [vp]),
+    "ads_acoustic_plan_choose_schedule": (C.c_int, [vp, C.c_size_t, i64p, i64p]),
+    "ads_checkpoint_schedule": (C.c_int, [i64, i64, i64p, i64, i64p]),
     "ads_acoustic_plan_forward": (C.c_int, [vp, vp, vp, vp, vp, C.c_size_t, i64, vp]),
     "ads_acoustic_plan_backward": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_size_t, i64, vp]),
     "ads_acoustic_plan_gradient": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, C.c_size_t, i64, vp]),

@@ -177,6 +177,27 @@ class AcousticPlan:
             raise MemoryError("no checkpoint interval fits the memory budget")
         return K
 
+    def set_snapshots(self, snapshots: int) -> None:
+        """Snapshot slots of the reversal schedule: < 0 one per segment start (one-level),
+        otherwise the binomial schedule with that many slots."""
+        _lib.check(self.lib.ads_acoustic_plan_set_snapshots(self.h, int(snapshots)))
+        self._ws_K = None  # a backward needs a forward made with the same schedule
+
+    @property
+    def snapshots(self) -> int:
+        return int(self.lib.ads_acoustic_plan_snapshots(self.h))
+
+    def choose_schedule(self, budget_bytes: Optional[int] = None):
+        """(K, S) with the least modelled time whose workspace fits the budget (default 90% of
+        the device memory a workspace can use); the plan keeps S."""
+        if budget_bytes is None:
+            budget_bytes = int(_lib.available_device_bytes() * 0.9)
+        K, S = _lib.i64(0), _lib.i64(0)
+        _lib.check(self.lib.ads_acoustic_plan_choose_schedule(self.h, int(budget_bytes),
+                                                              C.byref(K), C.byref(S)))
+        self._ws_K = None
+        return int(K.value), int(S.value)
+
     def workspace(self, K: int) -> torch.Tensor:
         need = self.workspace_size(K)
         if self._ws is None or self._ws.numel() < need:

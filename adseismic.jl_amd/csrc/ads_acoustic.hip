@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "ads_common.h"
+#include "ads_schedule.h"
 #include "ads_stencil.h"
 
 namespace ads {
@@ -1413,6 +1414,12 @@ struct AcPlanBase {
     int64_t srcv_elems = 0, trace_elems = 0;
     std::vector<int> active;  // active[n] = #shots with nstep >= n
     std::vector<double> sx, ty;
+    // phi/psi vanish on the quiet box (no damping profile acts there): checkpoint records keep
+    // the nfr cells outside it
+    FrameBox fbox{};
+    int64_t nfr = 0;
+    // snapshot slots of the reversal schedule (< 0: one per segment start, the one-level scheme)
+    int64_t snapshots = -1;
     std::vector<uint8_t> pmlx, pmly, nearx, neary;
     double f_pml = 0.0;  // fraction of interior cells inside the PML strips
     // defaults: the benchmark shape (forward: the streaming two-level rectangle)
@@ -1803,14 +1810,26 @@ template <typename T> struct AcPlan : AcPlanBase {
 
     // ---- workspace layout for checkpoint interval K ----
     struct WS {
-        int64_t K, NS, M;  // interval, slots, segments
+        int64_t K, NS, M, S;  // interval, slots, segments, snapshot slots
         // phi/psi of level n in buffer n mod 3: a two-level pass writes level n+1 while other
         // workgroups of the same launch still read level n-1
         T *seg, *phi[3], *psi[3], *P[2], *F[2], *G[2], *gC, *ckpt, *C, *resid;
         double *loss, *part;
+        std::vector<size_t> ck_off;  // element offset of each snapshot slot in ckpt
     };
+    int64_t seg_level(const WS &w, int64_t m) const { return std::min<int64_t>(1 + m * w.K, NMAX); }
+    int64_t snap_slots(int64_t M, int64_t snaps) const
+    {
+        const int64_t one = std::max<int64_t>(M - 1, 0);
+        return snaps < 0 ? one : std::min(snaps, one);
+    }
+    // a record of the state at level a for `act` shots: u^a, u^{a-1} (full grid), phi^a and
+    // psi^a (cells outside the quiet box)
+    size_t rec_elems(int act) const { return (size_t)act * (2 * (size_t)Npad + 2 * (size_t)nfr); }
+    int rec_act(int64_t a) const { return active[std::min<int64_t>(a + 1, NMAX)]; }
     int64_t seg_count(int64_t K) const { return NMAX >= 2 ? (NMAX - 1 + K - 1) / K : 0; }
-    void layout(WS &w, Carve &cv, int64_t K) const
+    void layout(WS &w, Carve &cv, int64_t K) const { layout(w, cv, K, snapshots); }
+    void layout(WS &w, Carve &cv, int64_t K, int64_t snaps) const
     {
         if (K < 1) K = 1;
         if (K > std::max<int64_t>(NMAX - 1, 1)) K = std::max<int64_t>(NMAX - 1, 1);
@@ -1829,7 +1848,17 @@ template <typename T> struct AcPlan : AcPlanBase {
             w.G[q] = (T *)cv.take(F);
         }
         w.gC = (T *)cv.take((size_t)nshots * Npad * sizeof(T));  // up to one array per shot
-        w.ckpt = (T *)cv.take(F * 4 * (size_t)std::max<int64_t>(w.M - 1, 0));
+        // snapshot slots: one-level, slot q holds state q+1 and is sized for the shots active
+        // there; binomial, any slot may hold any state and is sized for the most active shots
+        w.S = snap_slots(w.M, snaps);
+        w.ck_off.assign((size_t)w.S, 0);
+        size_t ck = 0;
+        for (int64_t q = 0; q < w.S; q++) {
+            w.ck_off[q] = ck;
+            const int64_t a = w.S >= w.M - 1 ? 1 + (q + 1) * w.K : 1 + w.K;
+            ck += (rec_elems(rec_act(a)) + 63) & ~size_t(63);
+        }
+        w.ckpt = (T *)cv.take(ck * sizeof(T));
         w.C = (T *)cv.take((size_t)Npad * sizeof(T));
         w.resid = (T *)cv.take((size_t)std::max<int64_t>(trace_elems, 1) * sizeof(T));
         w.loss = (double *)cv.take(64);
@@ -1837,12 +1866,60 @@ template <typename T> struct AcPlan : AcPlanBase {
     }
     Lv<T> phl(const WS &w, int64_t n) const { return Lv<T>{w.phi[n % 3], Npad}; }
     Lv<T> psl(const WS &w, int64_t n) const { return Lv<T>{w.psi[n % 3], Npad}; }
-    size_t ws_size(int64_t K) const
+    size_t ws_size(int64_t K) const { return ws_size(K, snapshots); }
+    size_t ws_size(int64_t K, int64_t snaps) const
     {
         WS w;
         Carve cv(nullptr, 0);
-        layout(w, cv, K);
+        layout(w, cv, K, snaps);
         return cv.off + 256;
+    }
+
+    // Fastest (K, S) whose workspace fits `budget`.  Cost model in shot-levels: the forward
+    // and the adjoint sweep each pass every active shot's levels once (an adjoint level costs
+    // ADJ_COST forward levels), and the reversal schedule adds its segment rebuilds.
+    static constexpr double ADJ_COST = 1.7;
+    int choose_schedule(size_t budget, int64_t *Kout, int64_t *Sout) const
+    {
+        const int64_t L = std::max<int64_t>(NMAX - 1, 1);
+        double total = 0.0;
+        for (int64_t n = 2; n <= NMAX; n++) total += active[n];
+        double best = 0.0;
+        int64_t bK = 0, bS = 0;
+        int64_t lastK = -1;
+        for (int64_t Mc = 1; Mc <= std::min<int64_t>(L, 4096); Mc++) {
+            const int64_t K = (L + Mc - 1) / Mc;
+            if (K == lastK) continue;
+            lastK = K;
+            const int64_t M = seg_count(K);
+            if (ws_size(K, 0) > budget) continue;
+            int64_t S = std::max<int64_t>(M - 1, 0);
+            if (ws_size(K, -1) > budget) {
+                const size_t rec = ((rec_elems(rec_act(1 + K)) + 63) & ~size_t(63)) * sizeof(T);
+                S = std::min<int64_t>((int64_t)((budget - ws_size(K, 0)) / std::max<size_t>(rec, 1)),
+                                      std::max<int64_t>(M - 2, 0));
+                while (S > 0 && ws_size(K, S) > budget) S--;
+            }
+            if (S == 0 && M > 64) continue;  // quadratic rebuilds: never the fastest here
+            std::vector<double> wseg((size_t)M, 0.0);
+            for (int64_t m = 0; m < M; m++) {
+                const int64_t a = std::min<int64_t>(1 + m * K, NMAX), b = std::min<int64_t>(a + K, NMAX);
+                for (int64_t n = a + 1; n <= b; n++) wseg[m] += active[n];
+            }
+            CostOps c{&wseg};
+            Reversal rv(M, S);
+            rv.run(c);
+            const double t = (1.0 + ADJ_COST) * total + c.extra;
+            if (bK == 0 || t < best * (1.0 - 1e-12)) {
+                best = t;
+                bK = K;
+                bS = S;
+            }
+        }
+        if (bK == 0) return ADS_ERR_WORKSPACE;
+        *Kout = bK;
+        *Sout = bS;
+        return ADS_OK;
     }
 
     Lv<T> slot(const WS &w, int64_t n) const
@@ -2044,37 +2121,48 @@ template <typename T> struct AcPlan : AcPlanBase {
         return ADS_OK;
     }
 
-    int save_ckpt(const WS &w, int64_t m, int64_t a, hipStream_t st)
+    int frame_copy(bool unpack, Lv<T> f, T *rec, int act, hipStream_t st)
     {
-        // state at level a: u^a, u^{a-1}, phi^a, psi^a (active shots are a prefix)
-        int act = active[std::min<int64_t>(a + 1, NMAX)];
-        size_t bytes = (size_t)act * Npad * sizeof(T);
-        T *dst = w.ckpt + (size_t)(m - 1) * 4 * nshots * Npad;
-        size_t F = (size_t)nshots * Npad;
-        ADS_HIP(hipMemcpyAsync(dst, slot(w, a).p, bytes, hipMemcpyDeviceToDevice, st));
-        ADS_HIP(hipMemcpyAsync(dst + F, slot(w, a - 1).p, bytes, hipMemcpyDeviceToDevice, st));
-        ADS_HIP(hipMemcpyAsync(dst + 2 * F, phl(w, a).p, bytes, hipMemcpyDeviceToDevice, st));
-        ADS_HIP(hipMemcpyAsync(dst + 3 * F, psl(w, a).p, bytes, hipMemcpyDeviceToDevice, st));
+        if (nfr <= 0 || act <= 0) return ADS_OK;
+        const dim3 g(cdiv(nfr, 256), (unsigned)act);
+        if (unpack) k_frame_copy<T, true><<<g, 256, 0, st>>>(f.p, Npad, rec, fbox, (int)Pitch);
+        else k_frame_copy<T, false><<<g, 256, 0, st>>>(f.p, Npad, rec, fbox, (int)Pitch);
+        ADS_LAUNCH_CHECK();
         return ADS_OK;
     }
 
-    int restore_ckpt(const WS &w, int64_t m, int64_t a, hipStream_t st)
+    // state at level a (u^a, u^{a-1}, phi^a, psi^a; active shots are a prefix) into slot q
+    int save_ckpt(const WS &w, int64_t q, int64_t a, hipStream_t st)
     {
-        int act = active[std::min<int64_t>(a + 1, NMAX)];
-        size_t bytes = (size_t)act * Npad * sizeof(T);
-        size_t F = (size_t)nshots * Npad;
-        if (m == 0) {  // level 1: everything zero
-            ADS_HIP(hipMemsetAsync(slot(w, 0).p, 0, bytes, st));
-            ADS_HIP(hipMemsetAsync(slot(w, 1).p, 0, bytes, st));
-            ADS_HIP(hipMemsetAsync(phl(w, 1).p, 0, bytes, st));
-            ADS_HIP(hipMemsetAsync(psl(w, 1).p, 0, bytes, st));
+        const int act = rec_act(a);
+        const size_t U = (size_t)act * Npad, R = (size_t)act * nfr;
+        T *dst = w.ckpt + w.ck_off[q];
+        ADS_HIP(hipMemcpyAsync(dst, slot(w, a).p, U * sizeof(T), hipMemcpyDeviceToDevice, st));
+        ADS_HIP(hipMemcpyAsync(dst + U, slot(w, a - 1).p, U * sizeof(T), hipMemcpyDeviceToDevice, st));
+        ADS_TRY(frame_copy(false, phl(w, a), dst + 2 * U, act, st));
+        ADS_TRY(frame_copy(false, psl(w, a), dst + 2 * U + R, act, st));
+        return ADS_OK;
+    }
+
+    // slot q (q < 0: the zero state at level 1) back into the slot ring and phi/psi buffers.
+    // The buffers' quiet-box cells are zero: every forward clears them and no pass writes a
+    // non-zero value there.
+    int restore_ckpt(const WS &w, int64_t q, int64_t a, hipStream_t st)
+    {
+        const int act = rec_act(a);
+        const size_t U = (size_t)act * Npad, R = (size_t)act * nfr;
+        if (q < 0) {
+            ADS_HIP(hipMemsetAsync(slot(w, 0).p, 0, U * sizeof(T), st));
+            ADS_HIP(hipMemsetAsync(slot(w, 1).p, 0, U * sizeof(T), st));
+            ADS_HIP(hipMemsetAsync(phl(w, 1).p, 0, U * sizeof(T), st));
+            ADS_HIP(hipMemsetAsync(psl(w, 1).p, 0, U * sizeof(T), st));
             return ADS_OK;
         }
-        const T *src = w.ckpt + (size_t)(m - 1) * 4 * nshots * Npad;
-        ADS_HIP(hipMemcpyAsync(slot(w, a).p, src, bytes, hipMemcpyDeviceToDevice, st));
-        ADS_HIP(hipMemcpyAsync(slot(w, a - 1).p, src + F, bytes, hipMemcpyDeviceToDevice, st));
-        ADS_HIP(hipMemcpyAsync(phl(w, a).p, src + 2 * F, bytes, hipMemcpyDeviceToDevice, st));
-        ADS_HIP(hipMemcpyAsync(psl(w, a).p, src + 3 * F, bytes, hipMemcpyDeviceToDevice, st));
+        const T *src = w.ckpt + w.ck_off[q];
+        ADS_HIP(hipMemcpyAsync(slot(w, a).p, src, U * sizeof(T), hipMemcpyDeviceToDevice, st));
+        ADS_HIP(hipMemcpyAsync(slot(w, a - 1).p, src + U, U * sizeof(T), hipMemcpyDeviceToDevice, st));
+        ADS_TRY(frame_copy(true, phl(w, a), const_cast<T *>(src) + 2 * U, act, st));
+        ADS_TRY(frame_copy(true, psl(w, a), const_cast<T *>(src) + 2 * U + R, act, st));
         return ADS_OK;
     }
 
@@ -2101,9 +2189,13 @@ template <typename T> struct AcPlan : AcPlanBase {
             ADS_HIP(hipMemsetAsync(w.psi[q], 0, F, st));
         }
         if (traces && trace_elems) ADS_HIP(hipMemsetAsync(traces, 0, trace_elems * sizeof(T), st));
+        // the reversal schedule's first descent: the states it keeps, in level order
+        Reversal rv(w.M, w.S);
+        const auto path = rv.first_path();
+        size_t pi = 0;
         for (int64_t m = 0; m < w.M; m++) {
-            int64_t a = 1 + m * w.K, b = std::min<int64_t>(a + w.K, NMAX);
-            if (m > 0) ADS_TRY(save_ckpt(w, m, a, st));
+            const int64_t a = seg_level(w, m), b = seg_level(w, m + 1);
+            if (pi < path.size() && path[pi].first == m) ADS_TRY(save_ckpt(w, path[pi++].second, a, st));
             ADS_TRY(run_forward(w, a, b, (const T *)srcv, (T *)traces, traces != nullptr, st));
         }
         return ADS_OK;
@@ -2139,20 +2231,33 @@ template <typename T> struct AcPlan : AcPlanBase {
         ADS_TRY(check_ws(w, ws, wsb, K));
         ADS_TRY(zero_adjoint_state(w, st));
         const T *sv = (const T *)srcv;
-        for (int64_t m = w.M - 1; m >= 0; m--) {
-            int64_t a = 1 + m * w.K, b = std::min<int64_t>(a + w.K, NMAX);
-            if (m < w.M - 1) {
-                ADS_TRY(restore_ckpt(w, m, a, st));
-                ADS_TRY(run_forward(w, a, b, sv, nullptr, 0, st));
-            }
-            if (resid && max_gi_cells > 0 && inj_sg != adj_shape.sg)
-                ADS_TRY(build_inject_tables(adj_shape.sg));  // host uploads: not inside a capture
-            GraphKey key{1, 0, a, b, w.NS, w.seg, sv, nullptr, resid};
-            ADS_TRY(replay(key, w.M, st, [&](hipStream_t s) -> int {
-                for (int64_t n = b; n > a; n--) ADS_TRY(adj_at(w, n, sv, resid, s));
+        struct Ops {
+            AcPlan *pl;
+            const WS &w;
+            const T *sv, *resid;
+            hipStream_t st;
+            int restore(int64_t m, int64_t q) { return pl->restore_ckpt(w, q, pl->seg_level(w, m), st); }
+            int save(int64_t m, int64_t q) { return pl->save_ckpt(w, q, pl->seg_level(w, m), st); }
+            int forward(int64_t m0, int64_t m1)  // segment by segment: one cached graph each
+            {
+                for (int64_t m = m0; m < m1; m++)
+                    ADS_TRY(pl->run_forward(w, pl->seg_level(w, m), pl->seg_level(w, m + 1), sv, nullptr, 0, st));
                 return ADS_OK;
-            }));
-        }
+            }
+            int adjoint(int64_t m)
+            {
+                const int64_t a = pl->seg_level(w, m), b = pl->seg_level(w, m + 1);
+                if (resid && pl->max_gi_cells > 0 && pl->inj_sg != pl->adj_shape.sg)
+                    ADS_TRY(pl->build_inject_tables(pl->adj_shape.sg));  // host uploads: not in a capture
+                GraphKey key{1, 0, a, b, w.NS, w.seg, sv, nullptr, resid};
+                return pl->replay(key, w.M, st, [&](hipStream_t s) -> int {
+                    for (int64_t n = b; n > a; n--) ADS_TRY(pl->adj_at(w, n, sv, resid, s));
+                    return ADS_OK;
+                });
+            }
+        } ops{this, w, sv, resid, st};
+        Reversal rv(w.M, w.S);
+        ADS_TRY(rv.run(ops));
         k_grad_out<T><<<dim3(cdiv(NY + 2, 256), NX + 2), 256, 0, st>>>(
             w.gC, Npad, (int)group_arrays(), (const T *)model, (T *)grad, (int)NX, (int)NY,
             (int)Pitch, p.square_velocity);
@@ -2416,6 +2521,8 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
     P->sx.resize(p->NX + 2);
     P->ty.resize(p->NY + 2);
     ADS_TRY(ads_acoustic_pml_profiles(p, P->sx.data(), P->ty.data()));
+    P->fbox = quiet_box(P->sx, P->ty);
+    P->nfr = P->fbox.cells();
     std::vector<T> sxT(P->sx.begin(), P->sx.end()), tyT(P->ty.begin(), P->ty.end());
     auto flags = [](const std::vector<T> &v, std::vector<uint8_t> &pm, std::vector<uint8_t> &nr) {
         size_t L = v.size();
@@ -2596,16 +2703,44 @@ size_t ads_acoustic_plan_workspace_size(const ads_acoustic_plan *plan, int64_t K
 int64_t ads_acoustic_plan_choose_interval(const ads_acoustic_plan *plan, size_t budget)
 {
     if (!plan || !plan->impl) return 0;
-    // Fewest segments M whose interval fits, then the shortest interval with that M: every
-    // segment but the last is recomputed, so K = ceil(levels / M) minimises the recomputation
-    // ((M - 1) K levels) among the intervals that need M segments.
+    // One-level schedule: the fewest segments M whose interval fits, then the shortest
+    // interval with that M: every segment but the last is recomputed, so K = ceil(levels / M)
+    // minimises the recomputation ((M - 1) K levels) among the intervals that need M segments.
     const int64_t L = std::max<int64_t>(plan->impl->NMAX - 1, 1);
     for (int64_t M = 1; M <= L; M++) {
         const int64_t K = (L + M - 1) / M;
-        if (ads_acoustic_plan_workspace_size(plan, K) <= budget) return K;
+        const size_t need = plan->impl->dtype == ADS_F64
+                                ? static_cast<AcPlan<double> *>(plan->impl.get())->ws_size(K, -1)
+                                : static_cast<AcPlan<float> *>(plan->impl.get())->ws_size(K, -1);
+        if (need <= budget) return K;
         if (K == 1) break;
     }
     return 0;
+}
+
+int ads_acoustic_plan_set_snapshots(ads_acoustic_plan *plan, int64_t snapshots)
+{
+    clear_error();
+    if (!plan || !plan->impl) ADS_FAIL(ADS_ERR_ARG, "null plan");
+    plan->impl->snapshots = snapshots < 0 ? -1 : snapshots;
+    return ADS_OK;
+}
+
+int64_t ads_acoustic_plan_snapshots(const ads_acoustic_plan *plan)
+{
+    return plan && plan->impl ? plan->impl->snapshots : -1;
+}
+
+int ads_acoustic_plan_choose_schedule(ads_acoustic_plan *plan, size_t budget, int64_t *K, int64_t *S)
+{
+    clear_error();
+    if (!plan || !plan->impl || !K || !S) ADS_FAIL(ADS_ERR_ARG, "null argument");
+    const int rc = plan->impl->dtype == ADS_F64
+                       ? static_cast<AcPlan<double> *>(plan->impl.get())->choose_schedule(budget, K, S)
+                       : static_cast<AcPlan<float> *>(plan->impl.get())->choose_schedule(budget, K, S);
+    if (rc != ADS_OK) ADS_FAIL(rc, "no checkpoint schedule fits %zu bytes", budget);
+    plan->impl->snapshots = *S;
+    return ADS_OK;
 }
 
 int ads_acoustic_plan_forward(ads_acoustic_plan *plan, const void *model, const void *srcv,

@@ -7,6 +7,7 @@
 #include <numeric>
 
 #include "ads_common.h"
+#include "ads_schedule.h"
 
 namespace ads {
 
@@ -175,6 +176,35 @@ static void acoustic_axis(int64_t len, double h, double origin, int64_t NXg, int
         }
         out[k] = v;
     }
+}
+
+int ads_checkpoint_schedule(int64_t M, int64_t S, int64_t *ops, int64_t cap, int64_t *nops)
+{
+    clear_error();
+    if (M < 0 || !nops || (cap > 0 && !ops)) ADS_FAIL(ADS_ERR_ARG, "bad argument");
+    struct Rec {
+        int64_t *ops, cap, n = 0;
+        void put(int64_t k, int64_t a, int64_t b)
+        {
+            if (n < cap) {
+                ops[3 * n] = k;
+                ops[3 * n + 1] = a;
+                ops[3 * n + 2] = b;
+            }
+            n++;
+        }
+        int restore(int64_t m, int64_t q) { put(0, m, q); return ADS_OK; }
+        int forward(int64_t m0, int64_t m1) { put(1, m0, m1); return ADS_OK; }
+        int save(int64_t m, int64_t q) { put(2, m, q); return ADS_OK; }
+        int adjoint(int64_t m) { put(3, m, 0); return ADS_OK; }
+    } rec{ops, cap};
+    Reversal rv(M, S < 0 ? std::max<int64_t>(M - 1, 0) : std::min(S, std::max<int64_t>(M - 1, 0)));
+    for (const auto &ps : rv.first_path()) rec.put(4, ps.first, ps.second);
+    rv.run(rec);
+    *nops = rec.n;
+    if (rec.n > cap) ADS_FAIL(ADS_ERR_SHAPE, "schedule has %lld operations, room for %lld",
+                              (long long)rec.n, (long long)cap);
+    return ADS_OK;
 }
 
 int ads_acoustic_pml_profiles(const ads_acoustic_params *p, double *sigma_x, double *tau_y)

@@ -114,6 +114,37 @@ def workload(ads, small=False, nstep=0, kernel=1):
     return p, c_true, c0, srcs, rcvs
 
 
+def workload_uneven(ads, seed=4, nshots=64, NX=2000, nstep_range=(3000, 6000)):
+    """2-D acoustic FWI gradient workload with uneven shots: NX x NX interior cells, h = 5 m,
+    dt = 0.5 ms (CFL 0.5 at 5000 m/s), 40-cell PML, smooth random velocity 1500-5000 m/s
+    (Gaussian-filtered noise, sigma 40 cells), `nshots` Ricker shots (8-12 Hz) at depth 3 evenly
+    spaced along the surface, record lengths uniform in nstep_range, spreads of 400-2000
+    receivers centred on each source.  Starting model: sigma = 30 cell smoothing.
+    Returns (params, c_true, c0, sources, receivers, record lengths)."""
+    from scipy.ndimage import gaussian_filter
+
+    rng = np.random.default_rng(seed)
+    NY = NX
+    dt, h = 5e-4, 5.0
+    nst = rng.integers(nstep_range[0], nstep_range[1] + 1, nshots)
+    p = ads.AcousticPropagatorParams(NX=NX, NY=NY, NSTEP=int(nst.max()), DELTAX=h, DELTAY=h,
+                                     DELTAT=dt, NPOINTS_PML=40, vp_ref=3000.0, PropagatorKernel=1)
+    noise = gaussian_filter(rng.standard_normal((NX + 2, NY + 2)), 40.0)
+    noise = (noise - noise.min()) / (noise.max() - noise.min())
+    c_true = 1500.0 + 3500.0 * noise
+    ads.check_cfl(p, float(c_true.max()))
+    c0 = gaussian_filter(c_true, 30.0)
+    xs = np.round(np.linspace(41, NX - 40, nshots)).astype(int)
+    srcs, rcvs = [], []
+    for s in range(nshots):
+        w = ads.ricker_freq(8.0 + 4.0 * rng.random(), dt, int(nst[s]), 1e4)
+        srcs.append(ads.AcousticSource([int(xs[s])], [3], w.reshape(-1, 1)))
+        nr = int(rng.integers(400, 2001))
+        lo = int(np.clip(xs[s] - nr // 2, 1, NX + 2 - nr))
+        rcvs.append(ads.AcousticReceiver(np.arange(lo, lo + nr), np.full(nr, 3)))
+    return p, c_true, c0, srcs, rcvs, nst
+
+
 def _cpu_gradient(O, p, c0, src, nsteps, threads):
     """Oracle forward + adjoint of one shot over the first nsteps steps on `threads` threads
     (1: the serial checker itself, scatter-form step VJPs; more: the gather-form sweep)."""

@@ -254,6 +254,30 @@ size_t ads_acoustic_plan_workspace_size(const ads_acoustic_plan *plan, int64_t K
  * interval fits in budget_bytes, and among the intervals with M segments the shortest,
  * K = ceil((NSTEP-1)/M), which minimises the recomputed levels ((M-1) K). */
 int64_t ads_acoustic_plan_choose_interval(const ads_acoustic_plan *plan, size_t budget_bytes);
+/* Checkpoint schedule: binomial / sqrt(N) checkpointing with recomputation, in place of the
+ * store-every-level tape of Core.jl:570-572, 614.  The NSTEP-1 levels are cut
+ * into M = ceil((NSTEP-1)/K) segments; a snapshot holds u^a, u^{a-1} on the full grid and
+ * phi^a, psi^a outside the box where both PML profiles vanish, for the shots active at a.
+ *   S < 0 (the default): one snapshot per segment start (one-level sqrt(N) scheme, each
+ *         segment but the last rebuilt once);
+ *   0 <= S < M-1: S snapshot slots reused by the binomial (Revolve-type) reversal schedule.
+ * ads_acoustic_plan_workspace_size(K) follows the plan's setting, and a backward must follow a
+ * forward made with the same K and setting. */
+int ads_acoustic_plan_set_snapshots(ads_acoustic_plan *plan, int64_t snapshots);
+int64_t ads_acoustic_plan_snapshots(const ads_acoustic_plan *plan);
+/* The checkpoint schedule over M segments with S snapshot slots (S < 0: one per segment
+ * start), as rows of three int64 (kind, a, b) in execution order:
+ *   4 (m, q): the forward pass keeps state m (the state at the start of segment m) in slot q;
+ *   then the adjoint sweep: 0 (m, q) restore state m from slot q (q < 0: the zero state);
+ *   1 (m0, m1) advance from state m0 through segments m0 .. m1-1; 2 (m, q) save state m in
+ *   slot q; 3 (m, -) adjoint sweep of segment m.  Host arrays; ADS_ERR_SHAPE (with *nops set)
+ *   when cap rows are too few. */
+int ads_checkpoint_schedule(int64_t M, int64_t S, int64_t *ops, int64_t cap, int64_t *nops);
+/* The (K, S) with the least modelled time (forward + adjoint + schedule rebuilds, weighted by
+ * the shots active at each level) whose workspace fits budget_bytes; sets the plan's snapshot
+ * count to S.  ADS_ERR_WORKSPACE if nothing fits. */
+int ads_acoustic_plan_choose_schedule(ads_acoustic_plan *plan, size_t budget_bytes, int64_t *K,
+                                      int64_t *S);
 
 /* Forward: traces of every shot (model = velocity or C as square_velocity says).  Keeps the
  * checkpoints in the workspace for a following ads_acoustic_plan_backward. */

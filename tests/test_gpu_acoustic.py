@@ -452,3 +452,104 @@ def test_graph_replay_matches_stream_launches(orc, ads, cuda, kernel):
         plan.set_graph(False)
         _ = plan.forward(ct * 0.99, srcv, 9)
         plan.set_graph(True)
+
+
+# ------------------------------------------------------------------------------------------
+# checkpoint schedules: binomial reversal with S snapshot slots
+# ------------------------------------------------------------------------------------------
+
+def _min_rebuilds(n, s, memo={}):
+    """Least segment advances to reverse n segments from a stored state with s free slots, by
+    exhaustive search over every first split (each segment is rebuilt once before its adjoint;
+    the initial forward has already made the first descent's advances)."""
+    key = (n, s)
+    if key not in memo:
+        if n == 1:
+            memo[key] = 1
+        elif s == 0:
+            memo[key] = n * (n + 1) // 2
+        else:
+            memo[key] = min(j + _min_rebuilds(n - j, s - 1) + _min_rebuilds(j, s)
+                            for j in range(1, n))
+    return memo[key]
+
+
+def _fwd_levels(lib, L):
+    import ctypes as C
+    tm, n, b = C.c_double(), C.c_int64(), C.c_double()
+    L.check(lib.ads_profile_query(0, C.byref(tm), C.byref(n), C.byref(b)))
+    return n.value
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_binomial_schedules_match_one_level(orc, ads, cuda, kernel, dtype):
+    """Every snapshot count S gives the one-level schedule's misfit and gradient bit for bit
+    (the rebuilt levels are the same arithmetic), the FP64 gradient matches the oracle, and the
+    forward levels run equal the initial pass plus K times the fewest segment rebuilds an
+    exhaustive search finds for (M segments, S slots) -- the schedule is optimal."""
+    import importlib
+    L = importlib.import_module("adseismic_amd._lib")
+    p, c, srcs, rcvs, nstep = make_case(orc, ads, NX=33, NY=29, NSTEP=121, nshots=3,
+                                        kernel=kernel, seed=5)
+    obs, g_ref = [], np.zeros_like(c)
+    for s in range(3):
+        o = oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s], cmod=c * 1.03)
+        g_ref += oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s], obs=o)[2]
+        obs.append(o)
+    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=dtype, nstep=nstep)
+    lib = plan.lib
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    ct, ob = T(c, dtype), plan.pack_traces(obs)
+    levels = p.NSTEP - 1
+    for K in (8, 10):                     # 15 and 12 segments of exactly K levels
+        M = levels // K
+        plan.set_snapshots(-1)
+        L1, g1, tr1 = plan.gradient(ct, srcv, ob, K, want_traces=True)
+        g1 = g1.cpu().numpy()
+        if dtype == torch.float64:
+            assert rel(g1, g_ref) < 1e-10
+        else:
+            assert rel(g1, g_ref) < 1e-3
+        sizes = []
+        for S in (0, 1, 2, 3, 5, M - 2, M - 1, M + 4):
+            plan.set_snapshots(S)
+            sizes.append(plan.workspace_size(K))
+            L.check(lib.ads_profile_begin())
+            LS, gS, trS = plan.gradient(ct, srcv, ob, K, want_traces=True)
+            torch.cuda.synchronize()
+            L.check(lib.ads_profile_end())
+            ran = _fwd_levels(lib, L)
+            assert LS.item() == L1.item(), (K, S)
+            assert torch.equal(trS, tr1), (K, S)
+            np.testing.assert_array_equal(gS.cpu().numpy(), g1, err_msg=f"K {K}, S {S}")
+            s_eff = min(S, M - 1)
+            want = levels + K * (_min_rebuilds(M, s_eff) - M)
+            assert ran == want, (K, S, ran, want)
+        assert sizes == sorted(sizes) and sizes[0] < sizes[-1]
+        assert sizes[-2] == sizes[-1]   # S >= M-1 is the one-level layout
+
+
+def test_choose_schedule_fits_and_prefers_fewer_rebuilds(orc, ads, cuda):
+    """choose_schedule returns a (K, S) whose workspace fits the budget; with room for the
+    whole record it rebuilds nothing, and a budget below every one-level layout still gets a
+    binomial schedule whose gradient equals the one-level gradient."""
+    p, c, srcs, rcvs, nstep = make_case(orc, ads, NX=33, NY=29, NSTEP=121, nshots=2, kernel=1,
+                                        seed=6, uneven=True)
+    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float32, nstep=nstep)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    obs = [oracle_shot(orc, p, c, srcs[s], rcvs[s], nstep[s], cmod=c * 1.03) for s in range(2)]
+    ct, ob = T(c, torch.float32), plan.pack_traces(obs)
+    levels = p.NSTEP - 1
+    plan.set_snapshots(-1)
+    one = {K: plan.workspace_size(K) for K in range(1, levels + 1)}
+    K, S = plan.choose_schedule(one[levels])
+    assert (K, S) == (levels, 0) and plan.workspace_size(K) <= one[levels]
+    L1, g1, _ = plan.gradient(ct, srcv, ob, K)
+    budget = min(one.values()) - 1                     # no one-level layout fits
+    K, S = plan.choose_schedule(budget)
+    assert plan.snapshots == S and S < -(-levels // K) - 1
+    assert plan.workspace_size(K) <= budget
+    LS, gS, _ = plan.gradient(ct, srcv, ob, K)
+    assert LS.item() == pytest.approx(L1.item(), rel=1e-6)
+    assert rel(gS.cpu().numpy(), g1.cpu().numpy()) < 1e-6
