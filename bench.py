@@ -1,21 +1,27 @@
 """Benchmark: batched FP32 acoustic FWI gradient (forward + hand-written adjoint) on one MI355X.
 
-Workload (the metric's configuration): 2-D acoustic, 1000 x 1000 interior cells, h = 5 m,
-dt = 0.5 ms, 4000 steps, 40-cell PML, random layered velocity model (8-20 layers,
-1500-4500 m/s increasing with depth, 3 Gaussian anomalies of +-15%, seed 2), 32 Ricker shots
-(8-15 Hz) evenly spaced along the surface, 1000 receivers per shot.  Observed data are simulated
-from the true model; the gradient is taken at a smoothed starting model.  One "step" is one
-complete FWI gradient of all 32 shots: forward with checkpoints, misfit, checkpoint
-recomputation and the adjoint sweep, all in one batched call.
+Headline workload: 2-D acoustic FWI gradient with checkpointing, 2000 x 2000 interior cells,
+h = 5 m, dt = 0.5 ms, 40-cell PML, smooth random velocity 1500-5000 m/s, 64 shots of uneven
+record length (uniform in 3000-6000 steps) and uneven receiver spreads (400-2000 receivers), all
+shots in one plan: one forward sweep keeping checkpoints (u^a, u^{a-1} in full, phi/psi outside
+the damping-free box), then the reversal schedule that rebuilds each segment from its checkpoint
+before its adjoint sweep.  One "step" is one complete FWI gradient of all 64 shots (forward,
+misfit, rebuilds and adjoint) in one batched call.  Observed data are simulated from the true
+model; the gradient is taken at a smoothed starting model.
 
 Prints ONE JSON line.  Besides value/ms_per_step it carries:
-  roofline      : the dominant stencil kernel's algorithmic HBM bytes per launch / its average
-                  launch duration (HIP events on the launch stream, over the timed region);
+  roofline      : the dominant stencil class's algorithmic HBM bytes per level / its average
+                  time per level (HIP events on the launch stream, over the timed region), with
+                  the rocprofv3-measured HBM bytes per level;
   cpu_baseline  : the FP64 CPU restatement (oracle/) on the process's threads, with a one-thread
                   run beside it (like the reference's serial CPU op), on a bounded sample of
-                  the same workload;
+                  the headline workload;
+  acoustic_1000x1000_32shots : the 1000 x 1000, 4000-step, 32-shot batched gradient, with
+                  PropagatorKernel 1 and (the reference's default) 0;
   plain_single_shot : the plain drop-in path (AcousticPropagatorSolver + SimulatedObservation_
-                  + autograd, full histories, one shot) on the same grid.
+                  + autograd, full histories, one shot) on the 1000 x 1000 grid, and the batched
+                  path's throughput over it;
+  elastic       : the 600 x 600, 3000-step, 16-shot elastic gradient (d/d rho, lambda, mu).
 """
 from __future__ import annotations
 
@@ -34,7 +40,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBPS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_acoustic_1000x1000_32shots.txt")
+TRAFFIC_FILES = {
+    "uneven": os.path.join(ROOT, "profiles", "traffic_acoustic_2000x2000_64shots.txt"),
+    "layered": os.path.join(ROOT, "profiles", "traffic_acoustic_1000x1000_32shots.txt"),
+}
 
 
 # Kernel shape of the acoustic plan on the benchmark workload: forward 8 rows per wavefront (the
@@ -47,24 +56,25 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_acoustic_1000x1000_32shot
 DEFAULT_SHAPE = "8,16,1,2,16,1"
 
 
-def measured_traffic(kernel, shape):
-    """HBM bytes per launch of a stencil class, from the committed rocprofv3 PMC summary
-    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a plain device copy of known size).
-    Lines: '<class> <bytes> <rows,shots,batch>'.  Returns (bytes, profiled shape); bytes is None
-    when the summary is missing or was taken at another kernel shape."""
+def measured_traffic(kernel, shape, path):
+    """Measured HBM bytes over algorithmic bytes of a stencil class, from the committed
+    rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a plain device copy
+    of known size).  Lines: '<class> <measured bytes per level> <rows,shots,batch> <algorithmic
+    bytes per level of the profiled run>'.  Returns (ratio, profiled shape); ratio is None when
+    the summary is missing or was taken at another kernel shape."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             lines = [ln.split() for ln in f if ln.strip() and not ln.startswith("#")]
     except OSError:
         return None, None
-    kv = {ln[0]: ln[1:] for ln in lines if len(ln) >= 3}
+    kv = {ln[0]: ln[1:] for ln in lines if len(ln) >= 4}
     if kernel not in kv:
         return None, None
     prof_shape = tuple(int(x) for x in kv[kernel][1].split(","))
     half = None if shape is None else tuple(shape[:3] if kernel == "ac_fwd_step" else shape[3:])
     if half != prof_shape:
         return None, prof_shape
-    return float(kv[kernel][0]), prof_shape
+    return float(kv[kernel][0]) / float(kv[kernel][2]), prof_shape
 
 
 def velocity_model(NX, NY, seed):
@@ -145,14 +155,14 @@ def workload_uneven(ads, seed=4, nshots=64, NX=2000, nstep_range=(3000, 6000)):
     return p, c_true, c0, srcs, rcvs, nst
 
 
-def _cpu_gradient(O, p, c0, src, nsteps, threads):
+def _cpu_gradient(O, p, c0, src, rcv, nsteps, threads):
     """Oracle forward + adjoint of one shot over the first nsteps steps on `threads` threads
     (1: the serial checker itself, scatter-form step VJPs; more: the gather-form sweep)."""
     O.set_num_threads(threads)
     NX, NY = p.NX, p.NY
     Sx, Sy = O.acoustic_pml(NX, NY, p.DELTAX, p.DELTAY, p.NPOINTS_PML, p.Rcoef, p.vp_ref)
     srcv = np.asarray(src.srcv)[:nsteps]
-    rcvi, rcvj = np.arange(2, NX + 2), np.full(NX, 3)
+    rcvi, rcvj = np.asarray(rcv.rcvi), np.asarray(rcv.rcvj)
     t0 = time.perf_counter()
     u, ph, ps = O.acoustic_forward(1, NX, NY, nsteps, p.DELTAT, p.DELTAX, p.DELTAY, Sx, Sy,
                                    c0 ** 2, src.srci, src.srcj, srcv)
@@ -168,7 +178,18 @@ def _cpu_gradient(O, p, c0, src, nsteps, threads):
     return NX * NY * (nsteps - 1) / t / 1e9, t
 
 
-def cpu_baseline(p, c0, src, nsteps):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(p, c0, src, rcv, nsteps, what):
     """Oracle (FP64) forward + adjoint of one shot over the first nsteps steps, on all the
     threads this process may use (OMP_NUM_THREADS caps the affinity set) and on one thread,
     like the reference's serial custom op."""
@@ -177,18 +198,19 @@ def cpu_baseline(p, c0, src, nsteps):
     threads = len(os.sched_getaffinity(0))
     if os.environ.get("OMP_NUM_THREADS", "").isdigit():
         threads = max(1, min(threads, int(os.environ["OMP_NUM_THREADS"])))
-    sample = (f"1 shot, {p.NX}x{p.NY} grid, first {nsteps} of {p.NSTEP} steps, FP64 "
-              "forward+adjoint (oracle/ads_oracle.c, PropagatorKernel 1)")
-    v1, t1 = _cpu_gradient(O, p, c0, src, nsteps, 1)
+    sample = (f"1 shot of {what}, {p.NX}x{p.NY} grid, first {nsteps} steps, "
+              f"{len(rcv.rcvi)} receivers, FP64 forward+adjoint (oracle/ads_oracle.c, "
+              "PropagatorKernel 1)")
+    v1, t1 = _cpu_gradient(O, p, c0, src, rcv, nsteps, 1)
     out = {"value": v1, "unit": "Gcell-steps/s", "cores": 1, "kind": "port",
            "sample": f"{sample}, one thread, {t1:.1f} s"}
     if threads > 1:
-        vm, tm = _cpu_gradient(O, p, c0, src, nsteps, threads)
+        vm, tm = _cpu_gradient(O, p, c0, src, rcv, nsteps, threads)
         out = {"value": vm, "unit": "Gcell-steps/s", "cores": threads, "kind": "port",
                "sample": f"{sample}, {threads} threads (OpenMP rows, gather-form adjoint), {tm:.1f} s",
                "single_thread": out}
     O.set_num_threads(1)
-    out["host_cpu"] = platform.processor() or platform.machine()
+    out["host_cpu"] = cpu_model()
     out["host_threads_available"] = len(os.sched_getaffinity(0))
     return out
 
@@ -304,28 +326,125 @@ def plain_single_shot(ads, p, c0, src, rcv, obs_shot):
                     "one shot, FP32)"}
 
 
+def acoustic_leg(ads, L, lib, args, p, c_true, c0, srcs, rcvs, nstep, steps, warmup, traffic):
+    """Batched FP32 acoustic gradient: one plan over every shot, the fastest checkpoint schedule
+    that fits 90% of the free device memory (or --interval/--snapshots), `warmup` untimed and
+    `steps` timed gradients.  Returns (result dict, observed traces of shot 0)."""
+    import torch
+
+    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float32, nstep=nstep)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    if args.interval:
+        plan.set_snapshots(args.snapshots)
+        K, S = args.interval, args.snapshots
+    else:
+        K, S = plan.choose_schedule(int(torch.cuda.mem_get_info()[0] * 0.90))
+    ws_bytes = plan.workspace_size(K)
+    M = -(-(int(plan.nstep.max()) - 1) // K)
+    ct = torch.as_tensor(c_true, dtype=torch.float32, device="cuda")
+    c0t = torch.as_tensor(c0, dtype=torch.float32, device="cuda")
+    obs = plan.forward(ct, srcv, K).clone()
+    torch.cuda.synchronize()
+    if args.autotune:
+        shape = plan.autotune(c0t, srcv, K, reps=3)
+    else:
+        shape = tuple(int(x) for x in args.shape.split(","))
+        plan.set_shape(shape)
+    for _ in range(warmup):
+        plan.gradient(c0t, srcv, obs, K)
+    torch.cuda.synchronize()
+    L.check(lib.ads_profile_begin())
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss, grad, _ = plan.gradient(c0t, srcv, obs, K)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    L.check(lib.ads_profile_end())
+    ms = t / steps * 1e3
+    cells = float(np.sum(p.NX * p.NY * np.asarray(plan.nstep, dtype=np.float64)))
+    if not math.isfinite(float(loss.item())):
+        raise RuntimeError("non-finite misfit")
+    prof = {}
+    for name, cls in (("ac_fwd_step", 0), ("ac_adj_step", 1)):
+        tm, n, b = C.c_double(), C.c_int64(), C.c_double()
+        L.check(lib.ads_profile_query(cls, C.byref(tm), C.byref(n), C.byref(b)))
+        prof[name] = (tm.value, n.value, b.value)
+    dom = max(prof, key=lambda k: prof[k][0])
+    tm, n, b = prof[dom]
+    achieved = (b / n) / (tm / n * 1e-3) / 1e9 if n else 0.0
+    ratio, prof_shape = measured_traffic(dom, shape, traffic) if (traffic and p.PropagatorKernel == 1) \
+        else (None, None)
+    # HBM bytes per level at this run's mix of active shots: the profiled measured/algorithmic
+    # ratio times this run's algorithmic bytes per level
+    hbm = round(ratio * b / n) if (ratio and n) else None
+    out = {
+        "value": round(cells / (ms * 1e-3) / 1e9, 4), "unit": "Gcell-steps/s",
+        "ms_per_gradient": round(ms, 3), "cell_steps": cells,
+        "schedule": {"interval": K, "segments": M, "snapshots": S,
+                     "kind": "one-level" if S >= M - 1 else "binomial",
+                     "workspace_GB": round(ws_bytes / 1e9, 2)},
+        "kernel_shape": {"fwd_rows": shape[0], "fwd_shots": shape[1], "fwd_batch": shape[2],
+                         "adj_rows": shape[3], "adj_shots": shape[4], "adj_batch": shape[5]},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
+                     "traffic": hbm,
+                     "traffic_source": os.path.relpath(traffic, ROOT) if hbm else
+                     (f"profiled at shape {prof_shape}, not this run's" if prof_shape else None),
+                     "avg_level_us": round(tm / n * 1e3, 2) if n else None, "levels": n,
+                     "alg_bytes_per_level": round(b / n) if n else None},
+        "kernels": {k: {"total_ms": round(v[0], 2), "levels": v[1],
+                        "avg_us": round(v[0] / v[1] * 1e3, 2) if v[1] else None,
+                        "alg_GBps": round(v[2] / (v[0] * 1e-3) / 1e9, 1) if v[0] else None}
+                    for k, v in prof.items()},
+        "loss": float(loss.item()),
+    }
+    # algorithmic bytes of the whole gradient: every cell-step once forward, w (4 + 4 f), and
+    # once adjoint, w (7 + 4 f) (f: interior fraction in the PML strips); rebuilt levels are
+    # overhead.  Over the gradient's wall time.
+    sx = np.zeros(p.NX + 2)
+    ty = np.zeros(p.NY + 2)
+    L.check(lib.ads_acoustic_pml_profiles(C.byref(p.to_c()), sx.ctypes.data, ty.ctypes.data))
+    f = float(np.mean((sx[1:-1, None] != 0) | (ty[None, 1:-1] != 0)))
+    alg = cells * 4.0 * (11.0 + 8.0 * f)
+    out["whole_gradient"] = {"alg_bytes_per_cell_step": round(4.0 * (11.0 + 8.0 * f), 2),
+                             "alg_GBps": round(alg / (ms * 1e-3) / 1e9, 1),
+                             "frac": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4)}
+    obs0 = plan.unpack_traces(obs)[0].cpu().numpy()
+    plan._ws = None
+    del grad, obs
+    torch.cuda.empty_cache()
+    return out, obs0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--small", action="store_true", help="reduced grid for quick checks")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=300)
-    ap.add_argument("--interval", type=int, default=0, help="checkpoint interval (0 = auto)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="headline leg only (no 1000x1000, plain, elastic or CPU legs)")
+    ap.add_argument("--cpu-steps", type=int, default=150)
+    ap.add_argument("--interval", type=int, default=0,
+                    help="checkpoint interval (0 = the schedule chooser's)")
+    ap.add_argument("--snapshots", type=int, default=-1,
+                    help="snapshot slots with --interval (-1: one per segment start)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the kernel shape with the plan's autotuner instead of --shape")
     ap.add_argument("--shape", default=DEFAULT_SHAPE,
                     help="kernel shape 'fr,fs,fb,ar,as,ab' (rows per wavefront, shots per "
-                         "workgroup, shots per batch of the forward and adjoint kernels); the "
-                         "default is the autotuner's pick for this workload, fixed so that the "
-                         "profiled HBM traffic refers to the same launches")
-    ap.add_argument("--nstep", type=int, default=0,
-                    help="override the record length (profiling runs only; not the metric)")
+                         "workgroup, shots per batch of the forward and adjoint kernels), fixed "
+                         "so that the profiled HBM traffic refers to the same launches")
+    ap.add_argument("--max-nstep", type=int, default=0,
+                    help="clip every record to this many steps (profiling runs only; not the "
+                         "metric)")
+    ap.add_argument("--workload", choices=("uneven", "layered"), default="uneven",
+                    help="headline workload: 2000x2000 x 64 uneven shots, or the 1000x1000 x 32 "
+                         "layered one (profiling runs)")
     ap.add_argument("--no-elastic", action="store_true", help="skip the elastic gradient leg")
     ap.add_argument("--propagator-kernel", type=int, default=1, choices=(0, 1),
-                    help="PropagatorKernel of the acoustic workload (0: the reference's default "
+                    help="PropagatorKernel of the headline workload (0: the reference's default "
                          "Core.jl form, phi/psi from the new u; 1: the custom-op form)")
     ap.add_argument("--elastic-only", action="store_true",
                     help="run only the elastic gradient leg (profiling runs; no bench line)")
@@ -352,8 +471,6 @@ def main():
         print(json.dumps({"elastic": elastic_leg(ads, L, lib, args.steps, args.elastic_nstep)}),
               flush=True)
         return
-    p, c_true, c0, srcs, rcvs = workload(ads, small=args.small, nstep=args.nstep,
-                                         kernel=args.propagator_kernel)
     if args.calib_copy:
         n = args.calib_copy * (1 << 20) // 4
         a = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -361,82 +478,65 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float32)
-    srcv = plan.pack_srcv([s.srcv for s in srcs])
-    K = args.interval or plan.choose_interval(int(torch.cuda.mem_get_info()[0] * 0.80))
-    ct = torch.as_tensor(c_true, dtype=torch.float32, device="cuda")
-    c0t = torch.as_tensor(c0, dtype=torch.float32, device="cuda")
-    obs = plan.forward(ct, srcv, K).clone()
-    torch.cuda.synchronize()
-    if args.autotune:
-        shape = plan.autotune(c0t, srcv, K, reps=3)
+    if args.workload == "uneven":
+        p, c_true, c0, srcs, rcvs, nst = workload_uneven(ads)
+        p.PropagatorKernel = args.propagator_kernel
+        what = (f"acoustic {p.NX}x{p.NY}, h=5 m, dt=0.5 ms, 40-cell PML, smooth random model "
+                f"1500-5000 m/s, {len(srcs)} shots of uneven record length "
+                f"({int(nst.min())}-{int(nst.max())} steps) and uneven receiver spreads "
+                f"({min(len(r.rcvi) for r in rcvs)}-{max(len(r.rcvi) for r in rcvs)} receivers), "
+                "FP32 forward + hand-written adjoint, all shots in one checkpointed batched call")
     else:
-        shape = tuple(int(x) for x in args.shape.split(","))
-        plan.set_shape(shape)
-    for _ in range(args.warmup):
-        plan.gradient(c0t, srcv, obs, K)
-    torch.cuda.synchronize()
-    L.check(lib.ads_profile_begin())
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        loss, grad, _ = plan.gradient(c0t, srcv, obs, K)
-    torch.cuda.synchronize()
-    t = time.perf_counter() - t0
-    L.check(lib.ads_profile_end())
-    ms = t / args.steps * 1e3
-    cells = float(np.sum(p.NX * p.NY * plan.nstep))
-    value = cells / (ms * 1e-3) / 1e9
-    if not math.isfinite(float(loss.item())):
-        raise RuntimeError("non-finite misfit")
-
-    prof = {}
-    for name, cls in (("ac_fwd_step", 0), ("ac_adj_step", 1)):
-        tm, n, b = C.c_double(), C.c_int64(), C.c_double()
-        L.check(lib.ads_profile_query(cls, C.byref(tm), C.byref(n), C.byref(b)))
-        prof[name] = (tm.value, n.value, b.value)
-    dom = max(prof, key=lambda k: prof[k][0])
-    tm, n, b = prof[dom]
-    achieved = (b / n) / (tm / n * 1e-3) / 1e9 if n else 0.0
-    # the committed PMC summary was taken with PropagatorKernel 1
-    traffic, traffic_shape = measured_traffic(dom, shape) if p.PropagatorKernel == 1 else (None, None)
+        p, c_true, c0, srcs, rcvs = workload(ads, kernel=args.propagator_kernel)
+        nst = np.full(len(srcs), p.NSTEP)
+        what = (f"acoustic {p.NX}x{p.NY}, h=5 m, dt=0.5 ms, {p.NSTEP} steps, 40-cell PML, "
+                f"{len(srcs)} shots x {p.NX} receivers, FP32 forward + hand-written adjoint, "
+                "one batched call per gradient")
+    if args.max_nstep:
+        nst = np.minimum(nst, args.max_nstep)
+        srcs = [ads.AcousticSource(s.srci, s.srcj, np.asarray(s.srcv)[:int(k)])
+                for s, k in zip(srcs, nst)]
+        p.NSTEP = int(nst.max())
+    head, _ = acoustic_leg(ads, L, lib, args, p, c_true, c0, srcs, rcvs, nst, args.steps,
+                           args.warmup, TRAFFIC_FILES[args.workload])
     out = {
         "metric": "FWI gradient Gcell-steps/s (fwd+adj), batched",
-        "value": round(value, 4), "unit": "Gcell-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": (f"acoustic {p.NX}x{p.NY}, h=5 m, dt=0.5 ms, {p.NSTEP} steps, "
-                                f"{p.NPOINTS_PML}-cell PML, {plan.nshots} shots x {p.NX} "
-                                "receivers, FP32 forward + hand-written adjoint, one batched "
-                                "call per gradient"),
-                   "PropagatorKernel": p.PropagatorKernel, "checkpoint_interval": K,
-                   "kernel_shape": {"fwd_rows": shape[0], "fwd_shots": shape[1],
-                                    "fwd_batch": shape[2], "adj_rows": shape[3],
-                                    "adj_shots": shape[4], "adj_batch": shape[5]} if shape else None,
-                   "shots": plan.nshots, "cells": p.NX * p.NY, "nstep": p.NSTEP},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                     "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
-                     "traffic": traffic, "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)
-                     if traffic else (f"profiled at shape {traffic_shape}, not this run's"
-                                      if traffic_shape else None),
-                     "avg_launch_us": round(tm / n * 1e3, 2) if n else None,
-                     "launches": n, "alg_bytes_per_launch": round(b / n) if n else None},
-        "kernels": {k: {"total_ms": round(v[0], 2), "launches": v[1],
-                        "avg_us": round(v[0] / v[1] * 1e3, 2) if v[1] else None,
-                        "alg_GBps": round(v[2] / (v[0] * 1e-3) / 1e9, 1) if v[0] else None}
-                    for k, v in prof.items()},
-        "loss": float(loss.item()),
+        "value": head.pop("value"), "unit": head.pop("unit"), "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head.pop("ms_per_gradient"),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": what, "PropagatorKernel": p.PropagatorKernel, "shots": len(srcs),
+                   "cells": p.NX * p.NY, "nstep_min": int(nst.min()), "nstep_max": int(nst.max()),
+                   "cell_steps": head.pop("cell_steps"), "schedule": head.pop("schedule"),
+                   "kernel_shape": head.pop("kernel_shape")},
+        "roofline": head.pop("roofline"),
     }
-    tr_true = plan.unpack_traces(obs)[0].cpu().numpy()
-    plan._ws = None  # release the batched workspace before the full-history run
-    del grad
-    torch.cuda.empty_cache()
-    if not args.no_elastic:
-        out["elastic"] = elastic_leg(ads, L, lib, 1, args.elastic_nstep)
-    if not args.no_plain:
-        out["plain_single_shot"] = plain_single_shot(ads, p, c0, srcs[0], rcvs[0], tr_true)
-        out["plain_single_shot"]["batched_over_plain"] = round(value / out["plain_single_shot"]["value"], 2)
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, c0, srcs[0], min(args.cpu_steps, p.NSTEP))
+    out.update(head)
+    if not args.no_extra and args.workload == "uneven" and not args.max_nstep:
+        layered = {}
+        pl, ct_l, c0_l, srcs_l, rcvs_l = workload(ads, kernel=1)
+        nst_l = np.full(len(srcs_l), pl.NSTEP)
+        r1, obs0 = acoustic_leg(ads, L, lib, args, pl, ct_l, c0_l, srcs_l, rcvs_l, nst_l, 2, 1,
+                                TRAFFIC_FILES["layered"])
+        layered["PropagatorKernel_1"] = r1
+        pl0, ct0, c00, srcs0, rcvs0 = workload(ads, kernel=0)
+        r0, _ = acoustic_leg(ads, L, lib, args, pl0, ct0, c00, srcs0, rcvs0, nst_l, 2, 1, None)
+        layered["PropagatorKernel_0"] = r0
+        layered["workload"] = (f"acoustic {pl.NX}x{pl.NY}, h=5 m, dt=0.5 ms, {pl.NSTEP} steps, "
+                               f"{pl.NPOINTS_PML}-cell PML, random layered model, {len(srcs_l)} "
+                               f"shots x {pl.NX} receivers, FP32")
+        out["acoustic_1000x1000_32shots"] = layered
+        if not args.no_plain:
+            pp = plain_single_shot(ads, pl, c0_l, srcs_l[0], rcvs_l[0], obs0)
+            pp["batched_over_plain"] = round(r1["value"] / pp["value"], 2)
+            pp["batched_path"] = "acoustic_1000x1000_32shots.PropagatorKernel_1"
+            out["plain_single_shot"] = pp
+        if not args.no_elastic:
+            out["elastic"] = elastic_leg(ads, L, lib, 1, args.elastic_nstep)
+        if not args.no_cpu:
+            out["cpu_baseline"] = cpu_baseline(p, c0, srcs[0], rcvs[0],
+                                               min(args.cpu_steps, int(nst.min())),
+                                               "the headline workload")
     print(json.dumps(out), flush=True)
 
 

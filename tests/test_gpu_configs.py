@@ -70,62 +70,73 @@ def test_plain_fp64_configuration(orc, ads, cuda):
 def test_large_uneven_batch_configuration(ads, cuda):
     """The 2000 x 2000, 64-shot configuration with uneven records (NSTEP_s ~ U{3000..6000},
     spreads of 400..2000 receivers centred on each source), FP32, dt = 0.5 ms (chosen: CFL
-    0.5 at 5000 m/s), 40-cell PML, smooth random model 1500..5000 m/s (seed 4).  Its
-    checkpointed workspace does not fit one plan (about 300 GB for 64 shots at the best
-    interval), so the shots go through plans of 16 and of 8 (acoustic_gradient_in_batches).
-    The oracle is out of reach at this size; checked instead through size-independent
-    properties: (1) the two batchings (different batch sizes, shot groupings and checkpoint
-    intervals) give the same misfit and gradient to FP32 reassociation; (2) a central finite
-    difference of the misfit along a perturbation of 0.5% of the model in the direction of the
-    (smoothed) gradient matches <gradient, perturbation> within 2%."""
+    0.5 at 5000 m/s), 40-cell PML, smooth random model 1500..5000 m/s -- bench.py's headline
+    workload -- with all 64 shots in ONE plan under the checkpoint schedule chosen for 90% of
+    the free device memory.  The oracle is out of reach at this size; checked instead through
+    size-independent properties: (1) plans of 16 shots (other shot groupings and schedules)
+    give the same misfit and gradient to FP32 reassociation; (2) the binomial schedule with 4
+    snapshot slots gives the chosen schedule's gradient bit for bit; (3) a central finite
+    difference of the misfit along a perturbation of 0.2% of the model in the direction of the
+    smoothed gradient, restricted to the imaging region below the acquisition PML band, matches
+    <gradient, perturbation> within 2%."""
+    import sys
     import time
     from scipy.ndimage import gaussian_filter
+    import os
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
 
-    rng = np.random.default_rng(4)
-    NX = NY = 2000
-    nshots = 64
-    dt, h = 5e-4, 5.0
-    nst = rng.integers(3000, 6001, nshots)
-    p = ads.AcousticPropagatorParams(NX=NX, NY=NY, NSTEP=int(nst.max()), DELTAX=h, DELTAY=h,
-                                     DELTAT=dt, NPOINTS_PML=40, vp_ref=3000.0, PropagatorKernel=1)
-    noise = gaussian_filter(rng.standard_normal((NX + 2, NY + 2)), 40.0)
-    noise = (noise - noise.min()) / (noise.max() - noise.min())
-    c_true = 1500.0 + 3500.0 * noise
-    ads.check_cfl(p, float(c_true.max()))
-    c0 = gaussian_filter(c_true, 30.0)
-    xs = np.round(np.linspace(41, NX - 40, nshots)).astype(int)
-    srcs, rcvs = [], []
-    for s in range(nshots):
-        w = ads.ricker_freq(8.0 + 4.0 * rng.random(), dt, int(nst[s]), 1e4)
-        srcs.append(ads.AcousticSource([int(xs[s])], [3], w.reshape(-1, 1)))
-        nr = int(rng.integers(400, 2001))
-        lo = int(np.clip(xs[s] - nr // 2, 1, NX + 2 - nr))
-        rcvs.append(ads.AcousticReceiver(np.arange(lo, lo + nr), np.full(nr, 3)))
+    p, c_true, c0, srcs, rcvs, nst = bench.workload_uneven(ads)
+    NX, NY = p.NX, p.NY
     T = lambda x: torch.as_tensor(x, dtype=torch.float32, device="cuda")
-    # observed data from the true model, 16 shots per plan
-    obs = ads.acoustic_traces_in_batches(p, srcs, rcvs, T(c_true), 16, nstep=nst)
+    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=torch.float32, nstep=nst)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    K, S = plan.choose_schedule(int(torch.cuda.mem_get_info()[0] * 0.9))
+    M = -(-(p.NSTEP - 1) // K)
+    print(f"one plan of 64 shots: K {K}, S {S} (M {M}), workspace "
+          f"{plan.workspace_size(K) / 1e9:.1f} GB", flush=True)
+    obs = plan.forward(T(c_true), srcv, K).clone()
     cells = NX * NY * int(nst.sum())
-    res = {}
-    for ms in (16, 8):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        res[ms] = ads.acoustic_gradient_in_batches(p, srcs, rcvs, obs, T(c0), ms, nstep=nst)
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        print(f"{ms} shots per plan: {el:.2f} s, {cells / el / 1e9:.1f} Gcell-steps/s", flush=True)
-    (L16, g16), (L8, g8) = res[16], res[8]
-    assert L16.item() == pytest.approx(L8.item(), rel=1e-6)
-    assert rel(g8.cpu(), g16.cpu()) < 1e-5
-    # directional derivative along the (smoothed) gradient itself, scaled to a perturbation of
-    # at most 0.5% of the model: the largest first-order signal against FP32 rounding
-    gs = gaussian_filter(g16.double().cpu().numpy(), 5.0)
-    d = gs * (0.005 * c0.min() / np.abs(gs).max())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L1, g1, _ = plan.gradient(T(c0), srcv, obs, K)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(f"one plan: {el:.2f} s, {cells / el / 1e9:.1f} Gcell-steps/s, misfit {L1.item():.6e}",
+          flush=True)
+    L1, g1 = L1.item(), g1.cpu().numpy()
+    # (2) binomial schedule with 4 slots: the same arithmetic, more rebuilds
+    plan.set_snapshots(4)
+    L4, g4, _ = plan.gradient(T(c0), srcv, obs, K)
+    assert L4.item() == L1
+    np.testing.assert_array_equal(g4.cpu().numpy(), g1)
+    plan.set_snapshots(S)
+    # (3) directional derivative in the imaging region
+    m = np.zeros_like(c0)
+    w = p.NPOINTS_PML + 2
+    m[w:NX + 2 - w, p.NPOINTS_PML + 20:NY + 2 - w] = 1.0
+    m = gaussian_filter(m, 10.0) * m
+    gs = gaussian_filter(g1 * m, 5.0) * m
+    d = gs * (0.002 * c0.min() / np.abs(gs).max())
+
     def misfit(cm):
-        tr = ads.acoustic_traces_in_batches(p, srcs, rcvs, T(cm), 16, nstep=nst)
-        return sum(float(torch.sum((a.double() - b.double()) ** 2)) for a, b in zip(tr, obs))
+        tr = plan.forward(T(cm), srcv, K)
+        return float(torch.sum((tr.double() - obs.double()) ** 2).item())
 
     fd = (misfit(c0 + d) - misfit(c0 - d)) / 2.0
-    an = float(torch.sum(g16.double() * torch.as_tensor(d, device="cuda")).item())
-    print(f"directional derivative: finite difference {fd:.6e}, gradient {an:.6e}, "
-          f"misfit {L16.item():.6e}", flush=True)
+    an = float(np.sum(g1.astype(np.float64) * d))
+    obs_per = [t.clone() for t in plan.unpack_traces(obs)]
+    plan._ws = None
+    del obs
+    torch.cuda.empty_cache()
+    print(f"directional derivative: finite difference {fd:.6e}, gradient {an:.6e}", flush=True)
+    # (1) plans of 16 shots
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L16, g16 = ads.acoustic_gradient_in_batches(p, srcs, rcvs, obs_per, T(c0), 16, nstep=nst)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(f"16 shots per plan: {el:.2f} s, {cells / el / 1e9:.1f} Gcell-steps/s", flush=True)
+    assert L16.item() == pytest.approx(L1, rel=1e-6)
+    assert rel(g16.cpu(), g1) < 1e-5
     assert abs(fd - an) <= 0.02 * abs(an)
