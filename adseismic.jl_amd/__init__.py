@@ -25,7 +25,7 @@ from .io_mat import (load_acoustic_model, load_acoustic_receiver, load_acoustic_
                      matread, safe_vec)
 from .fwi import (AcousticFWI, Adam_, LBFGS_, acoustic_gradient_in_batches,  # noqa: F401
                   acoustic_traces_in_batches, compute_loss_and_grads)
-from .nn import Generator, add_initial_model  # noqa: F401
+from .nn import Generator, UpsamplingGenerator, add_initial_model  # noqa: F401
 from .mpi_acoustic import (MPIAcousticPropagator, MPIAcousticPropagatorParams,  # noqa: F401
                            MPIAcousticPropagatorSolver, MPIAcousticReceiver, MPIAcousticSource,
                            MPISimulatedObservation_acoustic, compute_PML_Params_mpi_acoustic,

@@ -2427,8 +2427,10 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
     P->Ndense = (p->NX + 2) * (p->NY + 2);
     P->nshots = g->nshots;
     P->k = make_coef<T>(p->DELTAT, p->DELTAX, p->DELTAY);
-    P->fwd_shape = Shape{1, (int)std::min<int64_t>(8, g->nshots), 1};
-    P->adj_shape = Shape{1, (int)std::min<int64_t>(8, g->nshots), 1};
+    // the benchmark shape (forward: the streaming two-level rectangle, 8 rows per wavefront;
+    // adjoint: 2 rows per wavefront), with at most 16 shots per workgroup group
+    P->fwd_shape = Shape{8, (int)std::min<int64_t>(16, g->nshots), 1};
+    P->adj_shape = Shape{2, (int)std::min<int64_t>(16, g->nshots), 1};
     const int64_t ns = g->nshots;
     if (g->src_ptr[0] != 0 || g->rcv_ptr[0] != 0) ADS_FAIL(ADS_ERR_ARG, "src_ptr/rcv_ptr must start at 0");
     std::vector<int64_t> nstep(ns);

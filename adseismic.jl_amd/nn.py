@@ -66,6 +66,46 @@ class Generator(torch.nn.Module):
         return x.squeeze()
 
 
+class UpsamplingGenerator(torch.nn.Module):
+    """A 4-layer upsampling CNN velocity parametrisation: a fixed latent tensor of shape
+    (latent_channels, h0, w0) drawn from a seeded standard normal; four times bilinear
+    upsampling (the last one onto the output grid) -> 3x3 convolution ("same", with bias) with
+    channels[0..3] outputs, LeakyReLU(0.1) after all but the last; velocity
+    v = vmin + (vmax - vmin) * sigmoid(x).  ``forward()`` returns the (H, W) velocity image.
+    The intermediate sizes double from (h0, w0) and the last upsampling lands on (H, W)."""
+
+    def __init__(self, out_shape, latent_shape=(8, 16, 16), channels=(64, 32, 16, 1),
+                 vmin: float = 1500.0, vmax: float = 4500.0, seed: int = 5,
+                 dtype=torch.float32, device="cuda"):
+        super().__init__()
+        self.out_shape = tuple(int(s) for s in out_shape)
+        self.vmin, self.vmax = float(vmin), float(vmax)
+        g = torch.Generator().manual_seed(int(seed))
+        z = torch.randn(1, *latent_shape, generator=g, dtype=torch.float64)
+        self.register_buffer("z", z.to(dtype))
+        convs, cin = [], int(latent_shape[0])
+        for cout in channels:
+            conv = torch.nn.Conv2d(cin, int(cout), 3, padding=1, bias=True)
+            torch.nn.init.xavier_uniform_(conv.weight, generator=g)
+            torch.nn.init.zeros_(conv.bias)
+            convs.append(conv)
+            cin = int(cout)
+        self.convs = torch.nn.ModuleList(convs)
+        h0, w0 = latent_shape[1], latent_shape[2]
+        n = len(channels)
+        self.sizes = [(h0 * 2 ** (k + 1), w0 * 2 ** (k + 1)) for k in range(n - 1)] + [self.out_shape]
+        self.to(dtype=dtype, device=device)
+
+    def forward(self) -> torch.Tensor:
+        x = self.z
+        for k, conv in enumerate(self.convs):
+            x = F.interpolate(x, size=self.sizes[k], mode="bilinear", align_corners=False)
+            x = conv(x)
+            if k < len(self.convs) - 1:
+                x = F.leaky_relu(x, 0.1)
+        return self.vmin + (self.vmax - self.vmin) * torch.sigmoid(x[0, 0])
+
+
 def add_initial_model(x: torch.Tensor, v: torch.Tensor) -> torch.Tensor:
     """add_initial_model(x, v), src/Utils.jl:140-166: ``v`` plus the image ``x`` (or each of a
     batch of images), centred: a smaller image is added into the middle of ``v``, a larger one

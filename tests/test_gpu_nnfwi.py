@@ -108,3 +108,93 @@ def test_nnfwi_step_at_400x800_24_shots(ads, cuda):
         losses.append(float(L))
         opt.step()
     assert min(losses[1:]) < losses[0]
+
+
+def _c5(ads, dtype):
+    """The network-parametrised configuration: 400 x 800 grid, h = 10 m, dt = 1 ms, 2500
+    steps, 20-cell PML (chosen), 24 shots of a 10 Hz Ricker at depth 3 evenly spaced along the
+    surface, 400 receivers each; velocity = 1500 + 3000 sigmoid(G(z)) with G the 4-layer
+    upsampling CNN (latent 8 x 16 x 16 from a standard normal, seed 5; 3x3 convolutions with
+    64/32/16/1 channels, LeakyReLU 0.1).  Observed data come from a layered model with a
+    Gaussian anomaly."""
+    NX, NY, NSTEP, nshots, npml = 400, 800, 2500, 24, 20
+    p = ads.AcousticPropagatorParams(NX=NX, NY=NY, NSTEP=NSTEP, DELTAX=10.0, DELTAY=10.0,
+                                     DELTAT=1e-3, NPOINTS_PML=npml, vp_ref=3000.0)
+    ads.check_cfl(p, 4500.0)
+    w = ads.ricker_freq(10.0, p.DELTAT, NSTEP, 1e4)
+    xs = np.round(np.linspace(npml + 2, NX + 1 - npml, nshots)).astype(int)
+    srcs = [ads.AcousticSource([int(x)], [3], w.reshape(-1, 1)) for x in xs]
+    rcvs = [ads.AcousticReceiver(np.arange(2, NX + 2), np.full(NX, 3)) for _ in xs]
+    plan = ads.AcousticPlan(p, srcs, rcvs, dtype=dtype)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    jj = np.arange(NY + 2)[None, :]
+    ii = np.arange(NX + 2)[:, None]
+    vtrue = 1800.0 + 2200.0 * (jj / (NY + 1)) + 0.0 * ii
+    vtrue = vtrue + 300.0 * np.exp(-((ii - NX / 2) ** 2 + (jj - NY / 2) ** 2) / (2 * 40.0 ** 2))
+    G = ads.UpsamplingGenerator((NX + 2, NY + 2), dtype=dtype)
+    return p, plan, srcv, torch.as_tensor(vtrue, dtype=dtype, device="cuda"), G
+
+
+@pytest.mark.timeout(600)
+def test_upsampling_generator_configuration(ads, cuda):
+    """FP64: dL/dweights through the HIP adjoint against central finite differences of the
+    misfit on 20 weights drawn uniformly at random over all the network's weights (relative
+    1e-4, or 1e-6 of the largest of the 20 where a weight's gradient is tiny).  FP32: one
+    NN-FWI iteration (generator, 24-shot forward, misfit, adjoint, backpropagation into the
+    weights) timed at full size."""
+    import time
+    from adseismic_amd.acoustic import acoustic_traces
+
+    # FP64 finite differences
+    p, plan, srcv, vtrue, G = _c5(ads, torch.float64)
+    K = plan.choose_interval()
+    obs = plan.forward(vtrue, srcv, K).clone()
+
+    def misfit():
+        return torch.sum((acoustic_traces(plan, G(), srcv, K) - obs) ** 2)
+
+    G.zero_grad()
+    L = misfit()
+    L.backward()
+    params = [q for q in G.parameters()]
+    sizes = [q.numel() for q in params]
+    rng = np.random.default_rng(20)
+    picks = sorted(rng.choice(sum(sizes), size=20, replace=False))
+    offs = np.cumsum([0] + sizes)
+    an, fd = [], []
+    with torch.no_grad():
+        for k in picks:
+            t = int(np.searchsorted(offs, k, side="right") - 1)
+            q, idx = params[t].view(-1), int(k - offs[t])
+            an.append(float(params[t].grad.view(-1)[idx]))
+            w0 = float(q[idx])
+            eps = 1e-4 * max(abs(w0), 1e-2)
+            q[idx] = w0 + eps
+            lp = float(misfit())
+            q[idx] = w0 - eps
+            lm = float(misfit())
+            q[idx] = w0
+            fd.append((lp - lm) / (2 * eps))
+    an, fd = np.array(an), np.array(fd)
+    print(f"C5 FP64: misfit {L.item():.6e}; 20 random weights: max |fd - g| / max|g| "
+          f"{np.max(np.abs(fd - an)) / np.max(np.abs(an)):.2e}", flush=True)
+    np.testing.assert_allclose(fd, an, rtol=1e-4, atol=1e-6 * np.max(np.abs(an)))
+    plan._ws = None
+    torch.cuda.empty_cache()
+    # FP32 iteration at full size
+    p, plan, srcv, vtrue, G = _c5(ads, torch.float32)
+    K = plan.choose_interval()
+    obs = plan.forward(vtrue, srcv, K).clone()
+    for it in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        G.zero_grad()
+        L = torch.sum((acoustic_traces(plan, G(), srcv, K) - obs) ** 2)
+        L.backward()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+    cells = p.NX * p.NY * p.NSTEP * 24
+    gn = float(sum(torch.sum(q.grad.double() ** 2) for q in G.parameters()) ** 0.5)
+    print(f"C5 FP32 NN-FWI iteration: {el * 1e3:.1f} ms ({cells / el / 1e9:.1f} Gcell-steps/s "
+          f"incl. the network), misfit {L.item():.6e}, |dL/dw| {gn:.3e}", flush=True)
+    assert math.isfinite(L.item()) and gn > 0
