@@ -80,8 +80,8 @@ template <typename T> __device__ inline bool fw_in(const ElCoef<T> &c, int fw, i
 // The padded grid is split once per plan into an interior rectangle, where every update range
 // holds and the CPML coefficients are the identity (a = 0, b = 1, so the memory variables stay
 // 0 and are neither read nor written), and the frame around it, which runs the full per-cell
-// scheme below.  Memory variables of rectangle cells are 0 in every slot (cleared once per
-// forward sweep).
+// scheme below.  The memory variables of rectangle cells are never read, so their slot
+// values are left as they are.
 
 // one cell of fw1/fw2 (level n-1 -> n), memory variables included
 template <typename T>
@@ -1054,6 +1054,363 @@ __global__ __launch_bounds__(256) void el_adjs_rb(ElAdj<T> a, ElCoef<T> c, TileM
     else adjs_rect_body(a, c, map, b - nfb);
 }
 
+// ---- streaming rectangle adjoint ----
+// A workgroup is SW_NW wavefronts, one shot each (one shot group), over one strip of SW_COLS
+// output columns and one chunk of up to `ch` output rows of the rectangle.  Lane l holds column
+// jL + strip*SW_COLS - 2 + l; lanes 2 .. WAVE-3 are the output columns, the two lanes on each
+// side carry the halo columns of the y-direction stencils.  Each wavefront walks its chunk row
+// by row, keeping the x-direction stencil inputs of rows i-2 .. i+2 in registers (one new row
+// loaded per step) and taking the y-direction neighbours of row i from the adjacent lanes.
+// t-values inside the rectangle are formed from the cotangents (t0 = t1, t2 = t3 there); on
+// the frame cells the halo reaches they come from the temporaries frame_a wrote, and outside
+// the interior they are 0, as in tv_vel / tv_str.  The parameter terms of the group's shots
+// are summed per cell in LDS and added to the group's accumulators once per workgroup.
+constexpr int SW_NW = 8, SW_CH_MAX = 32, SW_COLS = WAVE - 4;
+
+struct StrmMap {
+    int nstrip, nchunk, ngroup, nb, ch;
+};
+__device__ inline bool decode_strm(const StrmMap &m, int &g, int &chunk, int &strip, int b)
+{
+    const int nb8 = (m.nb + NXCD - 1) / NXCD;
+    const int L = (b % NXCD) * nb8 + b / NXCD;
+    if (L >= m.nb) return false;
+    strip = L % m.nstrip;
+    const int r = L / m.nstrip;
+    chunk = r % m.nchunk;
+    g = r / m.nchunk;
+    return true;
+}
+template <typename T> __device__ inline T lane_at(T v, int src) { return __shfl(v, src, WAVE); }
+
+// shot-invariant inputs of the streaming kernels (workspace arrays, one Npad grid each)
+template <typename T> struct ElPar {
+    const T *irho, *irb, *lb, *mb, *mb2;  // 1/rho, 1/rho-bar, lambda-bar, mu-bar (fw1), mu-bar (fw2)
+};
+
+// Each row step issues the loads of the row it will need one step later (window row i+3 and
+// the row-(i+1) cotangents), so their latency overlaps the current row's arithmetic.
+
+// velocity phase: fw3/fw4 VJP on rectangle cells
+template <typename T>
+__device__ inline void adjv_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, const ElPar<T> &q,
+                                      const StrmMap &mp, int bid)
+{
+    __shared__ T gacc[2][SW_CH_MAX][WAVE];
+    int g, chunk, strip;
+    if (!decode_strm(mp, g, chunk, strip, bid)) return;
+    for (int k = threadIdx.x; k < 2 * SW_CH_MAX * WAVE; k += blockDim.x) (&gacc[0][0][0])[k] = T(0);
+    __syncthreads();
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int i0 = a.R.iL + chunk * mp.ch, i1 = min(i0 + mp.ch, a.R.iR);
+    const int jb = a.R.jL + strip * SW_COLS - 2;
+    const int j = jb + lane;
+    const int jc = j <= c.NY + 1 ? j : c.NY + 1;  // loads of lanes beyond the grid stay inside it
+    const bool jin = j >= a.R.jL && j < a.R.jR;
+    const bool jint = j >= 1 && j <= c.NY;
+    const bool out = lane >= 2 && lane < WAVE - 2 && jin;
+    const int s = g * a.Sg + w;
+    if (w < a.Sg && s < a.act) {
+        const int P = c.P;
+        const int64_t ts = a.tstride;
+        const T dt = c.dt;
+        const T *__restrict__ vxa = a.A.f(VX, s);
+        const T *__restrict__ vya = a.A.f(VY, s);
+        T *__restrict__ axx = a.A.f(SXX, s);
+        T *__restrict__ axy = a.A.f(SXY, s);
+        T *__restrict__ ayy = a.A.f(SYY, s);
+        const T *__restrict__ sxx = a.Sn.f(SXX, s);
+        const T *__restrict__ sxy = a.Sn.f(SXY, s);
+        const T *__restrict__ syy = a.Sn.f(SYY, s);
+        const T *__restrict__ tb = a.Tb + (int64_t)s * a.npad;
+        // raw loads of one window row, and of one output row
+        struct Rw { T a, b, x, y, r, q; };
+        struct Ro { T yy, t1, t3, gxx, gxy, gyy; };
+        auto fetch_w = [&](int r, Rw &o) {
+            const int64_t rj = (int64_t)r * P + jc;
+            o.x = sxx[rj];
+            o.y = sxy[rj];
+            if (r >= a.R.iL && r < a.R.iR) {  // uniform over the wavefront
+                o.a = vxa[rj];
+                o.b = vya[rj];
+                o.r = q.irho[rj];
+                o.q = q.irb[rj];
+            } else {
+                const bool inter = jint && r >= 1 && r <= c.NX;
+                o.a = inter ? tb[rj] : T(0);
+                o.b = inter ? tb[2 * ts + rj] : T(0);
+                o.r = o.q = T(0);
+            }
+        };
+        auto fetch_o = [&](int i, Ro &o) {
+            const int64_t ij = (int64_t)i * P + jc;
+            o.yy = syy[ij];
+            o.t1 = !jin && jint ? tb[ts + ij] : T(0);
+            o.t3 = !jin && jint ? tb[3 * ts + ij] : T(0);
+            o.gxx = out ? axx[ij] : T(0);
+            o.gxy = out ? axy[ij] : T(0);
+            o.gyy = out ? ayy[ij] : T(0);
+        };
+        // windows over rows i-2 .. i+2 (index 0 .. 4): t0, t2, sxx, sxy, 1/rho, 1/rho-bar;
+        // t-values inside the rectangle from the cotangents, elsewhere the temporaries
+        T w0[5], w2[5], wx[5], wy[5], wr[5], wb[5];
+        auto finish = [&](int r, const Rw &o, int k) {
+            wx[k] = o.x;
+            wy[k] = o.y;
+            wr[k] = o.r;
+            wb[k] = o.q;
+            const bool inr = r >= a.R.iL && r < a.R.iR;
+            w0[k] = inr ? o.a * dt * o.r : o.a;
+            w2[k] = inr ? o.b * dt * o.q : o.b;
+        };
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            Rw t;
+            fetch_w(i0 - 2 + k, t);
+            finish(i0 - 2 + k, t, k);
+        }
+        Rw rn;
+        Ro on;
+        fetch_w(i0 + 2, rn);
+        fetch_o(i0, on);
+        for (int i = i0; i < i1; i++) {
+            const Rw rc = rn;
+            const Ro oc = on;
+            if (i + 1 < i1) {
+                fetch_w(i + 3, rn);
+                fetch_o(i + 1, on);
+            }
+            finish(i + 2, rc, 4);
+            // row i, y-direction inputs: t1, t3 (= t0, t2 inside the rectangle), sxy, syy
+            const T t1 = jin ? w0[2] : oc.t1;
+            const T t3 = jin ? w2[2] : oc.t3;
+            const T xy = wy[2], yy = oc.yy;
+            const T t1m = lane_at(t1, lane - 1), t1p = lane_at(t1, lane + 1), t1pp = lane_at(t1, lane + 2);
+            const T t3mm = lane_at(t3, lane - 2), t3m = lane_at(t3, lane - 1), t3p = lane_at(t3, lane + 1);
+            const T xymm = lane_at(xy, lane - 2), xym = lane_at(xy, lane - 1), xyp = lane_at(xy, lane + 1);
+            const T yym = lane_at(yy, lane - 1), yyp = lane_at(yy, lane + 1), yypp = lane_at(yy, lane + 2);
+            if (out) {
+                const int64_t ij = (int64_t)i * P + jc;
+                // parameter terms (fw3: rho; fw4: rho-bar)
+                const T S3 = d4(wx[2], wx[1], wx[3], wx[0]) * c.ihx + d4(xy, xym, xyp, xymm) * c.ihy;
+                const T S4 = d4(wy[3], wy[2], wy[4], wy[1]) * c.ihx + d4(yyp, yy, yypp, yym) * c.ihy;
+                atomicAdd(&gacc[0][i - i0][lane], -w0[2] * S3 * wr[2]);
+                atomicAdd(&gacc[1][i - i0][lane], -w2[2] * S4 * wb[2]);
+                // sxx <- (D-x)^T t0 ; sxy <- (D-y)^T t1 + (D+x)^T t2 ; syy <- (D+y)^T t3
+                axx[ij] = oc.gxx + d4(w0[2], w0[3], w0[1], w0[4]) * c.ihx;
+                axy[ij] = oc.gxy + d4(t1, t1p, t1m, t1pp) * c.ihy + d4(w2[1], w2[2], w2[0], w2[3]) * c.ihx;
+                ayy[ij] = oc.gyy + d4(t3m, t3, t3mm, t3p) * c.ihy;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                w0[k] = w0[k + 1];
+                w2[k] = w2[k + 1];
+                wx[k] = wx[k + 1];
+                wy[k] = wy[k + 1];
+                wr[k] = wr[k + 1];
+                wb[k] = wb[k + 1];
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < (i1 - i0) * WAVE; k += blockDim.x) {
+        const int r = k / WAVE, l = k % WAVE, jj = jb + l;
+        if (l < 2 || l >= WAVE - 2 || jj >= a.R.jR) continue;
+        const int64_t ij = (int64_t)(i0 + r) * c.P + jj;
+        g_at(a, G_RHO, g, ij) += gacc[0][r][l];
+        g_at(a, G_RB, g, ij) += gacc[1][r][l];
+    }
+}
+
+// stress phase: fw1/fw2 VJP on rectangle cells
+template <typename T>
+__device__ inline void adjs_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, const ElPar<T> &q,
+                                      const StrmMap &mp, int bid)
+{
+    __shared__ T gacc[3][SW_CH_MAX][WAVE];
+    int g, chunk, strip;
+    if (!decode_strm(mp, g, chunk, strip, bid)) return;
+    for (int k = threadIdx.x; k < 3 * SW_CH_MAX * WAVE; k += blockDim.x) (&gacc[0][0][0])[k] = T(0);
+    __syncthreads();
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int i0 = a.R.iL + chunk * mp.ch, i1 = min(i0 + mp.ch, a.R.iR);
+    const int jb = a.R.jL + strip * SW_COLS - 2;
+    const int j = jb + lane;
+    const int jc = j <= c.NY + 1 ? j : c.NY + 1;
+    const bool jin = j >= a.R.jL && j < a.R.jR;
+    const bool jint = j >= 1 && j <= c.NY;
+    const bool out = lane >= 2 && lane < WAVE - 2 && jin;
+    const int s = g * a.Sg + w;
+    if (w < a.Sg && s < a.act) {
+        const int P = c.P;
+        const int64_t ts = a.tstride;
+        const T dt = c.dt;
+        T *__restrict__ avx = a.A.f(VX, s);
+        T *__restrict__ avy = a.A.f(VY, s);
+        const T *__restrict__ axx = a.A.f(SXX, s);
+        const T *__restrict__ ayy = a.A.f(SYY, s);
+        const T *__restrict__ axy = a.A.f(SXY, s);
+        const T *__restrict__ vx = a.Sm.f(VX, s);
+        const T *__restrict__ vy = a.Sm.f(VY, s);
+        const T *__restrict__ tb = a.Tb + (int64_t)s * a.npad;
+        // raw loads of one window row (inside the rectangle's rows: the stress cotangents and the
+        // parameters; outside: the temporaries t0..t3), and of one output row
+        struct Rw { T u, v, e0, e1, e2, e3, e4, e5; };
+        struct Ro { T t1, t3, gvx, gvy; };
+        auto fetch_w = [&](int r, Rw &o) {
+            const int64_t rj = (int64_t)r * P + jc;
+            o.u = vx[rj];
+            o.v = vy[rj];
+            if (r >= a.R.iL && r < a.R.iR) {  // uniform over the wavefront
+                o.e0 = axx[rj];
+                o.e1 = ayy[rj];
+                o.e2 = axy[rj];
+                o.e3 = q.lb[rj];
+                o.e4 = q.mb[rj];
+                o.e5 = q.mb2[rj];
+            } else {
+                const bool inter = jint && r >= 1 && r <= c.NX;
+                o.e0 = inter ? tb[rj] : T(0);
+                o.e1 = inter ? tb[ts + rj] : T(0);
+                o.e2 = inter ? tb[2 * ts + rj] : T(0);
+                o.e3 = inter ? tb[3 * ts + rj] : T(0);
+                o.e4 = o.e5 = T(0);
+            }
+        };
+        auto fetch_o = [&](int i, Ro &o) {
+            const int64_t ij = (int64_t)i * P + jc;
+            o.t1 = !jin && jint ? tb[ts + ij] : T(0);
+            o.t3 = !jin && jint ? tb[3 * ts + ij] : T(0);
+            o.gvx = out ? avx[ij] : T(0);
+            o.gvy = out ? avy[ij] : T(0);
+        };
+        // windows over rows i-2 .. i+2: t0, t1, t2 (t3 = t2 inside the rectangle), the stress
+        // cotangents sxx*dt, syy*dt, sxy of the row, and vx, vy of level n-1
+        T w0[5], w1[5], w2[5], gx[5], gy[5], gs[5], wu[5], wv[5];
+        auto finish = [&](int r, const Rw &o, int k) {
+            wu[k] = o.u;
+            wv[k] = o.v;
+            if (r >= a.R.iL && r < a.R.iR) {
+                gx[k] = o.e0 * dt;
+                gy[k] = o.e1 * dt;
+                gs[k] = o.e2;
+                const T lb = o.e3, lm = lb + T(2) * o.e4;
+                w0[k] = lm * gx[k] + lb * gy[k];
+                w1[k] = lb * gx[k] + lm * gy[k];
+                w2[k] = gs[k] * o.e5 * dt;
+            } else {
+                gx[k] = gy[k] = gs[k] = T(0);
+                w0[k] = o.e0;
+                w1[k] = o.e1;
+                w2[k] = o.e2;  // t3 of such rows is never needed: only row i's t3 is read
+            }
+        };
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            Rw t;
+            fetch_w(i0 - 2 + k, t);
+            finish(i0 - 2 + k, t, k);
+        }
+        Rw rn;
+        Ro on;
+        fetch_w(i0 + 2, rn);
+        fetch_o(i0, on);
+        for (int i = i0; i < i1; i++) {
+            const Rw rc = rn;
+            const Ro oc = on;
+            if (i + 1 < i1) {
+                fetch_w(i + 3, rn);
+                fetch_o(i + 1, on);
+            }
+            finish(i + 2, rc, 4);
+            const T t1 = jin ? w1[2] : oc.t1, t3 = jin ? w2[2] : oc.t3, u = wu[2], v = wv[2];
+            const T t1m = lane_at(t1, lane - 1), t1p = lane_at(t1, lane + 1), t1pp = lane_at(t1, lane + 2);
+            const T t3mm = lane_at(t3, lane - 2), t3m = lane_at(t3, lane - 1), t3p = lane_at(t3, lane + 1);
+            const T um = lane_at(u, lane - 1), up = lane_at(u, lane + 1), upp = lane_at(u, lane + 2);
+            const T vmm = lane_at(v, lane - 2), vm = lane_at(v, lane - 1), vp = lane_at(v, lane + 1);
+            if (out) {
+                const int64_t ij = (int64_t)i * P + jc;
+                // parameter terms: fw1 (lambda-bar, mu-bar) and fw2 (mu-bar), level n-1 velocities
+                const T X = d4(wu[3], wu[2], wu[4], wu[1]) * c.ihx;
+                const T Y = d4(v, vm, vp, vmm) * c.ihy;
+                const T glm = X * gx[2] + Y * gy[2];
+                atomicAdd(&gacc[0][i - i0][lane], Y * gx[2] + X * gy[2] + glm);
+                atomicAdd(&gacc[1][i - i0][lane], T(2) * glm);
+                const T dx = d4(wv[2], wv[1], wv[3], wv[0]) * c.ihx;
+                const T dy = d4(up, u, upp, um) * c.ihy;
+                atomicAdd(&gacc[2][i - i0][lane], gs[2] * (dx + dy) * dt);
+                // vx <- (D+x)^T t0 + (D+y)^T t3 ; vy <- (D-y)^T t1 + (D-x)^T t2
+                avx[ij] = oc.gvx + d4(w0[1], w0[2], w0[0], w0[3]) * c.ihx + d4(t3m, t3, t3mm, t3p) * c.ihy;
+                avy[ij] = oc.gvy + d4(t1, t1p, t1m, t1pp) * c.ihy + d4(w2[2], w2[3], w2[1], w2[4]) * c.ihx;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                w0[k] = w0[k + 1];
+                w1[k] = w1[k + 1];
+                w2[k] = w2[k + 1];
+                gx[k] = gx[k + 1];
+                gy[k] = gy[k + 1];
+                gs[k] = gs[k + 1];
+                wu[k] = wu[k + 1];
+                wv[k] = wv[k + 1];
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < (i1 - i0) * WAVE; k += blockDim.x) {
+        const int r = k / WAVE, l = k % WAVE, jj = jb + l;
+        if (l < 2 || l >= WAVE - 2 || jj >= a.R.jR) continue;
+        const int64_t ij = (int64_t)(i0 + r) * c.P + jj;
+        g_at(a, G_LB, g, ij) += gacc[0][r][l];
+        g_at(a, G_MB1, g, ij) += gacc[1][r][l];
+        g_at(a, G_MB2, g, ij) += gacc[2][r][l];
+    }
+}
+
+// The streaming rectangle alone, or fused with frame_b (independent cells, same phase):
+// workgroups [0, nfb) take the frame items (nib item blocks per shot group, NWAVES items per
+// workgroup), the rest the rectangle strips.  Both halves use WAVE * a.Sg threads; frame_b
+// workgroups of fewer than NWAVES wavefronts are launched only when Sg == NWAVES.
+template <typename T>
+__global__ __launch_bounds__(WAVE *SW_NW) void el_adjv_strm(ElAdj<T> a, ElCoef<T> c, ElPar<T> q, StrmMap mp,
+                                                            const int4 *__restrict__ items, int nitems,
+                                                            int nib, int nfb)
+{
+    const int b = (int)blockIdx.x;
+    if (b < nfb) adjv_frame_b_body(a, c, items, nitems, b % nib, b / nib);
+    else adjv_strm_body(a, c, q, mp, b - nfb);
+}
+template <typename T>
+__global__ __launch_bounds__(WAVE *SW_NW) void el_adjs_strm(ElAdj<T> a, ElCoef<T> c, ElPar<T> q, StrmMap mp,
+                                                            const int4 *__restrict__ items, int nitems,
+                                                            int nib, int nfb)
+{
+    const int b = (int)blockIdx.x;
+    if (b < nfb) adjs_frame_b_body(a, c, items, nitems, b % nib, b / nib);
+    else adjs_strm_body(a, c, q, mp, b - nfb);
+}
+
+// Shot-invariant inputs of the streaming kernels on every padded cell: 1/rho, 1/rho-bar
+// (0 where the parameter is 0, the padding ring), lambda-bar and mu-bar of fw1
+// ((i, j), (i+1, j) averages, Core.jl:109-110), mu-bar of fw2 ((i, j), (i, j+1), :141); the
+// block form takes the parameters themselves.
+template <typename T>
+__global__ void el_adj_params(const T *__restrict__ rho, const T *__restrict__ rb,
+                              const T *__restrict__ lam, const T *__restrict__ mu, T *__restrict__ irho,
+                              T *__restrict__ irb, T *__restrict__ lb, T *__restrict__ mb,
+                              T *__restrict__ mb2, int NX, int NY, int P, int sub)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= P) return;
+    const int64_t ij = (int64_t)i * P + j;
+    irho[ij] = rho[ij] != T(0) ? T(1) / rho[ij] : T(0);
+    irb[ij] = rb[ij] != T(0) ? T(1) / rb[ij] : T(0);
+    const bool dn = i <= NX, rt = j <= NY;  // (i+1, j) and (i, j+1) inside the padded grid
+    lb[ij] = sub ? lam[ij] : (dn ? T(0.5) * (lam[ij + P] + lam[ij]) : T(0));
+    mb[ij] = sub ? mu[ij] : (dn ? T(0.5) * (mu[ij + P] + mu[ij]) : T(0));
+    mb2[ij] = sub ? mu[ij] : (rt ? T(0.5) * (mu[ij] + mu[ij + 1]) : T(0));
+}
+
 // rho-bar of fw4 at every cell of [1, NX-1] x [1, NY-1] (Core.jl:201), 0 elsewhere
 template <typename T>
 __global__ void el_rho_bar(const T *__restrict__ rho, T *__restrict__ rb, int NX, int NY, int P, int sub)
@@ -1160,7 +1517,11 @@ struct ElPlanBase {
     int nitems[2] = {0, 0};
     DevBuf d_ring[2];  // the rectangle's cells within two of its edge (adjoint t-values)
     int nring[2] = {0, 0};
-    int rpt = 1, sg = 4;  // rectangle kernels: rows per wavefront, shots per workgroup
+    int rpt = 1, sg = 4;  // forward rectangle kernels: rows per wavefront, shots per workgroup
+    // adjoint: shots per group (one wavefront each in the streaming rectangle kernels, whose
+    // parameter terms are summed per group), streaming (1) or LDS-tiled (0) rectangle kernels,
+    // and the streaming kernels' rows per workgroup
+    int asg = NWAVES, strm = 1, strm_ch = 8;
     int sub = 0;          // one MPIElastic subdomain (see ElCoef)
     int64_t PX = 0, PY = 0;  // CPML profile lengths (NX, NY; the block size in the subdomain form)
 };
@@ -1193,11 +1554,11 @@ template <typename T> struct ElPlan : ElPlanBase {
 
     struct WS {
         int64_t K, NS, M;
-        T *seg, *ckpt, *adj, *tmp, *grad, *rho, *lam, *mu, *rb, *resid;
+        T *seg, *ckpt, *adj, *tmp, *grad, *rho, *lam, *mu, *rb, *par, *resid;
         double *loss, *part;
     };
     int64_t seg_count(int64_t K) const { return NMAX >= 1 ? (NMAX + K - 1) / K : 0; }
-    int64_t adj_groups() const { return (int64_t)cdiv(nshots, std::max(sg, 1)); }
+    int64_t adj_groups() const { return (int64_t)cdiv(nshots, std::max(asg, 1)); }
     void layout(WS &w, Carve &cv, int64_t K) const
     {
         if (K < 1) K = 1;
@@ -1212,6 +1573,7 @@ template <typename T> struct ElPlan : ElPlanBase {
         w.tmp = (T *)cv.take(F * 4);
         w.grad = (T *)cv.take((size_t)G_COUNT * adj_groups() * Npad * sizeof(T));
         w.rb = (T *)cv.take((size_t)Npad * sizeof(T));
+        w.par = (T *)cv.take((size_t)5 * Npad * sizeof(T));  // ElPar: irho, irb, lb, mb, mb2
         w.rho = (T *)cv.take((size_t)Npad * sizeof(T));
         w.lam = (T *)cv.take((size_t)Npad * sizeof(T));
         w.mu = (T *)cv.take((size_t)Npad * sizeof(T));
@@ -1366,10 +1728,10 @@ template <typename T> struct ElPlan : ElPlanBase {
         WS w;
         ADS_TRY(check_ws(w, ws, wsb, K));
         ADS_TRY(prepare_models(w.rho, w.lam, w.mu, rho, lam, mu, st));
+        // Level 0 is zero everywhere.  The CPML memory variables and their cotangents are read
+        // and written on frame cells only (frame kernels); rectangle cells never touch them, so
+        // the other slots need no clearing.
         ADS_HIP(hipMemsetAsync(slot(w, 0).p[0], 0, (size_t)NF * nshots * Npad * sizeof(T), st));
-        // rectangle cells never write their (zero) memory variables: clear them in every slot
-        for (int64_t q = 1; q < w.NS; q++)
-            ADS_HIP(hipMemsetAsync(slot(w, q).p[M1], 0, (size_t)8 * nshots * Npad * sizeof(T), st));
         if (traces && trace_elems) ADS_HIP(hipMemsetAsync(traces, 0, trace_elems * sizeof(T), st));
         for (int64_t m = 0; m < w.M; m++) {
             int64_t a = m * w.K, b = std::min<int64_t>(a + w.K, NMAX);
@@ -1414,7 +1776,7 @@ template <typename T> struct ElPlan : ElPlanBase {
         a.rb = w.rb;
         a.R = rect[li];
         a.act = act;
-        a.Sg = std::min(sg, act);
+        a.Sg = std::min(asg, act);
         const int ng = (int)cdiv(act, a.Sg);
         const dim3 fg(cdiv(nitems[li], NWAVES), ng);
         const int4 *items = d_items[li].template as<int4>();
@@ -1432,6 +1794,40 @@ template <typename T> struct ElPlan : ElPlanBase {
         // stresses (3), reads and writes the stress cotangents (6), reads rho; stress phase reads
         // the stress cotangents (3) and velocities (2), reads and writes the velocity cotangents
         // (4), reads lambda and mu; CPML memory values and cotangents in their strips
+        const ElPar<T> par{w.par, w.par + Npad, w.par + 2 * Npad, w.par + 3 * Npad, w.par + 4 * Npad};
+        if (rr && strm && li == 0 && a.Sg <= SW_NW) {
+            StrmMap sm;
+            sm.ch = std::max(1, std::min(strm_ch, SW_CH_MAX));
+            sm.nstrip = (int)cdiv(a.R.jR - a.R.jL, SW_COLS);
+            sm.nchunk = (int)cdiv(a.R.iR - a.R.iL, sm.ch);
+            sm.ngroup = ng;
+            sm.nb = sm.nstrip * sm.nchunk * sm.ngroup;
+            const unsigned sgrid = cdiv(sm.nb, NXCD) * NXCD;
+            const unsigned sblock = (unsigned)(WAVE * a.Sg);
+            const int nib = (int)cdiv(nitems[li], NWAVES), nfa = nib * ng;
+            const int nrb = (int)cdiv(nring[li], NWAVES);
+            const int4 *ring = d_ring[li].template as<int4>();
+            // frame_b rides in the streaming launch when its workgroups have NWAVES wavefronts
+            const bool fuse = nitems[li] > 0 && a.Sg == NWAVES;
+            const int nfb = fuse ? nfa : 0;
+            int pr = prof_start(ADS_PROF_EL_ADJ_VEL, st);
+            if (nfa) el_adjv_a<T><<<nfa, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li], std::max(nib, 1), ring,
+                                                                 0, 1, nfa);
+            if (nitems[li] && !fuse) el_adjv_frame_b<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
+            el_adjv_strm<T><<<(unsigned)nfb + sgrid, sblock, 0, st>>>(a, c, par, sm, items, nitems[li],
+                                                                      std::max(nib, 1), nfb);
+            prof_stop(pr, st, cells * sizeof(T) * (12.0 + 12.0 * (f_x + f_y)));
+            pr = prof_start(ADS_PROF_EL_ADJ_STRESS, st);
+            const unsigned ga = (unsigned)(nfa + nrb * ng);
+            if (ga) el_adjs_a<T><<<ga, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li], std::max(nib, 1), ring,
+                                                               nring[li], std::max(nrb, 1), nfa);
+            if (nitems[li] && !fuse) el_adjs_frame_b<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
+            el_adjs_strm<T><<<(unsigned)nfb + sgrid, sblock, 0, st>>>(a, c, par, sm, items, nitems[li],
+                                                                      std::max(nib, 1), nfb);
+            prof_stop(pr, st, cells * sizeof(T) * (11.0 + 12.0 * (f_x + f_y)));
+            ADS_LAUNCH_CHECK();
+            return ADS_OK;
+        }
         int pr = prof_start(ADS_PROF_EL_ADJ_VEL, st);
         const int nib = (int)cdiv(nitems[li], NWAVES), nfb = nib * ng;
         // velocity phase: frame_a alone (its gathers recompute rectangle t-values, two loads
@@ -1469,6 +1865,9 @@ template <typename T> struct ElPlan : ElPlanBase {
         ADS_HIP(hipMemsetAsync(w.grad, 0, (size_t)G_COUNT * adj_groups() * Npad * sizeof(T), st));
         ADS_HIP(hipMemsetAsync(w.tmp, 0, F * 4, st));
         el_rho_bar<T><<<dim3(cdiv(Pitch, 256), NX + 2), 256, 0, st>>>(w.rho, w.rb, (int)NX, (int)NY, (int)Pitch, sub);
+        el_adj_params<T><<<dim3(cdiv(Pitch, 256), NX + 2), 256, 0, st>>>(
+            w.rho, w.rb, w.lam, w.mu, w.par, w.par + Npad, w.par + 2 * Npad, w.par + 3 * Npad,
+            w.par + 4 * Npad, (int)NX, (int)NY, (int)Pitch, sub);
         const T *sv = (const T *)srcv;
         const El<T> A = block(w.adj);
         const int64_t L = NMAX + 1;
@@ -1798,6 +2197,18 @@ int64_t ads_elastic_plan_choose_interval(const ads_elastic_plan *plan, size_t bu
         if (K == 1) break;
     }
     return 0;
+}
+
+int ads_elastic_plan_set_adjoint_kernels(ads_elastic_plan *plan, int streaming, int rows, int shots)
+{
+    clear_error();
+    if (!plan || !plan->impl) ADS_FAIL(ADS_ERR_ARG, "null plan");
+    if (rows < 1 || rows > SW_CH_MAX || shots < 1 || shots > SW_NW)
+        ADS_FAIL(ADS_ERR_ARG, "rows must be in [1, %d] and shots in [1, %d]", SW_CH_MAX, SW_NW);
+    plan->impl->strm = streaming ? 1 : 0;
+    plan->impl->strm_ch = rows;
+    plan->impl->asg = shots;
+    return ADS_OK;
 }
 
 int ads_elastic_plan_forward(ads_elastic_plan *plan, const void *rho, const void *lam,

@@ -100,6 +100,13 @@ class ElasticPlan:
     def workspace_size(self, K: int) -> int:
         return int(self.lib.ads_elastic_plan_workspace_size(self.h, int(K)))
 
+    def set_adjoint_kernels(self, streaming: bool = True, rows: int = 8, shots: int = 4) -> None:
+        """Adjoint rectangle kernels (ads_elastic_plan_set_adjoint_kernels); changes the
+        workspace layout, so call it before the forward."""
+        _lib.check(self.lib.ads_elastic_plan_set_adjoint_kernels(self.h, int(bool(streaming)),
+                                                                 int(rows), int(shots)))
+        self._ws_K = None
+
     def choose_interval(self, budget_bytes: Optional[int] = None) -> int:
         if budget_bytes is None:
             budget_bytes = int(_lib.available_device_bytes() * 0.6)

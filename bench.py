@@ -254,13 +254,16 @@ def elastic_workload(ads, nstep=0):
     return p, true, start, srcs, rcvs
 
 
-def elastic_leg(ads, L, lib, steps, nstep=0):
-    """Batched FP32 elastic FWI gradient (d/d rho, lambda, mu) throughput and kernel roofline."""
+def elastic_leg(ads, L, lib, steps, nstep=0, adjoint_kernels=None):
+    """Batched FP32 elastic FWI gradient (d/d rho, lambda, mu) throughput and kernel roofline.
+    adjoint_kernels: (streaming, rows, shots) for ads_elastic_plan_set_adjoint_kernels."""
     import ctypes as C
     import torch
 
     p, true, start, srcs, rcvs = elastic_workload(ads, nstep)
     plan = ads.ElasticPlan(p, srcs, rcvs, dtype=torch.float32)
+    if adjoint_kernels:
+        plan.set_adjoint_kernels(*adjoint_kernels)
     srcv = plan.pack_srcv([s.srcv for s in srcs])
     K = plan.choose_interval(int(torch.cuda.mem_get_info()[0] * 0.80))
     tt = [torch.as_tensor(x, dtype=torch.float32, device="cuda") for x in true]
@@ -450,6 +453,9 @@ def main():
                     help="run only the elastic gradient leg (profiling runs; no bench line)")
     ap.add_argument("--elastic-nstep", type=int, default=0,
                     help="override the elastic record length (profiling runs only)")
+    ap.add_argument("--elastic-adjoint", default="",
+                    help="elastic adjoint rectangle kernels 'streaming,rows,shots' (default: the "
+                         "plan's, 1,8,4)")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -468,7 +474,8 @@ def main():
     lib = ads.load_library()
     torch.cuda.init()
     if args.elastic_only:
-        print(json.dumps({"elastic": elastic_leg(ads, L, lib, args.steps, args.elastic_nstep)}),
+        ek = tuple(int(x) for x in args.elastic_adjoint.split(",")) if args.elastic_adjoint else None
+        print(json.dumps({"elastic": elastic_leg(ads, L, lib, args.steps, args.elastic_nstep, ek)}),
               flush=True)
         return
     if args.calib_copy:

@@ -242,3 +242,27 @@ def test_mpi_elastic_block_plan_and_dropin_f64(orc, ads, cuda):
     torch.sum((out - T(obs)) ** 2).backward()
     for t, b, name in zip(mg, g_ref, ("rho", "lambda", "mu")):
         assert rel(t.grad.cpu(), b) < 1e-10, (name, rel(t.grad.cpu(), b))
+
+
+@pytest.mark.parametrize("kernels", [(1, 8, 4), (1, 16, 8), (1, 3, 2), (1, 32, 1), (0, 16, 4)])
+def test_elastic_adjoint_rectangle_kernels_f64(orc, ads, cuda, kernels):
+    """Every adjoint rectangle variant gives the oracle's gradient: the streaming kernels with
+    frame_b in the same launch (4 shots per group), with frame_b launched apart (8, 2, 1 shots
+    per group; 5 uneven shots, so the last group is partial and, near the end, fewer shots are
+    active than a group holds), chunk heights that do not divide the rectangle, and the
+    LDS-tiled kernels."""
+    p, models, srcs, rcvs, nstep = make_case(orc, ads, NX=41, NY=37, NSTEP=50, nshots=5,
+                                             uneven=True, seed=3)
+    obs, g_ref = [], [0.0, 0.0, 0.0]
+    for s in range(5):
+        o, _ = oracle(orc, p, models, srcs[s], rcvs[s], nstep[s], scale=(1.02, 0.97, 1.04))
+        _, g, _ = oracle(orc, p, models, srcs[s], rcvs[s], nstep[s], obs=o)
+        obs.append(o)
+        g_ref = [a + b for a, b in zip(g_ref, g)]
+    plan = ads.ElasticPlan(p, srcs, rcvs, dtype=torch.float64, nstep=nstep)
+    plan.set_adjoint_kernels(*kernels)
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    mt = [T(m) for m in models]
+    L, g, _ = plan.gradient(*mt, srcv, plan.pack_traces(obs), 11)
+    errs = [rel(a.cpu(), b) for a, b in zip(g, g_ref)]
+    assert max(errs) < 1e-10, (kernels, errs)
