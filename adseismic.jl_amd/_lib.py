@@ -107,6 +107,9 @@ _SIGS = {
     "ads_elastic_plan_workspace_size": (C.c_size_t, [vp, i64]),
     "ads_elastic_plan_choose_interval": (i64, [vp, C.c_size_t]),
     "ads_elastic_plan_set_adjoint_kernels": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+    "ads_elastic_plan_set_snapshots": (C.c_int, [vp, i64]),
+    "ads_elastic_plan_snapshots": (i64, [vp]),
+    "ads_elastic_plan_choose_schedule": (C.c_int, [vp, C.c_size_t, i64p, i64p]),
     "ads_elastic_plan_forward": (C.c_int, [vp] * 6 + [vp, C.c_size_t, i64, vp]),
     "ads_elastic_plan_backward": (C.c_int, [vp] * 9 + [vp, C.c_size_t, i64, vp]),
     "ads_elastic_plan_gradient": (C.c_int, [vp] * 12 + [C.c_size_t, i64, vp]),

@@ -1881,45 +1881,20 @@ template <typename T> struct AcPlan : AcPlanBase {
     static constexpr double ADJ_COST = 1.7;
     int choose_schedule(size_t budget, int64_t *Kout, int64_t *Sout) const
     {
-        const int64_t L = std::max<int64_t>(NMAX - 1, 1);
         double total = 0.0;
         for (int64_t n = 2; n <= NMAX; n++) total += active[n];
-        double best = 0.0;
-        int64_t bK = 0, bS = 0;
-        int64_t lastK = -1;
-        for (int64_t Mc = 1; Mc <= std::min<int64_t>(L, 4096); Mc++) {
-            const int64_t K = (L + Mc - 1) / Mc;
-            if (K == lastK) continue;
-            lastK = K;
-            const int64_t M = seg_count(K);
-            if (ws_size(K, 0) > budget) continue;
-            int64_t S = std::max<int64_t>(M - 1, 0);
-            if (ws_size(K, -1) > budget) {
-                const size_t rec = ((rec_elems(rec_act(1 + K)) + 63) & ~size_t(63)) * sizeof(T);
-                S = std::min<int64_t>((int64_t)((budget - ws_size(K, 0)) / std::max<size_t>(rec, 1)),
-                                      std::max<int64_t>(M - 2, 0));
-                while (S > 0 && ws_size(K, S) > budget) S--;
-            }
-            if (S == 0 && M > 64) continue;  // quadratic rebuilds: never the fastest here
-            std::vector<double> wseg((size_t)M, 0.0);
+        auto ws = [&](int64_t K, int64_t S) { return ws_size(K, S); };
+        auto work = [&](int64_t K, int64_t M) {
+            std::vector<double> w((size_t)M, 0.0);
             for (int64_t m = 0; m < M; m++) {
                 const int64_t a = std::min<int64_t>(1 + m * K, NMAX), b = std::min<int64_t>(a + K, NMAX);
-                for (int64_t n = a + 1; n <= b; n++) wseg[m] += active[n];
+                for (int64_t n = a + 1; n <= b; n++) w[m] += active[n];
             }
-            CostOps c{&wseg};
-            Reversal rv(M, S);
-            rv.run(c);
-            const double t = (1.0 + ADJ_COST) * total + c.extra;
-            if (bK == 0 || t < best * (1.0 - 1e-12)) {
-                best = t;
-                bK = K;
-                bS = S;
-            }
-        }
-        if (bK == 0) return ADS_ERR_WORKSPACE;
-        *Kout = bK;
-        *Sout = bS;
-        return ADS_OK;
+            return w;
+        };
+        return ads::choose_schedule(budget, NMAX - 1, total, ADJ_COST, ws, work, *Kout, *Sout)
+                   ? ADS_OK
+                   : ADS_ERR_WORKSPACE;
     }
 
     Lv<T> slot(const WS &w, int64_t n) const

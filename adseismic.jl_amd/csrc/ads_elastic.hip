@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "ads_common.h"
+#include "ads_schedule.h"
 #include "ads_stencil.h"
 
 namespace ads {
@@ -1063,9 +1064,11 @@ __global__ __launch_bounds__(256) void el_adjs_rb(ElAdj<T> a, ElCoef<T> c, TileM
 // loaded per step) and taking the y-direction neighbours of row i from the adjacent lanes.
 // t-values inside the rectangle are formed from the cotangents (t0 = t1, t2 = t3 there); on
 // the frame cells the halo reaches they come from the temporaries frame_a wrote, and outside
-// the interior they are 0, as in tv_vel / tv_str.  The parameter terms of the group's shots
-// are summed per cell in LDS and added to the group's accumulators once per workgroup.
-constexpr int SW_NW = 8, SW_CH_MAX = 32, SW_COLS = WAVE - 4;
+// the interior they are 0, as in tv_vel / tv_str.  Each wavefront writes its shot's parameter
+// terms per cell to its own LDS slice; after one barrier the slices are added in wavefront
+// order (no atomics: the sums, like every other sum of the plan, are bitwise reproducible) and
+// added to the group's accumulators once per workgroup.  LDS: shots x terms x rows x WAVE.
+constexpr int SW_NW = 8, SW_CH_MAX = 32, SW_COLS = WAVE - 4, SW_LDS_ROWS = 64;  // shots x rows <= 64
 
 struct StrmMap {
     int nstrip, nchunk, ngroup, nb, ch;
@@ -1096,11 +1099,16 @@ template <typename T>
 __device__ inline void adjv_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, const ElPar<T> &q,
                                       const StrmMap &mp, int bid)
 {
-    __shared__ T gacc[2][SW_CH_MAX][WAVE];
+    extern __shared__ __align__(16) unsigned char strm_lds[];
+    T *__restrict__ gacc = reinterpret_cast<T *>(strm_lds);  // [wave][term][row][lane]
     int g, chunk, strip;
     if (!decode_strm(mp, g, chunk, strip, bid)) return;
-    for (int k = threadIdx.x; k < 2 * SW_CH_MAX * WAVE; k += blockDim.x) (&gacc[0][0][0])[k] = T(0);
+    const int nw = (int)blockDim.x / WAVE;
+    for (int k = threadIdx.x; k < nw * 2 * mp.ch * WAVE; k += blockDim.x) gacc[k] = T(0);
     __syncthreads();
+    auto gslot = [&](int wv, int term, int r, int l) -> T & {
+        return gacc[((wv * 2 + term) * mp.ch + r) * WAVE + l];
+    };
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int i0 = a.R.iL + chunk * mp.ch, i1 = min(i0 + mp.ch, a.R.iR);
     const int jb = a.R.jL + strip * SW_COLS - 2;
@@ -1194,8 +1202,8 @@ __device__ inline void adjv_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                 // parameter terms (fw3: rho; fw4: rho-bar)
                 const T S3 = d4(wx[2], wx[1], wx[3], wx[0]) * c.ihx + d4(xy, xym, xyp, xymm) * c.ihy;
                 const T S4 = d4(wy[3], wy[2], wy[4], wy[1]) * c.ihx + d4(yyp, yy, yypp, yym) * c.ihy;
-                atomicAdd(&gacc[0][i - i0][lane], -w0[2] * S3 * wr[2]);
-                atomicAdd(&gacc[1][i - i0][lane], -w2[2] * S4 * wb[2]);
+                gslot(w, 0, i - i0, lane) = -w0[2] * S3 * wr[2];
+                gslot(w, 1, i - i0, lane) = -w2[2] * S4 * wb[2];
                 // sxx <- (D-x)^T t0 ; sxy <- (D-y)^T t1 + (D+x)^T t2 ; syy <- (D+y)^T t3
                 axx[ij] = oc.gxx + d4(w0[2], w0[3], w0[1], w0[4]) * c.ihx;
                 axy[ij] = oc.gxy + d4(t1, t1p, t1m, t1pp) * c.ihy + d4(w2[1], w2[2], w2[0], w2[3]) * c.ihx;
@@ -1217,8 +1225,13 @@ __device__ inline void adjv_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, con
         const int r = k / WAVE, l = k % WAVE, jj = jb + l;
         if (l < 2 || l >= WAVE - 2 || jj >= a.R.jR) continue;
         const int64_t ij = (int64_t)(i0 + r) * c.P + jj;
-        g_at(a, G_RHO, g, ij) += gacc[0][r][l];
-        g_at(a, G_RB, g, ij) += gacc[1][r][l];
+        T s0 = T(0), s1 = T(0);
+        for (int wv = 0; wv < nw; wv++) {
+            s0 += gslot(wv, 0, r, l);
+            s1 += gslot(wv, 1, r, l);
+        }
+        g_at(a, G_RHO, g, ij) += s0;
+        g_at(a, G_RB, g, ij) += s1;
     }
 }
 
@@ -1227,11 +1240,16 @@ template <typename T>
 __device__ inline void adjs_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, const ElPar<T> &q,
                                       const StrmMap &mp, int bid)
 {
-    __shared__ T gacc[3][SW_CH_MAX][WAVE];
+    extern __shared__ __align__(16) unsigned char strm_lds[];
+    T *__restrict__ gacc = reinterpret_cast<T *>(strm_lds);  // [wave][term][row][lane]
     int g, chunk, strip;
     if (!decode_strm(mp, g, chunk, strip, bid)) return;
-    for (int k = threadIdx.x; k < 3 * SW_CH_MAX * WAVE; k += blockDim.x) (&gacc[0][0][0])[k] = T(0);
+    const int nw = (int)blockDim.x / WAVE;
+    for (int k = threadIdx.x; k < nw * 3 * mp.ch * WAVE; k += blockDim.x) gacc[k] = T(0);
     __syncthreads();
+    auto gslot = [&](int wv, int term, int r, int l) -> T & {
+        return gacc[((wv * 3 + term) * mp.ch + r) * WAVE + l];
+    };
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int i0 = a.R.iL + chunk * mp.ch, i1 = min(i0 + mp.ch, a.R.iR);
     const int jb = a.R.jL + strip * SW_COLS - 2;
@@ -1334,11 +1352,11 @@ __device__ inline void adjs_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                 const T X = d4(wu[3], wu[2], wu[4], wu[1]) * c.ihx;
                 const T Y = d4(v, vm, vp, vmm) * c.ihy;
                 const T glm = X * gx[2] + Y * gy[2];
-                atomicAdd(&gacc[0][i - i0][lane], Y * gx[2] + X * gy[2] + glm);
-                atomicAdd(&gacc[1][i - i0][lane], T(2) * glm);
+                gslot(w, 0, i - i0, lane) = Y * gx[2] + X * gy[2] + glm;
+                gslot(w, 1, i - i0, lane) = T(2) * glm;
                 const T dx = d4(wv[2], wv[1], wv[3], wv[0]) * c.ihx;
                 const T dy = d4(up, u, upp, um) * c.ihy;
-                atomicAdd(&gacc[2][i - i0][lane], gs[2] * (dx + dy) * dt);
+                gslot(w, 2, i - i0, lane) = gs[2] * (dx + dy) * dt;
                 // vx <- (D+x)^T t0 + (D+y)^T t3 ; vy <- (D-y)^T t1 + (D-x)^T t2
                 avx[ij] = oc.gvx + d4(w0[1], w0[2], w0[0], w0[3]) * c.ihx + d4(t3m, t3, t3mm, t3p) * c.ihy;
                 avy[ij] = oc.gvy + d4(t1, t1p, t1m, t1pp) * c.ihy + d4(w2[2], w2[3], w2[1], w2[4]) * c.ihx;
@@ -1361,9 +1379,15 @@ __device__ inline void adjs_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, con
         const int r = k / WAVE, l = k % WAVE, jj = jb + l;
         if (l < 2 || l >= WAVE - 2 || jj >= a.R.jR) continue;
         const int64_t ij = (int64_t)(i0 + r) * c.P + jj;
-        g_at(a, G_LB, g, ij) += gacc[0][r][l];
-        g_at(a, G_MB1, g, ij) += gacc[1][r][l];
-        g_at(a, G_MB2, g, ij) += gacc[2][r][l];
+        T s0 = T(0), s1 = T(0), s2 = T(0);
+        for (int wv = 0; wv < nw; wv++) {
+            s0 += gslot(wv, 0, r, l);
+            s1 += gslot(wv, 1, r, l);
+            s2 += gslot(wv, 2, r, l);
+        }
+        g_at(a, G_LB, g, ij) += s0;
+        g_at(a, G_MB1, g, ij) += s1;
+        g_at(a, G_MB2, g, ij) += s2;
     }
 }
 
@@ -1522,6 +1546,8 @@ struct ElPlanBase {
     // parameter terms are summed per group), streaming (1) or LDS-tiled (0) rectangle kernels,
     // and the streaming kernels' rows per workgroup
     int asg = NWAVES, strm = 1, strm_ch = 8;
+    // snapshot slots of the reversal schedule (< 0: one per segment start, the one-level scheme)
+    int64_t snapshots = -1;
     int sub = 0;          // one MPIElastic subdomain (see ElCoef)
     int64_t PX = 0, PY = 0;  // CPML profile lengths (NX, NY; the block size in the subdomain form)
 };
@@ -1553,22 +1579,25 @@ template <typename T> struct ElPlan : ElPlanBase {
     dim3 grid(int act) const { return dim3(cdiv(NY + 2, 64), cdiv(NX + 2, 4), act); }
 
     struct WS {
-        int64_t K, NS, M;
+        int64_t K, NS, M, S;  // interval, slots, segments, snapshot slots
         T *seg, *ckpt, *adj, *tmp, *grad, *rho, *lam, *mu, *rb, *par, *resid;
         double *loss, *part;
     };
     int64_t seg_count(int64_t K) const { return NMAX >= 1 ? (NMAX + K - 1) / K : 0; }
     int64_t adj_groups() const { return (int64_t)cdiv(nshots, std::max(asg, 1)); }
-    void layout(WS &w, Carve &cv, int64_t K) const
+    void layout(WS &w, Carve &cv, int64_t K) const { layout(w, cv, K, snapshots); }
+    void layout(WS &w, Carve &cv, int64_t K, int64_t snaps) const
     {
         if (K < 1) K = 1;
         if (K > std::max<int64_t>(NMAX, 1)) K = std::max<int64_t>(NMAX, 1);
         w.K = K;
         w.M = seg_count(K);
         w.NS = std::min<int64_t>(K + 1, NMAX + 1);
+        const int64_t one = std::max<int64_t>(w.M - 1, 0);
+        w.S = snaps < 0 ? one : std::min(snaps, one);
         const size_t F = (size_t)nshots * Npad * sizeof(T);
         w.seg = (T *)cv.take(F * NF * w.NS);
-        w.ckpt = (T *)cv.take(F * NF * (size_t)std::max<int64_t>(w.M - 1, 0));
+        w.ckpt = (T *)cv.take(F * NF * (size_t)w.S);  // slot q: a whole level of every shot
         w.adj = (T *)cv.take(F * NF);
         w.tmp = (T *)cv.take(F * 4);
         w.grad = (T *)cv.take((size_t)G_COUNT * adj_groups() * Npad * sizeof(T));
@@ -1581,12 +1610,31 @@ template <typename T> struct ElPlan : ElPlanBase {
         w.loss = (double *)cv.take(64);
         w.part = (double *)cv.take(MISFIT_BLOCKS * sizeof(double));
     }
-    size_t ws_size(int64_t K) const
+    size_t ws_size(int64_t K) const { return ws_size(K, snapshots); }
+    size_t ws_size(int64_t K, int64_t snaps) const
     {
         WS w;
         Carve cv(nullptr, 0);
-        layout(w, cv, K);
+        layout(w, cv, K, snaps);
         return cv.off + 256;
+    }
+    static constexpr double ADJ_COST = 2.5;  // an adjoint level costs about 2.5 forward levels
+    int choose_schedule(size_t budget, int64_t *Kout, int64_t *Sout) const
+    {
+        double total = 0.0;
+        for (int64_t n = 1; n <= NMAX; n++) total += active[n];
+        auto ws = [&](int64_t K, int64_t S) { return ws_size(K, S); };
+        auto work = [&](int64_t K, int64_t M) {
+            std::vector<double> w((size_t)M, 0.0);
+            for (int64_t m = 0; m < M; m++) {
+                const int64_t a = m * K, b = std::min<int64_t>(a + K, NMAX);
+                for (int64_t n = a + 1; n <= b; n++) w[m] += active[n];
+            }
+            return w;
+        };
+        return ads::choose_schedule(budget, NMAX, total, ADJ_COST, ws, work, *Kout, *Sout)
+                   ? ADS_OK
+                   : ADS_ERR_WORKSPACE;
     }
     int check_ws(WS &w, void *ws, size_t wsb, int64_t K)
     {
@@ -1717,9 +1765,9 @@ template <typename T> struct ElPlan : ElPlanBase {
                                    hipMemcpyDeviceToDevice, st));
         return ADS_OK;
     }
-    El<T> ckpt(const WS &w, int64_t m) const
+    El<T> ckpt(const WS &w, int64_t q) const  // snapshot slot q
     {
-        return block(w.ckpt + (size_t)(m - 1) * NF * nshots * Npad);
+        return block(w.ckpt + (size_t)q * NF * nshots * Npad);
     }
 
     int forward(const void *rho, const void *lam, const void *mu, const void *srcv, void *traces,
@@ -1733,9 +1781,14 @@ template <typename T> struct ElPlan : ElPlanBase {
         // the other slots need no clearing.
         ADS_HIP(hipMemsetAsync(slot(w, 0).p[0], 0, (size_t)NF * nshots * Npad * sizeof(T), st));
         if (traces && trace_elems) ADS_HIP(hipMemsetAsync(traces, 0, trace_elems * sizeof(T), st));
+        // the reversal schedule's first descent: the states it keeps, in level order
+        Reversal rv(w.M, w.S);
+        const auto path = rv.first_path();
+        size_t pi = 0;
         for (int64_t m = 0; m < w.M; m++) {
             int64_t a = m * w.K, b = std::min<int64_t>(a + w.K, NMAX);
-            if (m > 0) ADS_TRY(copy_state(ckpt(w, m), slot(w, a), active[std::min<int64_t>(a + 1, NMAX)], st));
+            if (pi < path.size() && path[pi].first == m)
+                ADS_TRY(copy_state(ckpt(w, path[pi++].second), slot(w, a), active[std::min<int64_t>(a + 1, NMAX)], st));
             ADS_TRY(run_forward(w, a, b, (const T *)srcv, (T *)traces, traces != nullptr, st));
         }
         return ADS_OK;
@@ -1795,7 +1848,10 @@ template <typename T> struct ElPlan : ElPlanBase {
         // the stress cotangents (3) and velocities (2), reads and writes the velocity cotangents
         // (4), reads lambda and mu; CPML memory values and cotangents in their strips
         const ElPar<T> par{w.par, w.par + Npad, w.par + 2 * Npad, w.par + 3 * Npad, w.par + 4 * Npad};
-        if (rr && strm && li == 0 && a.Sg <= SW_NW) {
+        // streaming kernels: their LDS (3 terms x shots x rows x WAVE values) must fit in 64 KiB,
+        // else the LDS-tiled kernels run
+        const size_t strm_lds = (size_t)3 * a.Sg * std::max(1, std::min(strm_ch, SW_CH_MAX)) * WAVE * sizeof(T);
+        if (rr && strm && li == 0 && a.Sg <= SW_NW && strm_lds <= 65536) {
             StrmMap sm;
             sm.ch = std::max(1, std::min(strm_ch, SW_CH_MAX));
             sm.nstrip = (int)cdiv(a.R.jR - a.R.jL, SW_COLS);
@@ -1814,16 +1870,17 @@ template <typename T> struct ElPlan : ElPlanBase {
             if (nfa) el_adjv_a<T><<<nfa, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li], std::max(nib, 1), ring,
                                                                  0, 1, nfa);
             if (nitems[li] && !fuse) el_adjv_frame_b<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
-            el_adjv_strm<T><<<(unsigned)nfb + sgrid, sblock, 0, st>>>(a, c, par, sm, items, nitems[li],
-                                                                      std::max(nib, 1), nfb);
+            const size_t lds = (size_t)a.Sg * sm.ch * WAVE * sizeof(T);  // x terms
+            el_adjv_strm<T><<<(unsigned)nfb + sgrid, sblock, 2 * lds, st>>>(a, c, par, sm, items, nitems[li],
+                                                                              std::max(nib, 1), nfb);
             prof_stop(pr, st, cells * sizeof(T) * (12.0 + 12.0 * (f_x + f_y)));
             pr = prof_start(ADS_PROF_EL_ADJ_STRESS, st);
             const unsigned ga = (unsigned)(nfa + nrb * ng);
             if (ga) el_adjs_a<T><<<ga, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li], std::max(nib, 1), ring,
                                                                nring[li], std::max(nrb, 1), nfa);
             if (nitems[li] && !fuse) el_adjs_frame_b<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
-            el_adjs_strm<T><<<(unsigned)nfb + sgrid, sblock, 0, st>>>(a, c, par, sm, items, nitems[li],
-                                                                      std::max(nib, 1), nfb);
+            el_adjs_strm<T><<<(unsigned)nfb + sgrid, sblock, 3 * lds, st>>>(a, c, par, sm, items, nitems[li],
+                                                                              std::max(nib, 1), nfb);
             prof_stop(pr, st, cells * sizeof(T) * (11.0 + 12.0 * (f_x + f_y)));
             ADS_LAUNCH_CHECK();
             return ADS_OK;
@@ -1871,33 +1928,56 @@ template <typename T> struct ElPlan : ElPlanBase {
         const T *sv = (const T *)srcv;
         const El<T> A = block(w.adj);
         const int64_t L = NMAX + 1;
-        for (int64_t m = w.M - 1; m >= 0; m--) {
-            int64_t a = m * w.K, b = std::min<int64_t>(a + w.K, NMAX);
-            if (m < w.M - 1) {
-                int act = active[std::min<int64_t>(a + 1, NMAX)];
-                if (m == 0) {
-                    El<T> z = slot(w, 0);
-                    for (int q = 0; q < NF; q++)
-                        ADS_HIP(hipMemsetAsync(z.p[q], 0, (size_t)act * Npad * sizeof(T), st));
-                } else {
-                    ADS_TRY(copy_state(slot(w, a), ckpt(w, m), act, st));
+        struct Ops {
+            ElPlan *pl;
+            const WS &w;
+            const El<T> &A;
+            const T *sv, *resid;
+            const void *dense_hist;
+            int64_t L;
+            hipStream_t st;
+            int act_at(int64_t m) const { return pl->active[std::min<int64_t>(m * w.K + 1, pl->NMAX)]; }
+            int restore(int64_t m, int64_t q)
+            {
+                const int act = act_at(m);
+                if (q < 0) {  // level 0: zero
+                    El<T> z = pl->slot(w, 0);
+                    for (int f = 0; f < NF; f++)
+                        ADS_HIP(hipMemsetAsync(z.p[f], 0, (size_t)act * pl->Npad * sizeof(T), st));
+                    return ADS_OK;
                 }
-                ADS_TRY(run_forward(w, a, b, sv, nullptr, 0, st));
+                return pl->copy_state(pl->slot(w, m * w.K), pl->ckpt(w, q), act, st);
             }
-            for (int64_t n = b; n > a; n--) {
-                ElC<T> D, *Dp = nullptr;
-                if (dense_hist) {  // [shot][field][level][Ndense]
-                    for (int q = 0; q < 5; q++) {
-                        D.p[q] = (const T *)dense_hist + ((size_t)q * L + n) * Ndense;
-                        D.st[q] = 5 * L * Ndense;
+            int save(int64_t m, int64_t q) { return pl->copy_state(pl->ckpt(w, q), pl->slot(w, m * w.K), act_at(m), st); }
+            int forward(int64_t m0, int64_t m1)
+            {
+                for (int64_t m = m0; m < m1; m++) {
+                    const int64_t a = m * w.K, b = std::min<int64_t>(a + w.K, pl->NMAX);
+                    ADS_TRY(pl->run_forward(w, a, b, sv, nullptr, 0, st));
+                }
+                return ADS_OK;
+            }
+            int adjoint(int64_t m)
+            {
+                const int64_t a = m * w.K, b = std::min<int64_t>(a + w.K, pl->NMAX);
+                for (int64_t n = b; n > a; n--) {
+                    ElC<T> D, *Dp = nullptr;
+                    if (dense_hist) {  // [shot][field][level][Ndense]
+                        for (int q = 0; q < 5; q++) {
+                            D.p[q] = (const T *)dense_hist + ((size_t)q * L + n) * pl->Ndense;
+                            D.st[q] = 5 * L * pl->Ndense;
+                        }
+                        for (int q = 5; q < NF; q++) D.p[q] = nullptr, D.st[q] = 0;
+                        Dp = &D;
                     }
-                    for (int q = 5; q < NF; q++) D.p[q] = nullptr, D.st[q] = 0;
-                    Dp = &D;
+                    ADS_TRY(pl->adj_level(A, pl->slot(w, n), pl->slot(w, n - 1), (int)n, pl->active[n],
+                                          resid, Dp, w, sv, (int)pl->Pitch, 0, st));
                 }
-                ADS_TRY(adj_level(A, slot(w, n), slot(w, n - 1), (int)n, active[n], resid, Dp, w,
-                                  sv, (int)Pitch, 0, st));
+                return ADS_OK;
             }
-        }
+        } ops{this, w, A, sv, resid, dense_hist, L, st};
+        Reversal rv(w.M, w.S);
+        ADS_TRY(rv.run(ops));
         dim3 g(cdiv(NY + 2, 256), NX + 2);
         el_grad_map<T><<<g, 256, 0, st>>>(w.grad, Npad, adj_groups() * Npad, (int)adj_groups(),
                                           (T *)grho, (T *)glam, (T *)gmu, (int)NX, (int)NY, (int)Pitch, sub);
@@ -2193,18 +2273,47 @@ int64_t ads_elastic_plan_choose_interval(const ads_elastic_plan *plan, size_t bu
     const int64_t L = std::max<int64_t>(plan->impl->NMAX, 1);
     for (int64_t M = 1; M <= L; M++) {
         const int64_t K = (L + M - 1) / M;
-        if (ads_elastic_plan_workspace_size(plan, K) <= budget) return K;
+        const size_t need = plan->impl->dtype == ADS_F64
+                                ? static_cast<ElPlan<double> *>(plan->impl.get())->ws_size(K, -1)
+                                : static_cast<ElPlan<float> *>(plan->impl.get())->ws_size(K, -1);
+        if (need <= budget) return K;
         if (K == 1) break;
     }
     return 0;
+}
+
+int ads_elastic_plan_set_snapshots(ads_elastic_plan *plan, int64_t snapshots)
+{
+    clear_error();
+    if (!plan || !plan->impl) ADS_FAIL(ADS_ERR_ARG, "null plan");
+    plan->impl->snapshots = snapshots < 0 ? -1 : snapshots;
+    return ADS_OK;
+}
+
+int64_t ads_elastic_plan_snapshots(const ads_elastic_plan *plan)
+{
+    return plan && plan->impl ? plan->impl->snapshots : -1;
+}
+
+int ads_elastic_plan_choose_schedule(ads_elastic_plan *plan, size_t budget, int64_t *K, int64_t *S)
+{
+    clear_error();
+    if (!plan || !plan->impl || !K || !S) ADS_FAIL(ADS_ERR_ARG, "null argument");
+    const int rc = plan->impl->dtype == ADS_F64
+                       ? static_cast<ElPlan<double> *>(plan->impl.get())->choose_schedule(budget, K, S)
+                       : static_cast<ElPlan<float> *>(plan->impl.get())->choose_schedule(budget, K, S);
+    if (rc != ADS_OK) ADS_FAIL(rc, "no checkpoint schedule fits %zu bytes", budget);
+    plan->impl->snapshots = *S;
+    return ADS_OK;
 }
 
 int ads_elastic_plan_set_adjoint_kernels(ads_elastic_plan *plan, int streaming, int rows, int shots)
 {
     clear_error();
     if (!plan || !plan->impl) ADS_FAIL(ADS_ERR_ARG, "null plan");
-    if (rows < 1 || rows > SW_CH_MAX || shots < 1 || shots > SW_NW)
-        ADS_FAIL(ADS_ERR_ARG, "rows must be in [1, %d] and shots in [1, %d]", SW_CH_MAX, SW_NW);
+    if (rows < 1 || rows > SW_CH_MAX || shots < 1 || shots > SW_NW || rows * shots > SW_LDS_ROWS)
+        ADS_FAIL(ADS_ERR_ARG, "rows must be in [1, %d], shots in [1, %d] and rows x shots <= %d",
+                 SW_CH_MAX, SW_NW, SW_LDS_ROWS);
     plan->impl->strm = streaming ? 1 : 0;
     plan->impl->strm_ch = rows;
     plan->impl->asg = shots;

@@ -170,6 +170,54 @@ struct CostOps {
     }
 };
 
+// The (K, S) with the least modelled time whose workspace fits `budget`, over L levels cut into
+// M = ceil(L / K) segments.  ws_size(K, S) is the workspace for interval K and S snapshot slots
+// (S < 0: one per segment start); seg_work(K, M) the work of each segment (the shots active at
+// its levels).  Model: the forward and the adjoint sweep pass every level once (an adjoint
+// level costs adj_cost forward levels) and the schedule adds its rebuilds.  Among the one-level
+// layouts that fit, every interval rebuilds all segments but the last, so the model prefers
+// the longest last segment; otherwise it takes the most slots that fit.
+template <class WsFn, class WorkFn>
+inline bool choose_schedule(size_t budget, int64_t L, double total_work, double adj_cost,
+                            WsFn ws_size, WorkFn seg_work, int64_t &Kout, int64_t &Sout)
+{
+    double best = 0.0;
+    int64_t bK = 0, bS = 0, lastK = -1;
+    L = std::max<int64_t>(L, 1);
+    for (int64_t Mc = 1; Mc <= std::min<int64_t>(L, 4096); Mc++) {
+        const int64_t K = (L + Mc - 1) / Mc;
+        if (K == lastK) continue;
+        lastK = K;
+        const int64_t M = (L + K - 1) / K;
+        if (ws_size(K, 0) > budget) continue;
+        int64_t S = std::max<int64_t>(M - 1, 0);
+        if (ws_size(K, -1) > budget) {  // the most slots that fit, below the one-level count
+            int64_t lo = 0, hi = std::max<int64_t>(M - 2, 0);
+            while (lo < hi) {
+                const int64_t mid = (lo + hi + 1) / 2;
+                if (ws_size(K, mid) <= budget) lo = mid;
+                else hi = mid - 1;
+            }
+            S = lo;
+        }
+        if (S == 0 && M > 64) continue;  // quadratic rebuilds: never the fastest here
+        const std::vector<double> w = seg_work(K, M);
+        CostOps c{&w};
+        Reversal rv(M, S);
+        rv.run(c);
+        const double t = (1.0 + adj_cost) * total_work + c.extra;
+        if (bK == 0 || t < best * (1.0 - 1e-12)) {
+            best = t;
+            bK = K;
+            bS = S;
+        }
+    }
+    if (bK == 0) return false;
+    Kout = bK;
+    Sout = bS;
+    return true;
+}
+
 // ---- compact records of the PML auxiliary fields ----
 // The auxiliary fields (acoustic phi/psi, elastic CPML memory variables) are zero on every
 // cell of the "quiet box" [qi0, qi1) x [qj0, qj1), where no damping profile acts; a record

@@ -107,6 +107,27 @@ class ElasticPlan:
                                                                  int(rows), int(shots)))
         self._ws_K = None
 
+    def set_snapshots(self, snapshots: int) -> None:
+        """Snapshot slots of the reversal schedule: < 0 one per segment start (one-level),
+        otherwise the binomial schedule with that many slots."""
+        _lib.check(self.lib.ads_elastic_plan_set_snapshots(self.h, int(snapshots)))
+        self._ws_K = None
+
+    @property
+    def snapshots(self) -> int:
+        return int(self.lib.ads_elastic_plan_snapshots(self.h))
+
+    def choose_schedule(self, budget_bytes: Optional[int] = None):
+        """(K, S) with the least modelled time whose workspace fits the budget (default 90% of
+        the device memory a workspace can use); the plan keeps S."""
+        if budget_bytes is None:
+            budget_bytes = int(_lib.available_device_bytes() * 0.9)
+        K, S = _lib.i64(0), _lib.i64(0)
+        _lib.check(self.lib.ads_elastic_plan_choose_schedule(self.h, int(budget_bytes),
+                                                             C.byref(K), C.byref(S)))
+        self._ws_K = None
+        return int(K.value), int(S.value)
+
     def choose_interval(self, budget_bytes: Optional[int] = None) -> int:
         if budget_bytes is None:
             budget_bytes = int(_lib.available_device_bytes() * 0.6)

@@ -265,7 +265,7 @@ def elastic_leg(ads, L, lib, steps, nstep=0, adjoint_kernels=None):
     if adjoint_kernels:
         plan.set_adjoint_kernels(*adjoint_kernels)
     srcv = plan.pack_srcv([s.srcv for s in srcs])
-    K = plan.choose_interval(int(torch.cuda.mem_get_info()[0] * 0.80))
+    K, S = plan.choose_schedule(int(torch.cuda.mem_get_info()[0] * 0.90))
     tt = [torch.as_tensor(x, dtype=torch.float32, device="cuda") for x in true]
     st = [torch.as_tensor(x, dtype=torch.float32, device="cuda") for x in start]
     obs = plan.forward(*tt, srcv, K).clone()
@@ -293,7 +293,7 @@ def elastic_leg(ads, L, lib, steps, nstep=0, adjoint_kernels=None):
                         "two-component receivers, FP32 forward + hand-written adjoint for "
                         "(rho, lambda, mu), one batched call per gradient"),
            "value": round(cells / t / 1e9, 4), "unit": "Gcell-steps/s",
-           "ms_per_gradient": round(t * 1e3, 2), "checkpoint_interval": K,
+           "ms_per_gradient": round(t * 1e3, 2), "checkpoint_interval": K, "snapshots": S,
            "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": PEAK_HBM_GBPS,
                         "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBPS, 4)},
            "kernels": kern, "loss": float(loss.item())}

@@ -358,12 +358,20 @@ size_t ads_elastic_plan_workspace_size(const ads_elastic_plan *plan, int64_t K);
 /* Adjoint rectangle kernels: streaming (1, the default: one wavefront per shot walking rows,
  * x-direction stencils from registers, y-direction from neighbouring lanes) or LDS-tiled (0);
  * rows per streaming workgroup (1..32, default 8); shots per adjoint group (1..8, default 4;
- * the parameter terms are accumulated per group).  Changes the workspace layout: call before
+ * the parameter terms are accumulated per group); rows x shots <= 64.  The streaming kernels
+ * need 3 x shots x rows x 64 values of LDS; above 64 KiB (FP64 past 32 rows x shots) the
+ * LDS-tiled kernels run.  Changes the workspace layout: call before
  * ads_elastic_plan_workspace_size and the forward. */
 int ads_elastic_plan_set_adjoint_kernels(ads_elastic_plan *plan, int streaming, int rows,
                                          int shots);
 /* As ads_acoustic_plan_choose_interval, over the NSTEP elastic levels. */
 int64_t ads_elastic_plan_choose_interval(const ads_elastic_plan *plan, size_t budget_bytes);
+/* As the acoustic snapshot calls, for the elastic plan: the NSTEP levels are cut into
+ * M = ceil(NSTEP/K) segments; a snapshot slot holds a whole level (13 arrays) of every shot. */
+int ads_elastic_plan_set_snapshots(ads_elastic_plan *plan, int64_t snapshots);
+int64_t ads_elastic_plan_snapshots(const ads_elastic_plan *plan);
+int ads_elastic_plan_choose_schedule(ads_elastic_plan *plan, size_t budget_bytes, int64_t *K,
+                                     int64_t *S);
 /* Forward: traces of every shot from rho, lambda, mu (device, N each). */
 int ads_elastic_plan_forward(ads_elastic_plan *plan, const void *rho, const void *lam,
                              const void *mu, const void *srcv, void *traces, void *workspace,

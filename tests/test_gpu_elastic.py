@@ -244,12 +244,13 @@ def test_mpi_elastic_block_plan_and_dropin_f64(orc, ads, cuda):
         assert rel(t.grad.cpu(), b) < 1e-10, (name, rel(t.grad.cpu(), b))
 
 
-@pytest.mark.parametrize("kernels", [(1, 8, 4), (1, 16, 8), (1, 3, 2), (1, 32, 1), (0, 16, 4)])
+@pytest.mark.parametrize("kernels", [(1, 8, 4), (1, 4, 8), (1, 3, 2), (1, 32, 1), (0, 16, 4)])
 def test_elastic_adjoint_rectangle_kernels_f64(orc, ads, cuda, kernels):
-    """Every adjoint rectangle variant gives the oracle's gradient: the streaming kernels with
-    frame_b in the same launch (4 shots per group), with frame_b launched apart (8, 2, 1 shots
-    per group; 5 uneven shots, so the last group is partial and, near the end, fewer shots are
-    active than a group holds), chunk heights that do not divide the rectangle, and the
+    """Every adjoint rectangle variant gives the oracle's gradient, and a second gradient on the
+    same plan is bit-identical to the first (no order-dependent sums): the streaming kernels
+    with frame_b in the same launch (4 shots per group), with frame_b launched apart (8, 2, 1
+    shots per group; 5 uneven shots, so the last group is partial and, near the end, fewer shots
+    are active than a group holds), chunk heights that do not divide the rectangle, and the
     LDS-tiled kernels."""
     p, models, srcs, rcvs, nstep = make_case(orc, ads, NX=41, NY=37, NSTEP=50, nshots=5,
                                              uneven=True, seed=3)
@@ -266,3 +267,68 @@ def test_elastic_adjoint_rectangle_kernels_f64(orc, ads, cuda, kernels):
     L, g, _ = plan.gradient(*mt, srcv, plan.pack_traces(obs), 11)
     errs = [rel(a.cpu(), b) for a, b in zip(g, g_ref)]
     assert max(errs) < 1e-10, (kernels, errs)
+    L2, g2, _ = plan.gradient(*mt, srcv, plan.pack_traces(obs), 11)
+    assert L2.item() == L.item()
+    for a, b in zip(g2, g):
+        assert torch.equal(a, b), kernels
+
+
+def _min_rebuilds(n, s, memo={}):
+    """Least segment advances to reverse n segments with s free slots (exhaustive search)."""
+    key = (n, s)
+    if key not in memo:
+        if n == 1:
+            memo[key] = 1
+        elif s == 0:
+            memo[key] = n * (n + 1) // 2
+        else:
+            memo[key] = min(j + _min_rebuilds(n - j, s - 1) + _min_rebuilds(j, s) for j in range(1, n))
+    return memo[key]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_elastic_binomial_schedules_match_one_level(orc, ads, cuda, dtype):
+    """Every snapshot count gives the one-level schedule's misfit and gradients bit for bit,
+    the FP64 gradient matches the oracle, and the forward levels run equal the initial pass plus
+    K times the fewest segment rebuilds an exhaustive search finds."""
+    import ctypes as C
+    import importlib
+    L = importlib.import_module("adseismic_amd._lib")
+    p, models, srcs, rcvs, nstep = make_case(orc, ads, NSTEP=60, nshots=3, seed=4)
+    obs, g_ref = [], [0.0, 0.0, 0.0]
+    for s in range(3):
+        o, _ = oracle(orc, p, models, srcs[s], rcvs[s], nstep[s], scale=(1.02, 0.97, 1.04))
+        _, g, _ = oracle(orc, p, models, srcs[s], rcvs[s], nstep[s], obs=o)
+        obs.append(o)
+        g_ref = [a + b for a, b in zip(g_ref, g)]
+    plan = ads.ElasticPlan(p, srcs, rcvs, dtype=dtype, nstep=nstep)
+    lib = plan.lib
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    mt = [T(m, dtype) for m in models]
+    ob = plan.pack_traces(obs)
+    for K in (6, 5):                       # 10 and 12 segments of exactly K levels
+        M = p.NSTEP // K
+        plan.set_snapshots(-1)
+        L1, g1, _ = plan.gradient(*mt, srcv, ob, K)
+        g1 = [x.cpu().numpy() for x in g1]
+        bar = 1e-10 if dtype == torch.float64 else 1e-3
+        assert max(rel(a, b) for a, b in zip(g1, g_ref)) < bar
+        for S in (0, 1, 2, 4, M - 2, M - 1):
+            plan.set_snapshots(S)
+            L.check(lib.ads_profile_begin())
+            LS, gS, _ = plan.gradient(*mt, srcv, ob, K)
+            torch.cuda.synchronize()
+            L.check(lib.ads_profile_end())
+            tm, n, b = C.c_double(), C.c_int64(), C.c_double()
+            L.check(lib.ads_profile_query(2, C.byref(tm), C.byref(n), C.byref(b)))
+            assert LS.item() == L1.item(), (K, S)
+            for a, bb in zip(gS, g1):
+                np.testing.assert_array_equal(a.cpu().numpy(), bb, err_msg=f"K {K}, S {S}")
+            want = p.NSTEP + K * (_min_rebuilds(M, min(S, M - 1)) - M)
+            assert n.value == want, (K, S, n.value, want)
+    plan.set_snapshots(-1)
+    budget = min(plan.workspace_size(k) for k in range(1, p.NSTEP + 1)) - 1  # no one-level fits
+    K, S = plan.choose_schedule(budget)
+    assert plan.snapshots == S and S < -(-p.NSTEP // K) - 1 and plan.workspace_size(K) <= budget
+    LS, gS, _ = plan.gradient(*mt, srcv, ob, K)
+    assert max(rel(a.cpu(), b) for a, b in zip(gS, g_ref)) < (1e-10 if dtype == torch.float64 else 1e-3)
