@@ -473,11 +473,6 @@ def main():
     L = importlib.import_module("adseismic_amd._lib")
     lib = ads.load_library()
     torch.cuda.init()
-    if args.elastic_only:
-        ek = tuple(int(x) for x in args.elastic_adjoint.split(",")) if args.elastic_adjoint else None
-        print(json.dumps({"elastic": elastic_leg(ads, L, lib, args.steps, args.elastic_nstep, ek)}),
-              flush=True)
-        return
     if args.calib_copy:
         n = args.calib_copy * (1 << 20) // 4
         a = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -485,6 +480,11 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.elastic_only:
+        ek = tuple(int(x) for x in args.elastic_adjoint.split(",")) if args.elastic_adjoint else None
+        print(json.dumps({"elastic": elastic_leg(ads, L, lib, args.steps, args.elastic_nstep, ek)}),
+              flush=True)
+        return
     if args.workload == "uneven":
         p, c_true, c0, srcs, rcvs, nst = workload_uneven(ads)
         p.PropagatorKernel = args.propagator_kernel
