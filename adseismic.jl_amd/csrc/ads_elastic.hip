@@ -1085,6 +1085,17 @@ __device__ inline bool decode_strm(const StrmMap &m, int &g, int &chunk, int &st
     return true;
 }
 template <typename T> __device__ inline T lane_at(T v, int src) { return __shfl(v, src, WAVE); }
+// Loads and stores at a 32-bit byte offset from a wave-uniform base: the base stays in scalar
+// registers and every load of one row shares the row's offset register (one field of one
+// shot is below 4 GiB; the host checks it before choosing these kernels).
+template <typename T> __device__ inline T ld_at(const T *base, uint32_t off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <typename T> __device__ inline void st_at(T *base, uint32_t off, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
+}
 
 // shot-invariant inputs of the streaming kernels (workspace arrays, one Npad grid each)
 template <typename T> struct ElPar {
@@ -1109,7 +1120,7 @@ __device__ inline void adjv_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, con
     auto gslot = [&](int wv, int term, int r, int l) -> T & {
         return gacc[((wv * 2 + term) * mp.ch + r) * WAVE + l];
     };
-    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
     const int i0 = a.R.iL + chunk * mp.ch, i1 = min(i0 + mp.ch, a.R.iR);
     const int jb = a.R.jL + strip * SW_COLS - 2;
     const int j = jb + lane;
@@ -1130,34 +1141,33 @@ __device__ inline void adjv_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, con
         const T *__restrict__ sxx = a.Sn.f(SXX, s);
         const T *__restrict__ sxy = a.Sn.f(SXY, s);
         const T *__restrict__ syy = a.Sn.f(SYY, s);
-        const T *__restrict__ tb = a.Tb + (int64_t)s * a.npad;
+        const T *__restrict__ tb0 = a.Tb + (int64_t)s * a.npad;
+        const T *__restrict__ tb1 = tb0 + ts;
+        const T *__restrict__ tb2 = tb0 + 2 * ts;
+        const T *__restrict__ tb3 = tb0 + 3 * ts;
+        const uint32_t pb = (uint32_t)P * (uint32_t)sizeof(T), jb4 = (uint32_t)jc * (uint32_t)sizeof(T);
         // raw loads of one window row, and of one output row
         struct Rw { T a, b, x, y, r, q; };
         struct Ro { T yy, t1, t3, gxx, gxy, gyy; };
         auto fetch_w = [&](int r, Rw &o) {
-            const int64_t rj = (int64_t)r * P + jc;
-            o.x = sxx[rj];
-            o.y = sxy[rj];
-            if (r >= a.R.iL && r < a.R.iR) {  // uniform over the wavefront
-                o.a = vxa[rj];
-                o.b = vya[rj];
-                o.r = q.irho[rj];
-                o.q = q.irb[rj];
-            } else {
-                const bool inter = jint && r >= 1 && r <= c.NX;
-                o.a = inter ? tb[rj] : T(0);
-                o.b = inter ? tb[2 * ts + rj] : T(0);
-                o.r = o.q = T(0);
-            }
+            const uint32_t rj = (uint32_t)r * pb + jb4;
+            const bool inr = r >= a.R.iL && r < a.R.iR;  // uniform over the wavefront
+            const bool ld = inr || (jint && r >= 1 && r <= c.NX);
+            o.x = ld_at(sxx, rj);
+            o.y = ld_at(sxy, rj);
+            o.a = ld ? ld_at(inr ? vxa : tb0, rj) : T(0);
+            o.b = ld ? ld_at(inr ? vya : tb2, rj) : T(0);
+            o.r = inr ? ld_at(q.irho, rj) : T(0);
+            o.q = inr ? ld_at(q.irb, rj) : T(0);
         };
         auto fetch_o = [&](int i, Ro &o) {
-            const int64_t ij = (int64_t)i * P + jc;
-            o.yy = syy[ij];
-            o.t1 = !jin && jint ? tb[ts + ij] : T(0);
-            o.t3 = !jin && jint ? tb[3 * ts + ij] : T(0);
-            o.gxx = out ? axx[ij] : T(0);
-            o.gxy = out ? axy[ij] : T(0);
-            o.gyy = out ? ayy[ij] : T(0);
+            const uint32_t ij = (uint32_t)i * pb + jb4;
+            o.yy = ld_at(syy, ij);
+            o.t1 = !jin && jint ? ld_at(tb1, ij) : T(0);
+            o.t3 = !jin && jint ? ld_at(tb3, ij) : T(0);
+            o.gxx = out ? ld_at<T>(axx, ij) : T(0);
+            o.gxy = out ? ld_at<T>(axy, ij) : T(0);
+            o.gyy = out ? ld_at<T>(ayy, ij) : T(0);
         };
         // windows over rows i-2 .. i+2 (index 0 .. 4): t0, t2, sxx, sxy, 1/rho, 1/rho-bar;
         // t-values inside the rectangle from the cotangents, elsewhere the temporaries
@@ -1198,16 +1208,16 @@ __device__ inline void adjv_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, con
             const T xymm = lane_at(xy, lane - 2), xym = lane_at(xy, lane - 1), xyp = lane_at(xy, lane + 1);
             const T yym = lane_at(yy, lane - 1), yyp = lane_at(yy, lane + 1), yypp = lane_at(yy, lane + 2);
             if (out) {
-                const int64_t ij = (int64_t)i * P + jc;
+                const uint32_t ij = (uint32_t)i * pb + jb4;
                 // parameter terms (fw3: rho; fw4: rho-bar)
                 const T S3 = d4(wx[2], wx[1], wx[3], wx[0]) * c.ihx + d4(xy, xym, xyp, xymm) * c.ihy;
                 const T S4 = d4(wy[3], wy[2], wy[4], wy[1]) * c.ihx + d4(yyp, yy, yypp, yym) * c.ihy;
                 gslot(w, 0, i - i0, lane) = -w0[2] * S3 * wr[2];
                 gslot(w, 1, i - i0, lane) = -w2[2] * S4 * wb[2];
                 // sxx <- (D-x)^T t0 ; sxy <- (D-y)^T t1 + (D+x)^T t2 ; syy <- (D+y)^T t3
-                axx[ij] = oc.gxx + d4(w0[2], w0[3], w0[1], w0[4]) * c.ihx;
-                axy[ij] = oc.gxy + d4(t1, t1p, t1m, t1pp) * c.ihy + d4(w2[1], w2[2], w2[0], w2[3]) * c.ihx;
-                ayy[ij] = oc.gyy + d4(t3m, t3, t3mm, t3p) * c.ihy;
+                st_at(axx, ij, oc.gxx + d4(w0[2], w0[3], w0[1], w0[4]) * c.ihx);
+                st_at(axy, ij, oc.gxy + d4(t1, t1p, t1m, t1pp) * c.ihy + d4(w2[1], w2[2], w2[0], w2[3]) * c.ihx);
+                st_at(ayy, ij, oc.gyy + d4(t3m, t3, t3mm, t3p) * c.ihy);
             }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -1250,7 +1260,7 @@ __device__ inline void adjs_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, con
     auto gslot = [&](int wv, int term, int r, int l) -> T & {
         return gacc[((wv * 3 + term) * mp.ch + r) * WAVE + l];
     };
-    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
     const int i0 = a.R.iL + chunk * mp.ch, i1 = min(i0 + mp.ch, a.R.iR);
     const int jb = a.R.jL + strip * SW_COLS - 2;
     const int j = jb + lane;
@@ -1270,37 +1280,34 @@ __device__ inline void adjs_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, con
         const T *__restrict__ axy = a.A.f(SXY, s);
         const T *__restrict__ vx = a.Sm.f(VX, s);
         const T *__restrict__ vy = a.Sm.f(VY, s);
-        const T *__restrict__ tb = a.Tb + (int64_t)s * a.npad;
+        const T *__restrict__ tb0 = a.Tb + (int64_t)s * a.npad;
+        const T *__restrict__ tb1 = tb0 + ts;
+        const T *__restrict__ tb2 = tb0 + 2 * ts;
+        const T *__restrict__ tb3 = tb0 + 3 * ts;
+        const uint32_t pb = (uint32_t)P * (uint32_t)sizeof(T), jb4 = (uint32_t)jc * (uint32_t)sizeof(T);
         // raw loads of one window row (inside the rectangle's rows: the stress cotangents and the
         // parameters; outside: the temporaries t0..t3), and of one output row
         struct Rw { T u, v, e0, e1, e2, e3, e4, e5; };
         struct Ro { T t1, t3, gvx, gvy; };
         auto fetch_w = [&](int r, Rw &o) {
-            const int64_t rj = (int64_t)r * P + jc;
-            o.u = vx[rj];
-            o.v = vy[rj];
-            if (r >= a.R.iL && r < a.R.iR) {  // uniform over the wavefront
-                o.e0 = axx[rj];
-                o.e1 = ayy[rj];
-                o.e2 = axy[rj];
-                o.e3 = q.lb[rj];
-                o.e4 = q.mb[rj];
-                o.e5 = q.mb2[rj];
-            } else {
-                const bool inter = jint && r >= 1 && r <= c.NX;
-                o.e0 = inter ? tb[rj] : T(0);
-                o.e1 = inter ? tb[ts + rj] : T(0);
-                o.e2 = inter ? tb[2 * ts + rj] : T(0);
-                o.e3 = inter ? tb[3 * ts + rj] : T(0);
-                o.e4 = o.e5 = T(0);
-            }
+            const uint32_t rj = (uint32_t)r * pb + jb4;
+            const bool inr = r >= a.R.iL && r < a.R.iR;  // uniform over the wavefront
+            const bool ld = inr || (jint && r >= 1 && r <= c.NX);
+            o.u = ld_at(vx, rj);
+            o.v = ld_at(vy, rj);
+            o.e0 = ld ? ld_at(inr ? axx : tb0, rj) : T(0);
+            o.e1 = ld ? ld_at(inr ? ayy : tb1, rj) : T(0);
+            o.e2 = ld ? ld_at(inr ? axy : tb2, rj) : T(0);
+            o.e3 = ld ? ld_at(inr ? q.lb : tb3, rj) : T(0);
+            o.e4 = inr ? ld_at(q.mb, rj) : T(0);
+            o.e5 = inr ? ld_at(q.mb2, rj) : T(0);
         };
         auto fetch_o = [&](int i, Ro &o) {
-            const int64_t ij = (int64_t)i * P + jc;
-            o.t1 = !jin && jint ? tb[ts + ij] : T(0);
-            o.t3 = !jin && jint ? tb[3 * ts + ij] : T(0);
-            o.gvx = out ? avx[ij] : T(0);
-            o.gvy = out ? avy[ij] : T(0);
+            const uint32_t ij = (uint32_t)i * pb + jb4;
+            o.t1 = !jin && jint ? ld_at(tb1, ij) : T(0);
+            o.t3 = !jin && jint ? ld_at(tb3, ij) : T(0);
+            o.gvx = out ? ld_at<T>(avx, ij) : T(0);
+            o.gvy = out ? ld_at<T>(avy, ij) : T(0);
         };
         // windows over rows i-2 .. i+2: t0, t1, t2 (t3 = t2 inside the rectangle), the stress
         // cotangents sxx*dt, syy*dt, sxy of the row, and vx, vy of level n-1
@@ -1347,7 +1354,7 @@ __device__ inline void adjs_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, con
             const T um = lane_at(u, lane - 1), up = lane_at(u, lane + 1), upp = lane_at(u, lane + 2);
             const T vmm = lane_at(v, lane - 2), vm = lane_at(v, lane - 1), vp = lane_at(v, lane + 1);
             if (out) {
-                const int64_t ij = (int64_t)i * P + jc;
+                const uint32_t ij = (uint32_t)i * pb + jb4;
                 // parameter terms: fw1 (lambda-bar, mu-bar) and fw2 (mu-bar), level n-1 velocities
                 const T X = d4(wu[3], wu[2], wu[4], wu[1]) * c.ihx;
                 const T Y = d4(v, vm, vp, vmm) * c.ihy;
@@ -1358,8 +1365,8 @@ __device__ inline void adjs_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                 const T dy = d4(up, u, upp, um) * c.ihy;
                 gslot(w, 2, i - i0, lane) = gs[2] * (dx + dy) * dt;
                 // vx <- (D+x)^T t0 + (D+y)^T t3 ; vy <- (D-y)^T t1 + (D-x)^T t2
-                avx[ij] = oc.gvx + d4(w0[1], w0[2], w0[0], w0[3]) * c.ihx + d4(t3m, t3, t3mm, t3p) * c.ihy;
-                avy[ij] = oc.gvy + d4(t1, t1p, t1m, t1pp) * c.ihy + d4(w2[2], w2[3], w2[1], w2[4]) * c.ihx;
+                st_at(avx, ij, oc.gvx + d4(w0[1], w0[2], w0[0], w0[3]) * c.ihx + d4(t3m, t3, t3mm, t3p) * c.ihy);
+                st_at(avy, ij, oc.gvy + d4(t1, t1p, t1m, t1pp) * c.ihy + d4(w2[2], w2[3], w2[1], w2[4]) * c.ihx);
             }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -1702,9 +1709,10 @@ template <typename T> struct ElPlan : ElPlanBase {
         if (act <= 0) return ADS_OK;
         const ElCoef<T> c = coef(P);
         const double cells = (double)act * (double)PX * (double)PY;  // interior (or block) cells
-        // algorithmic bytes: stress kernel reads v (2) + sigma (3) + lam, mu (2), writes sigma (3),
-        // memory m1..m4 read+write in their strips; velocity kernel reads sigma (3) + v (2) + rho,
-        // writes v (2), m5..m8 in their strips
+        // algorithmic bytes, the whole-step count w (13 + 8 f_x + 8 f_y) with each phase charged
+        // for the fields it updates: stress phase sigma read + write (6), lam, mu (2) and m1..m4
+        // in their strips; velocity phase v read + write (4), rho (1) and m5..m8.  The reads of
+        // the other phase's fields are the stencil neighbours a fused step would reuse on chip.
         const Rect &R = rect[li];
         const bool rr = R.iR > R.iL && R.jR > R.jL;
         const int Sg = std::min(sg, act);
@@ -1718,7 +1726,7 @@ template <typename T> struct ElPlan : ElPlanBase {
         if (nitems[li] && !fuse)
             el_stress_frame<T><<<fg, WAVE * NWAVES, 0, st>>>(cst(I), O, lam, mu, c,
                                                               d_items[li].template as<int4>(), nitems[li]);
-        prof_stop(pr, st, cells * sizeof(T) * (10.0 + 4.0 * f_x + 4.0 * f_y));
+        prof_stop(pr, st, cells * sizeof(T) * (8.0 + 4.0 * f_x + 4.0 * f_y));
         pr = prof_start(ADS_PROF_EL_FWD_VEL, st);
         if (rr) {
             if (li == 0) launch_rect<VECP>(1, R, cst(I), O, rho, lam, mu, c, act, Sg, li, fuse, st);
@@ -1727,7 +1735,7 @@ template <typename T> struct ElPlan : ElPlanBase {
         if (nitems[li] && !fuse)
             el_vel_frame<T><<<fg, WAVE * NWAVES, 0, st>>>(cst(I), O, rho, c,
                                                            d_items[li].template as<int4>(), nitems[li]);
-        prof_stop(pr, st, cells * sizeof(T) * (8.0 + 4.0 * f_x + 4.0 * f_y));
+        prof_stop(pr, st, cells * sizeof(T) * (5.0 + 4.0 * f_x + 4.0 * f_y));
         el_src_rcv<T><<<act, 256, 0, st>>>(O, n, d_us_ptr.as<int64_t>(), d_us_key[li].template as<int64_t>(),
                                            d_us_eptr.as<int64_t>(), d_us_ent.as<int64_t>(), srcv,
                                            d_srcv_off.as<int64_t>(), d_nsrc.as<int64_t>(),
@@ -1843,15 +1851,18 @@ template <typename T> struct ElPlan : ElPlanBase {
         }
         const unsigned rg = cdiv(m.nb, NXCD) * NXCD;
         const double cells = (double)act * (double)PX * (double)PY;  // interior (or block) cells
-        // algorithmic bytes per cell: velocity phase reads the velocity cotangents (2) and the
-        // stresses (3), reads and writes the stress cotangents (6), reads rho; stress phase reads
-        // the stress cotangents (3) and velocities (2), reads and writes the velocity cotangents
-        // (4), reads lambda and mu; CPML memory values and cotangents in their strips
+        // algorithmic bytes, the whole-step count w (24 + 8 f_x + 8 f_y) with each phase charged
+        // for what it updates: velocity phase the stress cotangents read + write (6), the
+        // stresses (3), rho (1) and its gradient read + write (2); stress phase the velocity
+        // cotangents read + write (4), the velocities (2), lambda, mu (2) and their gradients
+        // read + write (4); each phase 4 CPML memory cotangents in their strips
         const ElPar<T> par{w.par, w.par + Npad, w.par + 2 * Npad, w.par + 3 * Npad, w.par + 4 * Npad};
         // streaming kernels: their LDS (3 terms x shots x rows x WAVE values) must fit in 64 KiB,
         // else the LDS-tiled kernels run
         const size_t strm_lds = (size_t)3 * a.Sg * std::max(1, std::min(strm_ch, SW_CH_MAX)) * WAVE * sizeof(T);
-        if (rr && strm && li == 0 && a.Sg <= SW_NW && strm_lds <= 65536) {
+        // (the streaming kernels address one field of one shot with 32-bit byte offsets)
+        const bool off32 = (uint64_t)Npad * sizeof(T) < (uint64_t(1) << 32);
+        if (rr && strm && li == 0 && a.Sg <= SW_NW && strm_lds <= 65536 && off32) {
             StrmMap sm;
             sm.ch = std::max(1, std::min(strm_ch, SW_CH_MAX));
             sm.nstrip = (int)cdiv(a.R.jR - a.R.jL, SW_COLS);
@@ -1873,7 +1884,7 @@ template <typename T> struct ElPlan : ElPlanBase {
             const size_t lds = (size_t)a.Sg * sm.ch * WAVE * sizeof(T);  // x terms
             el_adjv_strm<T><<<(unsigned)nfb + sgrid, sblock, 2 * lds, st>>>(a, c, par, sm, items, nitems[li],
                                                                               std::max(nib, 1), nfb);
-            prof_stop(pr, st, cells * sizeof(T) * (12.0 + 12.0 * (f_x + f_y)));
+            prof_stop(pr, st, cells * sizeof(T) * (12.0 + 4.0 * (f_x + f_y)));
             pr = prof_start(ADS_PROF_EL_ADJ_STRESS, st);
             const unsigned ga = (unsigned)(nfa + nrb * ng);
             if (ga) el_adjs_a<T><<<ga, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li], std::max(nib, 1), ring,
@@ -1881,7 +1892,7 @@ template <typename T> struct ElPlan : ElPlanBase {
             if (nitems[li] && !fuse) el_adjs_frame_b<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
             el_adjs_strm<T><<<(unsigned)nfb + sgrid, sblock, 3 * lds, st>>>(a, c, par, sm, items, nitems[li],
                                                                               std::max(nib, 1), nfb);
-            prof_stop(pr, st, cells * sizeof(T) * (11.0 + 12.0 * (f_x + f_y)));
+            prof_stop(pr, st, cells * sizeof(T) * (12.0 + 4.0 * (f_x + f_y)));
             ADS_LAUNCH_CHECK();
             return ADS_OK;
         }
@@ -1899,7 +1910,7 @@ template <typename T> struct ElPlan : ElPlanBase {
             el_adjv_rb<T><<<(unsigned)nfb + rg, WAVE * NWAVES, 0, st>>>(a, c, m, items, nitems[li], nib, nfb);
         else if (rr) el_adjv_rect<T><<<rg, WAVE * NWAVES, 0, st>>>(a, c, m);
         else if (nitems[li]) el_adjv_frame_b<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
-        prof_stop(pr, st, cells * sizeof(T) * (12.0 + 12.0 * (f_x + f_y)));
+        prof_stop(pr, st, cells * sizeof(T) * (12.0 + 4.0 * (f_x + f_y)));
         pr = prof_start(ADS_PROF_EL_ADJ_STRESS, st);
         if (ga) el_adjs_a<T><<<ga, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li], std::max(nib, 1), ring,
                                                            nring[li], std::max(nrb, 1), nfa);
@@ -1907,7 +1918,7 @@ template <typename T> struct ElPlan : ElPlanBase {
             el_adjs_rb<T><<<(unsigned)nfb + rg, WAVE * NWAVES, 0, st>>>(a, c, m, items, nitems[li], nib, nfb);
         else if (rr) el_adjs_rect<T><<<rg, WAVE * NWAVES, 0, st>>>(a, c, m);
         else if (nitems[li]) el_adjs_frame_b<T><<<fg, WAVE * NWAVES, 0, st>>>(a, c, items, nitems[li]);
-        prof_stop(pr, st, cells * sizeof(T) * (11.0 + 12.0 * (f_x + f_y)));
+        prof_stop(pr, st, cells * sizeof(T) * (12.0 + 4.0 * (f_x + f_y)));
         ADS_LAUNCH_CHECK();
         return ADS_OK;
     }

@@ -297,6 +297,16 @@ def elastic_leg(ads, L, lib, steps, nstep=0, adjoint_kernels=None):
            "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": PEAK_HBM_GBPS,
                         "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBPS, 4)},
            "kernels": kern, "loss": float(loss.item())}
+    # whole gradient: every cell-step once forward, w (13 + 8 f_x + 8 f_y), and once adjoint,
+    # w (24 + 8 f_x + 8 f_y) (f_x, f_y: interior fractions in the x / y CPML strips); rebuilt
+    # levels are overhead.  Over the gradient's wall time.
+    ax, _, _, ay, _, _ = ads.compute_PML_Params_elastic(p)
+    f_x = float(np.mean((ax[0] != 0) | (ax[1] != 0)))
+    f_y = float(np.mean((ay[0] != 0) | (ay[1] != 0)))
+    bpc = 4.0 * (37.0 + 16.0 * (f_x + f_y))
+    out["whole_gradient"] = {"alg_bytes_per_cell_step": round(bpc, 2),
+                             "alg_GBps": round(cells * bpc / t / 1e9, 1),
+                             "frac": round(cells * bpc / t / 1e9 / PEAK_HBM_GBPS, 4)}
     plan._ws = None
     torch.cuda.empty_cache()
     return out

@@ -407,7 +407,9 @@ int ads_elastic_forward_history(ads_elastic_plan *plan, const void *rho, const v
  * the summed duration, the launch count and the algorithmic HBM bytes of those launches
  * (compulsory per-cell traffic: acoustic forward w(4 + 4 f_pml), adjoint w(7 + 4 f_pml) for
  * PropagatorKernel 1; w(4 + 2 f_pml) and w(7 + 2 f_pml) for kernel 0 whose phi/psi updates
- * run in a separate pass; elastic as documented per class).  A launch that advances two time
+ * run in a separate pass; elastic forward w(8 + 4 f_x + 4 f_y) stress and w(5 + 4 f_x + 4 f_y)
+ * velocity phase, adjoint w(12 + 4 f_x + 4 f_y) per phase: the whole-step counts
+ * w(13 + 8 f_x + 8 f_y) and w(24 + 8 f_x + 8 f_y) split by the fields each phase updates).  A launch that advances two time
  * levels (the acoustic forward's two-level pass) counts as two launches.  Not thread-safe across
  * concurrent callers; meant for one benchmarking thread.
  * ==================================================================================== */
