@@ -314,7 +314,7 @@ __device__ inline void fwd_rect2s_body(const FwdArgs<T> &a, const Lv<T> &u2, con
 {
     int g, strip, tj;
     if (!decode_tile(a.map, g, strip, tj, bid)) return;
-    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1), wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x / WAVE);
     const int s = g * NWAVES + wv;
     if (s >= a.act) return;  // wave-uniform; no barrier below
     const int ja = R.jL + tj * WAVE * VEC;  // first column of the segment
@@ -333,14 +333,16 @@ __device__ inline void fwd_rect2s_body(const FwdArgs<T> &a, const Lv<T> &u2, con
     const T *__restrict__ wo = a.wold.at(s);
     T *__restrict__ u1 = a.u.at(s);
     T *__restrict__ up = u2.at(s);
+    // byte offsets: row x starts at x * pb, this lane's vector at jo
+    const uint32_t pb = (uint32_t)P * (uint32_t)sizeof(T), jo = (uint32_t)j0 * (uint32_t)sizeof(T);
     // u^{n-1} row x (rows R.iL-1 .. R.iR lie inside [0, NX+1]) with columns ja-1 and jb
     auto loadW = [&](int x, Rect2Row<T, VEC> &o) {
         const bool ok = x >= R.iL - 1 && x <= R.iR;
-        const int64_t q = (int64_t)x * P;
-        if (act && ok) vld(o.v, w + q + j0);
+        const uint32_t q = (uint32_t)x * pb;
+        if (act && ok) vld(o.v, at_off(w, q + jo));
         else vzero(o.v);
-        o.l = le && ok ? w[q + ja - 1] : T(0);
-        o.r = re && ok ? w[q + jb] : T(0);
+        o.l = le && ok ? at_off(w, q)[ja - 1] : T(0);
+        o.r = re && ok ? at_off(w, q)[jb] : T(0);
     };
     // inputs of level n on row x besides u^{n-1}: u^{n-2}, C, and for the halo columns
     // u^{n-1} at ja-2 / jb+1, u^{n-2} and C at ja-1 / jb
@@ -350,20 +352,20 @@ __device__ inline void fwd_rect2s_body(const FwdArgs<T> &a, const Lv<T> &u2, con
     };
     auto loadIn = [&](int x, In &o) {
         const bool ok = x >= R.iL && x < R.iR;
-        const int64_t q = (int64_t)x * P;
+        const uint32_t q = (uint32_t)x * pb;
         if (act && ok) {
-            vld(o.wo, wo + q + j0);
-            vld(o.c, a.C + q + j0);
+            vld(o.wo, at_off(wo, q + jo));
+            vld(o.c, at_off(a.C, q + jo));
         } else {
             vzero(o.wo);
             vzero(o.c);
         }
-        o.cl = hl && ok ? a.C[q + ja - 1] : T(0);
-        o.cr = hr && ok ? a.C[q + jb] : T(0);
-        o.wl2 = hl && ok ? w[q + ja - 2] : T(0);
-        o.wr2 = hr && ok ? w[q + jb + 1] : T(0);
-        o.wol = hl && ok ? wo[q + ja - 1] : T(0);
-        o.wor = hr && ok ? wo[q + jb] : T(0);
+        o.cl = hl && ok ? at_off(a.C, q)[ja - 1] : T(0);
+        o.cr = hr && ok ? at_off(a.C, q)[jb] : T(0);
+        o.wl2 = hl && ok ? at_off(w, q)[ja - 2] : T(0);
+        o.wr2 = hr && ok ? at_off(w, q)[jb + 1] : T(0);
+        o.wol = hl && ok ? at_off(wo, q)[ja - 1] : T(0);
+        o.wor = hr && ok ? at_off(wo, q)[jb] : T(0);
     };
     // level n on row x from u^{n-1} rows x-1, x, x+1
     auto level_n = [&](const Rect2Row<T, VEC> &Wn, const Rect2Row<T, VEC> &Wc,
@@ -412,7 +414,7 @@ __device__ inline void fwd_rect2s_body(const FwdArgs<T> &a, const Lv<T> &u2, con
         loadIn(x + 1, I2);
         Rect2Row<T, VEC> U2;  // level n on row x
         level_n(W0, W1, W2, I1, U2);
-        if (x >= ib && x < ie && act) vst(u1 + (int64_t)x * P + j0, U2.v);
+        if (x >= ib && x < ie && act) vst(at_off(u1, (uint32_t)x * pb + jo), U2.v);
         const int y = x - 1;  // level n+1 on row y from level n rows y-1, y, y+1
         if (y >= ib && y > R.iL && y < R.iR - 1) {
             T ul, ur;
@@ -427,13 +429,13 @@ __device__ inline void fwd_rect2s_body(const FwdArgs<T> &a, const Lv<T> &u2, con
                 out[e] = T(2) * uc - W0.v[e] + C1[e] * lap;
             }
             if (act) {
-                const int64_t o = (int64_t)y * P + j0;
+                T *__restrict__ po = at_off(up, (uint32_t)y * pb + jo);
                 if (j0 > R.jL && j0 + VEC < R.jR) {
-                    vst(up + o, out);
+                    vst(po, out);
                 } else {
 #pragma unroll
                     for (int e = 0; e < VEC; e++)
-                        if (j0 + e > R.jL && j0 + e < R.jR - 1) up[o + e] = out[e];
+                        if (j0 + e > R.jL && j0 + e < R.jR - 1) po[e] = out[e];
                 }
             }
         }
@@ -989,9 +991,11 @@ __device__ inline void adj_rect_body(const AdjArgs<T> &a, const Rect &R, int bid
     const bool dense = DENSE && a.dense.p != nullptr;
     const AcCoef<T> k = a.k;
     T Cr[RPT + 2][VEC], cl[RPT], cr[RPT];
+    // byte offsets: row x starts at x * pb, this lane's vector at jo
+    const uint32_t pb = (uint32_t)P * (uint32_t)sizeof(T), jo = (uint32_t)j0 * (uint32_t)sizeof(T);
 #pragma unroll
     for (int q = 0; q < RPT + 2; q++) {
-        if (act && q <= nr + 1) vld(Cr[q], a.C + (int64_t)(ib - 1 + q) * P + j0);
+        if (act && q <= nr + 1) vld(Cr[q], at_off(a.C, (uint32_t)(ib - 1 + q) * pb + jo));
         else vzero(Cr[q]);
     }
 #pragma unroll
@@ -1016,10 +1020,10 @@ __device__ inline void adj_rect_body(const AdjArgs<T> &a, const Rect &R, int bid
             const T *__restrict__ p2 = a.Pn.at(s + b);
 #pragma unroll
             for (int q = 0; q < RPT + 2; q++) {
-                const int64_t row = (int64_t)(ib - 1 + q) * P;
+                const uint32_t row = (uint32_t)(ib - 1 + q) * pb + jo;
                 if (act && q <= nr + 1) {
-                    vld(P1[b][q], p1 + row + j0);
-                    vld(U[b][q], um + row + j0);
+                    vld(P1[b][q], at_off(p1, row));
+                    vld(U[b][q], at_off(um, row));
                 } else {
                     vzero(P1[b][q]);
                     vzero(U[b][q]);
@@ -1027,18 +1031,18 @@ __device__ inline void adj_rect_body(const AdjArgs<T> &a, const Rect &R, int bid
             }
 #pragma unroll
             for (int r = 0; r < RPT; r++) {
-                const int64_t row = (int64_t)(ib + r) * P;
+                const uint32_t ro = (uint32_t)(ib + r) * pb + jo;
                 const bool ok = act && r < nr;
-                if (ok) vld(P2[b][r], p2 + row + j0);
+                if (ok) vld(P2[b][r], at_off(p2, ro));
                 else vzero(P2[b][r]);
                 if (dense) {
-                    if (ok) vld(DN[b][r], a.dense.at(s + b) + row + j0);
+                    if (ok) vld(DN[b][r], at_off(a.dense.at(s + b), ro));
                     else vzero(DN[b][r]);
                 }
-                PL[b][r] = le && r < nr ? p1[row + j0 - 1] : T(0);
-                PR[b][r] = re && r < nr ? p1[row + j0 + VEC] : T(0);
-                UL[b][r] = le && r < nr ? um[row + j0 - 1] : T(0);
-                UR[b][r] = re && r < nr ? um[row + j0 + VEC] : T(0);
+                PL[b][r] = le && r < nr ? at_off(p1, ro)[-1] : T(0);
+                PR[b][r] = re && r < nr ? at_off(p1, ro)[VEC] : T(0);
+                UL[b][r] = le && r < nr ? at_off(um, ro)[-1] : T(0);
+                UR[b][r] = re && r < nr ? at_off(um, ro)[VEC] : T(0);
             }
         }
 #pragma unroll
@@ -1070,7 +1074,7 @@ __device__ inline void adj_rect_body(const AdjArgs<T> &a, const Rect &R, int bid
                     const T ue = e == VEC - 1 ? ur : U[b][r + 1][e + 1];
                     gacc[r][e] += Pv * (k.cx * (U[b][r + 2][e] + U[b][r][e] - T(2) * uc) + k.cy * (ue + uw - T(2) * uc));
                 }
-                if (act) vst(Pout + (int64_t)(ib + r) * P + j0, out);
+                if (act) vst(at_off(Pout, (uint32_t)(ib + r) * pb + jo), out);
             }
         }
     }
@@ -1079,7 +1083,7 @@ __device__ inline void adj_rect_body(const AdjArgs<T> &a, const Rect &R, int bid
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
         if (r >= nr) break;
-        T *p = gC + (int64_t)(ib + r) * P + j0;
+        T *p = at_off(gC, (uint32_t)(ib + r) * pb + jo);
         T cur[VEC];
         vld(cur, p);
 #pragma unroll
@@ -2399,6 +2403,9 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
     P->NY = p->NY;
     P->Pitch = ((p->NY + 2 + WAVE - 1) / WAVE) * WAVE;
     P->Npad = (p->NX + 2) * P->Pitch;
+    // the stencil kernels address one field of one shot with 32-bit byte offsets
+    if ((uint64_t)P->Npad * sizeof(T) >= (uint64_t(1) << 32))
+        ADS_FAIL(ADS_ERR_ARG, "grid too large: one padded field of one shot must be below 4 GiB");
     P->Ndense = (p->NX + 2) * (p->NY + 2);
     P->nshots = g->nshots;
     P->k = make_coef<T>(p->DELTAT, p->DELTAX, p->DELTAY);

@@ -67,6 +67,17 @@ template <typename T, int VEC> __device__ inline void vzero(T (&a)[VEC])
 #pragma unroll
     for (int e = 0; e < VEC; e++) a[e] = T(0);
 }
+// A wave-uniform base plus a 32-bit byte offset: the base stays in scalar registers and one
+// offset register serves every array indexed by the same cell (one field of one shot is below
+// 4 GiB; the plans check it).
+template <typename T> __device__ inline T *at_off(T *p, uint32_t off)
+{
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(p) + off);
+}
+template <typename T> __device__ inline const T *at_off(const T *p, uint32_t off)
+{
+    return reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + off);
+}
 
 // The grid is split once per plan into a rectangle of interior cells none of which needs the
 // PML formula (rows [iL, iR) x columns [jL, jR); jL, jR multiples of the vector width) and the
