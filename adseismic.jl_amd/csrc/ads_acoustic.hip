@@ -114,7 +114,7 @@ __device__ inline void fwd_rect_body(const FwdArgs<T> &a, const Rect &R, int bid
 {
     int g, strip, tj;
     if (!decode_tile(a.map, g, strip, tj, bid)) return;
-    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1), wv = wave_id();
     const int j0 = R.jL + (tj * WAVE + lane) * VEC;
     const int ib = R.iL + (strip * NWAVES + wv) * RPT;
     if (ib >= R.iR) return;  // wave-uniform
@@ -190,7 +190,7 @@ __device__ inline void fwd_rect2_body(const FwdArgs<T> &a, const Lv<T> &u2, cons
 {
     int g, strip, tj;
     if (!decode_tile(a.map, g, strip, tj, bid)) return;
-    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1), wv = wave_id();
     const int ja = R.jL + tj * WAVE * VEC;  // first column of the segment
     const int j0 = ja + lane * VEC;
     const int ib = R.iL + (strip * NWAVES + wv) * RPT;
@@ -314,7 +314,7 @@ __device__ inline void fwd_rect2s_body(const FwdArgs<T> &a, const Lv<T> &u2, con
 {
     int g, strip, tj;
     if (!decode_tile(a.map, g, strip, tj, bid)) return;
-    const int lane = threadIdx.x & (WAVE - 1), wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x / WAVE);
+    const int lane = threadIdx.x & (WAVE - 1), wv = wave_id();
     const int s = g * NWAVES + wv;
     if (s >= a.act) return;  // wave-uniform; no barrier below
     const int ja = R.jL + tj * WAVE * VEC;  // first column of the segment
@@ -461,7 +461,7 @@ __device__ inline void fwd_frame_body(const FwdArgs<T> &a, const TileMap &map,
 {
     int g, blk, tj;
     if (!decode_tile(map, g, blk, tj, bid)) return;
-    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1), wv = wave_id();
     const int it = blk;  // one item per workgroup; its 4 wavefronts take every 4th shot
     if (it >= nitems) return;  // workgroup-uniform
     const int4 m = items[it];
@@ -562,7 +562,7 @@ __device__ inline void fwd_frame2_body(const FwdArgs<T> &a, const Lv<T> &u2, con
     constexpr int NB = FRPT + 2;  // level-n rows: the item's and one on each side
     int g, blk, tj;
     if (!decode_tile(map, g, blk, tj, bid)) return;
-    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1), wv = wave_id();
     const int it = blk;
     if (it >= nitems) return;  // workgroup-uniform
     const int4 m = items[it];
@@ -700,7 +700,7 @@ __device__ inline void fwd_frame0h_body(const FwdArgs<T> &a, const TileMap &map,
     constexpr int NB = FRPT + 2;
     int g, blk, tj;
     if (!decode_tile(map, g, blk, tj, bid)) return;
-    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1), wv = wave_id();
     const int it = blk;
     if (it >= nitems) return;  // workgroup-uniform
     const int4 m = items[it];
@@ -980,7 +980,7 @@ __device__ inline void adj_rect_body(const AdjArgs<T> &a, const Rect &R, int bid
     constexpr bool DENSE = VEC == 1;  // the dense cotangent only exists in the history layout
     int g, strip, tj;
     if (!decode_tile(a.map, g, strip, tj, bid)) return;
-    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1), wv = wave_id();
     const int j0 = R.jL + (tj * WAVE + lane) * VEC;
     const int ib = R.iL + (strip * NWAVES + wv) * RPT;
     if (ib >= R.iR) return;  // wave-uniform
@@ -1101,7 +1101,7 @@ __device__ inline void adj_frame_body(const AdjArgs<T> &a, const TileMap &map,
 {
     int g, blk, tj;
     if (!decode_tile(map, g, blk, tj, bid)) return;
-    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1), wv = wave_id();
     const int it = blk;  // one item per workgroup; its 4 wavefronts take every 4th shot
     if (it >= nitems) return;  // workgroup-uniform
     const int4 m = items[it];
@@ -1350,7 +1350,7 @@ __global__ __launch_bounds__(256) void ac_adj_inject(
 {
     const int g = blockIdx.y;
     const int lane = threadIdx.x & (WAVE - 1);
-    const int64_t c = gi_ptr[g] + (int64_t)blockIdx.x * NWAVES + threadIdx.x / WAVE;
+    const int64_t c = gi_ptr[g] + (int64_t)blockIdx.x * NWAVES + wave_id();
     if (c >= gi_ptr[g + 1]) return;  // wave-uniform
     const int64_t cell = gi_cell[c];
     const int i = (int)(cell / P), j = (int)(cell % P);

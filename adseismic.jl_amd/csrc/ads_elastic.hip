@@ -169,7 +169,7 @@ __global__ __launch_bounds__(WAVE *NWAVES) void el_stress_frame(ElC<T> I, El<T> 
                                                                 const T *__restrict__ mu, ElCoef<T> c,
                                                                 const int4 *__restrict__ items, int nitems)
 {
-    const int it = blockIdx.x * NWAVES + threadIdx.x / WAVE;
+    const int it = blockIdx.x * NWAVES + wave_id();
     if (it >= nitems) return;
     const int4 m = items[it];
     const int j = m.z + (threadIdx.x & (WAVE - 1));
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(WAVE *NWAVES) void el_vel_frame(ElC<T> I, El<T> O, 
                                                              ElCoef<T> c, const int4 *__restrict__ items,
                                                              int nitems)
 {
-    const int it = blockIdx.x * NWAVES + threadIdx.x / WAVE;
+    const int it = blockIdx.x * NWAVES + wave_id();
     if (it >= nitems) return;
     const int4 m = items[it];
     const int j = m.z + (threadIdx.x & (WAVE - 1));
@@ -233,7 +233,7 @@ __device__ inline void el_stress_rect_body(const ElC<T> &I, const El<T> &O, cons
 {
     int g, strip, tj;
     if (!decode_tile(map, g, strip, tj, bid)) return;
-    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1), wv = wave_id();
     const int jb = R.jL + tj * WAVE * VEC, j0 = jb + lane * VEC;
     const int ib = R.iL + (strip * NWAVES + wv) * RPT;
     if (ib >= R.iR) return;  // wave-uniform
@@ -329,7 +329,7 @@ __device__ inline void el_vel_rect_body(const ElC<T> &I, const El<T> &O, const T
 {
     int g, strip, tj;
     if (!decode_tile(map, g, strip, tj, bid)) return;
-    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1), wv = wave_id();
     const int jb = R.jL + tj * WAVE * VEC, j0 = jb + lane * VEC;
     const int ib = R.iL + (strip * NWAVES + wv) * RPT;
     if (ib >= R.iR) return;  // wave-uniform
@@ -435,7 +435,7 @@ __global__ __launch_bounds__(WAVE *NWAVES) void el_stress_level(ElC<T> I, El<T> 
         el_stress_rect_body<T, RPT, VEC>(I, O, lam, mu, c, R, map, act, Sg, b - nfb);
         return;
     }
-    const int it = (b % nib) * NWAVES + threadIdx.x / WAVE, s = b / nib;
+    const int it = (b % nib) * NWAVES + wave_id(), s = b / nib;
     if (it >= nitems || s >= act) return;
     const int4 m = items[it];
     const int j = m.z + (threadIdx.x & (WAVE - 1));
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(WAVE *NWAVES) void el_vel_level(ElC<T> I, El<T> O, 
         el_vel_rect_body<T, RPT, VEC>(I, O, rho, c, R, map, act, Sg, b - nfb);
         return;
     }
-    const int it = (b % nib) * NWAVES + threadIdx.x / WAVE, s = b / nib;
+    const int it = (b % nib) * NWAVES + wave_id(), s = b / nib;
     if (it >= nitems || s >= act) return;
     const int4 m = items[it];
     const int j = m.z + (threadIdx.x & (WAVE - 1));
@@ -714,7 +714,7 @@ template <typename T>
 __device__ inline void adjv_frame_a_body(const ElAdj<T> &a, const ElCoef<T> &c,
                                          const int4 *__restrict__ items, int nitems, int bx, int g)
 {
-    const int it = bx * NWAVES + threadIdx.x / WAVE;
+    const int it = bx * NWAVES + wave_id();
     if (it >= nitems) return;
     const int4 m = items[it];
     const int j = m.z + (threadIdx.x & (WAVE - 1));
@@ -737,7 +737,7 @@ template <typename T>
 __device__ inline void adjs_ring_body(const ElAdj<T> &a, const ElCoef<T> &c,
                                       const int4 *__restrict__ items, int nitems, int bx, int g)
 {
-    const int it = bx * NWAVES + threadIdx.x / WAVE;
+    const int it = bx * NWAVES + wave_id();
     if (it >= nitems) return;
     const int4 m = items[it];
     const int j = m.z + (threadIdx.x & (WAVE - 1));
@@ -780,7 +780,7 @@ template <typename T>
 __device__ inline void adjv_frame_b_body(const ElAdj<T> &a, const ElCoef<T> &c,
                                          const int4 *__restrict__ items, int nitems, int bx, int g)
 {
-    const int it = bx * NWAVES + threadIdx.x / WAVE;
+    const int it = bx * NWAVES + wave_id();
     if (it >= nitems) return;
     const int4 m = items[it];
     const int j = m.z + (threadIdx.x & (WAVE - 1));
@@ -799,7 +799,7 @@ template <typename T>
 __device__ inline void adjs_frame_a_body(const ElAdj<T> &a, const ElCoef<T> &c,
                                          const int4 *__restrict__ items, int nitems, int bx, int g)
 {
-    const int it = bx * NWAVES + threadIdx.x / WAVE;
+    const int it = bx * NWAVES + wave_id();
     if (it >= nitems) return;
     const int4 m = items[it];
     const int j = m.z + (threadIdx.x & (WAVE - 1));
@@ -819,7 +819,7 @@ template <typename T>
 __device__ inline void adjs_frame_b_body(const ElAdj<T> &a, const ElCoef<T> &c,
                                          const int4 *__restrict__ items, int nitems, int bx, int g)
 {
-    const int it = bx * NWAVES + threadIdx.x / WAVE;
+    const int it = bx * NWAVES + wave_id();
     if (it >= nitems) return;
     const int4 m = items[it];
     const int j = m.z + (threadIdx.x & (WAVE - 1));
@@ -1120,7 +1120,7 @@ __device__ inline void adjv_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, con
     auto gslot = [&](int wv, int term, int r, int l) -> T & {
         return gacc[((wv * 2 + term) * mp.ch + r) * WAVE + l];
     };
-    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
+    const int w = wave_id(), lane = threadIdx.x % WAVE;
     const int i0 = a.R.iL + chunk * mp.ch, i1 = min(i0 + mp.ch, a.R.iR);
     const int jb = a.R.jL + strip * SW_COLS - 2;
     const int j = jb + lane;
@@ -1260,7 +1260,7 @@ __device__ inline void adjs_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, con
     auto gslot = [&](int wv, int term, int r, int l) -> T & {
         return gacc[((wv * 3 + term) * mp.ch + r) * WAVE + l];
     };
-    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
+    const int w = wave_id(), lane = threadIdx.x % WAVE;
     const int i0 = a.R.iL + chunk * mp.ch, i1 = min(i0 + mp.ch, a.R.iR);
     const int jb = a.R.jL + strip * SW_COLS - 2;
     const int j = jb + lane;

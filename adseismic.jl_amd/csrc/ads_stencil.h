@@ -67,6 +67,10 @@ template <typename T, int VEC> __device__ inline void vzero(T (&a)[VEC])
 #pragma unroll
     for (int e = 0; e < VEC; e++) a[e] = T(0);
 }
+// Index of the calling wavefront in its workgroup, as a wave-uniform (scalar) value, so that
+// what is derived from it (shot, item, row loop bounds, base pointers) stays in scalar registers.
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane((int)threadIdx.x / WAVE); }
+
 // A wave-uniform base plus a 32-bit byte offset: the base stays in scalar registers and one
 // offset register serves every array indexed by the same cell (one field of one shot is below
 // 4 GiB; the plans check it).
