@@ -546,20 +546,34 @@ __device__ inline void fwd_frame_body(const FwdArgs<T> &a, const TileMap &map,
     }
 }
 
-// Two-level frame work (kernel 1, sources from the per-cell table).  An item is nr <= FRPT rows
-// of the complement of the shrunk rectangle by at most WAVE - 2 columns [m.z, m.w).  Lane L
-// holds column m.z - 1 + L: every lane computes level n at its column on the item's rows and
-// one row above and below (from levels n-1, n-2 and phi/psi^{n-1}); lanes 1 .. m.w - m.z then
-// compute level n+1 on the item's rows from those values, the first and last lane supplying
-// the halo columns.  Level n is stored only outside R (the rectangle pass stores it inside).
-// Every value is fwd_frame_body's expression in the same order, so the results equal
-// single-level stepping bit for bit.
-template <typename T>
-__device__ inline void fwd_frame2_body(const FwdArgs<T> &a, const Lv<T> &u2, const Lv<T> &phi2,
-                                       const Lv<T> &psi2, const Rect &R, const TileMap &map,
-                                       const int4 *__restrict__ items, int nitems, int bid)
+// Rows per item of the two-level frame work: 4, 8, .. 64 (FRAME2_H item lists).  Taller items
+// read fewer halo rows, but each wavefront walks its item once per shot of its quarter of the
+// group, so with few active shots tall items make serial chains longer than the rectangle's
+// work.  The launch takes the largest height not above the number of active shots.
+constexpr int FRAME2_H = 5;
+inline int frame2_height(int k) { return 4 << k; }
+inline int frame2_pick(int act)
 {
-    constexpr int NB = FRPT + 2;  // level-n rows: the item's and one on each side
+    int k = 0;
+    while (k + 1 < FRAME2_H && frame2_height(k + 1) <= act) k++;
+    return k;
+}
+
+// Two-level frame work (kernel 1, sources from the per-cell table), streaming form.  An item is
+// nr rows of at most WAVE - 2 columns [m.z, m.w) of the complement of the rectangle shrunk by
+// one cell; lane L holds column m.z - 1 + L, the first and last lanes supplying the halo
+// columns.  For each shot the wavefront walks the level-n rows x = ib-1 .. ib+nr once, holding
+// u^{n-1} and phi^{n-1} of rows x-1 .. x+1 and level n of rows x-2 .. x in registers, with the
+// next row's inputs loaded one step ahead: every lane computes level n at its column (from
+// levels n-1, n-2 and phi/psi^{n-1}); once level n of rows y-1 .. y+1 is known (y = x-1) it
+// stores level n on row y (outside R: the rectangle pass stores it inside) and lanes 1 ..
+// m.w - m.z compute level n+1 there.  Every value is fwd_frame_body's expression in the same
+// order, so the results equal single-level stepping bit for bit.
+template <typename T>
+__device__ inline void fwd_frame2s_body(const FwdArgs<T> &a, const Lv<T> &u2, const Lv<T> &phi2,
+                                        const Lv<T> &psi2, const Rect &R, const TileMap &map,
+                                        const int4 *__restrict__ items, int nitems, int bid)
+{
     int g, blk, tj;
     if (!decode_tile(map, g, blk, tj, bid)) return;
     const int lane = threadIdx.x & (WAVE - 1), wv = wave_id();
@@ -579,110 +593,141 @@ __device__ inline void fwd_frame2_body(const FwdArgs<T> &a, const Lv<T> &u2, con
     const T tu = act ? a.ty[j] : T(0);
     const bool ny = act && a.neary[j] != 0;
     const bool jR = j >= R.jL && j < R.jR;
-    bool rin[NB], nx[NB], lvl[NB];
-    T sg[NB], Cr[NB];
-    int cidx[NB];
-    bool lp = false, lpi = false;  // near cells on the block's rows / on the item's rows
-#pragma unroll
-    for (int q = 0; q < NB; q++) {
-        const int i = ib - 1 + q;
-        lvl[q] = q <= nr + 1 && i >= 0 && i <= NX + 1;
-        rin[q] = lvl[q] && i >= 1 && i <= NX;
-        nx[q] = lvl[q] && a.nearx[i] != 0;
-        sg[q] = lvl[q] ? a.sx[i] : T(0);
-        Cr[q] = act && rin[q] ? a.C[(int64_t)i * P + j] : T(0);
-        cidx[q] = act && lvl[q] && a.src.cell_idx ? a.src.cell_idx[(int64_t)i * P + j] : -1;
-        const bool nr_ = rin[q] && jin && (nx[q] || ny);
-        lp |= nr_;
-        if (q >= 1 && q <= nr) lpi |= nr_;
+    // near cells on the block's rows ib-1 .. ib+nr (lp) and on the item's rows (lpi): some row
+    // is interior (rb, ri) / interior and near in x (nb, ni)
+    bool rb = false, nb = false, ri = false, ni = false;
+    for (int x = max(ib - 1, 0); x <= min(ib + nr, NX + 1); x++) {  // uniform
+        const bool rin = x >= 1 && x <= NX;
+        const bool nxx = a.nearx[x] != 0;
+        rb = rb || rin;
+        nb = nb || (rin && nxx);
+        if (x >= ib && x < ib + nr) {
+            ri = ri || rin;
+            ni = ni || (rin && nxx);
+        }
     }
-    lp = lp && act;
-    lpi = lpi && act;
+    const bool lp = act && jin && ((ny && rb) || nb);
+    const bool lpi = act && jin && ((ny && ri) || ni);
     SrcIn<T> src2 = a.src;
     src2.n = a.src.n + 1;
+    // inputs of level n on row x besides u^{n-1} and phi^{n-1}
+    struct RowIn {
+        T wo, ps, el, er, sl, sr, c, sg;
+        int ci;
+        bool lvl, rin, nx;
+    };
     const int s1 = min(a.act, (g + 1) * a.Sg);
     for (int s = g * a.Sg + wv; s < s1; s += NWAVES) {
         const T *__restrict__ w = a.w.at(s);
         const T *__restrict__ wo = a.wold.at(s);
         const T *__restrict__ phi = a.phi.at(s);
         const T *__restrict__ psi = a.psi.at(s);
-        // level n-1 on rows ib-2 .. ib+nr+1, level n-2 and psi^{n-1} on the block's rows
-        T W[NB + 2], PH[NB + 2], WO[NB], PS[NB], EL[NB], ER[NB], SL[NB], SR[NB];
-#pragma unroll
-        for (int q = 0; q < NB + 2; q++) {
-            const int i = ib - 2 + q;
-            const bool ok = act && q <= nr + 3 && i >= 0 && i <= NX + 1;
-            W[q] = ok ? w[(int64_t)i * P + j] : T(0);
-            PH[q] = ok && lp ? phi[(int64_t)i * P + j] : T(0);
-        }
-#pragma unroll
-        for (int q = 0; q < NB; q++) {
-            const int64_t row = (int64_t)(ib - 1 + q) * P;
-            WO[q] = act && rin[q] ? wo[row + j] : T(0);
-            PS[q] = lp && rin[q] ? psi[row + j] : T(0);
-            EL[q] = le && rin[q] ? w[row + j - 1] : T(0);
-            ER[q] = reok && rin[q] ? w[row + j + 1] : T(0);
-            SL[q] = le && lp && rin[q] ? psi[row + j - 1] : T(0);
-            SR[q] = reok && lp && rin[q] ? psi[row + j + 1] : T(0);
-        }
-        // level n on the block
-        T Un[NB], PHn[NB], PSn[NB];
-#pragma unroll
-        for (int q = 0; q < NB; q++) {
-            T wv1[1] = {W[q + 1]}, pv1[1] = {PS[q]};
-            T ww, we, pw, pe;
-            lr_of(wv1, EL[q], ER[q], lane, re, ww, we);
-            lr_of(pv1, SL[q], SR[q], lane, re, pw, pe);
-            const T wc = W[q + 1], wn = W[q], ws = W[q + 2];
-            const T lap = k.cx * (ws + wn - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
-            const T s_ = sg[q], Cij = Cr[q];
-            const bool in = rin[q] && jin;
-            const bool near = nx[q] || ny;
-            const T plain = T(2) * wc - WO[q] + Cij * lap;
-            const T num = (T(2) - s_ * tu * k.dt2) * wc + Cij * lap + k.ex * (PH[q + 2] - PH[q]) +
-                          k.ey * (pe - pw) - (T(1) - (s_ + tu) * k.hdt) * WO[q];
-            T out = in ? (near ? num / (T(1) + (s_ + tu) * k.hdt) : plain) : T(0);
-            if (cidx[q] >= 0) out = add_sources(a.src, cidx[q], s, out, k.dt2);
-            Un[q] = act && lvl[q] ? out : T(0);
-            const bool pml = in && (s_ != T(0) || tu != T(0));
-            PHn[q] = lp && pml ? (T(1) - k.dt * s_) * PH[q + 1] + Cij * (tu - s_) * k.fx * (ws - wn) : T(0);
-            PSn[q] = lp && pml ? (T(1) - k.dt * tu) * PS[q] + Cij * (s_ - tu) * k.fy * (we - ww) : T(0);
-        }
         T *__restrict__ u = a.u.at(s);
         T *__restrict__ uo2 = u2.at(s);
-        // level n stores (item cells outside R) and level n+1 on the item
-#pragma unroll
-        for (int r = 0; r < FRPT; r++) {
-            if (r >= nr) break;  // uniform
-            const int q = r + 1, i = ib + r;
-            const int64_t row = (int64_t)i * P;
-            if (own && !(jR && i >= R.iL && i < R.iR)) {
-                u[row + j] = Un[q];
-                if (lpi) {
-                    a.phio.at(s)[row + j] = PHn[q];
-                    a.psio.at(s)[row + j] = PSn[q];
+        // u^{n-1} and phi^{n-1} of row x (rows ib-2 .. ib+nr+1 inside [0, NX+1])
+        auto ldW = [&](int x, T &wx, T &px) {
+            const bool ok = act && x >= 0 && x <= NX + 1 && x <= ib + nr + 1;
+            wx = ok ? w[(int64_t)x * P + j] : T(0);
+            px = ok && lp ? phi[(int64_t)x * P + j] : T(0);
+        };
+        auto ldR = [&](int x, RowIn &o) {
+            o.lvl = x >= 0 && x <= NX + 1;
+            o.rin = o.lvl && x >= 1 && x <= NX;
+            o.nx = o.lvl && a.nearx[x] != 0;
+            o.sg = o.lvl ? a.sx[x] : T(0);
+            const int64_t row = (int64_t)x * P;
+            o.c = act && o.rin ? a.C[row + j] : T(0);
+            o.ci = act && o.lvl && a.src.cell_idx ? a.src.cell_idx[row + j] : -1;
+            o.wo = act && o.rin ? wo[row + j] : T(0);
+            o.ps = lp && o.rin ? psi[row + j] : T(0);
+            o.el = le && o.rin ? w[row + j - 1] : T(0);
+            o.er = reok && o.rin ? w[row + j + 1] : T(0);
+            o.sl = le && lp && o.rin ? psi[row + j - 1] : T(0);
+            o.sr = reok && lp && o.rin ? psi[row + j + 1] : T(0);
+        };
+        // window: u^{n-1} / phi^{n-1} of rows x-1 (w0, p0), x (w1, p1), x+1 (w2, p2); level n of
+        // rows x-2 (um, hm) and x-1 (uc, hc, sc); rc = inputs of row x, ry = those of row x-1
+        T w0, w1, w2, p0, p1, p2;
+        ldW(ib - 2, w0, p0);
+        ldW(ib - 1, w1, p1);
+        ldW(ib, w2, p2);
+        RowIn rc, ry;
+        ldR(ib - 1, rc);
+        ry = rc;
+        T um = T(0), uc = T(0), hm = T(0), hc = T(0), sc = T(0);
+        for (int x = ib - 1; x <= ib + nr; x++) {  // uniform
+            // next row's inputs first, so their latency overlaps this step
+            T w3 = T(0), p3 = T(0);
+            RowIn rn = rc;
+            if (x + 2 <= ib + nr + 1) ldW(x + 2, w3, p3);
+            if (x + 1 <= ib + nr) ldR(x + 1, rn);
+            // level n on row x
+            T U, H, S;
+            {
+                T wv1[1] = {w1}, pv1[1] = {rc.ps};
+                T ww, we, pw, pe;
+                lr_of(wv1, rc.el, rc.er, lane, re, ww, we);
+                lr_of(pv1, rc.sl, rc.sr, lane, re, pw, pe);
+                const T wc = w1, wn = w0, ws = w2;
+                const T lap = k.cx * (ws + wn - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
+                const T s_ = rc.sg, Cij = rc.c;
+                const bool in = rc.rin && jin;
+                const bool near = rc.nx || ny;
+                const T plain = T(2) * wc - rc.wo + Cij * lap;
+                const T num = (T(2) - s_ * tu * k.dt2) * wc + Cij * lap + k.ex * (p2 - p0) +
+                              k.ey * (pe - pw) - (T(1) - (s_ + tu) * k.hdt) * rc.wo;
+                T out = in ? (near ? num / (T(1) + (s_ + tu) * k.hdt) : plain) : T(0);
+                if (rc.ci >= 0) out = add_sources(a.src, rc.ci, s, out, k.dt2);
+                U = act && rc.lvl ? out : T(0);
+                const bool pml = in && (s_ != T(0) || tu != T(0));
+                H = lp && pml ? (T(1) - k.dt * s_) * p1 + Cij * (tu - s_) * k.fx * (ws - wn) : T(0);
+                S = lp && pml ? (T(1) - k.dt * tu) * rc.ps + Cij * (s_ - tu) * k.fy * (we - ww) : T(0);
+            }
+            if (x >= ib + 1) {  // uniform: level n of rows y-1 .. y+1 is known
+                const int y = x - 1;
+                const int64_t row = (int64_t)y * P;
+                if (own && !(jR && y >= R.iL && y < R.iR)) {
+                    u[row + j] = uc;
+                    if (lpi) {
+                        a.phio.at(s)[row + j] = hc;
+                        a.psio.at(s)[row + j] = sc;
+                    }
+                }
+                T uv1[1] = {uc}, pv1[1] = {sc};
+                T ww, we, pw, pe;
+                lr_of(uv1, T(0), T(0), lane, false, ww, we);
+                lr_of(pv1, T(0), T(0), lane, false, pw, pe);
+                const T wc = uc, wn = um, ws = U;
+                const T lap = k.cx * (ws + wn - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
+                const T s_ = ry.sg, Cij = ry.c;
+                const bool in = ry.rin && jin;
+                const bool near = ry.nx || ny;
+                const T plain = T(2) * wc - w0 + Cij * lap;
+                const T num = (T(2) - s_ * tu * k.dt2) * wc + Cij * lap + k.ex * (H - hm) +
+                              k.ey * (pe - pw) - (T(1) - (s_ + tu) * k.hdt) * w0;
+                T out = in ? (near ? num / (T(1) + (s_ + tu) * k.hdt) : plain) : T(0);
+                if (ry.ci >= 0) out = add_sources(src2, ry.ci, s, out, k.dt2);
+                if (own) uo2[row + j] = out;
+                if (own && lpi) {
+                    const bool pml = in && (s_ != T(0) || tu != T(0));
+                    phi2.at(s)[row + j] = pml ? (T(1) - k.dt * s_) * hc + Cij * (tu - s_) * k.fx * (ws - wn) : T(0);
+                    psi2.at(s)[row + j] = pml ? (T(1) - k.dt * tu) * sc + Cij * (s_ - tu) * k.fy * (we - ww) : T(0);
                 }
             }
-            T uv1[1] = {Un[q]}, pv1[1] = {PSn[q]};
-            T ww, we, pw, pe;
-            lr_of(uv1, T(0), T(0), lane, false, ww, we);
-            lr_of(pv1, T(0), T(0), lane, false, pw, pe);
-            const T wc = Un[q], wn = Un[q - 1], ws = Un[q + 1];
-            const T lap = k.cx * (ws + wn - T(2) * wc) + k.cy * (we + ww - T(2) * wc);
-            const T s_ = sg[q], Cij = Cr[q];
-            const bool in = rin[q] && jin;
-            const bool near = nx[q] || ny;
-            const T plain = T(2) * wc - W[q + 1] + Cij * lap;
-            const T num = (T(2) - s_ * tu * k.dt2) * wc + Cij * lap + k.ex * (PHn[q + 1] - PHn[q - 1]) +
-                          k.ey * (pe - pw) - (T(1) - (s_ + tu) * k.hdt) * W[q + 1];
-            T out = in ? (near ? num / (T(1) + (s_ + tu) * k.hdt) : plain) : T(0);
-            if (cidx[q] >= 0) out = add_sources(src2, cidx[q], s, out, k.dt2);
-            if (own) uo2[row + j] = out;
-            if (own && lpi) {
-                const bool pml = in && (s_ != T(0) || tu != T(0));
-                phi2.at(s)[row + j] = pml ? (T(1) - k.dt * s_) * PHn[q] + Cij * (tu - s_) * k.fx * (ws - wn) : T(0);
-                psi2.at(s)[row + j] = pml ? (T(1) - k.dt * tu) * PSn[q] + Cij * (s_ - tu) * k.fy * (we - ww) : T(0);
-            }
+            // rotate
+            um = uc;
+            uc = U;
+            hm = hc;
+            hc = H;
+            sc = S;
+            w0 = w1;
+            w1 = w2;
+            w2 = w3;
+            p0 = p1;
+            p1 = p2;
+            p2 = p3;
+            ry = rc;
+            rc = rn;
         }
     }
 }
@@ -1285,7 +1330,7 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_pair(const FwdArgs<T> a, 
                                                              int nitems, int nfb)
 {
     const int b = (int)blockIdx.x;
-    if (b < nfb) fwd_frame2_body<T>(a, u2, phi2, psi2, R, fmap, items, nitems, b);
+    if (b < nfb) fwd_frame2s_body<T>(a, u2, phi2, psi2, R, fmap, items, nitems, b);
     else fwd_rect2_body<T, RPT, VEC>(a, u2, R, b - nfb);
 }
 // Levels n and n+1 of the rectangle (streaming body) with level n of the frame (kernel 0).
@@ -1308,7 +1353,7 @@ __global__ __launch_bounds__(WAVE *NWAVES) void ac_fwd_pair_s(const FwdArgs<T> a
                                                                int nitems, int nfb)
 {
     const int b = (int)blockIdx.x;
-    if (b < nfb) fwd_frame2_body<T>(a, u2, phi2, psi2, R, fmap, items, nitems, b);
+    if (b < nfb) fwd_frame2s_body<T>(a, u2, phi2, psi2, R, fmap, items, nitems, b);
     else fwd_rect2s_body<T, VEC, ROWS>(a, u2, R, b - nfb);
 }
 template <typename T, int RPT, int VEC, int SB>
@@ -1459,8 +1504,8 @@ struct AcPlanBase {
     // recorded per forward segment (ac_record)
     bool src_fused = false;
     // items of the two-level frame pass (complement of rect1, WAVE - 2 columns each)
-    DevBuf d_items2;
-    int nitems2 = 0;
+    DevBuf d_items2[FRAME2_H];  // two-level frame items of height frame2_height(k)
+    int nitems2[FRAME2_H] = {};
     // kernel 0 with the sources in the frame work: items of WAVE - 2 columns covering the
     // complement of rect[0] and of rect1, for the frame body that also updates phi/psi
     DevBuf d_items_h, d_items1h;
@@ -1995,13 +2040,14 @@ template <typename T> struct AcPlan : AcPlanBase {
         a.map = rect_tiles(R, VECP, RPT, a.act, a.Sg);
         TileMap fm;
         fm.ntj = 1;
-        fm.nstrips = nitems2;
+        const int h = frame2_pick(a.act);
+        fm.nstrips = nitems2[h];
         fm.ngroups = (int)cdiv(a.act, a.Sg);
         fm.nb = fm.ntj * fm.nstrips * fm.ngroups;
         const int nfb = (int)grid_of(fm);
         const unsigned nb = (unsigned)nfb + grid_of(a.map);
         ac_fwd_pair<T, RPT, VECP><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, phi2, psi2, R, fm,
-                                                                d_items2.template as<int4>(), nitems2, nfb);
+                                                                d_items2[h].template as<int4>(), nitems2[h], nfb);
     }
     template <int ROWS>
     void launch_pair_s(FwdArgs<T> a, Lv<T> u2, Lv<T> phi2, Lv<T> psi2, hipStream_t st)
@@ -2013,13 +2059,14 @@ template <typename T> struct AcPlan : AcPlanBase {
         a.map.nb = a.map.ntj * a.map.nstrips * a.map.ngroups;
         TileMap fm;
         fm.ntj = 1;
-        fm.nstrips = nitems2;
+        const int h = frame2_pick(a.act);
+        fm.nstrips = nitems2[h];
         fm.ngroups = (int)cdiv(a.act, a.Sg);
         fm.nb = fm.ntj * fm.nstrips * fm.ngroups;
         const int nfb = (int)grid_of(fm);
         const unsigned nb = (unsigned)nfb + grid_of(a.map);
         ac_fwd_pair_s<T, VECP, ROWS><<<nb, WAVE * NWAVES, 0, st>>>(a, u2, phi2, psi2, R, fm,
-                                                                   d_items2.template as<int4>(), nitems2, nfb);
+                                                                   d_items2[h].template as<int4>(), nitems2[h], nfb);
     }
     int fwd_pair(const WS &w, int n, int act, const T *srcv, T *traces, int record, hipStream_t st,
                  const Shape *sh)
@@ -2029,7 +2076,7 @@ template <typename T> struct AcPlan : AcPlanBase {
         FwdArgs<T> a = fwd_args(L, act, slot(w, n), cst(slot(w, n - 1)), cst(slot(w, n - 2)),
                                 cst(phl(w, n - 1)), cst(psl(w, n - 1)), phl(w, n), psl(w, n), w.C, shape,
                                 n, srcv);
-        if (variant == 1 && nitems2 > 0 && a.src.cell_idx) {
+        if (variant == 1 && nitems2[0] > 0 && a.src.cell_idx) {
             int pr = prof_start(ADS_PROF_AC_FWD, st);
             if (shape.rpt == 1) launch_pair<1>(a, slot(w, n + 1), phl(w, n + 1), psl(w, n + 1), st);
             else if (shape.rpt == 4) launch_pair<4>(a, slot(w, n + 1), phl(w, n + 1), psl(w, n + 1), st);
@@ -2584,9 +2631,11 @@ static int build_plan(const ads_acoustic_params *p, const ads_shot_geometry *g, 
         P->nitems1 = P->blk_ok ? (int)items.size() : 0;
         if (P->blk_ok) ADS_TRY(P->d_items1.upload(items));
         if (P->blk_ok && P->src_fused && P->variant == 1) {
-            frame_items(P->rect1, p->NX, p->NY, items, FRPT, WAVE - 2);
-            P->nitems2 = (int)items.size();
-            ADS_TRY(P->d_items2.upload(items));
+            for (int h = 0; h < FRAME2_H; h++) {
+                frame_items(P->rect1, p->NX, p->NY, items, frame2_height(h), WAVE - 2);
+                P->nitems2[h] = (int)items.size();
+                ADS_TRY(P->d_items2[h].upload(items));
+            }
         }
         if (P->src_fused && P->variant == 0) {
             frame_items(P->rect[0], p->NX, p->NY, items, FRPT, WAVE - 2);
