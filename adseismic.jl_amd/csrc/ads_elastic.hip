@@ -463,8 +463,11 @@ __global__ __launch_bounds__(WAVE *NWAVES) void el_vel_level(ElC<T> I, El<T> O, 
 
 // Sources at level n (unique (cell, type) keys, entries in source order), then receivers:
 // trace[r * (nstep_s + 1) + n] = field_type_r[cell_r] (GetReceive.cpp:13-45 layout).
-// mode 0: add; 1: add, then record the receivers; 2: remove the stress-type sources again
-// (the adjoint needs the stresses of level n as fw3/fw4 read them, before injection).
+// mode 0: add; 1: add, then record the receivers; 2: put back the stresses of level n as
+// fw3/fw4 read them, before injection (the adjoint needs those).  pre (one value per key,
+// may be null) keeps the stress at each stress-type key before the add (modes 0 and 1) and
+// gives it back in mode 2, so the adjoint sees the exact pre-injection value; without it,
+// mode 2 subtracts the sources again, which rounds in FP32.
 template <typename T>
 __global__ __launch_bounds__(256) void el_src_rcv(
     El<T> O, int n, const int64_t *__restrict__ us_ptr, const int64_t *__restrict__ us_key,
@@ -472,7 +475,7 @@ __global__ __launch_bounds__(256) void el_src_rcv(
     const T *__restrict__ srcv, const int64_t *__restrict__ srcv_off,
     const int64_t *__restrict__ nsrc, const int64_t *__restrict__ rcv_ptr,
     const int64_t *__restrict__ rcv_key, const int64_t *__restrict__ nstep,
-    T *__restrict__ traces, const int64_t *__restrict__ trace_off, int mode)
+    T *__restrict__ traces, const int64_t *__restrict__ trace_off, int mode, T *__restrict__ pre)
 {
     const int s = blockIdx.x;
     const int64_t ns = nsrc[s];
@@ -483,8 +486,13 @@ __global__ __launch_bounds__(256) void el_src_rcv(
         const int64_t cell = key >> 3;
         if (mode == 2 && type < SXX) continue;  // removal touches stresses only
         T *__restrict__ f = O.f(type, s);
+        if (mode == 2 && pre) {
+            f[cell] = pre[c];
+            continue;
+        }
         T acc = T(0);
         for (int64_t e = us_eptr[c]; e < us_eptr[c + 1]; e++) acc += row[us_ent[e]];
+        if (mode != 2 && pre && type >= SXX) pre[c] = f[cell];
         f[cell] = mode == 2 ? f[cell] - acc : f[cell] + acc;
     }
     if (mode != 1) return;
@@ -1542,6 +1550,7 @@ struct ElPlanBase {
     DevBuf d_rcv_ptr, d_trace_off, d_nstep;
     DevBuf d_us_key[2], d_ur_key[2], d_rcv_key[2];  // [0] pitched, [1] dense; key = cell*8+type
     int64_t max_ur = 0;
+    int64_t nkeys = 0;  // unique (cell, type) source keys over all shots
     // interior rectangle (identity CPML, every update range) and frame items, per layout
     Rect rect[2]{};
     DevBuf d_items[2];
@@ -1588,6 +1597,7 @@ template <typename T> struct ElPlan : ElPlanBase {
     struct WS {
         int64_t K, NS, M, S;  // interval, slots, segments, snapshot slots
         T *seg, *ckpt, *adj, *tmp, *grad, *rho, *lam, *mu, *rb, *par, *resid;
+        T *pre;  // per level slot: the stress at each stress-type source key before injection
         double *loss, *part;
     };
     int64_t seg_count(int64_t K) const { return NMAX >= 1 ? (NMAX + K - 1) / K : 0; }
@@ -1614,6 +1624,7 @@ template <typename T> struct ElPlan : ElPlanBase {
         w.lam = (T *)cv.take((size_t)Npad * sizeof(T));
         w.mu = (T *)cv.take((size_t)Npad * sizeof(T));
         w.resid = (T *)cv.take((size_t)std::max<int64_t>(trace_elems, 1) * sizeof(T));
+        w.pre = (T *)cv.take((size_t)w.NS * (size_t)std::max<int64_t>(nkeys, 1) * sizeof(T));
         w.loss = (double *)cv.take(64);
         w.part = (double *)cv.take(MISFIT_BLOCKS * sizeof(double));
     }
@@ -1665,6 +1676,9 @@ template <typename T> struct ElPlan : ElPlanBase {
     {
         return block(w.seg + (size_t)(n % w.NS) * NF * nshots * Npad);
     }
+    // pre-injection stresses of the level in slot n (written by the forward step of level n;
+    // the adjoint only reads levels that the last forward over their segment wrote)
+    T *pre_slot(const WS &w, int64_t n) const { return w.pre + (size_t)(n % w.NS) * std::max<int64_t>(nkeys, 1); }
 
     static constexpr int VECP = sizeof(T) == 4 ? 4 : 2;  // pitched layout: 16-byte vectors
     // The rectangle of layout li, with the frame items in the same launch when `fuse` (the
@@ -1704,7 +1718,8 @@ template <typename T> struct ElPlan : ElPlanBase {
     }
 
     int step(const El<T> &I, const El<T> &O, int n, int act, const T *rho, const T *lam,
-             const T *mu, const T *srcv, T *traces, int record, int P, int li, hipStream_t st)
+             const T *mu, const T *srcv, T *traces, int record, int P, int li, hipStream_t st,
+             T *pre = nullptr)
     {
         if (act <= 0) return ADS_OK;
         const ElCoef<T> c = coef(P);
@@ -1741,7 +1756,7 @@ template <typename T> struct ElPlan : ElPlanBase {
                                            d_srcv_off.as<int64_t>(), d_nsrc.as<int64_t>(),
                                            d_rcv_ptr.as<int64_t>(), d_rcv_key[li].template as<int64_t>(),
                                            d_nstep.as<int64_t>(), traces,
-                                           d_trace_off.as<int64_t>(), record ? 1 : 0);
+                                           d_trace_off.as<int64_t>(), record ? 1 : 0, pre);
         ADS_LAUNCH_CHECK();
         return ADS_OK;
     }
@@ -1762,7 +1777,7 @@ template <typename T> struct ElPlan : ElPlanBase {
     {
         for (int64_t n = lo + 1; n <= hi; n++)
             ADS_TRY(step(slot(w, n - 1), slot(w, n), (int)n, active[n], w.rho, w.lam, w.mu, srcv,
-                         traces, record, (int)Pitch, 0, st));
+                         traces, record, (int)Pitch, 0, st, pre_slot(w, n)));
         return ADS_OK;
     }
 
@@ -1821,7 +1836,7 @@ template <typename T> struct ElPlan : ElPlanBase {
                                            d_srcv_off.as<int64_t>(), d_nsrc.as<int64_t>(),
                                            d_rcv_ptr.as<int64_t>(), d_rcv_key[li].template as<int64_t>(),
                                            d_nstep.as<int64_t>(), nullptr,
-                                           d_trace_off.as<int64_t>(), 2);
+                                           d_trace_off.as<int64_t>(), 2, pre_slot(w, n));
         ElAdj<T> a;
         a.A = A;
         a.Sn = cst(Sn);
@@ -2151,6 +2166,7 @@ static int build_el_plan(const ads_elastic_params *p, const ads_shot_geometry *g
         for (size_t e = 0; e < ri.size(); e++) rkey[e] = (ri[e] * pitches[L] + rj[e]) * 8 + rt[e];
         CellLists us = build_cell_lists(ns, sp.data(), skey), ur = build_cell_lists(ns, rp.data(), rkey);
         if (L == 0) {
+            P->nkeys = us.shot_ptr[ns];
             ADS_TRY(P->d_us_ptr.upload(us.shot_ptr));
             ADS_TRY(P->d_us_eptr.upload(us.ent_ptr));
             ADS_TRY(P->d_us_ent.upload(us.ent));
