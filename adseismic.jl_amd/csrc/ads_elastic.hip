@@ -37,21 +37,28 @@ namespace ads {
 constexpr int NF = 13;  // fields per level
 enum { VX = 0, VY, SXX, SYY, SXY, M1, M2, M3, M4, M5, M6, M7, M8 };
 
-// a level of the 13 arrays: field pointers and per-field shot strides
+// a level of the 13 arrays: field pointers and per-field shot strides.  The memory variables
+// M1..M8 are either full grids like the fields (mb.H == 0) or kept only on the cells outside
+// the box mb where every CPML profile vanishes (there they stay 0): mv() gives their index.
 template <typename T> struct El {
     T *p[NF];
     int64_t st[NF];
+    FrameBox mb;
     __device__ __host__ T *f(int q, int s) const { return p[q] + (int64_t)s * st[q]; }
+    __device__ __host__ int64_t mv(int i, int j, int64_t ij) const { return mb.H ? mb.index(i, j) : ij; }
 };
 template <typename T> struct ElC {
     const T *p[NF];
     int64_t st[NF];
+    FrameBox mb;
     __device__ __host__ const T *f(int q, int s) const { return p[q] + (int64_t)s * st[q]; }
+    __device__ __host__ int64_t mv(int i, int j, int64_t ij) const { return mb.H ? mb.index(i, j) : ij; }
 };
 template <typename T> ElC<T> cst(const El<T> &a)
 {
     ElC<T> c;
     for (int q = 0; q < NF; q++) c.p[q] = a.p[q], c.st[q] = a.st[q];
+    c.mb = a.mb;
     return c;
 }
 
@@ -96,6 +103,8 @@ __device__ inline void el_stress_cell(const ElC<T> &I, const El<T> &O, const T *
     const T *__restrict__ vy = I.f(VY, s);
     T sxx = I.f(SXX, s)[ij], syy = I.f(SYY, s)[ij], sxy = I.f(SXY, s)[ij];
     T m1 = 0, m2 = 0, m3 = 0, m4 = 0;
+    const int64_t mi = I.mv(i, j, ij), mo = O.mv(i, j, ij);  // memory variables (< 0: all 0)
+    auto mold = [&](int q) { return mi >= 0 ? I.f(q, s)[mi] : T(0); };
     const int pi = i - 1 - c.sub, pj = j - 1 - c.sub;  // profile index
     if (fw_in(c, 1, i, j)) {
         const T lb = c.sub ? lam[ij] : T(0.5) * (lam[ij + P] + lam[ij]);
@@ -103,8 +112,8 @@ __device__ inline void el_stress_cell(const ElC<T> &I, const El<T> &O, const T *
         const T lm = lb + T(2) * mb;
         T dx = (T(27) * vx[ij + P] - T(27) * vx[ij] - vx[ij + 2 * P] + vx[ij - P]) * c.ihx;
         T dy = (T(27) * vy[ij] - T(27) * vy[ij - 1] - vy[ij + 1] + vy[ij - 2]) * c.ihy;
-        m1 = c.bxh[pi] * I.f(M1, s)[ij] + c.axh[pi] * dx;
-        m2 = c.byf[pj] * I.f(M2, s)[ij] + c.ayf[pj] * dy;
+        m1 = c.bxh[pi] * mold(M1) + c.axh[pi] * dx;
+        m2 = c.byf[pj] * mold(M2) + c.ayf[pj] * dy;
         dx += m1;
         dy += m2;
         sxx += (lm * dx + lb * dy) * c.dt;
@@ -114,17 +123,19 @@ __device__ inline void el_stress_cell(const ElC<T> &I, const El<T> &O, const T *
         const T mb = c.sub ? mu[ij] : T(0.5) * (mu[ij] + mu[ij + 1]);
         T dx = (T(27) * vy[ij] - T(27) * vy[ij - P] - vy[ij + P] + vy[ij - 2 * P]) * c.ihx;
         T dy = (T(27) * vx[ij + 1] - T(27) * vx[ij] - vx[ij + 2] + vx[ij - 1]) * c.ihy;
-        m3 = c.bxf[pi] * I.f(M3, s)[ij] + c.axf[pi] * dx;
-        m4 = c.byh[pj] * I.f(M4, s)[ij] + c.ayh[pj] * dy;
+        m3 = c.bxf[pi] * mold(M3) + c.axf[pi] * dx;
+        m4 = c.byh[pj] * mold(M4) + c.ayh[pj] * dy;
         sxy += mb * (dx + m3 + dy + m4) * c.dt;
     }
     O.f(SXX, s)[ij] = sxx;
     O.f(SYY, s)[ij] = syy;
     O.f(SXY, s)[ij] = sxy;
-    O.f(M1, s)[ij] = m1;
-    O.f(M2, s)[ij] = m2;
-    O.f(M3, s)[ij] = m3;
-    O.f(M4, s)[ij] = m4;
+    if (mo >= 0) {
+        O.f(M1, s)[mo] = m1;
+        O.f(M2, s)[mo] = m2;
+        O.f(M3, s)[mo] = m3;
+        O.f(M4, s)[mo] = m4;
+    }
 }
 
 // one cell of fw3/fw4 (reads the new stresses of level n)
@@ -139,28 +150,32 @@ __device__ inline void el_vel_cell(const ElC<T> &I, const El<T> &O, const T *__r
     const T *__restrict__ sxy = O.f(SXY, s);
     T vx = I.f(VX, s)[ij], vy = I.f(VY, s)[ij];
     T m5 = 0, m6 = 0, m7 = 0, m8 = 0;
+    const int64_t mi = I.mv(i, j, ij), mo = O.mv(i, j, ij);  // memory variables (< 0: all 0)
+    auto mold = [&](int q) { return mi >= 0 ? I.f(q, s)[mi] : T(0); };
     const int pi = i - 1 - c.sub, pj = j - 1 - c.sub;  // profile index
     if (fw_in(c, 3, i, j)) {
         T dx = (T(27) * sxx[ij] - T(27) * sxx[ij - P] - sxx[ij + P] + sxx[ij - 2 * P]) * c.ihx;
         T dy = (T(27) * sxy[ij] - T(27) * sxy[ij - 1] - sxy[ij + 1] + sxy[ij - 2]) * c.ihy;
-        m5 = c.bxf[pi] * I.f(M5, s)[ij] + c.axf[pi] * dx;
-        m6 = c.byf[pj] * I.f(M6, s)[ij] + c.ayf[pj] * dy;
+        m5 = c.bxf[pi] * mold(M5) + c.axf[pi] * dx;
+        m6 = c.byf[pj] * mold(M6) + c.ayf[pj] * dy;
         vx += (dx + m5 + dy + m6) * c.dt / rho[ij];
     }
     if (fw_in(c, 4, i, j)) {
         const T rb = c.sub ? rho[ij] : T(0.25) * (rho[ij] + rho[ij + P] + rho[ij + P + 1] + rho[ij + 1]);
         T dx = (T(27) * sxy[ij + P] - T(27) * sxy[ij] - sxy[ij + 2 * P] + sxy[ij - P]) * c.ihx;
         T dy = (T(27) * syy[ij + 1] - T(27) * syy[ij] - syy[ij + 2] + syy[ij - 1]) * c.ihy;
-        m7 = c.bxh[pi] * I.f(M7, s)[ij] + c.axh[pi] * dx;
-        m8 = c.byh[pj] * I.f(M8, s)[ij] + c.ayh[pj] * dy;
+        m7 = c.bxh[pi] * mold(M7) + c.axh[pi] * dx;
+        m8 = c.byh[pj] * mold(M8) + c.ayh[pj] * dy;
         vy += (dx + m7 + dy + m8) * c.dt / rb;
     }
     O.f(VX, s)[ij] = vx;
     O.f(VY, s)[ij] = vy;
-    O.f(M5, s)[ij] = m5;
-    O.f(M6, s)[ij] = m6;
-    O.f(M7, s)[ij] = m7;
-    O.f(M8, s)[ij] = m8;
+    if (mo >= 0) {
+        O.f(M5, s)[mo] = m5;
+        O.f(M6, s)[mo] = m6;
+        O.f(M7, s)[mo] = m7;
+        O.f(M8, s)[mo] = m8;
+    }
 }
 
 // frame: one column per lane, FRPT rows per wavefront item, one shot per grid z
@@ -586,11 +601,13 @@ __device__ inline void adjv_a_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s,
     const T *__restrict__ sxx = a.Sn.f(SXX, s);
     const T *__restrict__ syy = a.Sn.f(SYY, s);
     const T *__restrict__ sxy = a.Sn.f(SXY, s);
+    const int64_t mi = a.Sn.mv(i, j, ij);  // forward memory variables of level n (< 0: all 0)
+    auto msn = [&](int q) { return mi >= 0 ? a.Sn.f(q, s)[mi] : T(0); };
     const int pi = i - 1 - c.sub, pj = j - 1 - c.sub;  // profile index
     if (fw_in(c, 3, i, j)) {
         const T dx = (T(27) * sxx[ij] - T(27) * sxx[ij - P] - sxx[ij + P] + sxx[ij - 2 * P]) * c.ihx;
         const T dy = (T(27) * sxy[ij] - T(27) * sxy[ij - 1] - sxy[ij + 1] + sxy[ij - 2]) * c.ihy;
-        const T S3 = dx + a.Sn.f(M5, s)[ij] + dy + a.Sn.f(M6, s)[ij];
+        const T S3 = dx + msn(M5) + dy + msn(M6);
         const T g = a.A.f(VX, s)[ij], r = a.rho[ij];
         gR += -g * S3 * c.dt / (r * r);
         const T e = g * c.dt / r;
@@ -604,7 +621,7 @@ __device__ inline void adjv_a_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s,
         const T rb = a.rb[ij];
         const T dx = (T(27) * sxy[ij + P] - T(27) * sxy[ij] - sxy[ij + 2 * P] + sxy[ij - P]) * c.ihx;
         const T dy = (T(27) * syy[ij + 1] - T(27) * syy[ij] - syy[ij + 2] + syy[ij - 1]) * c.ihy;
-        const T S4 = dx + a.Sn.f(M7, s)[ij] + dy + a.Sn.f(M8, s)[ij];
+        const T S4 = dx + msn(M7) + dy + msn(M8);
         const T g = a.A.f(VY, s)[ij];
         gB += -g * S4 * c.dt / (rb * rb);
         const T e = g * c.dt / rb;
@@ -656,6 +673,8 @@ __device__ inline void adjs_a_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s,
     const T *__restrict__ vy = a.Sm.f(VY, s);
     T *a1 = a.A.f(M1, s), *a2 = a.A.f(M2, s), *a3 = a.A.f(M3, s), *a4 = a.A.f(M4, s);
     T t0 = 0, t1 = 0, t2 = 0, t3 = 0, n1 = 0, n2 = 0, n3 = 0, n4 = 0;
+    const int64_t mi = a.Sn.mv(i, j, ij);  // forward memory variables of level n (< 0: all 0)
+    auto msn = [&](int q) { return mi >= 0 ? a.Sn.f(q, s)[mi] : T(0); };
     const int pi = i - 1 - c.sub, pj = j - 1 - c.sub;  // profile index
     if (fw_in(c, 1, i, j)) {
         const T lb = c.sub ? a.lam[ij] : T(0.5) * (a.lam[ij + P] + a.lam[ij]);
@@ -663,7 +682,7 @@ __device__ inline void adjs_a_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s,
         const T lm = lb + T(2) * mb;
         const T dx = (T(27) * vx[ij + P] - T(27) * vx[ij] - vx[ij + 2 * P] + vx[ij - P]) * c.ihx;
         const T dy = (T(27) * vy[ij] - T(27) * vy[ij - 1] - vy[ij + 1] + vy[ij - 2]) * c.ihy;
-        const T X = dx + a.Sn.f(M1, s)[ij], Y = dy + a.Sn.f(M2, s)[ij];
+        const T X = dx + msn(M1), Y = dy + msn(M2);
         const T gx = a.A.f(SXX, s)[ij] * c.dt, gy = a.A.f(SYY, s)[ij] * c.dt;
         const T glm = X * gx + Y * gy;
         gL += Y * gx + X * gy + glm;  // d/d lambda-bar
@@ -680,7 +699,7 @@ __device__ inline void adjs_a_cell(const ElAdj<T> &a, const ElCoef<T> &c, int s,
         const T dx = (T(27) * vy[ij] - T(27) * vy[ij - P] - vy[ij + P] + vy[ij - 2 * P]) * c.ihx;
         const T dy = (T(27) * vx[ij + 1] - T(27) * vx[ij] - vx[ij + 2] + vx[ij - 1]) * c.ihy;
         const T g = a.A.f(SXY, s)[ij];
-        gM2 += g * (dx + a.Sn.f(M3, s)[ij] + dy + a.Sn.f(M4, s)[ij]) * c.dt;  // d/d mu-bar (fw2)
+        gM2 += g * (dx + msn(M3) + dy + msn(M4)) * c.dt;  // d/d mu-bar (fw2)
         const T gs = g * mb * c.dt;
         const T g3 = a3[ij] + gs, g4 = a4[ij] + gs;
         t2 = gs + c.axf[pi] * g3;
@@ -1551,6 +1570,12 @@ struct ElPlanBase {
     DevBuf d_us_key[2], d_ur_key[2], d_rcv_key[2];  // [0] pitched, [1] dense; key = cell*8+type
     int64_t max_ur = 0;
     int64_t nkeys = 0;  // unique (cell, type) source keys over all shots
+    // level slots and checkpoints keep the memory variables only outside this box, where some
+    // CPML profile is non-zero (inside it they stay 0): nmv cells per shot and variable
+    FrameBox mbox;
+    int64_t nmv = 0;
+    int64_t level_elems() const { return nshots * (5 * Npad + 8 * nmv); }  // one level, all shots
+    int64_t field_elems(int q) const { return q < M1 ? Npad : nmv; }
     // interior rectangle (identity CPML, every update range) and frame items, per layout
     Rect rect[2]{};
     DevBuf d_items[2];
@@ -1612,9 +1637,9 @@ template <typename T> struct ElPlan : ElPlanBase {
         w.NS = std::min<int64_t>(K + 1, NMAX + 1);
         const int64_t one = std::max<int64_t>(w.M - 1, 0);
         w.S = snaps < 0 ? one : std::min(snaps, one);
-        const size_t F = (size_t)nshots * Npad * sizeof(T);
-        w.seg = (T *)cv.take(F * NF * w.NS);
-        w.ckpt = (T *)cv.take(F * NF * (size_t)w.S);  // slot q: a whole level of every shot
+        const size_t F = (size_t)nshots * Npad * sizeof(T), LV = (size_t)level_elems() * sizeof(T);
+        w.seg = (T *)cv.take(LV * w.NS);
+        w.ckpt = (T *)cv.take(LV * (size_t)w.S);  // slot q: a whole level of every shot
         w.adj = (T *)cv.take(F * NF);
         w.tmp = (T *)cv.take(F * 4);
         w.grad = (T *)cv.take((size_t)G_COUNT * adj_groups() * Npad * sizeof(T));
@@ -1672,9 +1697,23 @@ template <typename T> struct ElPlan : ElPlanBase {
         }
         return e;
     }
+    // a level slot or checkpoint: the fields [field][shot][Npad], then the memory variables
+    // [variable][shot][nmv] outside mbox
+    El<T> level(T *base) const
+    {
+        El<T> e;
+        T *q0 = base;
+        for (int q = 0; q < NF; q++) {
+            e.p[q] = q0;
+            e.st[q] = field_elems(q);
+            q0 += (size_t)nshots * field_elems(q);
+        }
+        e.mb = mbox;
+        return e;
+    }
     El<T> slot(const WS &w, int64_t n) const
     {
-        return block(w.seg + (size_t)(n % w.NS) * NF * nshots * Npad);
+        return level(w.seg + (size_t)(n % w.NS) * level_elems());
     }
     // pre-injection stresses of the level in slot n (written by the forward step of level n;
     // the adjoint only reads levels that the last forward over their segment wrote)
@@ -1784,13 +1823,13 @@ template <typename T> struct ElPlan : ElPlanBase {
     int copy_state(const El<T> &dst, const El<T> &src, int act, hipStream_t st)
     {
         for (int q = 0; q < NF; q++)
-            ADS_HIP(hipMemcpyAsync(dst.p[q], src.p[q], (size_t)act * Npad * sizeof(T),
+            ADS_HIP(hipMemcpyAsync(dst.p[q], src.p[q], (size_t)act * field_elems(q) * sizeof(T),
                                    hipMemcpyDeviceToDevice, st));
         return ADS_OK;
     }
     El<T> ckpt(const WS &w, int64_t q) const  // snapshot slot q
     {
-        return block(w.ckpt + (size_t)q * NF * nshots * Npad);
+        return level(w.ckpt + (size_t)q * level_elems());
     }
 
     int forward(const void *rho, const void *lam, const void *mu, const void *srcv, void *traces,
@@ -1802,7 +1841,7 @@ template <typename T> struct ElPlan : ElPlanBase {
         // Level 0 is zero everywhere.  The CPML memory variables and their cotangents are read
         // and written on frame cells only (frame kernels); rectangle cells never touch them, so
         // the other slots need no clearing.
-        ADS_HIP(hipMemsetAsync(slot(w, 0).p[0], 0, (size_t)NF * nshots * Npad * sizeof(T), st));
+        ADS_HIP(hipMemsetAsync(slot(w, 0).p[0], 0, (size_t)level_elems() * sizeof(T), st));
         if (traces && trace_elems) ADS_HIP(hipMemsetAsync(traces, 0, trace_elems * sizeof(T), st));
         // the reversal schedule's first descent: the states it keeps, in level order
         Reversal rv(w.M, w.S);
@@ -1969,7 +2008,7 @@ template <typename T> struct ElPlan : ElPlanBase {
                 if (q < 0) {  // level 0: zero
                     El<T> z = pl->slot(w, 0);
                     for (int f = 0; f < NF; f++)
-                        ADS_HIP(hipMemsetAsync(z.p[f], 0, (size_t)act * pl->Npad * sizeof(T), st));
+                        ADS_HIP(hipMemsetAsync(z.p[f], 0, (size_t)act * pl->field_elems(f) * sizeof(T), st));
                     return ADS_OK;
                 }
                 return pl->copy_state(pl->slot(w, m * w.K), pl->ckpt(w, q), act, st);
@@ -2189,6 +2228,19 @@ static int build_el_plan(const ads_elastic_params *p, const ads_shot_geometry *g
     for (int64_t k = 0; k < 2 * p->NY; k++) prof.push_back((T)ay[k]);  // ayf, ayh
     for (int64_t k = 0; k < 2 * p->NY; k++) prof.push_back((T)by[k]);  // byf, byh
     ADS_TRY(P->d_prof.upload(prof));
+    {  // the box where every a-profile vanishes: the memory variables stay 0 inside it
+        std::vector<double> px((size_t)P->NX + 2, 1.0), py((size_t)P->NY + 2, 1.0);
+        for (int64_t i = 1 + sub; i <= P->NX - sub; i++) {
+            const int64_t q = i - 1 - sub;  // profile index
+            px[i] = std::fabs(ax[q]) + std::fabs(ax[p->NX + q]);
+        }
+        for (int64_t j = 1 + sub; j <= P->NY - sub; j++) {
+            const int64_t q = j - 1 - sub;
+            py[j] = std::fabs(ay[q]) + std::fabs(ay[p->NY + q]);
+        }
+        P->mbox = quiet_box(px, py);
+        P->nmv = P->mbox.cells();
+    }
     int64_t cx = 0, cy = 0;
     for (int64_t k = 0; k < p->NX; k++) cx += (ax[k] != 0.0 || ax[p->NX + k] != 0.0);
     for (int64_t k = 0; k < p->NY; k++) cy += (ay[k] != 0.0 || ay[p->NY + k] != 0.0);

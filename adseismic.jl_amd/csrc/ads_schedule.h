@@ -229,6 +229,18 @@ struct FrameBox {
     {
         return (int64_t)H * W - (int64_t)(qi1 - qi0) * (qj1 - qj0);
     }
+    // (i, j) -> its index f among the cells outside the box, or -1 inside the box (the
+    // inverse of cell())
+    __host__ __device__ int64_t index(int i, int j) const
+    {
+        const int64_t top = (int64_t)qi0 * W;
+        if (i < qi0) return (int64_t)i * W + j;
+        const int wm = qj0 + (W - qj1);
+        const int64_t mid = (int64_t)(qi1 - qi0) * wm;
+        if (i >= qi1) return top + mid + (int64_t)(i - qi1) * W + j;
+        if (j >= qj0 && j < qj1) return -1;
+        return top + (int64_t)(i - qi0) * wm + (j < qj0 ? j : qj0 + (j - qj1));
+    }
     // f-th cell outside the box -> (i, j); f in [0, cells())
     __host__ __device__ void cell(int64_t f, int &i, int &j) const
     {
