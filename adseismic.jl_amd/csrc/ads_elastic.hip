@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "ads_common.h"
@@ -537,6 +538,7 @@ template <typename T> struct ElAdj {
     El<T> A;          // cotangents of the 13 state arrays
     ElC<T> Sn, Sm;    // forward state of levels n (this level's sources removed) and n-1
     T *Tb;            // [4][shot][Npad]: t0..t3 on frame cells
+    T *mo[8];         // full-grid kernels: where M1..M8 cotangents of level n-1 go ([shot][Npad])
     T *G;             // [G_COUNT][group][Npad]
     int64_t npad, tstride, gstride;  // Npad, nshots * Npad, ngroups * Npad
     const T *rho, *lam, *mu, *rb;    // rb: rho-bar of fw4 (Core.jl:201)
@@ -1132,6 +1134,432 @@ template <typename T> struct ElPar {
 // Each row step issues the loads of the row it will need one step later (window row i+3 and
 // the row-(i+1) cotangents), so their latency overlaps the current row's arithmetic.
 
+// Full-grid streaming form: the same walk over every cell of the padded grid, the CPML frame
+// included, so no frame_a / frame_b pass and no temporaries.  t-values are formed in the
+// registers of the lanes that need them: t0 and t2 when a row enters the window, t1 and t3 on
+// the output row, each with the memory-variable cotangent term (a * (m + e), the profile a 0
+// outside the CPML strips) and 0 outside the cell's update range (as tv_vel / tv_str).  The
+// memory-variable cotangents are read from the A arrays and their updates written to the
+// second set (mo), so a halo lane never reads a value another wavefront has already replaced;
+// the sets swap every level.  A memory-variable cotangent whose a-profile is 0 at its row (x)
+// or column (y) never reaches a t-value, so it is neither read nor written there.  A
+// wavefront whose window rows and lanes all lie inside R (identity CPML, every update range)
+// runs the plain specialisation of the walk: no range tests, no memory variables.
+template <typename T> struct ProfRow {  // CPML coefficients of a row (x) or column (y), 0 off range
+    T af, ah, bf, bh;
+};
+template <typename T> __device__ inline ProfRow<T> prof_row(const ElCoef<T> &c, int r)
+{
+    const int pi = r - 1 - c.sub;
+    const bool ok = r >= 1 + c.sub && r <= c.NX - c.sub;
+    return ProfRow<T>{ok ? c.axf[pi] : T(0), ok ? c.axh[pi] : T(0), ok ? c.bxf[pi] : T(0), ok ? c.bxh[pi] : T(0)};
+}
+template <typename T> __device__ inline ProfRow<T> prof_col(const ElCoef<T> &c, int j)
+{
+    const int pj = j - 1 - c.sub;
+    const bool ok = j >= 1 + c.sub && j <= c.NY - c.sub;
+    return ProfRow<T>{ok ? c.ayf[pj] : T(0), ok ? c.ayh[pj] : T(0), ok ? c.byf[pj] : T(0), ok ? c.byh[pj] : T(0)};
+}
+// The update ranges of fw_in split into row and column parts: fw1 = A_row & A_col, fw2 = B_row
+// & B_col, fw3 = B_row & A_col, fw4 = A_row & B_col.
+struct FwPart {
+    bool A, B;
+};
+template <typename T> __device__ inline FwPart fw_row(const ElCoef<T> &c, int i)
+{
+    if (c.sub) return FwPart{i >= 2 && i <= c.NX - 1, i >= 2 && i <= c.NX - 1};
+    return FwPart{i >= 1 && i <= c.NX - 1, i >= 2 && i <= c.NX};
+}
+template <typename T> __device__ inline FwPart fw_col(const ElCoef<T> &c, int j)
+{
+    if (c.sub) return FwPart{j >= 2 && j <= c.NY - 1, j >= 2 && j <= c.NY - 1};
+    return FwPart{j >= 2 && j <= c.NY, j >= 1 && j <= c.NY - 1};
+}
+
+// velocity phase, full grid: fw3/fw4 VJP on every cell
+template <typename T>
+__device__ inline void adjv_full_body(const ElAdj<T> &a, const ElCoef<T> &c, const ElPar<T> &q,
+                                      const StrmMap &mp, int bid)
+{
+    extern __shared__ __align__(16) unsigned char strm_lds[];
+    T *__restrict__ gacc = reinterpret_cast<T *>(strm_lds);  // [wave][term][row][lane]
+    int g, chunk, strip;
+    if (!decode_strm(mp, g, chunk, strip, bid)) return;
+    const int nw = (int)blockDim.x / WAVE;
+    for (int k = threadIdx.x; k < nw * 2 * mp.ch * WAVE; k += blockDim.x) gacc[k] = T(0);
+    __syncthreads();
+    auto gslot = [&](int wv, int term, int r, int l) -> T & {
+        return gacc[((wv * 2 + term) * mp.ch + r) * WAVE + l];
+    };
+    const int w = wave_id(), lane = threadIdx.x % WAVE;
+    const int NXp = c.NX + 1, NYp = c.NY + 1;  // last padded row / column
+    const int i0 = chunk * mp.ch, i1 = min(i0 + mp.ch, NXp + 1);
+    const int jb = strip * SW_COLS - 2;
+    const int j = jb + lane;
+    const int jc = j < 0 ? 0 : (j > NYp ? NYp : j);  // loads of lanes beyond the grid stay inside it
+    const bool out = lane >= 2 && lane < WAVE - 2 && j <= NYp;
+    const bool gen = !(i0 - 2 >= a.R.iL && i1 + 2 <= a.R.iR && jb >= a.R.jL && jb + WAVE <= a.R.jR);
+    const int s = g * a.Sg + w;
+    if (w < a.Sg && s < a.act) {
+        const int P = c.P;
+        const T dt = c.dt;
+        const T *__restrict__ vxa = a.A.f(VX, s);
+        const T *__restrict__ vya = a.A.f(VY, s);
+        T *__restrict__ axx = a.A.f(SXX, s);
+        T *__restrict__ axy = a.A.f(SXY, s);
+        T *__restrict__ ayy = a.A.f(SYY, s);
+        const T *__restrict__ sxx = a.Sn.f(SXX, s);
+        const T *__restrict__ sxy = a.Sn.f(SXY, s);
+        const T *__restrict__ syy = a.Sn.f(SYY, s);
+        const uint32_t pb = (uint32_t)P * (uint32_t)sizeof(T), jb4 = (uint32_t)jc * (uint32_t)sizeof(T);
+        auto walk = [&](auto genc) {
+            constexpr bool GEN = decltype(genc)::value;
+            const ProfRow<T> py = GEN ? prof_col(c, j) : ProfRow<T>{};
+            const FwPart fc = GEN ? fw_col(c, j) : FwPart{true, true};
+            // raw loads of one window row, and of one output row
+            struct Rw { T a, b, x, y, r, q, m5, m7; ProfRow<T> px; };
+            struct Ro { T yy, m6, m8, gxx, gxy, gyy, r, q; };
+            auto fetch_w = [&](int r, Rw &o) {
+                const bool ok = !GEN || (r >= 0 && r <= NXp);  // uniform over the wavefront
+                const uint32_t rj = (uint32_t)(ok ? r : 0) * pb + jb4;
+                o.x = ok ? ld_at(sxx, rj) : T(0);
+                o.y = ok ? ld_at(sxy, rj) : T(0);
+                o.a = ok ? ld_at(vxa, rj) : T(0);
+                o.b = ok ? ld_at(vya, rj) : T(0);
+                o.r = ok ? ld_at(q.irho, rj) : T(0);
+                o.q = ok ? ld_at(q.irb, rj) : T(0);
+                // (cells whose profile is 0 keep the 0 of the clear: no test needed)
+                o.m5 = GEN && ok ? ld_at(a.A.f(M5, s), rj) : T(0);
+                o.m7 = GEN && ok ? ld_at(a.A.f(M7, s), rj) : T(0);
+                o.px = GEN ? prof_row(c, r) : ProfRow<T>{};
+            };
+            auto fetch_o = [&](int i, Ro &o) {
+                const uint32_t ij = (uint32_t)i * pb + jb4;
+                o.yy = ld_at(syy, ij);
+                o.r = ld_at(q.irho, ij);
+                o.q = ld_at(q.irb, ij);
+                o.m6 = GEN ? ld_at(a.A.f(M6, s), ij) : T(0);
+                o.m8 = GEN ? ld_at(a.A.f(M8, s), ij) : T(0);
+                o.gxx = out ? ld_at<T>(axx, ij) : T(0);
+                o.gxy = out ? ld_at<T>(axy, ij) : T(0);
+                o.gyy = out ? ld_at<T>(ayy, ij) : T(0);
+            };
+            // windows over rows i-2 .. i+2 (index 0 .. 4): t0, t2, e3 = g_vx dt / rho,
+            // e4 = g_vy dt / rho-bar, the forward sxx, sxy; g5 = m5 + e3, g7 = m7 + e4
+            T w0[5], w2[5], e3w[5], e4w[5], wx[5], wy[5], g5w[5], g7w[5];
+            auto finish = [&](int r, const Rw &o, int k) {
+                wx[k] = o.x;
+                wy[k] = o.y;
+                const T e3 = o.a * dt * o.r, e4 = o.b * dt * o.q;
+                e3w[k] = e3;
+                e4w[k] = e4;
+                if constexpr (GEN) {
+                    const FwPart fr = fw_row(c, r);
+                    const T g5 = o.m5 + e3, g7 = o.m7 + e4;
+                    g5w[k] = g5;
+                    g7w[k] = g7;
+                    w0[k] = fr.B && fc.A ? e3 + o.px.af * g5 : T(0);
+                    w2[k] = fr.A && fc.B ? e4 + o.px.ah * g7 : T(0);
+                } else {
+                    w0[k] = e3;
+                    w2[k] = e4;
+                }
+            };
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                Rw t;
+                fetch_w(i0 - 2 + k, t);
+                finish(i0 - 2 + k, t, k);
+            }
+            Rw rn;
+            Ro on;
+            fetch_w(i0 + 2, rn);
+            fetch_o(i0, on);
+            for (int i = i0; i < i1; i++) {
+                const Rw rc = rn;
+                const Ro oc = on;
+                if (i + 1 < i1) {
+                    fetch_w(i + 3, rn);
+                    fetch_o(i + 1, on);
+                }
+                finish(i + 2, rc, 4);
+                const FwPart fr = GEN ? fw_row(c, i) : FwPart{true, true};
+                const bool f3 = fr.B && fc.A, f4 = fr.A && fc.B;
+                const T e3 = e3w[2], e4 = e4w[2];
+                // row i, y-direction t-values
+                const T g6 = oc.m6 + e3, g8 = oc.m8 + e4;
+                const T t1 = GEN ? (f3 ? e3 + py.af * g6 : T(0)) : e3;
+                const T t3 = GEN ? (f4 ? e4 + py.ah * g8 : T(0)) : e4;
+                const T xy = wy[2], yy = oc.yy;
+                const T t1m = lane_at(t1, lane - 1), t1p = lane_at(t1, lane + 1), t1pp = lane_at(t1, lane + 2);
+                const T t3mm = lane_at(t3, lane - 2), t3m = lane_at(t3, lane - 1), t3p = lane_at(t3, lane + 1);
+                const T xymm = lane_at(xy, lane - 2), xym = lane_at(xy, lane - 1), xyp = lane_at(xy, lane + 1);
+                const T yym = lane_at(yy, lane - 1), yyp = lane_at(yy, lane + 1), yypp = lane_at(yy, lane + 2);
+                if (out) {
+                    const uint32_t ij = (uint32_t)i * pb + jb4;
+                    // parameter terms (fw3: rho; fw4: rho-bar), with the forward memory
+                    // variables of level n outside the damping-free box
+                    T m5 = T(0), m6 = T(0), m7 = T(0), m8 = T(0);
+                    if constexpr (GEN) {
+                        const int64_t mi = a.Sn.mv(i, j, (int64_t)i * P + j);
+                        if (mi >= 0) {
+                            m5 = a.Sn.f(M5, s)[mi];
+                            m6 = a.Sn.f(M6, s)[mi];
+                            m7 = a.Sn.f(M7, s)[mi];
+                            m8 = a.Sn.f(M8, s)[mi];
+                        }
+                    }
+                    const T S3 = d4(wx[2], wx[1], wx[3], wx[0]) * c.ihx + m5 + d4(xy, xym, xyp, xymm) * c.ihy + m6;
+                    const T S4 = d4(wy[3], wy[2], wy[4], wy[1]) * c.ihx + m7 + d4(yyp, yy, yypp, yym) * c.ihy + m8;
+                    gslot(w, 0, i - i0, lane) = f3 ? -e3 * S3 * oc.r : T(0);
+                    gslot(w, 1, i - i0, lane) = f4 ? -e4 * S4 * oc.q : T(0);
+                    if constexpr (GEN) {
+                        // memory-variable cotangents of level n-1 (those whose profile is
+                        // non-zero)
+                        const ProfRow<T> px = prof_row(c, i);
+                        if (px.af != T(0)) st_at(a.mo[M5 - M1] + (int64_t)s * a.npad, ij, f3 ? px.bf * g5w[2] : T(0));
+                        if (py.af != T(0)) st_at(a.mo[M6 - M1] + (int64_t)s * a.npad, ij, f3 ? py.bf * g6 : T(0));
+                        if (px.ah != T(0)) st_at(a.mo[M7 - M1] + (int64_t)s * a.npad, ij, f4 ? px.bh * g7w[2] : T(0));
+                        if (py.ah != T(0)) st_at(a.mo[M8 - M1] + (int64_t)s * a.npad, ij, f4 ? py.bh * g8 : T(0));
+                    }
+                    // sxx <- (D-x)^T t0 ; sxy <- (D-y)^T t1 + (D+x)^T t2 ; syy <- (D+y)^T t3
+                    st_at(axx, ij, oc.gxx + d4(w0[2], w0[3], w0[1], w0[4]) * c.ihx);
+                    st_at(axy, ij, oc.gxy + d4(t1, t1p, t1m, t1pp) * c.ihy + d4(w2[1], w2[2], w2[0], w2[3]) * c.ihx);
+                    st_at(ayy, ij, oc.gyy + d4(t3m, t3, t3mm, t3p) * c.ihy);
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    w0[k] = w0[k + 1];
+                    w2[k] = w2[k + 1];
+                    e3w[k] = e3w[k + 1];
+                    e4w[k] = e4w[k + 1];
+                    wx[k] = wx[k + 1];
+                    wy[k] = wy[k + 1];
+                    if constexpr (GEN) {
+                        g5w[k] = g5w[k + 1];
+                        g7w[k] = g7w[k + 1];
+                    }
+                }
+            }
+        };
+        if (gen) walk(std::true_type{});
+        else walk(std::false_type{});
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < (i1 - i0) * WAVE; k += blockDim.x) {
+        const int r = k / WAVE, l = k % WAVE, jj = jb + l;
+        if (l < 2 || l >= WAVE - 2 || jj > NYp) continue;
+        const int64_t ij = (int64_t)(i0 + r) * c.P + jj;
+        T s0 = T(0), s1 = T(0);
+        for (int wv = 0; wv < nw; wv++) {
+            s0 += gslot(wv, 0, r, l);
+            s1 += gslot(wv, 1, r, l);
+        }
+        g_at(a, G_RHO, g, ij) += s0;
+        g_at(a, G_RB, g, ij) += s1;
+    }
+}
+
+// stress phase, full grid: fw1/fw2 VJP on every cell
+template <typename T>
+__device__ inline void adjs_full_body(const ElAdj<T> &a, const ElCoef<T> &c, const ElPar<T> &q,
+                                      const StrmMap &mp, int bid)
+{
+    extern __shared__ __align__(16) unsigned char strm_lds[];
+    T *__restrict__ gacc = reinterpret_cast<T *>(strm_lds);  // [wave][term][row][lane]
+    int g, chunk, strip;
+    if (!decode_strm(mp, g, chunk, strip, bid)) return;
+    const int nw = (int)blockDim.x / WAVE;
+    for (int k = threadIdx.x; k < nw * 3 * mp.ch * WAVE; k += blockDim.x) gacc[k] = T(0);
+    __syncthreads();
+    auto gslot = [&](int wv, int term, int r, int l) -> T & {
+        return gacc[((wv * 3 + term) * mp.ch + r) * WAVE + l];
+    };
+    const int w = wave_id(), lane = threadIdx.x % WAVE;
+    const int NXp = c.NX + 1, NYp = c.NY + 1;
+    const int i0 = chunk * mp.ch, i1 = min(i0 + mp.ch, NXp + 1);
+    const int jb = strip * SW_COLS - 2;
+    const int j = jb + lane;
+    const int jc = j < 0 ? 0 : (j > NYp ? NYp : j);
+    const bool out = lane >= 2 && lane < WAVE - 2 && j <= NYp;
+    const bool gen = !(i0 - 2 >= a.R.iL && i1 + 2 <= a.R.iR && jb >= a.R.jL && jb + WAVE <= a.R.jR);
+    const int s = g * a.Sg + w;
+    if (w < a.Sg && s < a.act) {
+        const int P = c.P;
+        const T dt = c.dt;
+        T *__restrict__ avx = a.A.f(VX, s);
+        T *__restrict__ avy = a.A.f(VY, s);
+        const T *__restrict__ axx = a.A.f(SXX, s);
+        const T *__restrict__ ayy = a.A.f(SYY, s);
+        const T *__restrict__ axy = a.A.f(SXY, s);
+        const T *__restrict__ vx = a.Sm.f(VX, s);
+        const T *__restrict__ vy = a.Sm.f(VY, s);
+        const uint32_t pb = (uint32_t)P * (uint32_t)sizeof(T), jb4 = (uint32_t)jc * (uint32_t)sizeof(T);
+        auto walk = [&](auto genc) {
+            constexpr bool GEN = decltype(genc)::value;
+            const ProfRow<T> py = GEN ? prof_col(c, j) : ProfRow<T>{};
+            const FwPart fc = GEN ? fw_col(c, j) : FwPart{true, true};
+            // raw loads of one window row (the stress cotangents, the averaged parameters, the
+            // memory-variable cotangents of x-profile rows, vx and vy of level n-1), and of one
+            // output row (its cotangents and parameters again, with those of y-profile
+            // columns, and the velocity cotangents it updates)
+            struct Rw { T u, v, xx, yy, xy, lb, mb, mb2, m1, m3; ProfRow<T> px; };
+            struct Ro { T xx, yy, xy, lb, mb, mb2, m1, m2, m3, m4, gvx, gvy; };
+            auto fetch_w = [&](int r, Rw &o) {
+                const bool ok = !GEN || (r >= 0 && r <= NXp);  // uniform over the wavefront
+                const uint32_t rj = (uint32_t)(ok ? r : 0) * pb + jb4;
+                o.u = ok ? ld_at(vx, rj) : T(0);
+                o.v = ok ? ld_at(vy, rj) : T(0);
+                o.xx = ok ? ld_at(axx, rj) : T(0);
+                o.yy = ok ? ld_at(ayy, rj) : T(0);
+                o.xy = ok ? ld_at(axy, rj) : T(0);
+                o.lb = ok ? ld_at(q.lb, rj) : T(0);
+                o.mb = ok ? ld_at(q.mb, rj) : T(0);
+                o.mb2 = ok ? ld_at(q.mb2, rj) : T(0);
+                // (cells whose profile is 0 keep the 0 of the clear: no test needed)
+                o.m1 = GEN && ok ? ld_at(a.A.f(M1, s), rj) : T(0);
+                o.m3 = GEN && ok ? ld_at(a.A.f(M3, s), rj) : T(0);
+                o.px = GEN ? prof_row(c, r) : ProfRow<T>{};
+            };
+            auto fetch_o = [&](int i, Ro &o) {
+                const uint32_t ij = (uint32_t)i * pb + jb4;
+                o.xx = ld_at(axx, ij);
+                o.yy = ld_at(ayy, ij);
+                o.xy = ld_at(axy, ij);
+                o.lb = ld_at(q.lb, ij);
+                o.mb = ld_at(q.mb, ij);
+                o.mb2 = ld_at(q.mb2, ij);
+                o.m1 = GEN ? ld_at(a.A.f(M1, s), ij) : T(0);
+                o.m3 = GEN ? ld_at(a.A.f(M3, s), ij) : T(0);
+                o.m2 = GEN ? ld_at(a.A.f(M2, s), ij) : T(0);
+                o.m4 = GEN ? ld_at(a.A.f(M4, s), ij) : T(0);
+                o.gvx = out ? ld_at<T>(avx, ij) : T(0);
+                o.gvy = out ? ld_at<T>(avy, ij) : T(0);
+            };
+            // windows over rows i-2 .. i+2: t0, t2, and vx, vy of level n-1
+            T w0[5], w2[5], wu[5], wv[5];
+            auto finish = [&](int r, const Rw &o, int k) {
+                wu[k] = o.u;
+                wv[k] = o.v;
+                const T gx = o.xx * dt, gy = o.yy * dt;
+                const T lm = o.lb + T(2) * o.mb;
+                const T gX = lm * gx + o.lb * gy;
+                const T gs = o.xy * o.mb2 * dt;
+                if constexpr (GEN) {
+                    const FwPart fr = fw_row(c, r);
+                    w0[k] = fr.A && fc.A ? gX + o.px.ah * (o.m1 + gX) : T(0);
+                    w2[k] = fr.B && fc.B ? gs + o.px.af * (o.m3 + gs) : T(0);
+                } else {
+                    w0[k] = gX;
+                    w2[k] = gs;
+                }
+            };
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                Rw t;
+                fetch_w(i0 - 2 + k, t);
+                finish(i0 - 2 + k, t, k);
+            }
+            Rw rn;
+            Ro on;
+            fetch_w(i0 + 2, rn);
+            fetch_o(i0, on);
+            for (int i = i0; i < i1; i++) {
+                const Rw rc = rn;
+                const Ro oc = on;
+                if (i + 1 < i1) {
+                    fetch_w(i + 3, rn);
+                    fetch_o(i + 1, on);
+                }
+                finish(i + 2, rc, 4);
+                const FwPart fr = GEN ? fw_row(c, i) : FwPart{true, true};
+                const bool f1 = fr.A && fc.A, f2 = fr.B && fc.B;
+                // row i: the cotangent products of fw1 / fw2 and the y-direction t-values
+                const T gx = oc.xx * dt, gy = oc.yy * dt;
+                const T lm = oc.lb + T(2) * oc.mb;
+                const T gX = lm * gx + oc.lb * gy, gY = oc.lb * gx + lm * gy;
+                const T gs = oc.xy * oc.mb2 * dt;
+                const T t1 = GEN ? (f1 ? gY + py.af * (oc.m2 + gY) : T(0)) : gY;
+                const T t3 = GEN ? (f2 ? gs + py.ah * (oc.m4 + gs) : T(0)) : gs;
+                const T u = wu[2], v = wv[2];
+                const T t1m = lane_at(t1, lane - 1), t1p = lane_at(t1, lane + 1), t1pp = lane_at(t1, lane + 2);
+                const T t3mm = lane_at(t3, lane - 2), t3m = lane_at(t3, lane - 1), t3p = lane_at(t3, lane + 1);
+                const T um = lane_at(u, lane - 1), up = lane_at(u, lane + 1), upp = lane_at(u, lane + 2);
+                const T vmm = lane_at(v, lane - 2), vm = lane_at(v, lane - 1), vp = lane_at(v, lane + 1);
+                if (out) {
+                    const uint32_t ij = (uint32_t)i * pb + jb4;
+                    // parameter terms: fw1 (lambda-bar, mu-bar) and fw2 (mu-bar), level n-1
+                    // velocities, with the forward memory variables of level n outside the box
+                    T m1 = T(0), m2 = T(0), m3 = T(0), m4 = T(0);
+                    if constexpr (GEN) {
+                        const int64_t mi = a.Sn.mv(i, j, (int64_t)i * P + j);
+                        if (mi >= 0) {
+                            m1 = a.Sn.f(M1, s)[mi];
+                            m2 = a.Sn.f(M2, s)[mi];
+                            m3 = a.Sn.f(M3, s)[mi];
+                            m4 = a.Sn.f(M4, s)[mi];
+                        }
+                    }
+                    const T X = d4(wu[3], wu[2], wu[4], wu[1]) * c.ihx + m1;
+                    const T Y = d4(v, vm, vp, vmm) * c.ihy + m2;
+                    const T glm = X * gx + Y * gy;
+                    gslot(w, 0, i - i0, lane) = f1 ? Y * gx + X * gy + glm : T(0);
+                    gslot(w, 1, i - i0, lane) = f1 ? T(2) * glm : T(0);
+                    const T dx = d4(wv[2], wv[1], wv[3], wv[0]) * c.ihx;
+                    const T dy = d4(up, u, upp, um) * c.ihy;
+                    gslot(w, 2, i - i0, lane) = f2 ? oc.xy * (dx + m3 + dy + m4) * dt : T(0);
+                    if constexpr (GEN) {
+                        // memory-variable cotangents of level n-1 (those whose profile is
+                        // non-zero)
+                        const ProfRow<T> px = prof_row(c, i);
+                        if (px.ah != T(0)) st_at(a.mo[M1 - M1] + (int64_t)s * a.npad, ij, f1 ? px.bh * (oc.m1 + gX) : T(0));
+                        if (py.af != T(0)) st_at(a.mo[M2 - M1] + (int64_t)s * a.npad, ij, f1 ? py.bf * (oc.m2 + gY) : T(0));
+                        if (px.af != T(0)) st_at(a.mo[M3 - M1] + (int64_t)s * a.npad, ij, f2 ? px.bf * (oc.m3 + gs) : T(0));
+                        if (py.ah != T(0)) st_at(a.mo[M4 - M1] + (int64_t)s * a.npad, ij, f2 ? py.bh * (oc.m4 + gs) : T(0));
+                    }
+                    // vx <- (D+x)^T t0 + (D+y)^T t3 ; vy <- (D-y)^T t1 + (D-x)^T t2
+                    st_at(avx, ij, oc.gvx + d4(w0[1], w0[2], w0[0], w0[3]) * c.ihx + d4(t3m, t3, t3mm, t3p) * c.ihy);
+                    st_at(avy, ij, oc.gvy + d4(t1, t1p, t1m, t1pp) * c.ihy + d4(w2[2], w2[3], w2[1], w2[4]) * c.ihx);
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    w0[k] = w0[k + 1];
+                    w2[k] = w2[k + 1];
+                    wu[k] = wu[k + 1];
+                    wv[k] = wv[k + 1];
+                }
+            }
+        };
+        if (gen) walk(std::true_type{});
+        else walk(std::false_type{});
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < (i1 - i0) * WAVE; k += blockDim.x) {
+        const int r = k / WAVE, l = k % WAVE, jj = jb + l;
+        if (l < 2 || l >= WAVE - 2 || jj > NYp) continue;
+        const int64_t ij = (int64_t)(i0 + r) * c.P + jj;
+        T s0 = T(0), s1 = T(0), s2 = T(0);
+        for (int wv = 0; wv < nw; wv++) {
+            s0 += gslot(wv, 0, r, l);
+            s1 += gslot(wv, 1, r, l);
+            s2 += gslot(wv, 2, r, l);
+        }
+        g_at(a, G_LB, g, ij) += s0;
+        g_at(a, G_MB1, g, ij) += s1;
+        g_at(a, G_MB2, g, ij) += s2;
+    }
+}
+template <typename T>
+__global__ __launch_bounds__(WAVE *SW_NW) void el_adjv_full(ElAdj<T> a, ElCoef<T> c, ElPar<T> q, StrmMap mp)
+{
+    adjv_full_body(a, c, q, mp, (int)blockIdx.x);
+}
+template <typename T>
+__global__ __launch_bounds__(WAVE *SW_NW) void el_adjs_full(ElAdj<T> a, ElCoef<T> c, ElPar<T> q, StrmMap mp)
+{
+    adjs_full_body(a, c, q, mp, (int)blockIdx.x);
+}
+
 // velocity phase: fw3/fw4 VJP on rectangle cells
 template <typename T>
 __device__ inline void adjv_strm_body(const ElAdj<T> &a, const ElCoef<T> &c, const ElPar<T> &q,
@@ -1586,7 +2014,7 @@ struct ElPlanBase {
     // adjoint: shots per group (one wavefront each in the streaming rectangle kernels, whose
     // parameter terms are summed per group), streaming (1) or LDS-tiled (0) rectangle kernels,
     // and the streaming kernels' rows per workgroup
-    int asg = NWAVES, strm = 1, strm_ch = 8;
+    int asg = NWAVES, strm = 2, strm_ch = 8;
     // snapshot slots of the reversal schedule (< 0: one per segment start, the one-level scheme)
     int64_t snapshots = -1;
     int sub = 0;          // one MPIElastic subdomain (see ElCoef)
@@ -1622,6 +2050,7 @@ template <typename T> struct ElPlan : ElPlanBase {
     struct WS {
         int64_t K, NS, M, S;  // interval, slots, segments, snapshot slots
         T *seg, *ckpt, *adj, *tmp, *grad, *rho, *lam, *mu, *rb, *par, *resid;
+        T *adjm;  // full-grid adjoint kernels: the second set of memory-variable cotangents
         T *pre;  // per level slot: the stress at each stress-type source key before injection
         double *loss, *part;
     };
@@ -1641,6 +2070,7 @@ template <typename T> struct ElPlan : ElPlanBase {
         w.seg = (T *)cv.take(LV * w.NS);
         w.ckpt = (T *)cv.take(LV * (size_t)w.S);  // slot q: a whole level of every shot
         w.adj = (T *)cv.take(F * NF);
+        w.adjm = (T *)cv.take(strm == 2 ? F * 8 : 0);
         w.tmp = (T *)cv.take(F * 4);
         w.grad = (T *)cv.take((size_t)G_COUNT * adj_groups() * Npad * sizeof(T));
         w.rb = (T *)cv.take((size_t)Npad * sizeof(T));
@@ -1916,6 +2346,38 @@ template <typename T> struct ElPlan : ElPlanBase {
         const size_t strm_lds = (size_t)3 * a.Sg * std::max(1, std::min(strm_ch, SW_CH_MAX)) * WAVE * sizeof(T);
         // (the streaming kernels address one field of one shot with 32-bit byte offsets)
         const bool off32 = (uint64_t)Npad * sizeof(T) < (uint64_t(1) << 32);
+        if (strm == 2 && li == 0 && a.Sg <= SW_NW && strm_lds <= 65536 && off32) {
+            // full grid, one launch per phase.  Level n reads the memory-variable cotangents
+            // that level n+1 wrote: the two sets alternate with the parity of n.
+            El<T> Ar = A;
+            for (int f = M1; f <= M8; f++) {
+                T *other = w.adjm + (size_t)(f - M1) * nshots * Npad;
+                if (n & 1) {
+                    Ar.p[f] = other;
+                    a.mo[f - M1] = A.p[f];
+                } else {
+                    a.mo[f - M1] = other;
+                }
+            }
+            a.A = Ar;
+            StrmMap sm;
+            sm.ch = std::max(1, std::min(strm_ch, SW_CH_MAX));
+            sm.nstrip = (int)cdiv(NY + 2, SW_COLS);
+            sm.nchunk = (int)cdiv(NX + 2, sm.ch);
+            sm.ngroup = ng;
+            sm.nb = sm.nstrip * sm.nchunk * sm.ngroup;
+            const unsigned sgrid = cdiv(sm.nb, NXCD) * NXCD;
+            const unsigned sblock = (unsigned)(WAVE * a.Sg);
+            const size_t lds = (size_t)a.Sg * sm.ch * WAVE * sizeof(T);  // x terms
+            int pr = prof_start(ADS_PROF_EL_ADJ_VEL, st);
+            el_adjv_full<T><<<sgrid, sblock, 2 * lds, st>>>(a, c, par, sm);
+            prof_stop(pr, st, cells * sizeof(T) * (12.0 + 4.0 * (f_x + f_y)));
+            pr = prof_start(ADS_PROF_EL_ADJ_STRESS, st);
+            el_adjs_full<T><<<sgrid, sblock, 3 * lds, st>>>(a, c, par, sm);
+            prof_stop(pr, st, cells * sizeof(T) * (12.0 + 4.0 * (f_x + f_y)));
+            ADS_LAUNCH_CHECK();
+            return ADS_OK;
+        }
         if (rr && strm && li == 0 && a.Sg <= SW_NW && strm_lds <= 65536 && off32) {
             StrmMap sm;
             sm.ch = std::max(1, std::min(strm_ch, SW_CH_MAX));
@@ -1984,6 +2446,7 @@ template <typename T> struct ElPlan : ElPlanBase {
         ADS_TRY(check_ws(w, ws, wsb, K));
         const size_t F = (size_t)nshots * Npad * sizeof(T);
         ADS_HIP(hipMemsetAsync(w.adj, 0, F * NF, st));
+        if (strm == 2) ADS_HIP(hipMemsetAsync(w.adjm, 0, F * 8, st));
         ADS_HIP(hipMemsetAsync(w.grad, 0, (size_t)G_COUNT * adj_groups() * Npad * sizeof(T), st));
         ADS_HIP(hipMemsetAsync(w.tmp, 0, F * 4, st));
         el_rho_bar<T><<<dim3(cdiv(Pitch, 256), NX + 2), 256, 0, st>>>(w.rho, w.rb, (int)NX, (int)NY, (int)Pitch, sub);
@@ -2390,10 +2853,11 @@ int ads_elastic_plan_set_adjoint_kernels(ads_elastic_plan *plan, int streaming, 
 {
     clear_error();
     if (!plan || !plan->impl) ADS_FAIL(ADS_ERR_ARG, "null plan");
-    if (rows < 1 || rows > SW_CH_MAX || shots < 1 || shots > SW_NW || rows * shots > SW_LDS_ROWS)
-        ADS_FAIL(ADS_ERR_ARG, "rows must be in [1, %d], shots in [1, %d] and rows x shots <= %d",
-                 SW_CH_MAX, SW_NW, SW_LDS_ROWS);
-    plan->impl->strm = streaming ? 1 : 0;
+    if (streaming < 0 || streaming > 2 || rows < 1 || rows > SW_CH_MAX || shots < 1 || shots > SW_NW ||
+        rows * shots > SW_LDS_ROWS)
+        ADS_FAIL(ADS_ERR_ARG, "streaming must be 0, 1 or 2, rows in [1, %d], shots in [1, %d] and "
+                 "rows x shots <= %d", SW_CH_MAX, SW_NW, SW_LDS_ROWS);
+    plan->impl->strm = streaming;
     plan->impl->strm_ch = rows;
     plan->impl->asg = shots;
     return ADS_OK;

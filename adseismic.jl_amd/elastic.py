@@ -100,10 +100,11 @@ class ElasticPlan:
     def workspace_size(self, K: int) -> int:
         return int(self.lib.ads_elastic_plan_workspace_size(self.h, int(K)))
 
-    def set_adjoint_kernels(self, streaming: bool = True, rows: int = 8, shots: int = 4) -> None:
-        """Adjoint rectangle kernels (ads_elastic_plan_set_adjoint_kernels); changes the
-        workspace layout, so call it before the forward."""
-        _lib.check(self.lib.ads_elastic_plan_set_adjoint_kernels(self.h, int(bool(streaming)),
+    def set_adjoint_kernels(self, streaming: int = 2, rows: int = 8, shots: int = 4) -> None:
+        """Adjoint kernels (ads_elastic_plan_set_adjoint_kernels): 2 full-grid streaming, 1
+        streaming rectangle with the frame passes, 0 LDS-tiled rectangle (True / False mean 1 /
+        0); changes the workspace layout, so call it before the forward."""
+        _lib.check(self.lib.ads_elastic_plan_set_adjoint_kernels(self.h, int(streaming),
                                                                  int(rows), int(shots)))
         self._ws_K = None
 

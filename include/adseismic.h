@@ -355,13 +355,15 @@ int ads_elastic_plan_destroy(ads_elastic_plan *plan);
 int ads_elastic_plan_sizes(const ads_elastic_plan *plan, int64_t *srcv_elems,
                            int64_t *trace_elems, int64_t *srcv_off, int64_t *trace_off);
 size_t ads_elastic_plan_workspace_size(const ads_elastic_plan *plan, int64_t K);
-/* Adjoint rectangle kernels: streaming (1, the default: one wavefront per shot walking rows,
- * x-direction stencils from registers, y-direction from neighbouring lanes) or LDS-tiled (0);
- * rows per streaming workgroup (1..32, default 8); shots per adjoint group (1..8, default 4;
- * the parameter terms are accumulated per group); rows x shots <= 64.  The streaming kernels
- * need 3 x shots x rows x 64 values of LDS; above 64 KiB (FP64 past 32 rows x shots) the
- * LDS-tiled kernels run.  Changes the workspace layout: call before
- * ads_elastic_plan_workspace_size and the forward. */
+/* Adjoint kernels: full-grid streaming (2, the default: one wavefront per shot walking rows
+ * over the whole grid, x-direction stencils from registers, y-direction from neighbouring
+ * lanes, the CPML memory cotangents formed on the fly), streaming over the interior rectangle
+ * with separate frame passes (1) or LDS-tiled over the rectangle (0); rows per streaming
+ * workgroup (1..32, default 8); shots per adjoint group (1..8, default 4; the parameter terms
+ * are accumulated per group); rows x shots <= 64.  The streaming kernels need 3 x shots x rows
+ * x 64 values of LDS; above 64 KiB (FP64 past 32 rows x shots) the LDS-tiled kernels run.
+ * Changes the workspace layout (mode 2 keeps a second set of memory-variable cotangents):
+ * call before ads_elastic_plan_workspace_size and the forward. */
 int ads_elastic_plan_set_adjoint_kernels(ads_elastic_plan *plan, int streaming, int rows,
                                          int shots);
 /* As ads_acoustic_plan_choose_interval, over the NSTEP elastic levels. */
