@@ -2055,6 +2055,14 @@ template <typename T> struct ElPlan : ElPlanBase {
         double *loss, *part;
     };
     int64_t seg_count(int64_t K) const { return NMAX >= 1 ? (NMAX + K - 1) / K : 0; }
+    // The full-grid adjoint kernels run at every level or at none: they keep the memory-variable
+    // cotangents in two alternating sets, the other kernels in one set in place, so the choice
+    // must not change with the number of active shots (their LDS is sized for the plan's group).
+    bool full_adjoint() const
+    {
+        const size_t lds = (size_t)3 * asg * std::max(1, std::min(strm_ch, SW_CH_MAX)) * WAVE * sizeof(T);
+        return strm == 2 && asg <= SW_NW && lds <= 65536 && (uint64_t)Npad * sizeof(T) < (uint64_t(1) << 32);
+    }
     int64_t adj_groups() const { return (int64_t)cdiv(nshots, std::max(asg, 1)); }
     void layout(WS &w, Carve &cv, int64_t K) const { layout(w, cv, K, snapshots); }
     void layout(WS &w, Carve &cv, int64_t K, int64_t snaps) const
@@ -2070,7 +2078,7 @@ template <typename T> struct ElPlan : ElPlanBase {
         w.seg = (T *)cv.take(LV * w.NS);
         w.ckpt = (T *)cv.take(LV * (size_t)w.S);  // slot q: a whole level of every shot
         w.adj = (T *)cv.take(F * NF);
-        w.adjm = (T *)cv.take(strm == 2 ? F * 8 : 0);
+        w.adjm = (T *)cv.take(full_adjoint() ? F * 8 : 0);
         w.tmp = (T *)cv.take(F * 4);
         w.grad = (T *)cv.take((size_t)G_COUNT * adj_groups() * Npad * sizeof(T));
         w.rb = (T *)cv.take((size_t)Npad * sizeof(T));
@@ -2346,7 +2354,7 @@ template <typename T> struct ElPlan : ElPlanBase {
         const size_t strm_lds = (size_t)3 * a.Sg * std::max(1, std::min(strm_ch, SW_CH_MAX)) * WAVE * sizeof(T);
         // (the streaming kernels address one field of one shot with 32-bit byte offsets)
         const bool off32 = (uint64_t)Npad * sizeof(T) < (uint64_t(1) << 32);
-        if (strm == 2 && li == 0 && a.Sg <= SW_NW && strm_lds <= 65536 && off32) {
+        if (full_adjoint() && li == 0) {
             // full grid, one launch per phase.  Level n reads the memory-variable cotangents
             // that level n+1 wrote: the two sets alternate with the parity of n.
             El<T> Ar = A;
@@ -2446,7 +2454,7 @@ template <typename T> struct ElPlan : ElPlanBase {
         ADS_TRY(check_ws(w, ws, wsb, K));
         const size_t F = (size_t)nshots * Npad * sizeof(T);
         ADS_HIP(hipMemsetAsync(w.adj, 0, F * NF, st));
-        if (strm == 2) ADS_HIP(hipMemsetAsync(w.adjm, 0, F * 8, st));
+        if (full_adjoint()) ADS_HIP(hipMemsetAsync(w.adjm, 0, F * 8, st));
         ADS_HIP(hipMemsetAsync(w.grad, 0, (size_t)G_COUNT * adj_groups() * Npad * sizeof(T), st));
         ADS_HIP(hipMemsetAsync(w.tmp, 0, F * 4, st));
         el_rho_bar<T><<<dim3(cdiv(Pitch, 256), NX + 2), 256, 0, st>>>(w.rho, w.rb, (int)NX, (int)NY, (int)Pitch, sub);

@@ -244,15 +244,18 @@ def test_mpi_elastic_block_plan_and_dropin_f64(orc, ads, cuda):
         assert rel(t.grad.cpu(), b) < 1e-10, (name, rel(t.grad.cpu(), b))
 
 
-@pytest.mark.parametrize("kernels", [(2, 8, 4), (2, 4, 8), (2, 3, 2), (2, 32, 1), (1, 8, 4),
-                                     (1, 4, 8), (1, 3, 2), (1, 32, 1), (0, 16, 4)])
+@pytest.mark.parametrize("kernels", [(2, 8, 4), (2, 4, 8), (2, 3, 2), (2, 32, 1), (2, 16, 4),
+                                     (1, 8, 4), (1, 4, 8), (1, 3, 2), (1, 32, 1), (0, 16, 4)])
 def test_elastic_adjoint_rectangle_kernels_f64(orc, ads, cuda, kernels):
     """Every adjoint kernel variant gives the oracle's gradient, and a second gradient on the
     same plan is bit-identical to the first (no order-dependent sums): the full-grid streaming
     kernels (2), the streaming rectangle kernels with frame_b in the same launch (4 shots per
     group) or launched apart (8, 2, 1 shots per group; 5 uneven shots, so the last group is
     partial and, near the end, fewer shots are active than a group holds), chunk heights that
-    do not divide the grid or the rectangle, and the LDS-tiled kernels (0)."""
+    do not divide the grid or the rectangle, and the LDS-tiled kernels (0).  (2, 16, 4) in FP64
+    needs more LDS than the streaming kernels may take with 4 shots: the plan then takes the
+    tiled and, with fewer active shots, the streaming rectangle kernels, never the full-grid
+    ones, whose memory-variable cotangents are kept apart."""
     p, models, srcs, rcvs, nstep = make_case(orc, ads, NX=41, NY=37, NSTEP=50, nshots=5,
                                              uneven=True, seed=3)
     obs, g_ref = [], [0.0, 0.0, 0.0]
