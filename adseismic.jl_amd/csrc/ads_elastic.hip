@@ -66,6 +66,10 @@ template <typename T> ElC<T> cst(const El<T> &a)
 template <typename T> struct ElCoef {
     T dt, ihx, ihy;  // 1/(24 hx), 1/(24 hy)
     const T *axf, *axh, *bxf, *bxh, *ayf, *ayh, *byf, *byh;
+    // the same coefficients packed per padded row r (prow[4 (r + 2) + 0..3] = axf, axh, bxf,
+    // bxh for r in [-2, NX+3], 0 off the profile range) and column (pcol[4 j + 0..3] = ayf,
+    // ayh, byf, byh for j in [0, NY+1])
+    const T *prow, *pcol;
     int NX, NY, P;
     // sub = 1: one MPIElastic subdomain (MPIElastic.jl:483-620): a block of (NX-2) x (NY-2)
     // cells at [2, NX-1] x [2, NY-1] inside a 2-cell zero halo; every fw updates the whole
@@ -1148,36 +1152,51 @@ template <typename T> struct ElPar {
 template <typename T> struct ProfRow {  // CPML coefficients of a row (x) or column (y), 0 off range
     T af, ah, bf, bh;
 };
-template <typename T> __device__ inline ProfRow<T> prof_row(const ElCoef<T> &c, int r)
+template <typename T> __device__ inline ProfRow<T> prof_row(const ElCoef<T> &c, int r)  // r in [-2, NX+3]
 {
-    const int pi = r - 1 - c.sub;
-    const bool ok = r >= 1 + c.sub && r <= c.NX - c.sub;
-    return ProfRow<T>{ok ? c.axf[pi] : T(0), ok ? c.axh[pi] : T(0), ok ? c.bxf[pi] : T(0), ok ? c.bxh[pi] : T(0)};
+    const T *p = c.prow + 4 * (r + 2);
+    return ProfRow<T>{p[0], p[1], p[2], p[3]};
 }
-template <typename T> __device__ inline ProfRow<T> prof_col(const ElCoef<T> &c, int j)
+template <typename T> __device__ inline ProfRow<T> prof_col(const ElCoef<T> &c, int j)  // j in [0, NY+1]
 {
-    const int pj = j - 1 - c.sub;
-    const bool ok = j >= 1 + c.sub && j <= c.NY - c.sub;
-    return ProfRow<T>{ok ? c.ayf[pj] : T(0), ok ? c.ayh[pj] : T(0), ok ? c.byf[pj] : T(0), ok ? c.byh[pj] : T(0)};
+    const T *p = c.pcol + 4 * j;
+    return ProfRow<T>{p[0], p[1], p[2], p[3]};
 }
 // The update ranges of fw_in split into row and column parts: fw1 = A_row & A_col, fw2 = B_row
 // & B_col, fw3 = B_row & A_col, fw4 = A_row & B_col.
 struct FwPart {
     bool A, B;
 };
-template <typename T> __device__ inline FwPart fw_row(const ElCoef<T> &c, int i)
+template <int SUB, typename T> __device__ inline FwPart fw_row(const ElCoef<T> &c, int i)
 {
-    if (c.sub) return FwPart{i >= 2 && i <= c.NX - 1, i >= 2 && i <= c.NX - 1};
+    if constexpr (SUB) return FwPart{i >= 2 && i <= c.NX - 1, i >= 2 && i <= c.NX - 1};
     return FwPart{i >= 1 && i <= c.NX - 1, i >= 2 && i <= c.NX};
 }
-template <typename T> __device__ inline FwPart fw_col(const ElCoef<T> &c, int j)
+template <int SUB, typename T> __device__ inline FwPart fw_col(const ElCoef<T> &c, int j)
 {
-    if (c.sub) return FwPart{j >= 2 && j <= c.NY - 1, j >= 2 && j <= c.NY - 1};
+    if constexpr (SUB) return FwPart{j >= 2 && j <= c.NY - 1, j >= 2 && j <= c.NY - 1};
     return FwPart{j >= 2 && j <= c.NY, j >= 1 && j <= c.NY - 1};
+}
+// Index of (i, j) among the cells outside the box mb (FrameBox::index) as a row part and a
+// column part: rows above and below the box are whole rows, the rows beside it keep the qj0
+// columns to its left and the W - qj1 to its right.  32-bit: one field of one shot.
+struct MvCol {
+    int full, mid;  // the column's offset in a whole row / in a row beside the box (-1: inside)
+};
+__device__ inline MvCol mv_col(const FrameBox &b, int j)
+{
+    return MvCol{j, j < b.qj0 ? j : (j >= b.qj1 ? b.qj0 + (j - b.qj1) : -1)};
+}
+__device__ inline int mv_at(const FrameBox &b, const MvCol &cj, int i)
+{
+    const int top = b.qi0 * b.W, wm = b.qj0 + (b.W - b.qj1);
+    if (i < b.qi0) return i * b.W + cj.full;
+    if (i >= b.qi1) return top + (b.qi1 - b.qi0) * wm + (i - b.qi1) * b.W + cj.full;
+    return cj.mid < 0 ? -1 : top + (i - b.qi0) * wm + cj.mid;
 }
 
 // velocity phase, full grid: fw3/fw4 VJP on every cell
-template <typename T>
+template <int SUB, typename T>
 __device__ inline void adjv_full_body(const ElAdj<T> &a, const ElCoef<T> &c, const ElPar<T> &q,
                                       const StrmMap &mp, int bid)
 {
@@ -1214,8 +1233,9 @@ __device__ inline void adjv_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
         const uint32_t pb = (uint32_t)P * (uint32_t)sizeof(T), jb4 = (uint32_t)jc * (uint32_t)sizeof(T);
         auto walk = [&](auto genc) {
             constexpr bool GEN = decltype(genc)::value;
-            const ProfRow<T> py = GEN ? prof_col(c, j) : ProfRow<T>{};
-            const FwPart fc = GEN ? fw_col(c, j) : FwPart{true, true};
+            const ProfRow<T> py = GEN ? prof_col(c, jc) : ProfRow<T>{};
+            const FwPart fc = GEN ? fw_col<SUB>(c, j) : FwPart{true, true};
+            const MvCol mc = mv_col(a.Sn.mb, jc);
             // raw loads of one window row, and of one output row
             struct Rw { T a, b, x, y, r, q, m5, m7; ProfRow<T> px; };
             struct Ro { T yy, m6, m8, gxx, gxy, gyy, r, q; };
@@ -1254,7 +1274,7 @@ __device__ inline void adjv_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                 e3w[k] = e3;
                 e4w[k] = e4;
                 if constexpr (GEN) {
-                    const FwPart fr = fw_row(c, r);
+                    const FwPart fr = fw_row<SUB>(c, r);
                     const T g5 = o.m5 + e3, g7 = o.m7 + e4;
                     g5w[k] = g5;
                     g7w[k] = g7;
@@ -1283,7 +1303,7 @@ __device__ inline void adjv_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                     fetch_o(i + 1, on);
                 }
                 finish(i + 2, rc, 4);
-                const FwPart fr = GEN ? fw_row(c, i) : FwPart{true, true};
+                const FwPart fr = GEN ? fw_row<SUB>(c, i) : FwPart{true, true};
                 const bool f3 = fr.B && fc.A, f4 = fr.A && fc.B;
                 const T e3 = e3w[2], e4 = e4w[2];
                 // row i, y-direction t-values
@@ -1301,7 +1321,7 @@ __device__ inline void adjv_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                     // variables of level n outside the damping-free box
                     T m5 = T(0), m6 = T(0), m7 = T(0), m8 = T(0);
                     if constexpr (GEN) {
-                        const int64_t mi = a.Sn.mv(i, j, (int64_t)i * P + j);
+                        const int mi = mv_at(a.Sn.mb, mc, i);
                         if (mi >= 0) {
                             m5 = a.Sn.f(M5, s)[mi];
                             m6 = a.Sn.f(M6, s)[mi];
@@ -1361,7 +1381,7 @@ __device__ inline void adjv_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
 }
 
 // stress phase, full grid: fw1/fw2 VJP on every cell
-template <typename T>
+template <int SUB, typename T>
 __device__ inline void adjs_full_body(const ElAdj<T> &a, const ElCoef<T> &c, const ElPar<T> &q,
                                       const StrmMap &mp, int bid)
 {
@@ -1397,8 +1417,9 @@ __device__ inline void adjs_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
         const uint32_t pb = (uint32_t)P * (uint32_t)sizeof(T), jb4 = (uint32_t)jc * (uint32_t)sizeof(T);
         auto walk = [&](auto genc) {
             constexpr bool GEN = decltype(genc)::value;
-            const ProfRow<T> py = GEN ? prof_col(c, j) : ProfRow<T>{};
-            const FwPart fc = GEN ? fw_col(c, j) : FwPart{true, true};
+            const ProfRow<T> py = GEN ? prof_col(c, jc) : ProfRow<T>{};
+            const FwPart fc = GEN ? fw_col<SUB>(c, j) : FwPart{true, true};
+            const MvCol mc = mv_col(a.Sn.mb, jc);
             // raw loads of one window row (the stress cotangents, the averaged parameters, the
             // memory-variable cotangents of x-profile rows, vx and vy of level n-1), and of one
             // output row (its cotangents and parameters again, with those of y-profile
@@ -1446,7 +1467,7 @@ __device__ inline void adjs_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                 const T gX = lm * gx + o.lb * gy;
                 const T gs = o.xy * o.mb2 * dt;
                 if constexpr (GEN) {
-                    const FwPart fr = fw_row(c, r);
+                    const FwPart fr = fw_row<SUB>(c, r);
                     w0[k] = fr.A && fc.A ? gX + o.px.ah * (o.m1 + gX) : T(0);
                     w2[k] = fr.B && fc.B ? gs + o.px.af * (o.m3 + gs) : T(0);
                 } else {
@@ -1472,7 +1493,7 @@ __device__ inline void adjs_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                     fetch_o(i + 1, on);
                 }
                 finish(i + 2, rc, 4);
-                const FwPart fr = GEN ? fw_row(c, i) : FwPart{true, true};
+                const FwPart fr = GEN ? fw_row<SUB>(c, i) : FwPart{true, true};
                 const bool f1 = fr.A && fc.A, f2 = fr.B && fc.B;
                 // row i: the cotangent products of fw1 / fw2 and the y-direction t-values
                 const T gx = oc.xx * dt, gy = oc.yy * dt;
@@ -1492,7 +1513,7 @@ __device__ inline void adjs_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                     // velocities, with the forward memory variables of level n outside the box
                     T m1 = T(0), m2 = T(0), m3 = T(0), m4 = T(0);
                     if constexpr (GEN) {
-                        const int64_t mi = a.Sn.mv(i, j, (int64_t)i * P + j);
+                        const int mi = mv_at(a.Sn.mb, mc, i);
                         if (mi >= 0) {
                             m1 = a.Sn.f(M1, s)[mi];
                             m2 = a.Sn.f(M2, s)[mi];
@@ -1549,15 +1570,15 @@ __device__ inline void adjs_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
         g_at(a, G_MB2, g, ij) += s2;
     }
 }
-template <typename T>
+template <typename T, int SUB>
 __global__ __launch_bounds__(WAVE *SW_NW) void el_adjv_full(ElAdj<T> a, ElCoef<T> c, ElPar<T> q, StrmMap mp)
 {
-    adjv_full_body(a, c, q, mp, (int)blockIdx.x);
+    adjv_full_body<SUB>(a, c, q, mp, (int)blockIdx.x);
 }
-template <typename T>
+template <typename T, int SUB>
 __global__ __launch_bounds__(WAVE *SW_NW) void el_adjs_full(ElAdj<T> a, ElCoef<T> c, ElPar<T> q, StrmMap mp)
 {
-    adjs_full_body(a, c, q, mp, (int)blockIdx.x);
+    adjs_full_body<SUB>(a, c, q, mp, (int)blockIdx.x);
 }
 
 // velocity phase: fw3/fw4 VJP on rectangle cells
@@ -2039,6 +2060,8 @@ template <typename T> struct ElPlan : ElPlanBase {
         c.ayh = b + 4 * PX + PY;
         c.byf = b + 4 * PX + 2 * PY;
         c.byh = b + 4 * PX + 3 * PY;
+        c.prow = b + 4 * PX + 4 * PY;
+        c.pcol = c.prow + 4 * (NX + 6);
         c.NX = (int)NX;
         c.NY = (int)NY;
         c.P = P;
@@ -2378,10 +2401,12 @@ template <typename T> struct ElPlan : ElPlanBase {
             const unsigned sblock = (unsigned)(WAVE * a.Sg);
             const size_t lds = (size_t)a.Sg * sm.ch * WAVE * sizeof(T);  // x terms
             int pr = prof_start(ADS_PROF_EL_ADJ_VEL, st);
-            el_adjv_full<T><<<sgrid, sblock, 2 * lds, st>>>(a, c, par, sm);
+            if (sub) el_adjv_full<T, 1><<<sgrid, sblock, 2 * lds, st>>>(a, c, par, sm);
+            else el_adjv_full<T, 0><<<sgrid, sblock, 2 * lds, st>>>(a, c, par, sm);
             prof_stop(pr, st, cells * sizeof(T) * (12.0 + 4.0 * (f_x + f_y)));
             pr = prof_start(ADS_PROF_EL_ADJ_STRESS, st);
-            el_adjs_full<T><<<sgrid, sblock, 3 * lds, st>>>(a, c, par, sm);
+            if (sub) el_adjs_full<T, 1><<<sgrid, sblock, 3 * lds, st>>>(a, c, par, sm);
+            else el_adjs_full<T, 0><<<sgrid, sblock, 3 * lds, st>>>(a, c, par, sm);
             prof_stop(pr, st, cells * sizeof(T) * (12.0 + 4.0 * (f_x + f_y)));
             ADS_LAUNCH_CHECK();
             return ADS_OK;
@@ -2698,6 +2723,22 @@ static int build_el_plan(const ads_elastic_params *p, const ads_shot_geometry *g
     for (int64_t k = 0; k < 2 * p->NX; k++) prof.push_back((T)bx[k]);  // bxf, bxh
     for (int64_t k = 0; k < 2 * p->NY; k++) prof.push_back((T)ay[k]);  // ayf, ayh
     for (int64_t k = 0; k < 2 * p->NY; k++) prof.push_back((T)by[k]);  // byf, byh
+    for (int64_t r = -2; r <= P->NX + 3; r++) {  // packed per padded row
+        const int64_t q = r - 1 - sub;
+        const bool ok = r >= 1 + sub && r <= P->NX - sub;
+        prof.push_back(ok ? (T)ax[q] : T(0));
+        prof.push_back(ok ? (T)ax[p->NX + q] : T(0));
+        prof.push_back(ok ? (T)bx[q] : T(0));
+        prof.push_back(ok ? (T)bx[p->NX + q] : T(0));
+    }
+    for (int64_t j = 0; j <= P->NY + 1; j++) {  // packed per padded column
+        const int64_t q = j - 1 - sub;
+        const bool ok = j >= 1 + sub && j <= P->NY - sub;
+        prof.push_back(ok ? (T)ay[q] : T(0));
+        prof.push_back(ok ? (T)ay[p->NY + q] : T(0));
+        prof.push_back(ok ? (T)by[q] : T(0));
+        prof.push_back(ok ? (T)by[p->NY + q] : T(0));
+    }
     ADS_TRY(P->d_prof.upload(prof));
     {  // the box where every a-profile vanishes: the memory variables stay 0 inside it
         std::vector<double> px((size_t)P->NX + 2, 1.0), py((size_t)P->NY + 2, 1.0);
