@@ -426,6 +426,165 @@ __device__ inline void el_vel_rect_body(const ElC<T> &I, const El<T> &O, const T
     }
 }
 
+// Frame work in streaming form (the fused forward launches).  An item is nr rows of at most
+// SW_FCOLS columns [m.z, m.w) outside the rectangle; lane L holds column m.z - 2 + L, so two
+// lanes on each side carry the columns the y-direction stencils reach.  For its shot the
+// wavefront walks the item's rows, holding the x-direction stencil inputs of the rows around
+// the current one in registers (one new row loaded per step) and taking the y-direction
+// neighbours from the adjacent lanes; each cell is el_stress_cell / el_vel_cell, the same
+// expressions, with its memory variables at their index in the level's layout.
+// columns / rows per streaming frame item (measured at 600^2 x 16 shots, forward level: 1, 2,
+// 4, 8, 16, 32 rows 128.2, 118.8, 115.7, 113.2, 121.4, 169.2 us: taller items leave too few
+// wavefronts, shorter ones re-read their halo rows)
+constexpr int SW_FCOLS = WAVE - 4, FRS_ROWS = 8;
+template <typename T> __device__ inline T fsh(T v, int src) { return __shfl(v, src, WAVE); }
+
+template <typename T>
+__device__ inline void stress_frame_strm(const ElC<T> &I, const El<T> &O, const T *__restrict__ lam,
+                                         const T *__restrict__ mu, const ElCoef<T> &c, int s, int4 m)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int j = m.z - 2 + lane;
+    const int P = c.P, NXp = c.NX + 1, NYp = c.NY + 1;
+    const int jc = j < 0 ? 0 : (j > NYp ? NYp : j);  // loads of lanes beyond the grid stay inside it
+    const bool own = lane >= 2 && j < m.w;
+    const int pj = j - 1 - c.sub;
+    const T *__restrict__ vx = I.f(VX, s);
+    const T *__restrict__ vy = I.f(VY, s);
+    // row r of an array (0 outside the grid; such values only reach cells outside fw1 / fw2)
+    auto rowv = [&](const T *__restrict__ f, int r) { return r >= 0 && r <= NXp ? f[(int64_t)r * P + jc] : T(0); };
+    const int i0 = m.x, i1 = m.x + m.y;
+    // windows: vx rows x-1 .. x+2 (X0 .. X3), vy rows x-2 .. x+1 (Y0 .. Y3), lam and mu rows x, x+1
+    T X0 = rowv(vx, i0 - 1), X1 = rowv(vx, i0), X2 = rowv(vx, i0 + 1), X3;
+    T Y0 = rowv(vy, i0 - 2), Y1 = rowv(vy, i0 - 1), Y2 = rowv(vy, i0), Y3;
+    T L0 = rowv(lam, i0), L1, U0 = rowv(mu, i0), U1;
+    for (int i = i0; i < i1; i++) {
+        X3 = rowv(vx, i + 2);
+        Y3 = rowv(vy, i + 1);
+        L1 = rowv(lam, i + 1);
+        U1 = rowv(mu, i + 1);
+        // row i's y-direction neighbours: vy at j-2 .. j+1, vx at j-1 .. j+2, mu at j+1
+        const T vym = fsh(Y2, lane - 1), vyp = fsh(Y2, lane + 1), vymm = fsh(Y2, lane - 2);
+        const T vxm = fsh(X1, lane - 1), vxp = fsh(X1, lane + 1), vxpp = fsh(X1, lane + 2);
+        const T mue = fsh(U0, lane + 1);
+        if (own) {
+            const int64_t ij = (int64_t)i * P + j;
+            T sxx = I.f(SXX, s)[ij], syy = I.f(SYY, s)[ij], sxy = I.f(SXY, s)[ij];
+            T m1 = 0, m2 = 0, m3 = 0, m4 = 0;
+            const int64_t mi = I.mv(i, j, ij), mo = O.mv(i, j, ij);  // memory variables (< 0: all 0)
+            auto mold = [&](int q) { return mi >= 0 ? I.f(q, s)[mi] : T(0); };
+            const int pi = i - 1 - c.sub;  // profile index
+            if (fw_in(c, 1, i, j)) {
+                const T lb = c.sub ? L0 : T(0.5) * (L1 + L0);
+                const T mb = c.sub ? U0 : T(0.5) * (U1 + U0);
+                const T lm = lb + T(2) * mb;
+                T dx = (T(27) * X2 - T(27) * X1 - X3 + X0) * c.ihx;
+                T dy = (T(27) * Y2 - T(27) * vym - vyp + vymm) * c.ihy;
+                m1 = c.bxh[pi] * mold(M1) + c.axh[pi] * dx;
+                m2 = c.byf[pj] * mold(M2) + c.ayf[pj] * dy;
+                dx += m1;
+                dy += m2;
+                sxx += (lm * dx + lb * dy) * c.dt;
+                syy += (lm * dy + lb * dx) * c.dt;
+            }
+            if (fw_in(c, 2, i, j)) {
+                const T mb = c.sub ? U0 : T(0.5) * (U0 + mue);
+                T dx = (T(27) * Y2 - T(27) * Y1 - Y3 + Y0) * c.ihx;
+                T dy = (T(27) * vxp - T(27) * X1 - vxpp + vxm) * c.ihy;
+                m3 = c.bxf[pi] * mold(M3) + c.axf[pi] * dx;
+                m4 = c.byh[pj] * mold(M4) + c.ayh[pj] * dy;
+                sxy += mb * (dx + m3 + dy + m4) * c.dt;
+            }
+            O.f(SXX, s)[ij] = sxx;
+            O.f(SYY, s)[ij] = syy;
+            O.f(SXY, s)[ij] = sxy;
+            if (mo >= 0) {
+                O.f(M1, s)[mo] = m1;
+                O.f(M2, s)[mo] = m2;
+                O.f(M3, s)[mo] = m3;
+                O.f(M4, s)[mo] = m4;
+            }
+        }
+        X0 = X1;
+        X1 = X2;
+        X2 = X3;
+        Y0 = Y1;
+        Y1 = Y2;
+        Y2 = Y3;
+        L0 = L1;
+        U0 = U1;
+    }
+}
+
+template <typename T>
+__device__ inline void vel_frame_strm(const ElC<T> &I, const El<T> &O, const T *__restrict__ rho,
+                                      const ElCoef<T> &c, int s, int4 m)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int j = m.z - 2 + lane;
+    const int P = c.P, NXp = c.NX + 1, NYp = c.NY + 1;
+    const int jc = j < 0 ? 0 : (j > NYp ? NYp : j);
+    const bool own = lane >= 2 && j < m.w;
+    const int pj = j - 1 - c.sub;
+    const T *__restrict__ sxx = O.f(SXX, s);
+    const T *__restrict__ syy = O.f(SYY, s);
+    const T *__restrict__ sxy = O.f(SXY, s);
+    auto rowv = [&](const T *__restrict__ f, int r) { return r >= 0 && r <= NXp ? f[(int64_t)r * P + jc] : T(0); };
+    const int i0 = m.x, i1 = m.x + m.y;
+    // windows: sxx rows x-2 .. x+1 (A0 .. A3), sxy rows x-1 .. x+2 (B0 .. B3), rho rows x, x+1
+    T A0 = rowv(sxx, i0 - 2), A1 = rowv(sxx, i0 - 1), A2 = rowv(sxx, i0), A3;
+    T B0 = rowv(sxy, i0 - 1), B1 = rowv(sxy, i0), B2 = rowv(sxy, i0 + 1), B3;
+    T R0 = rowv(rho, i0), R1;
+    for (int i = i0; i < i1; i++) {
+        A3 = rowv(sxx, i + 1);
+        B3 = rowv(sxy, i + 2);
+        R1 = rowv(rho, i + 1);
+        const T YY = rowv(syy, i);
+        // row i's y-direction neighbours: sxy at j-2 .. j+1, syy at j-1 .. j+2, rho at j+1
+        const T xym = fsh(B1, lane - 1), xyp = fsh(B1, lane + 1), xymm = fsh(B1, lane - 2);
+        const T yym = fsh(YY, lane - 1), yyp = fsh(YY, lane + 1), yypp = fsh(YY, lane + 2);
+        const T re0 = fsh(R0, lane + 1), re1 = fsh(R1, lane + 1);
+        if (own) {
+            const int64_t ij = (int64_t)i * P + j;
+            T vxn = I.f(VX, s)[ij], vyn = I.f(VY, s)[ij];
+            T m5 = 0, m6 = 0, m7 = 0, m8 = 0;
+            const int64_t mi = I.mv(i, j, ij), mo = O.mv(i, j, ij);  // memory variables (< 0: all 0)
+            auto mold = [&](int q) { return mi >= 0 ? I.f(q, s)[mi] : T(0); };
+            const int pi = i - 1 - c.sub;  // profile index
+            if (fw_in(c, 3, i, j)) {
+                T dx = (T(27) * A2 - T(27) * A1 - A3 + A0) * c.ihx;
+                T dy = (T(27) * B1 - T(27) * xym - xyp + xymm) * c.ihy;
+                m5 = c.bxf[pi] * mold(M5) + c.axf[pi] * dx;
+                m6 = c.byf[pj] * mold(M6) + c.ayf[pj] * dy;
+                vxn += (dx + m5 + dy + m6) * c.dt / R0;
+            }
+            if (fw_in(c, 4, i, j)) {
+                const T rb = c.sub ? R0 : T(0.25) * (R0 + R1 + re1 + re0);
+                T dx = (T(27) * B2 - T(27) * B1 - B3 + B0) * c.ihx;
+                T dy = (T(27) * yyp - T(27) * YY - yypp + yym) * c.ihy;
+                m7 = c.bxh[pi] * mold(M7) + c.axh[pi] * dx;
+                m8 = c.byh[pj] * mold(M8) + c.ayh[pj] * dy;
+                vyn += (dx + m7 + dy + m8) * c.dt / rb;
+            }
+            O.f(VX, s)[ij] = vxn;
+            O.f(VY, s)[ij] = vyn;
+            if (mo >= 0) {
+                O.f(M5, s)[mo] = m5;
+                O.f(M6, s)[mo] = m6;
+                O.f(M7, s)[mo] = m7;
+                O.f(M8, s)[mo] = m8;
+            }
+        }
+        A0 = A1;
+        A1 = A2;
+        A2 = A3;
+        B0 = B1;
+        B1 = B2;
+        B2 = B3;
+        R0 = R1;
+    }
+}
+
 // Kernels: the rectangle alone, and the rectangle with the frame in one launch (pitched
 // layout): workgroups [0, nfb) take the frame items (nib item blocks of NWAVES items per shot),
 // the rest the rectangle tiles, so the latency-bound frame work overlaps the rectangle's.
@@ -456,11 +615,8 @@ __global__ __launch_bounds__(WAVE *NWAVES) void el_stress_level(ElC<T> I, El<T> 
         return;
     }
     const int it = (b % nib) * NWAVES + wave_id(), s = b / nib;
-    if (it >= nitems || s >= act) return;
-    const int4 m = items[it];
-    const int j = m.z + (threadIdx.x & (WAVE - 1));
-    if (j >= m.w) return;
-    for (int r = 0; r < m.y; r++) el_stress_cell(I, O, lam, mu, c, s, m.x + r, j);
+    if (it >= nitems || s >= act) return;  // wave-uniform
+    stress_frame_strm(I, O, lam, mu, c, s, items[it]);
 }
 template <typename T, int RPT, int VEC>
 __global__ __launch_bounds__(WAVE *NWAVES) void el_vel_level(ElC<T> I, El<T> O, const T *__restrict__ rho,
@@ -474,11 +630,8 @@ __global__ __launch_bounds__(WAVE *NWAVES) void el_vel_level(ElC<T> I, El<T> O, 
         return;
     }
     const int it = (b % nib) * NWAVES + wave_id(), s = b / nib;
-    if (it >= nitems || s >= act) return;
-    const int4 m = items[it];
-    const int j = m.z + (threadIdx.x & (WAVE - 1));
-    if (j >= m.w) return;
-    for (int r = 0; r < m.y; r++) el_vel_cell(I, O, rho, c, s, m.x + r, j);
+    if (it >= nitems || s >= act) return;  // wave-uniform
+    vel_frame_strm(I, O, rho, c, s, items[it]);
 }
 
 // Sources at level n (unique (cell, type) keys, entries in source order), then receivers:
@@ -2029,6 +2182,8 @@ struct ElPlanBase {
     Rect rect[2]{};
     DevBuf d_items[2];
     int nitems[2] = {0, 0};
+    DevBuf d_sitems;  // the pitched layout's frame as streaming items (FRS_ROWS x SW_FCOLS)
+    int nsitems = 0;
     DevBuf d_ring[2];  // the rectangle's cells within two of its edge (adjoint t-values)
     int nring[2] = {0, 0};
     int rpt = 1, sg = 4;  // forward rectangle kernels: rows per wavefront, shots per workgroup
@@ -2194,15 +2349,15 @@ template <typename T> struct ElPlan : ElPlanBase {
         m.ngroups = (int)cdiv(act, Sg);
         m.nb = m.ntj * m.nstrips * m.ngroups;
         const unsigned nb = cdiv(m.nb, NXCD) * NXCD;
-        if (fuse) {
-            const int nib = (int)cdiv(nitems[li], NWAVES), nfb = nib * act;
-            const int4 *it = d_items[li].template as<int4>();
+        if (fuse) {  // the frame as streaming items (the pitched layout's rectangle)
+            const int nib = (int)cdiv(nsitems, NWAVES), nfb = nib * act;
+            const int4 *it = d_sitems.template as<int4>();
             if (phase == 0)
                 el_stress_level<T, RPT, VEC><<<nb + nfb, WAVE * NWAVES, 0, st>>>(I, O, lam, mu, c, R, m, act, Sg,
-                                                                                it, nitems[li], nib, nfb);
+                                                                                it, nsitems, nib, nfb);
             else
                 el_vel_level<T, RPT, VEC><<<nb + nfb, WAVE * NWAVES, 0, st>>>(I, O, rho, c, R, m, act, Sg,
-                                                                             it, nitems[li], nib, nfb);
+                                                                             it, nsitems, nib, nfb);
             return;
         }
         if (phase == 0) el_stress_rect<T, RPT, VEC><<<nb, WAVE * NWAVES, 0, st>>>(I, O, lam, mu, c, R, m, act, Sg);
@@ -2774,6 +2929,12 @@ static int build_el_plan(const ads_elastic_params *p, const ads_shot_geometry *g
             split_grid(px, py, NX, NY, li == 0 ? ElPlan<T>::VECP : 1, P->rect[li], items);
             P->nitems[li] = (int)items.size();
             ADS_TRY(P->d_items[li].upload(items));
+            if (li == 0 && P->nitems[0] > 0) {
+                std::vector<int4> sitems;
+                frame_items(P->rect[0], NX, NY, sitems, FRS_ROWS, SW_FCOLS);
+                P->nsitems = (int)sitems.size();
+                ADS_TRY(P->d_sitems.upload(sitems));
+            }
             ring_items(P->rect[li], 2, items);
             P->nring[li] = (int)items.size();
             if (!items.empty()) ADS_TRY(P->d_ring[li].upload(items));
