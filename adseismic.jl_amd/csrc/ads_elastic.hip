@@ -642,15 +642,14 @@ __global__ __launch_bounds__(WAVE *NWAVES) void el_vel_level(ElC<T> I, El<T> O, 
 // gives it back in mode 2, so the adjoint sees the exact pre-injection value; without it,
 // mode 2 subtracts the sources again, which rounds in FP32.
 template <typename T>
-__global__ __launch_bounds__(256) void el_src_rcv(
-    El<T> O, int n, const int64_t *__restrict__ us_ptr, const int64_t *__restrict__ us_key,
+__device__ inline void el_src_rcv_body(
+    const El<T> &O, int n, const int64_t *__restrict__ us_ptr, const int64_t *__restrict__ us_key,
     const int64_t *__restrict__ us_eptr, const int64_t *__restrict__ us_ent,
     const T *__restrict__ srcv, const int64_t *__restrict__ srcv_off,
     const int64_t *__restrict__ nsrc, const int64_t *__restrict__ rcv_ptr,
     const int64_t *__restrict__ rcv_key, const int64_t *__restrict__ nstep,
-    T *__restrict__ traces, const int64_t *__restrict__ trace_off, int mode, T *__restrict__ pre)
+    T *__restrict__ traces, const int64_t *__restrict__ trace_off, int mode, T *__restrict__ pre, int s)
 {
-    const int s = blockIdx.x;
     const int64_t ns = nsrc[s];
     const T *__restrict__ row = srcv + srcv_off[s] + (int64_t)(n - 1) * ns;
     for (int64_t c = us_ptr[s] + threadIdx.x; c < us_ptr[s + 1]; c += blockDim.x) {
@@ -676,6 +675,18 @@ __global__ __launch_bounds__(256) void el_src_rcv(
         const int64_t key = rcv_key[r0 + r];
         tr[r * nt + n] = O.f((int)(key & 7), s)[key >> 3];
     }
+}
+template <typename T>
+__global__ __launch_bounds__(256) void el_src_rcv(
+    El<T> O, int n, const int64_t *__restrict__ us_ptr, const int64_t *__restrict__ us_key,
+    const int64_t *__restrict__ us_eptr, const int64_t *__restrict__ us_ent,
+    const T *__restrict__ srcv, const int64_t *__restrict__ srcv_off,
+    const int64_t *__restrict__ nsrc, const int64_t *__restrict__ rcv_ptr,
+    const int64_t *__restrict__ rcv_key, const int64_t *__restrict__ nstep,
+    T *__restrict__ traces, const int64_t *__restrict__ trace_off, int mode, T *__restrict__ pre)
+{
+    el_src_rcv_body(O, n, us_ptr, us_key, us_eptr, us_ent, srcv, srcv_off, nsrc, rcv_ptr, rcv_key, nstep,
+                    traces, trace_off, mode, pre, (int)blockIdx.x);
 }
 
 // ------------------------------- adjoint -------------------------------------------------
@@ -2117,15 +2128,28 @@ __global__ void el_grad_map(const T *__restrict__ G, int64_t npad, int64_t gstri
     gmu[d] = m;
 }
 
-// cotangent injection of the trace residual at level n (unique (cell, type) per shot)
+// Start of an adjoint level, one launch: blocks [0, nxi) inject the trace residual of level
+// n into the cotangents (unique (cell, type) per shot, blockIdx.y = shot), block nxi puts
+// back the pre-injection stresses of the level's state (el_src_rcv mode 2).  The two touch
+// different arrays.
+struct SrcTab {
+    const int64_t *us_ptr, *us_key, *us_eptr, *us_ent, *srcv_off, *nsrc, *rcv_ptr, *rcv_key, *nstep,
+        *trace_off;
+};
 template <typename T>
-__global__ __launch_bounds__(256) void el_adj_inject(
-    El<T> A, int n, const int64_t *__restrict__ ur_ptr, const int64_t *__restrict__ ur_key,
+__global__ __launch_bounds__(256) void el_adj_start(
+    El<T> A, El<T> Sn, int n, const int64_t *__restrict__ ur_ptr, const int64_t *__restrict__ ur_key,
     const int64_t *__restrict__ ur_eptr, const int64_t *__restrict__ ur_ent,
     const int64_t *__restrict__ nstep, const T *__restrict__ resid,
-    const int64_t *__restrict__ trace_off)
+    const int64_t *__restrict__ trace_off, int nxi, SrcTab st, const T *__restrict__ srcv,
+    T *__restrict__ pre)
 {
     const int s = blockIdx.y;
+    if ((int)blockIdx.x >= nxi) {
+        el_src_rcv_body(Sn, n, st.us_ptr, st.us_key, st.us_eptr, st.us_ent, srcv, st.srcv_off, st.nsrc,
+                        st.rcv_ptr, st.rcv_key, st.nstep, (T *)nullptr, st.trace_off, 2, pre, s);
+        return;
+    }
     const int64_t c = ur_ptr[s] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ur_ptr[s + 1]) return;
     const int64_t key = ur_key[c], nt = nstep[s] + 1;
@@ -2479,19 +2503,19 @@ template <typename T> struct ElPlan : ElPlanBase {
     {
         if (act <= 0) return ADS_OK;
         const ElCoef<T> c = coef(P);
-        if (resid && max_ur > 0) {
-            dim3 g(cdiv(max_ur, 256), act);
-            el_adj_inject<T><<<g, 256, 0, st>>>(A, n, d_ur_ptr.as<int64_t>(), d_ur_key[li].template as<int64_t>(),
-                                                d_ur_eptr.as<int64_t>(), d_ur_ent.as<int64_t>(),
-                                                d_nstep.as<int64_t>(), resid, d_trace_off.as<int64_t>());
+        {
+            const int nxi = resid && max_ur > 0 ? (int)cdiv(max_ur, 256) : 0;
+            const SrcTab tab{d_us_ptr.as<int64_t>(),    d_us_key[li].template as<int64_t>(),
+                             d_us_eptr.as<int64_t>(),   d_us_ent.as<int64_t>(),
+                             d_srcv_off.as<int64_t>(),  d_nsrc.as<int64_t>(),
+                             d_rcv_ptr.as<int64_t>(),   d_rcv_key[li].template as<int64_t>(),
+                             d_nstep.as<int64_t>(),     d_trace_off.as<int64_t>()};
+            el_adj_start<T><<<dim3(nxi + 1, act), 256, 0, st>>>(
+                A, Sn, n, d_ur_ptr.as<int64_t>(), d_ur_key[li].template as<int64_t>(), d_ur_eptr.as<int64_t>(),
+                d_ur_ent.as<int64_t>(), d_nstep.as<int64_t>(), resid, d_trace_off.as<int64_t>(), nxi, tab, srcv,
+                pre_slot(w, n));
         }
         if (dense) el_adj_dense<T><<<grid(act), 256, 0, st>>>(A, *dense, (int)NX, (int)NY, P);
-        el_src_rcv<T><<<act, 256, 0, st>>>(Sn, n, d_us_ptr.as<int64_t>(), d_us_key[li].template as<int64_t>(),
-                                           d_us_eptr.as<int64_t>(), d_us_ent.as<int64_t>(), srcv,
-                                           d_srcv_off.as<int64_t>(), d_nsrc.as<int64_t>(),
-                                           d_rcv_ptr.as<int64_t>(), d_rcv_key[li].template as<int64_t>(),
-                                           d_nstep.as<int64_t>(), nullptr,
-                                           d_trace_off.as<int64_t>(), 2, pre_slot(w, n));
         ElAdj<T> a;
         a.A = A;
         a.Sn = cst(Sn);
