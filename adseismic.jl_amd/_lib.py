@@ -117,6 +117,7 @@ _SIGS = {
     "ads_elastic_plan_history_scratch_size": (C.c_size_t, [vp]),
     "ads_elastic_forward_history": (C.c_int, [vp] * 8),
     "ads_profile_begin": (C.c_int, []),
+    "ads_profile_begin_sampled": (C.c_int, [C.c_int]),
     "ads_profile_end": (C.c_int, []),
     "ads_profile_query": (C.c_int, [C.c_int, C.POINTER(f64), i64p, C.POINTER(f64)]),
 }

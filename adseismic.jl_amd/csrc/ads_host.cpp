@@ -37,6 +37,8 @@ std::vector<ProfRec> g_recs;
 std::vector<hipEvent_t> g_pool;
 double g_ms[ADS_PROF_COUNT], g_bytes[ADS_PROF_COUNT];
 int64_t g_n[ADS_PROF_COUNT];
+int64_t g_calls[ADS_PROF_COUNT];  // launches seen per class since the last begin
+int g_every = 1;                  // time launch numbers 0, every, 2 every, ... of each class
 
 hipEvent_t pool_get()
 {
@@ -55,8 +57,9 @@ bool prof_enabled() { return g_prof_on; }
 
 int prof_start(int cls, hipStream_t st)
 {
-    if (!g_prof_on) return -1;
+    if (!g_prof_on || cls < 0 || cls >= ADS_PROF_COUNT) return -1;
     std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (g_calls[cls]++ % g_every != 0) return -1;
     ProfRec r{cls, pool_get(), pool_get(), 0.0, 1};
     if (!r.a || !r.b) return -1;
     if (hipEventRecord(r.a, st) != hipSuccess) return -1;
@@ -112,15 +115,19 @@ const char *ads_last_error(void) { return g_err; }
 
 int ads_version(void) { return 100; /* 0.1.0 */ }
 
-int ads_profile_begin(void)
+int ads_profile_begin_sampled(int every)
 {
     clear_error();
+    if (every < 1) ADS_FAIL(ADS_ERR_ARG, "profile sampling stride %d must be >= 1", every);
     std::lock_guard<std::mutex> lk(g_prof_mu);
-    for (int k = 0; k < ADS_PROF_COUNT; k++) g_ms[k] = g_bytes[k] = 0.0, g_n[k] = 0;
+    for (int k = 0; k < ADS_PROF_COUNT; k++) g_ms[k] = g_bytes[k] = 0.0, g_n[k] = g_calls[k] = 0;
     g_recs.clear();
+    g_every = every;
     g_prof_on = true;
     return ADS_OK;
 }
+
+int ads_profile_begin(void) { return ads_profile_begin_sampled(1); }
 
 int ads_profile_end(void)
 {

@@ -412,8 +412,12 @@ int ads_elastic_forward_history(ads_elastic_plan *plan, const void *rho, const v
  * run in a separate pass; elastic forward w(8 + 4 f_x + 4 f_y) stress and w(5 + 4 f_x + 4 f_y)
  * velocity phase, adjoint w(12 + 4 f_x + 4 f_y) per phase: the whole-step counts
  * w(13 + 8 f_x + 8 f_y) and w(24 + 8 f_x + 8 f_y) split by the fields each phase updates).  A launch that advances two time
- * levels (the acoustic forward's two-level pass) counts as two launches.  Not thread-safe across
- * concurrent callers; meant for one benchmarking thread.
+ * levels (the acoustic forward's two-level pass) counts as two launches.
+ * ads_profile_begin_sampled(every) brackets only launches 0, every, 2*every, ... of each class
+ * (counted from the begin call), so that a long timed run carries few events; the queried
+ * duration, launch count and bytes then cover those launches alone.  ads_profile_begin() is
+ * ads_profile_begin_sampled(1).  Not thread-safe across concurrent callers; meant for one
+ * benchmarking thread.
  * ==================================================================================== */
 enum {
     ADS_PROF_AC_FWD = 0,       /* acoustic forward step kernel */
@@ -425,6 +429,7 @@ enum {
     ADS_PROF_COUNT = 6
 };
 int ads_profile_begin(void);
+int ads_profile_begin_sampled(int every);
 int ads_profile_end(void);
 int ads_profile_query(int kernel_class, double *total_ms, int64_t *launches, double *alg_bytes);
 

@@ -336,3 +336,40 @@ def test_elastic_binomial_schedules_match_one_level(orc, ads, cuda, dtype):
     assert plan.snapshots == S and S < -(-p.NSTEP // K) - 1 and plan.workspace_size(K) <= budget
     LS, gS, _ = plan.gradient(*mt, srcv, ob, K)
     assert max(rel(a.cpu(), b) for a, b in zip(gS, g_ref)) < (1e-10 if dtype == torch.float64 else 1e-3)
+
+
+def test_profile_sampling_brackets_every_kth_launch(orc, ads, cuda):
+    """ads_profile_begin_sampled(k) times launches 0, k, 2k, ... of each class: an elastic
+    gradient (one level per launch) then reports ceil(N / k) of the N launches ads_profile_begin
+    counts, with the same algorithmic bytes per launch; a stride below 1 is refused."""
+    import ctypes as C
+    import importlib
+    L = importlib.import_module("adseismic_amd._lib")
+    p, models, srcs, rcvs, nstep = make_case(orc, ads, NSTEP=40, nshots=2, seed=6)
+    plan = ads.ElasticPlan(p, srcs, rcvs, dtype=torch.float32, nstep=nstep)
+    lib = plan.lib
+    srcv = plan.pack_srcv([s.srcv for s in srcs])
+    mt = [T(m, torch.float32) for m in models]
+    ob = plan.forward(*[T(m * 1.02, torch.float32) for m in models], srcv, 7).clone()
+
+    def counts(begin):
+        L.check(begin())
+        plan.gradient(*mt, srcv, ob, 7)
+        torch.cuda.synchronize()
+        L.check(lib.ads_profile_end())
+        out = []
+        for cls in (2, 3, 4, 5):
+            tm, n, b = C.c_double(), C.c_int64(), C.c_double()
+            L.check(lib.ads_profile_query(cls, C.byref(tm), C.byref(n), C.byref(b)))
+            out.append((tm.value, n.value, b.value))
+        return out
+
+    full = counts(lib.ads_profile_begin)
+    for k in (1, 3, 4):
+        part = counts(lambda: lib.ads_profile_begin_sampled(k))
+        for (tf, nf, bf), (tp, np_, bp) in zip(full, part):
+            assert nf > 0 and np_ == -(-nf // k), (k, nf, np_)
+            assert tp > 0.0
+            assert abs(bp / np_ - bf / nf) <= 1e-9 * (bf / nf)
+    assert lib.ads_profile_begin_sampled(0) != 0
+    assert "stride" in lib.ads_last_error().decode()
