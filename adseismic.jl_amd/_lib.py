@@ -96,6 +96,7 @@ _SIGS = {
     "ads_acoustic_plan_set_shape": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "ads_acoustic_plan_set_graph": (C.c_int, [vp, C.c_int]),
     "ads_acoustic_plan_set_blocking": (C.c_int, [vp, C.c_int]),
+    "ads_acoustic_plan_set_frame_rows": (C.c_int, [vp, C.c_int]),
     "ads_acoustic_plan_blocking": (C.c_int, [vp]),
     "ads_acoustic_plan_history_scratch_size": (C.c_size_t, [vp]),
     "ads_acoustic_plan_forward_history": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),

@@ -275,6 +275,11 @@ class AcousticPlan:
         the default; bit-identical to 1)."""
         _lib.check(self.lib.ads_acoustic_plan_set_blocking(self.h, int(levels)))
 
+    def set_frame_rows(self, rows: int = 0):
+        """Rows per item of the two-level PML-frame pass: 4, 8, 16, 32 or 64, or 0 to choose
+        per launch from the active shots and the launch's size (bit-identical results)."""
+        _lib.check(self.lib.ads_acoustic_plan_set_frame_rows(self.h, int(rows)))
+
     @property
     def blocking(self) -> int:
         """Levels per pass in effect: 2 only when no source lies in the interior rectangle."""

@@ -547,15 +547,23 @@ __device__ inline void fwd_frame_body(const FwdArgs<T> &a, const TileMap &map,
 }
 
 // Rows per item of the two-level frame work: 4, 8, .. 64 (FRAME2_H item lists).  Taller items
-// read fewer halo rows, but each wavefront walks its item once per shot of its quarter of the
-// group, so with few active shots tall items make serial chains longer than the rectangle's
-// work.  The launch takes the largest height not above the number of active shots.
+// read fewer halo rows, but each wavefront walks its item, h + 2 rows, once per shot of its
+// quarter of the group, one shot after another.  With few frame wavefronts beside the
+// rectangle's work that walk is the launch's critical path, about 2.3 us per row and shot for
+// the two levels (32-row items at 1000^2 x 32 shots: 156 us per level against 98 us with
+// 8-row items; at 2000^2 x 64 shots 64-row items are the fastest, 610 against 634 us).  The
+// launch takes the tallest height not above the number of active shots whose walk fits in
+// half the pair's streaming time at 5 TB/s.  per_wave: shots each wavefront walks;
+// pair_bytes: algorithmic bytes of the two levels.
 constexpr int FRAME2_H = 5;
 inline int frame2_height(int k) { return 4 << k; }
-inline int frame2_pick(int act)
+inline int frame2_pick(int act, int per_wave, double pair_bytes)
 {
+    const double budget_us = 0.5 * pair_bytes / 5.0e6;
     int k = 0;
-    while (k + 1 < FRAME2_H && frame2_height(k + 1) <= act) k++;
+    while (k + 1 < FRAME2_H && frame2_height(k + 1) <= act &&
+           2.3 * (frame2_height(k + 1) + 2) * per_wave <= budget_us)
+        k++;
     return k;
 }
 
@@ -1506,6 +1514,7 @@ struct AcPlanBase {
     // items of the two-level frame pass (complement of rect1, WAVE - 2 columns each)
     DevBuf d_items2[FRAME2_H];  // two-level frame items of height frame2_height(k)
     int nitems2[FRAME2_H] = {};
+    int frame2_rows = 0;        // forced item height (ads_acoustic_plan_set_frame_rows), 0: chosen per launch
     // kernel 0 with the sources in the frame work: items of WAVE - 2 columns covering the
     // complement of rect[0] and of rect1, for the frame body that also updates phi/psi
     DevBuf d_items_h, d_items1h;
@@ -1716,6 +1725,13 @@ template <typename T> struct AcPlan : AcPlanBase {
     double fwd_level_bytes(int act) const
     {
         return (double)act * (double)NX * (double)NY * sizeof(T) * (4.0 + (variant == 1 ? 4.0 : 2.0) * f_pml);
+    }
+    // item list of the two-level frame work for a launch over act shots in groups of Sg
+    int frame2_index(int act, int Sg) const
+    {
+        for (int k = 0; k < FRAME2_H; k++)
+            if (frame2_height(k) == frame2_rows) return k;
+        return frame2_pick(act, (int)cdiv(std::min(act, Sg), NWAVES), 2.0 * fwd_level_bytes(act));
     }
     FwdArgs<T> fwd_args(const Layout &L, int act, Lv<T> uo, Lv<const T> w, Lv<const T> wo,
                         Lv<const T> phi, Lv<const T> psi, Lv<T> phio, Lv<T> psio, const T *C,
@@ -2040,7 +2056,7 @@ template <typename T> struct AcPlan : AcPlanBase {
         a.map = rect_tiles(R, VECP, RPT, a.act, a.Sg);
         TileMap fm;
         fm.ntj = 1;
-        const int h = frame2_pick(a.act);
+        const int h = frame2_index(a.act, a.Sg);
         fm.nstrips = nitems2[h];
         fm.ngroups = (int)cdiv(a.act, a.Sg);
         fm.nb = fm.ntj * fm.nstrips * fm.ngroups;
@@ -2059,7 +2075,7 @@ template <typename T> struct AcPlan : AcPlanBase {
         a.map.nb = a.map.ntj * a.map.nstrips * a.map.ngroups;
         TileMap fm;
         fm.ntj = 1;
-        const int h = frame2_pick(a.act);
+        const int h = frame2_index(a.act, a.Sg);
         fm.nstrips = nitems2[h];
         fm.ngroups = (int)cdiv(a.act, a.Sg);
         fm.nb = fm.ntj * fm.nstrips * fm.ngroups;
@@ -2861,6 +2877,18 @@ int ads_acoustic_plan_set_graph(ads_acoustic_plan *plan, int enable)
     return dispatch(plan, [&](auto &P) -> int {
         P.use_graph = enable ? 1 : 0;
         if (!enable) P.drop_graphs();
+        return ADS_OK;
+    });
+}
+
+int ads_acoustic_plan_set_frame_rows(ads_acoustic_plan *plan, int rows)
+{
+    clear_error();
+    if (rows != 0 && rows != 4 && rows != 8 && rows != 16 && rows != 32 && rows != 64)
+        ADS_FAIL(ADS_ERR_ARG, "frame rows must be 0, 4, 8, 16, 32 or 64, got %d", rows);
+    return dispatch(plan, [&](auto &P) -> int {
+        if (P.frame2_rows != rows) P.drop_graphs();
+        P.frame2_rows = rows;
         return ADS_OK;
     });
 }

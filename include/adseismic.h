@@ -330,6 +330,12 @@ int ads_acoustic_plan_set_graph(ads_acoustic_plan *plan, int enable);
 int ads_acoustic_plan_set_blocking(ads_acoustic_plan *plan, int levels);
 int ads_acoustic_plan_blocking(const ads_acoustic_plan *plan);
 
+/* Rows per item of the two-level PML-frame pass (no counterpart in the reference): 4, 8, 16,
+ * 32 or 64 force that height; 0 (the default) chooses per launch the tallest height not above
+ * the number of active shots whose per-wavefront walk over the item's rows, shot after shot,
+ * stays well inside the launch's streaming time.  Results are bit-identical for every value. */
+int ads_acoustic_plan_set_frame_rows(ads_acoustic_plan *plan, int rows);
+
 /* Plain drop-in of AcousticPropagatorSolver (Core.jl:562-620) on the plan's shots (every shot
  * must have nstep == NSTEP): writes the full histories u, phi, psi, each laid out
  * [shot][level 0..NSTEP][N] (phi/psi may be NULL).  scratch: device buffer of

@@ -284,10 +284,11 @@ def test_two_level_blocking_is_bit_identical(orc, ads, cuda, kernel, dtype):
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_two_level_frame_heights_bit_identical(orc, ads, cuda, dtype):
-    """The two-level frame work takes items of 4 .. 64 rows, the height chosen from the number
-    of active shots.  66 shots with records of 500 - 7 s steps let that number fall from 66 to
-    1 within one gradient, so every height runs; single- and two-level stepping must still
-    agree bit for bit, with the streaming and the register-blocked rectangle."""
+    """The two-level frame work takes items of 4 .. 64 rows, forced per plan or chosen per
+    launch.  66 shots with records of 500 - 7 s steps let the number of active shots fall from
+    66 to 1 within one gradient; with every forced height and with the per-launch choice,
+    single- and two-level stepping must agree bit for bit, with the streaming and the
+    register-blocked rectangle."""
     p, c, srcs, rcvs, nstep = make_case(orc, ads, NX=70, NY=300, NSTEP=500, npml=8, nshots=66,
                                         kernel=1, seed=77, uneven=True)
     plan = ads.AcousticPlan(p, srcs, rcvs, dtype=dtype, nstep=nstep)
@@ -297,16 +298,19 @@ def test_two_level_frame_heights_bit_identical(orc, ads, cuda, dtype):
     obs = plan.forward(ct * 1.02, srcv, 1000).clone()
     for shape in [(8, 16, 1, 2, 16, 1), (2, 16, 1, 2, 16, 1)]:
         plan.set_shape(shape)
-        res = []
-        for levels in (1, 2):
-            plan.set_blocking(levels)
-            loss, g, tr = plan.gradient(ct, srcv, obs, 1000, want_traces=True)
-            res.append((loss.clone(), g.clone(), tr.clone()))
-        (l1, g1, t1), (l2, g2, t2) = res
-        assert torch.equal(t1, t2), shape
-        assert torch.equal(g1, g2), shape
-        assert torch.equal(l1, l2), shape
-    plan.set_blocking(2)
+        plan.set_blocking(1)
+        l1, g1, t1 = plan.gradient(ct, srcv, obs, 1000, want_traces=True)
+        l1, g1, t1 = l1.clone(), g1.clone(), t1.clone()
+        plan.set_blocking(2)
+        for rows in (4, 8, 16, 32, 64, 0):
+            plan.set_frame_rows(rows)
+            l2, g2, t2 = plan.gradient(ct, srcv, obs, 1000, want_traces=True)
+            assert torch.equal(t1, t2), (shape, rows)
+            assert torch.equal(g1, g2), (shape, rows)
+            assert torch.equal(l1, l2), (shape, rows)
+    with pytest.raises(Exception, match="frame rows"):
+        plan.set_frame_rows(12)
+    plan.set_frame_rows(0)
 
 
 @pytest.mark.parametrize("kernel", [0, 1])
