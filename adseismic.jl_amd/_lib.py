@@ -108,6 +108,7 @@ _SIGS = {
     "ads_elastic_plan_workspace_size": (C.c_size_t, [vp, i64]),
     "ads_elastic_plan_choose_interval": (i64, [vp, C.c_size_t]),
     "ads_elastic_plan_set_adjoint_kernels": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+    "ads_elastic_plan_set_forward_kernels": (C.c_int, [vp, C.c_int, C.c_int]),
     "ads_elastic_plan_set_snapshots": (C.c_int, [vp, i64]),
     "ads_elastic_plan_snapshots": (i64, [vp]),
     "ads_elastic_plan_choose_schedule": (C.c_int, [vp, C.c_size_t, i64p, i64p]),

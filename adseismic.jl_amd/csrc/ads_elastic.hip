@@ -1359,6 +1359,279 @@ __device__ inline int mv_at(const FrameBox &b, const MvCol &cj, int i)
     return cj.mid < 0 ? -1 : top + (i - b.qi0) * wm + cj.mid;
 }
 
+// ---- fused streaming forward step ----
+// One launch per level over the whole padded grid: fw1/fw2 and then fw3/fw4 of the same rows,
+// the new stresses kept in registers for the velocity update instead of a round trip through
+// HBM.  A workgroup is NWAVES wavefronts, one shot each, over one strip of FS_COLS output
+// columns and one chunk of up to ch output rows.  Lane l holds column jb + l, jb = strip *
+// FS_COLS - 4: the stresses of level n are formed on lanes 2 .. WAVE-3 (their y-direction
+// velocity neighbours come from the adjacent lanes), the velocities on the output lanes
+// 4 .. WAVE-5 (their stress neighbours from lanes 2 .. WAVE-3).  The wavefront walks rows
+// r = i0-2 .. i1+1: the stresses of row r from the level n-1 velocities of rows r-2 .. r+2 in
+// a register window, then the velocities of row r-2 from the new stresses of rows r-4 .. r in
+// a second window.  The stresses of the two rows above and below the chunk and of the halo
+// lanes are formed again, with the same arithmetic, by the wavefronts that own them, and only
+// the owner stores them.  A wavefront whose rows and lanes all lie inside R runs the plain
+// walk (no range tests, no memory variables; the CPML is the identity there).  Sources and
+// receivers stay in el_src_rcv after the step: the velocity update reads the stresses as
+// they are before the stress sources are added.
+constexpr int FS_COLS = WAVE - 8;
+
+// Lane shifts as DPP wavefront shifts (VALU moves, no LDS): up1 gives lane l the value of lane
+// l + 1 (wave_shl:1), dn1 that of lane l - 1 (wave_shr:1); a lane shifted in from beyond the
+// wavefront reads 0.
+template <int CTRL> __device__ inline float dpp_move(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL> __device__ inline double dpp_move(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <typename T> __device__ inline T up1(T v) { return dpp_move<0x130>(v); }
+template <typename T> __device__ inline T dn1(T v) { return dpp_move<0x138>(v); }
+
+template <int SUB, typename T>
+__device__ inline void fwd_full_body(const ElC<T> &I, const El<T> &O, const T *__restrict__ rho,
+                                     const T *__restrict__ lam, const T *__restrict__ mu,
+                                     const ElCoef<T> &c, const Rect &R, const StrmMap &mp, int act,
+                                     int bid)
+{
+    int g, chunk, strip;
+    if (!decode_strm(mp, g, chunk, strip, bid)) return;
+    const int s = g * NWAVES + wave_id();
+    if (s >= act) return;  // wave-uniform; the kernel has no barrier
+    const int lane = threadIdx.x % WAVE;
+    const int NXp = c.NX + 1, NYp = c.NY + 1;  // last padded row / column
+    const int i0 = chunk * mp.ch, i1 = min(i0 + mp.ch, NXp + 1);
+    const int jb = strip * FS_COLS - 4, j = jb + lane;
+    const int jc = j < 0 ? 0 : (j > NYp ? NYp : j);  // loads of lanes beyond the grid stay inside it
+    const bool own = lane >= 4 && lane < WAVE - 4 && j <= NYp;
+    const bool sl = lane >= 2 && lane < WAVE - 2;  // lanes whose stresses are used
+    const bool gen = !(i0 - 4 >= 0 && i1 + 3 <= NXp && i0 - 2 >= R.iL && i1 + 2 <= R.iR &&
+                       jb + 2 >= R.jL && jb + WAVE - 2 <= R.jR);
+    const T dt = c.dt, ihx = c.ihx, ihy = c.ihy;
+    const uint32_t pb = (uint32_t)c.P * (uint32_t)sizeof(T), jb4 = (uint32_t)jc * (uint32_t)sizeof(T);
+    const T *__restrict__ vx = I.f(VX, s);
+    const T *__restrict__ vy = I.f(VY, s);
+    const T *__restrict__ ixx = I.f(SXX, s);
+    const T *__restrict__ iyy = I.f(SYY, s);
+    const T *__restrict__ ixy = I.f(SXY, s);
+    T *__restrict__ ovx = O.f(VX, s);
+    T *__restrict__ ovy = O.f(VY, s);
+    T *__restrict__ oxx = O.f(SXX, s);
+    T *__restrict__ oyy = O.f(SYY, s);
+    T *__restrict__ oxy = O.f(SXY, s);
+    auto walk = [&](auto genc) {
+        constexpr bool GEN = decltype(genc)::value;
+        const T *im[8];  // memory variables of level n-1 and n (general walk only)
+        T *om[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            im[q] = GEN ? I.f(M1 + q, s) : nullptr;
+            om[q] = GEN ? O.f(M1 + q, s) : nullptr;
+        }
+        const ProfRow<T> py = GEN ? prof_col(c, jc) : ProfRow<T>{};
+        const FwPart fc = GEN ? fw_col<SUB>(c, j) : FwPart{true, true};
+        const MvCol mci = mv_col(I.mb, jc), mco = mv_col(O.mb, jc);
+        // memory-variable index of (r, jc): compact outside the box, or the full grid
+        auto mvi = [&](const FrameBox &b, const MvCol &m, int r) -> int {
+            return b.H ? mv_at(b, m, r) : r * c.P + jc;
+        };
+        // row r of an array (0 off the grid's rows in the general walk)
+        auto row = [&](const T *__restrict__ f, int r) -> T {
+            const bool ok = !GEN || (r >= 0 && r <= NXp);  // uniform over the wavefront
+            return ok ? ld_at(f, (uint32_t)(ok ? r : 0) * pb + jb4) : T(0);
+        };
+        // the loads of step r: the window rows entering, the level n-1 stresses (and memory
+        // variables m1..m4) of row r, those of m5..m8 of row r-2
+        struct Rw { T x, y, l, u, xx, yy, xy, rh, m1, m2, m3, m4, m5, m6, m7, m8; };
+        auto fetch = [&](int r, Rw &o) {
+            o.x = row(vx, r + 2);
+            o.y = row(vy, r + 1);
+            o.l = row(lam, r + 1);
+            o.u = row(mu, r + 1);
+            o.xx = row(ixx, r);
+            o.yy = row(iyy, r);
+            o.xy = row(ixy, r);
+            o.rh = row(rho, r - 1);
+            o.m1 = o.m2 = o.m3 = o.m4 = o.m5 = o.m6 = o.m7 = o.m8 = T(0);
+            if constexpr (GEN) {
+                const int mi = sl && r >= 0 && r <= NXp ? mvi(I.mb, mci, r) : -1;
+                if (mi >= 0) {
+                    const uint32_t o4 = (uint32_t)mi * (uint32_t)sizeof(T);
+                    o.m1 = ld_at(im[0], o4);
+                    o.m2 = ld_at(im[1], o4);
+                    o.m3 = ld_at(im[2], o4);
+                    o.m4 = ld_at(im[3], o4);
+                }
+                const int mv = own && r - 2 >= i0 ? mvi(I.mb, mci, r - 2) : -1;
+                if (mv >= 0) {
+                    const uint32_t o4 = (uint32_t)mv * (uint32_t)sizeof(T);
+                    o.m5 = ld_at(im[4], o4);
+                    o.m6 = ld_at(im[5], o4);
+                    o.m7 = ld_at(im[6], o4);
+                    o.m8 = ld_at(im[7], o4);
+                }
+            }
+        };
+        const int r0 = i0 - 2;
+        // windows: WX rows r-2 .. r+2, WY rows r-2 .. r+1, LA / MU rows r, r+1, RH rows r-2,
+        // r-1 (level n-1 and parameters); XX rows r-4 .. r, XY rows r-3 .. r, YY rows r-2 .. r
+        // (new stresses)
+        T WX[5], WY[4], LA[2], MU[2], RH[2];
+        T XX[5] = {}, XY[4] = {}, YY[3] = {};
+#pragma unroll
+        for (int k = 0; k < 4; k++) WX[k] = row(vx, r0 - 2 + k);
+#pragma unroll
+        for (int k = 0; k < 3; k++) WY[k] = row(vy, r0 - 2 + k);
+        LA[0] = row(lam, r0);
+        MU[0] = row(mu, r0);
+        RH[0] = row(rho, r0 - 2);
+        Rw nx;
+        fetch(r0, nx);
+        for (int r = r0; r <= i1 + 1; r++) {
+            const Rw cu = nx;
+            if (r <= i1) fetch(r + 1, nx);
+            WX[4] = cu.x;
+            WY[3] = cu.y;
+            LA[1] = cu.l;
+            MU[1] = cu.u;
+            RH[1] = cu.rh;
+            // fw1 / fw2 on row r
+            const T X = WX[2], Y = WY[2];
+            const T vym = dn1(Y), vyp = up1(Y), vymm = dn1(vym);
+            const T vxm = dn1(X), vxp = up1(X), vxpp = up1(vxp);
+            const T mue = up1(MU[0]);
+            const FwPart fr = GEN ? fw_row<SUB>(c, r) : FwPart{true, true};
+            const ProfRow<T> px = GEN ? prof_row(c, r) : ProfRow<T>{};
+            T sxx = cu.xx, syy = cu.yy, sxy = cu.xy;
+            T m1 = T(0), m2 = T(0), m3 = T(0), m4 = T(0);
+            if (!GEN || (fr.A && fc.A)) {
+                const T lb = SUB ? LA[0] : T(0.5) * (LA[1] + LA[0]);
+                const T mb = SUB ? MU[0] : T(0.5) * (MU[1] + MU[0]);
+                const T lm = lb + T(2) * mb;
+                T dx = (T(27) * WX[3] - T(27) * X - WX[4] + WX[1]) * ihx;
+                T dy = (T(27) * Y - T(27) * vym - vyp + vymm) * ihy;
+                if constexpr (GEN) {
+                    m1 = px.bh * cu.m1 + px.ah * dx;
+                    m2 = py.bf * cu.m2 + py.af * dy;
+                    dx += m1;
+                    dy += m2;
+                }
+                sxx += (lm * dx + lb * dy) * dt;
+                syy += (lm * dy + lb * dx) * dt;
+            }
+            if (!GEN || (fr.B && fc.B)) {
+                const T mb = SUB ? MU[0] : T(0.5) * (MU[0] + mue);
+                const T dx = (T(27) * Y - T(27) * WY[1] - WY[3] + WY[0]) * ihx;
+                const T dy = (T(27) * vxp - T(27) * X - vxpp + vxm) * ihy;
+                if constexpr (GEN) {
+                    m3 = px.bf * cu.m3 + px.af * dx;
+                    m4 = py.bh * cu.m4 + py.ah * dy;
+                    sxy += mb * (dx + m3 + dy + m4) * dt;
+                } else {
+                    sxy += mb * (dx + dy) * dt;
+                }
+            }
+            if (own && r >= i0 && r < i1) {
+                const uint32_t ij = (uint32_t)r * pb + jb4;
+                st_at(oxx, ij, sxx);
+                st_at(oyy, ij, syy);
+                st_at(oxy, ij, sxy);
+                if constexpr (GEN) {
+                    const int mo = mvi(O.mb, mco, r);
+                    if (mo >= 0) {
+                        const uint32_t o4 = (uint32_t)mo * (uint32_t)sizeof(T);
+                        st_at(om[0], o4, m1);
+                        st_at(om[1], o4, m2);
+                        st_at(om[2], o4, m3);
+                        st_at(om[3], o4, m4);
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) XX[k] = XX[k + 1];
+#pragma unroll
+            for (int k = 0; k < 3; k++) XY[k] = XY[k + 1];
+#pragma unroll
+            for (int k = 0; k < 2; k++) YY[k] = YY[k + 1];
+            XX[4] = sxx;
+            XY[3] = sxy;
+            YY[2] = syy;
+            // fw3 / fw4 on row i = r - 2
+            const int i = r - 2;
+            if (i >= i0) {  // uniform
+                const T B = XY[1], Q = YY[0];
+                const T xym = dn1(B), xyp = up1(B), xymm = dn1(xym);
+                const T yym = dn1(Q), yyp = up1(Q), yypp = up1(yyp);
+                const T re0 = up1(RH[0]), re1 = up1(RH[1]);
+                const FwPart fv = GEN ? fw_row<SUB>(c, i) : FwPart{true, true};
+                const ProfRow<T> pv = GEN ? prof_row(c, i) : ProfRow<T>{};
+                T vxn = WX[0], vyn = WY[0];
+                T m5 = T(0), m6 = T(0), m7 = T(0), m8 = T(0);
+                if (!GEN || (fv.B && fc.A)) {
+                    const T dx = (T(27) * XX[2] - T(27) * XX[1] - XX[3] + XX[0]) * ihx;
+                    const T dy = (T(27) * B - T(27) * xym - xyp + xymm) * ihy;
+                    if constexpr (GEN) {
+                        m5 = pv.bf * cu.m5 + pv.af * dx;
+                        m6 = py.bf * cu.m6 + py.af * dy;
+                        vxn += (dx + m5 + dy + m6) * dt / RH[0];
+                    } else {
+                        vxn += (dx + dy) * dt / RH[0];
+                    }
+                }
+                if (!GEN || (fv.A && fc.B)) {
+                    const T rb = SUB ? RH[0] : T(0.25) * (RH[0] + RH[1] + re1 + re0);
+                    const T dx = (T(27) * XY[2] - T(27) * B - XY[3] + XY[0]) * ihx;
+                    const T dy = (T(27) * yyp - T(27) * Q - yypp + yym) * ihy;
+                    if constexpr (GEN) {
+                        m7 = pv.bh * cu.m7 + pv.ah * dx;
+                        m8 = py.bh * cu.m8 + py.ah * dy;
+                        vyn += (dx + m7 + dy + m8) * dt / rb;
+                    } else {
+                        vyn += (dx + dy) * dt / rb;
+                    }
+                }
+                if (own && i < i1) {
+                    const uint32_t ij = (uint32_t)i * pb + jb4;
+                    st_at(ovx, ij, vxn);
+                    st_at(ovy, ij, vyn);
+                    if constexpr (GEN) {
+                        const int mo = mvi(O.mb, mco, i);
+                        if (mo >= 0) {
+                            const uint32_t o4 = (uint32_t)mo * (uint32_t)sizeof(T);
+                            st_at(om[4], o4, m5);
+                            st_at(om[5], o4, m6);
+                            st_at(om[6], o4, m7);
+                            st_at(om[7], o4, m8);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) WX[k] = WX[k + 1];
+#pragma unroll
+            for (int k = 0; k < 3; k++) WY[k] = WY[k + 1];
+            LA[0] = LA[1];
+            MU[0] = MU[1];
+            RH[0] = RH[1];
+        }
+    };
+    if (gen) walk(std::true_type{});
+    else walk(std::false_type{});
+}
+template <typename T, int SUB>
+__global__ __launch_bounds__(WAVE *NWAVES) void el_fwd_full(ElC<T> I, El<T> O, const T *__restrict__ rho,
+                                                            const T *__restrict__ lam, const T *__restrict__ mu,
+                                                            ElCoef<T> c, Rect R, StrmMap mp, int act)
+{
+    fwd_full_body<SUB, T>(I, O, rho, lam, mu, c, R, mp, act, (int)blockIdx.x);
+}
+
 // velocity phase, full grid: fw3/fw4 VJP on every cell
 template <int SUB, typename T>
 __device__ inline void adjv_full_body(const ElAdj<T> &a, const ElCoef<T> &c, const ElPar<T> &q,
@@ -2215,6 +2488,9 @@ struct ElPlanBase {
     // parameter terms are summed per group), streaming (1) or LDS-tiled (0) rectangle kernels,
     // and the streaming kernels' rows per workgroup
     int asg = NWAVES, strm = 2, strm_ch = 8;
+    // forward: the fused streaming step (el_fwd_full) on the pitched layout, and its rows per
+    // wavefront chunk (0: chosen per launch)
+    int ffused = 1, fch = 0;
     // snapshot slots of the reversal schedule (< 0: one per segment start, the one-level scheme)
     int64_t snapshots = -1;
     int sub = 0;          // one MPIElastic subdomain (see ElCoef)
@@ -2412,6 +2688,30 @@ template <typename T> struct ElPlan : ElPlanBase {
         const int Sg = std::min(sg, act);
         const dim3 fg(cdiv(nitems[li], NWAVES), 1, act);
         const bool fuse = rr && li == 0 && nitems[li] > 0;  // rectangle + frame in one launch
+        if (li == 0 && ffused && (uint64_t)Npad * sizeof(T) < (uint64_t(1) << 32)) {
+            // one launch for both phases (the whole step's bytes, charged to the stress class)
+            // Taller chunks re-read fewer halo rows (8 rows: 1.19x the step's compulsory bytes,
+            // 32 rows: 1.0x at 600^2 x 16 shots) but give fewer wavefronts, each walking its
+            // rows one after another: below about 12 per SIMD the walk's latency is exposed
+            // (32 rows there: 130 against 104 us per level).  The tallest of 8, 16 and 32 rows
+            // that keeps that many wavefronts.
+            StrmMap mp;
+            mp.nstrip = (int)cdiv(NY + 2, FS_COLS);
+            mp.ch = fch;
+            if (mp.ch <= 0) {
+                mp.ch = 8;
+                for (int h = 16; h <= 32; h *= 2)
+                    if ((int64_t)mp.nstrip * cdiv(NX + 2, h) * act >= 12288) mp.ch = h;
+            }
+            mp.nchunk = (int)cdiv(NX + 2, mp.ch);
+            mp.ngroup = (int)cdiv(act, NWAVES);
+            mp.nb = mp.nstrip * mp.nchunk * mp.ngroup;
+            const unsigned nb = (unsigned)(cdiv(mp.nb, NXCD) * NXCD);
+            const int pr = prof_start(ADS_PROF_EL_FWD_STRESS, st);
+            if (sub) el_fwd_full<T, 1><<<nb, WAVE * NWAVES, 0, st>>>(cst(I), O, rho, lam, mu, c, R, mp, act);
+            else el_fwd_full<T, 0><<<nb, WAVE * NWAVES, 0, st>>>(cst(I), O, rho, lam, mu, c, R, mp, act);
+            prof_stop(pr, st, cells * sizeof(T) * (13.0 + 8.0 * f_x + 8.0 * f_y));
+        } else {
         int pr = prof_start(ADS_PROF_EL_FWD_STRESS, st);
         if (rr) {
             if (li == 0) launch_rect<VECP>(0, R, cst(I), O, rho, lam, mu, c, act, Sg, li, fuse, st);
@@ -2430,6 +2730,7 @@ template <typename T> struct ElPlan : ElPlanBase {
             el_vel_frame<T><<<fg, WAVE * NWAVES, 0, st>>>(cst(I), O, rho, c,
                                                            d_items[li].template as<int4>(), nitems[li]);
         prof_stop(pr, st, cells * sizeof(T) * (5.0 + 4.0 * f_x + 4.0 * f_y));
+        }
         el_src_rcv<T><<<act, 256, 0, st>>>(O, n, d_us_ptr.as<int64_t>(), d_us_key[li].template as<int64_t>(),
                                            d_us_eptr.as<int64_t>(), d_us_ent.as<int64_t>(), srcv,
                                            d_srcv_off.as<int64_t>(), d_nsrc.as<int64_t>(),
@@ -3094,6 +3395,17 @@ int ads_elastic_plan_set_adjoint_kernels(ads_elastic_plan *plan, int streaming, 
     plan->impl->strm = streaming;
     plan->impl->strm_ch = rows;
     plan->impl->asg = shots;
+    return ADS_OK;
+}
+
+int ads_elastic_plan_set_forward_kernels(ads_elastic_plan *plan, int fused, int rows)
+{
+    clear_error();
+    if (!plan || !plan->impl) ADS_FAIL(ADS_ERR_ARG, "null plan");
+    if (fused < 0 || fused > 1 || rows < 0 || rows > 1024)
+        ADS_FAIL(ADS_ERR_ARG, "fused must be 0 or 1 and rows in [0, 1024]");
+    plan->impl->ffused = fused;
+    plan->impl->fch = rows;
     return ADS_OK;
 }
 

@@ -108,6 +108,12 @@ class ElasticPlan:
                                                                  int(rows), int(shots)))
         self._ws_K = None
 
+    def set_forward_kernels(self, fused: int = 1, rows: int = 0) -> None:
+        """Forward kernels (ads_elastic_plan_set_forward_kernels): 1 one fused launch per level
+        walking `rows` rows per wavefront (0: chosen from the grid and the active shots), 0 the
+        stress and velocity phases as two launches."""
+        _lib.check(self.lib.ads_elastic_plan_set_forward_kernels(self.h, int(fused), int(rows)))
+
     def set_snapshots(self, snapshots: int) -> None:
         """Snapshot slots of the reversal schedule: < 0 one per segment start (one-level),
         otherwise the binomial schedule with that many slots."""

@@ -372,6 +372,13 @@ size_t ads_elastic_plan_workspace_size(const ads_elastic_plan *plan, int64_t K);
  * call before ads_elastic_plan_workspace_size and the forward. */
 int ads_elastic_plan_set_adjoint_kernels(ads_elastic_plan *plan, int streaming, int rows,
                                          int shots);
+/* Forward kernels (no counterpart in the reference): fused = 1 (the default) steps a level in
+ * one launch over the whole grid, the stress update two rows ahead of the velocity update in
+ * registers, each wavefront walking `rows` rows of one shot (0, the default: 8, 16 or 32 rows,
+ * the tallest that leaves enough wavefronts to fill the GPU); fused = 0 runs the
+ * stress and the velocity phase as two launches (interior rectangle tiles and the CPML frame
+ * as streaming items).  Sources and receivers are applied after either. */
+int ads_elastic_plan_set_forward_kernels(ads_elastic_plan *plan, int fused, int rows);
 /* As ads_acoustic_plan_choose_interval, over the NSTEP elastic levels. */
 int64_t ads_elastic_plan_choose_interval(const ads_elastic_plan *plan, size_t budget_bytes);
 /* As the acoustic snapshot calls, for the elastic plan: the NSTEP levels are cut into
