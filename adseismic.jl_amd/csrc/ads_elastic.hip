@@ -1748,10 +1748,10 @@ __device__ inline void adjv_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                 const T t1 = GEN ? (f3 ? e3 + py.af * g6 : T(0)) : e3;
                 const T t3 = GEN ? (f4 ? e4 + py.ah * g8 : T(0)) : e4;
                 const T xy = wy[2], yy = oc.yy;
-                const T t1m = lane_at(t1, lane - 1), t1p = lane_at(t1, lane + 1), t1pp = lane_at(t1, lane + 2);
-                const T t3mm = lane_at(t3, lane - 2), t3m = lane_at(t3, lane - 1), t3p = lane_at(t3, lane + 1);
-                const T xymm = lane_at(xy, lane - 2), xym = lane_at(xy, lane - 1), xyp = lane_at(xy, lane + 1);
-                const T yym = lane_at(yy, lane - 1), yyp = lane_at(yy, lane + 1), yypp = lane_at(yy, lane + 2);
+                const T t1m = dn1(t1), t1p = up1(t1), t1pp = up1(t1p);
+                const T t3m = dn1(t3), t3mm = dn1(t3m), t3p = up1(t3);
+                const T xym = dn1(xy), xymm = dn1(xym), xyp = up1(xy);
+                const T yym = dn1(yy), yyp = up1(yy), yypp = up1(yyp);
                 if (out) {
                     const uint32_t ij = (uint32_t)i * pb + jb4;
                     // parameter terms (fw3: rho; fw4: rho-bar), with the forward memory
@@ -1940,10 +1940,10 @@ __device__ inline void adjs_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                 const T t1 = GEN ? (f1 ? gY + py.af * (oc.m2 + gY) : T(0)) : gY;
                 const T t3 = GEN ? (f2 ? gs + py.ah * (oc.m4 + gs) : T(0)) : gs;
                 const T u = wu[2], v = wv[2];
-                const T t1m = lane_at(t1, lane - 1), t1p = lane_at(t1, lane + 1), t1pp = lane_at(t1, lane + 2);
-                const T t3mm = lane_at(t3, lane - 2), t3m = lane_at(t3, lane - 1), t3p = lane_at(t3, lane + 1);
-                const T um = lane_at(u, lane - 1), up = lane_at(u, lane + 1), upp = lane_at(u, lane + 2);
-                const T vmm = lane_at(v, lane - 2), vm = lane_at(v, lane - 1), vp = lane_at(v, lane + 1);
+                const T t1m = dn1(t1), t1p = up1(t1), t1pp = up1(t1p);
+                const T t3m = dn1(t3), t3mm = dn1(t3m), t3p = up1(t3);
+                const T um = dn1(u), up = up1(u), upp = up1(up);
+                const T vm = dn1(v), vmm = dn1(vm), vp = up1(v);
                 if (out) {
                     const uint32_t ij = (uint32_t)i * pb + jb4;
                     // parameter terms: fw1 (lambda-bar, mu-bar) and fw2 (mu-bar), level n-1
