@@ -671,9 +671,24 @@ __device__ inline void el_src_rcv_body(
     __syncthreads();
     const int64_t r0 = rcv_ptr[s], nr = rcv_ptr[s + 1] - r0, nt = nstep[s] + 1;
     T *__restrict__ tr = traces + trace_off[s];
-    for (int64_t r = threadIdx.x; r < nr; r += blockDim.x) {
-        const int64_t key = rcv_key[r0 + r];
-        tr[r * nt + n] = O.f((int)(key & 7), s)[key >> 3];
+    // RB receivers per thread and pass: every key load, then every field load, then the
+    // stores, so a pass waits on two load latencies instead of two per receiver
+    constexpr int RB = 8;
+    for (int64_t rb = threadIdx.x; rb < nr; rb += (int64_t)RB * blockDim.x) {
+        int64_t key[RB];
+        T v[RB];
+#pragma unroll
+        for (int k = 0; k < RB; k++) {
+            const int64_t r = rb + (int64_t)k * blockDim.x;
+            key[k] = r < nr ? rcv_key[r0 + r] : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < RB; k++) v[k] = key[k] >= 0 ? O.f((int)(key[k] & 7), s)[key[k] >> 3] : T(0);
+#pragma unroll
+        for (int k = 0; k < RB; k++) {
+            const int64_t r = rb + (int64_t)k * blockDim.x;
+            if (r < nr) tr[r * nt + n] = v[k];
+        }
     }
 }
 template <typename T>
