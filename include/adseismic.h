@@ -424,7 +424,9 @@ int ads_elastic_forward_history(ads_elastic_plan *plan, const void *rho, const v
  * PropagatorKernel 1; w(4 + 2 f_pml) and w(7 + 2 f_pml) for kernel 0 whose phi/psi updates
  * run in a separate pass; elastic forward w(8 + 4 f_x + 4 f_y) stress and w(5 + 4 f_x + 4 f_y)
  * velocity phase, adjoint w(12 + 4 f_x + 4 f_y) per phase: the whole-step counts
- * w(13 + 8 f_x + 8 f_y) and w(24 + 8 f_x + 8 f_y) split by the fields each phase updates).  A launch that advances two time
+ * w(13 + 8 f_x + 8 f_y) and w(24 + 8 f_x + 8 f_y) split by the fields each phase updates; the
+ * fused elastic forward step, the default, is timed in ADS_PROF_EL_FWD_STRESS with the
+ * whole-step count and leaves ADS_PROF_EL_FWD_VEL empty).  A launch that advances two time
  * levels (the acoustic forward's two-level pass) counts as two launches.
  * ads_profile_begin_sampled(every) brackets only launches 0, every, 2*every, ... of each class
  * (counted from the begin call), so that a long timed run carries few events; the queried
