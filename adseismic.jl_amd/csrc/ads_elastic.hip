@@ -1683,8 +1683,12 @@ __device__ inline void adjv_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
         const T *__restrict__ sxy = a.Sn.f(SXY, s);
         const T *__restrict__ syy = a.Sn.f(SYY, s);
         const uint32_t pb = (uint32_t)P * (uint32_t)sizeof(T), jb4 = (uint32_t)jc * (uint32_t)sizeof(T);
-        auto walk = [&](auto genc) {
+        // DN: walk down the chunk (even chunks), else up (odd chunks), so the window rows two
+        // adjacent chunks share are read by both wavefronts at about the same time
+        auto walk = [&](auto genc, auto dnc) {
             constexpr bool GEN = decltype(genc)::value;
+            constexpr bool DN = decltype(dnc)::value;
+            constexpr int d = DN ? 1 : -1;
             const ProfRow<T> py = GEN ? prof_col(c, jc) : ProfRow<T>{};
             const FwPart fc = GEN ? fw_col<SUB>(c, j) : FwPart{true, true};
             const MvCol mc = mv_col(a.Sn.mb, jc);
@@ -1737,24 +1741,27 @@ __device__ inline void adjv_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                     w2[k] = e4;
                 }
             };
+            const int ia = DN ? i0 : i1 - 1, nrow = i1 - i0;  // first output row, rows
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
+            for (int k = 0; k < 4; k++) {  // the window rows but the one the first step takes
+                const int kk = DN ? k : k + 1;
                 Rw t;
-                fetch_w(i0 - 2 + k, t);
-                finish(i0 - 2 + k, t, k);
+                fetch_w(ia - 2 + kk, t);
+                finish(ia - 2 + kk, t, kk);
             }
             Rw rn;
             Ro on;
-            fetch_w(i0 + 2, rn);
-            fetch_o(i0, on);
-            for (int i = i0; i < i1; i++) {
+            fetch_w(ia + 2 * d, rn);
+            fetch_o(ia, on);
+            int i = ia;
+            for (int m = 0; m < nrow; m++, i += d) {
                 const Rw rc = rn;
                 const Ro oc = on;
-                if (i + 1 < i1) {
-                    fetch_w(i + 3, rn);
-                    fetch_o(i + 1, on);
+                if (m + 1 < nrow) {
+                    fetch_w(i + 3 * d, rn);
+                    fetch_o(i + d, on);
                 }
-                finish(i + 2, rc, 4);
+                finish(i + 2 * d, rc, DN ? 4 : 0);
                 const FwPart fr = GEN ? fw_row<SUB>(c, i) : FwPart{true, true};
                 const bool f3 = fr.B && fc.A, f4 = fr.A && fc.B;
                 const T e3 = e3w[2], e4 = e4w[2];
@@ -1799,23 +1806,45 @@ __device__ inline void adjv_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                     st_at(axy, ij, oc.gxy + d4(t1, t1p, t1m, t1pp) * c.ihy + d4(w2[1], w2[2], w2[0], w2[3]) * c.ihx);
                     st_at(ayy, ij, oc.gyy + d4(t3m, t3, t3mm, t3p) * c.ihy);
                 }
+                if constexpr (DN) {
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    w0[k] = w0[k + 1];
-                    w2[k] = w2[k + 1];
-                    e3w[k] = e3w[k + 1];
-                    e4w[k] = e4w[k + 1];
-                    wx[k] = wx[k + 1];
-                    wy[k] = wy[k + 1];
-                    if constexpr (GEN) {
-                        g5w[k] = g5w[k + 1];
-                        g7w[k] = g7w[k + 1];
+                    for (int k = 0; k < 4; k++) {
+                        w0[k] = w0[k + 1];
+                        w2[k] = w2[k + 1];
+                        e3w[k] = e3w[k + 1];
+                        e4w[k] = e4w[k + 1];
+                        wx[k] = wx[k + 1];
+                        wy[k] = wy[k + 1];
+                        if constexpr (GEN) {
+                            g5w[k] = g5w[k + 1];
+                            g7w[k] = g7w[k + 1];
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 4; k > 0; k--) {
+                        w0[k] = w0[k - 1];
+                        w2[k] = w2[k - 1];
+                        e3w[k] = e3w[k - 1];
+                        e4w[k] = e4w[k - 1];
+                        wx[k] = wx[k - 1];
+                        wy[k] = wy[k - 1];
+                        if constexpr (GEN) {
+                            g5w[k] = g5w[k - 1];
+                            g7w[k] = g7w[k - 1];
+                        }
                     }
                 }
             }
         };
-        if (gen) walk(std::true_type{});
-        else walk(std::false_type{});
+        const bool dn = (chunk & 1) == 0;
+        if (gen) {
+            if (dn) walk(std::true_type{}, std::true_type{});
+            else walk(std::true_type{}, std::false_type{});
+        } else {
+            if (dn) walk(std::false_type{}, std::true_type{});
+            else walk(std::false_type{}, std::false_type{});
+        }
     }
     __syncthreads();
     for (int k = threadIdx.x; k < (i1 - i0) * WAVE; k += blockDim.x) {
@@ -1867,8 +1896,12 @@ __device__ inline void adjs_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
         const T *__restrict__ vx = a.Sm.f(VX, s);
         const T *__restrict__ vy = a.Sm.f(VY, s);
         const uint32_t pb = (uint32_t)P * (uint32_t)sizeof(T), jb4 = (uint32_t)jc * (uint32_t)sizeof(T);
-        auto walk = [&](auto genc) {
+        // DN: walk down the chunk (even chunks), else up (odd chunks), so the window rows two
+        // adjacent chunks share are read by both wavefronts at about the same time
+        auto walk = [&](auto genc, auto dnc) {
             constexpr bool GEN = decltype(genc)::value;
+            constexpr bool DN = decltype(dnc)::value;
+            constexpr int d = DN ? 1 : -1;
             const ProfRow<T> py = GEN ? prof_col(c, jc) : ProfRow<T>{};
             const FwPart fc = GEN ? fw_col<SUB>(c, j) : FwPart{true, true};
             const MvCol mc = mv_col(a.Sn.mb, jc);
@@ -1927,24 +1960,27 @@ __device__ inline void adjs_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                     w2[k] = gs;
                 }
             };
+            const int ia = DN ? i0 : i1 - 1, nrow = i1 - i0;  // first output row, rows
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
+            for (int k = 0; k < 4; k++) {  // the window rows but the one the first step takes
+                const int kk = DN ? k : k + 1;
                 Rw t;
-                fetch_w(i0 - 2 + k, t);
-                finish(i0 - 2 + k, t, k);
+                fetch_w(ia - 2 + kk, t);
+                finish(ia - 2 + kk, t, kk);
             }
             Rw rn;
             Ro on;
-            fetch_w(i0 + 2, rn);
-            fetch_o(i0, on);
-            for (int i = i0; i < i1; i++) {
+            fetch_w(ia + 2 * d, rn);
+            fetch_o(ia, on);
+            int i = ia;
+            for (int m = 0; m < nrow; m++, i += d) {
                 const Rw rc = rn;
                 const Ro oc = on;
-                if (i + 1 < i1) {
-                    fetch_w(i + 3, rn);
-                    fetch_o(i + 1, on);
+                if (m + 1 < nrow) {
+                    fetch_w(i + 3 * d, rn);
+                    fetch_o(i + d, on);
                 }
-                finish(i + 2, rc, 4);
+                finish(i + 2 * d, rc, DN ? 4 : 0);
                 const FwPart fr = GEN ? fw_row<SUB>(c, i) : FwPart{true, true};
                 const bool f1 = fr.A && fc.A, f2 = fr.B && fc.B;
                 // row i: the cotangent products of fw1 / fw2 and the y-direction t-values
@@ -1994,17 +2030,33 @@ __device__ inline void adjs_full_body(const ElAdj<T> &a, const ElCoef<T> &c, con
                     st_at(avx, ij, oc.gvx + d4(w0[1], w0[2], w0[0], w0[3]) * c.ihx + d4(t3m, t3, t3mm, t3p) * c.ihy);
                     st_at(avy, ij, oc.gvy + d4(t1, t1p, t1m, t1pp) * c.ihy + d4(w2[2], w2[3], w2[1], w2[4]) * c.ihx);
                 }
+                if constexpr (DN) {
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    w0[k] = w0[k + 1];
-                    w2[k] = w2[k + 1];
-                    wu[k] = wu[k + 1];
-                    wv[k] = wv[k + 1];
+                    for (int k = 0; k < 4; k++) {
+                        w0[k] = w0[k + 1];
+                        w2[k] = w2[k + 1];
+                        wu[k] = wu[k + 1];
+                        wv[k] = wv[k + 1];
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 4; k > 0; k--) {
+                        w0[k] = w0[k - 1];
+                        w2[k] = w2[k - 1];
+                        wu[k] = wu[k - 1];
+                        wv[k] = wv[k - 1];
+                    }
                 }
             }
         };
-        if (gen) walk(std::true_type{});
-        else walk(std::false_type{});
+        const bool dn = (chunk & 1) == 0;
+        if (gen) {
+            if (dn) walk(std::true_type{}, std::true_type{});
+            else walk(std::true_type{}, std::false_type{});
+        } else {
+            if (dn) walk(std::false_type{}, std::true_type{});
+            else walk(std::false_type{}, std::false_type{});
+        }
     }
     __syncthreads();
     for (int k = threadIdx.x; k < (i1 - i0) * WAVE; k += blockDim.x) {
