@@ -321,6 +321,32 @@ def test_elastic_fused_forward_matches_two_phase(orc, ads, cuda, dtype):
         plan.set_forward_kernels(2, 8)
 
 
+def test_mpi_elastic_fused_forward_matches_two_phase(orc, ads, cuda):
+    """The fused forward step in the subdomain form (no parameter averaging, the block's own
+    update range and profile index) agrees with the two-phase step on an 80 x 80 block, wide
+    enough for plain wavefronts inside its interior rectangle."""
+    p, models, src, rcv, pml = _mpi_case(ads, orc, n=80, NSTEP=40, seed=43)
+    plan = ads.ElasticPlan(p, [ads.ElasticSource(src.srci, src.srcj, src.srctype, src.srcv)],
+                           [ads.ElasticReceiver(rcv.rcvi, rcv.rcvj, rcv.rcvtype)],
+                           dtype=torch.float64)
+    srcv = plan.pack_srcv([src.srcv])
+    mt = [T(m) for m in models]
+    tm = [T(m * f) for m, f in zip(models, (1.02, 0.97, 1.04))]
+    plan.set_forward_kernels(0)
+    obs = plan.forward(*tm, srcv, 9).clone()
+    L0, g0, t0 = plan.gradient(*mt, srcv, obs, 9, want_traces=True)
+    g0 = [x.clone() for x in g0]
+    t0 = t0.clone()
+    for rows in (8, 3, 0):
+        plan.set_forward_kernels(1, rows)
+        assert rel(plan.forward(*tm, srcv, 9).cpu(), obs.cpu()) < 1e-12, rows
+        L1, g1, t1 = plan.gradient(*mt, srcv, obs, 9, want_traces=True)
+        assert rel(t1.cpu(), t0.cpu()) < 1e-12, rows
+        assert abs(L1.item() - L0.item()) <= 1e-12 * abs(L0.item()), rows
+        for a, b in zip(g1, g0):
+            assert rel(a.cpu(), b.cpu()) < 1e-11, rows
+
+
 def _min_rebuilds(n, s, memo={}):
     """Least segment advances to reverse n segments with s free slots (exhaustive search)."""
     key = (n, s)
