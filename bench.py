@@ -44,6 +44,13 @@ PEAK_HBM_GBPS = 8000.0
 # Kernel-class timing brackets launches 0, 7, 14, ... of each class over the timed region with
 # HIP events (ads_profile_begin_sampled): events on every launch cost 1-5% of the throughput.
 PROF_EVERY = 7
+_T_START = time.perf_counter()
+
+
+def progress(msg):
+    """A progress line on stderr (stdout carries only the JSON line): a long run reports each
+    finished gradient, so a watcher can tell it from a hung one."""
+    print(f"bench.py [{time.perf_counter() - _T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
 TRAFFIC_FILES = {
     "uneven": os.path.join(ROOT, "profiles", "traffic_acoustic_2000x2000_64shots.txt"),
     "layered": os.path.join(ROOT, "profiles", "traffic_acoustic_1000x1000_32shots.txt"),
@@ -368,6 +375,8 @@ def acoustic_leg(ads, L, lib, args, p, c_true, c0, srcs, rcvs, nstep, steps, war
     M = -(-(int(plan.nstep.max()) - 1) // K)
     ct = torch.as_tensor(c_true, dtype=torch.float32, device="cuda")
     c0t = torch.as_tensor(c0, dtype=torch.float32, device="cuda")
+    what = f"acoustic {p.NX}x{p.NY}, {len(srcs)} shots, kernel {p.PropagatorKernel}"
+    progress(f"{what}: simulating the observed data (K = {K}, {M} segments)")
     obs = plan.forward(ct, srcv, K).clone()
     torch.cuda.synchronize()
     if args.autotune:
@@ -375,13 +384,17 @@ def acoustic_leg(ads, L, lib, args, p, c_true, c0, srcs, rcvs, nstep, steps, war
     else:
         shape = tuple(int(x) for x in args.shape.split(","))
         plan.set_shape(shape)
-    for _ in range(warmup):
+    for k in range(warmup):
         plan.gradient(c0t, srcv, obs, K)
+        torch.cuda.synchronize()
+        progress(f"{what}: warmup gradient {k + 1}/{warmup} done")
     torch.cuda.synchronize()
     L.check(lib.ads_profile_begin_sampled(PROF_EVERY))
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
         loss, grad, _ = plan.gradient(c0t, srcv, obs, K)
+        torch.cuda.synchronize()  # (one wait per gradient of seconds: no measurable cost)
+        progress(f"{what}: timed gradient {k + 1}/{steps} done")
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     L.check(lib.ads_profile_end())
@@ -490,6 +503,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    progress("loading torch and the HIP library")
     import torch
     import __graft_entry__ as g
 
@@ -560,13 +574,16 @@ def main():
                                f"shots x {pl.NX} receivers, FP32")
         out["acoustic_1000x1000_32shots"] = layered
         if not args.no_plain:
+            progress("plain single-shot path")
             pp = plain_single_shot(ads, pl, c0_l, srcs_l[0], rcvs_l[0], obs0)
             pp["batched_over_plain"] = round(r1["value"] / pp["value"], 2)
             pp["batched_path"] = "acoustic_1000x1000_32shots.PropagatorKernel_1"
             out["plain_single_shot"] = pp
         if not args.no_elastic:
+            progress("elastic 600x600, 16 shots")
             out["elastic"] = elastic_leg(ads, L, lib, 1, args.elastic_nstep)
         if not args.no_cpu:
+            progress("CPU baseline")
             out["cpu_baseline"] = cpu_baseline(p, c0, srcs[0], rcvs[0],
                                                min(args.cpu_steps, int(nst.min())),
                                                "the headline workload")
