@@ -1394,7 +1394,9 @@ constexpr int FS_COLS = WAVE - 8;
 
 // Lane shifts as DPP wavefront shifts (VALU moves, no LDS): up1 gives lane l the value of lane
 // l + 1 (wave_shl:1), dn1 that of lane l - 1 (wave_shr:1); a lane shifted in from beyond the
-// wavefront reads 0.
+// wavefront reads 0.  DPP reads an inactive source lane as that 0 too, so the walks call them
+// where every lane of the wavefront is active (the builtin is convergent: the compiler keeps
+// it there rather than sinking it into the per-lane branches that use the result).
 template <int CTRL> __device__ inline float dpp_move(float v)
 {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
